@@ -1,0 +1,89 @@
+"""Build libfj_mi355x.so for gfx950: generate the per-class HIP units, compile
+them in parallel with hipcc, link one shared library in-tree.
+
+    python -m felizjueves_amd.build [--lmax 2] [--jobs N]
+"""
+
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import time
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+from . import gen_hip
+from .hgp import fock_classes
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+OBJ = HERE / "_build"
+LIB = HERE / "libfj_mi355x.so"
+ARCH = "gfx950"
+LMAX = 2
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+    # packed FP32 VALU forms are kept out of every code object built here
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-variable", "-Wno-unused-parameter",
+]
+
+
+def sources(lmax: int = LMAX) -> list[Path]:
+    classes = fock_classes(lmax)
+    files = gen_hip.generate(classes)
+    gen_hip.write_header(classes)
+    return [CSRC / "fj_runtime.hip", *files]
+
+
+def _compile(src: Path, extra: list[str]) -> tuple[Path, float, str]:
+    OBJ.mkdir(exist_ok=True)
+    obj = OBJ / (src.stem + ".o")
+    deps = [src, CSRC / "fj_device.h", HERE.parent / "include" / "fj_mi355x.h",
+            CSRC / "gen" / "fj_boys_table.h"]
+    if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps if d.exists()):
+        return obj, 0.0, ""
+    t = time.time()
+    r = subprocess.run([HIPCC, *FLAGS, *extra, "-c", str(src), "-o", str(obj)],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-4000:]}")
+    return obj, time.time() - t, r.stderr
+
+
+def build(lmax: int = LMAX, jobs: int | None = None, extra: list[str] | None = None,
+          verbose: bool = True) -> Path:
+    srcs = sources(lmax)
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    t0 = time.time()
+    # biggest units first so the long compiles start early
+    srcs.sort(key=lambda p: -p.stat().st_size)
+    with ThreadPoolExecutor(jobs) as ex:
+        res = list(ex.map(lambda s: _compile(s, extra or []), srcs))
+    if verbose:
+        slow = sorted(res, key=lambda r: -r[1])[:3]
+        print(f"[fj build] {len(res)} units in {time.time() - t0:.1f}s; slowest: " +
+              ", ".join(f"{p.name} {dt:.1f}s" for p, dt, _ in slow), flush=True)
+    objs = [str(p) for p, _, _ in res]
+    tmp = LIB.with_suffix(".so.tmp")
+    r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lmax", type=int, default=LMAX)
+    ap.add_argument("--jobs", type=int, default=None)
+    a = ap.parse_args()
+    print(build(a.lmax, a.jobs))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+// fj_device.h -- shared device code of the gfx950 Fock-build library:
+// argument records, the Boys function, wavefront reductions, error codes.
+// Written for CDNA4 (gfx950, wave64) only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fj_mi355x.h"
+#include "gen/fj_boys_table.h"
+
+#define FJ_SREC 9                 // staged bra record: P(3) X(3) zeta U 1/zeta
+#define FJ_MAX_PRIM_PAIRS 128     // primitive pairs per contracted pair
+#define FJ_MAX_THREADS 256
+#define FJ_DEFAULT_THREADS 64
+#define FJ_TWO_OVER_SQRT_PI 1.1283791670955125739
+#define FJ_PI 3.141592653589793238
+#define FJ_BOYS_ROW 24
+#define FJ_BOYS_TMAX 40.0
+
+// Launch arguments shared by the per-class kernels (both C-ABI entries).
+struct FjFArgs {
+    FjSide bra, ket;
+    const int* bra_list;   // optional: workgroup x -> bra pair index
+    int n_sh, n_ao, diag;
+    double tau;
+    const double* Dsh;
+    const double* D;
+    double* F;
+};
+
+// First primitive record and degree of pair i of one side.
+__device__ __forceinline__ void fj_side_range(const FjSide& s, int i, int& k0, int& k) {
+    if (s.off) {
+        k0 = s.off[i];
+        k = s.off[i + 1] - k0;
+    } else {
+        k0 = i * s.k;
+        k = s.k;
+    }
+}
+
+// Sum of v over the 64 lanes of the wavefront (butterfly, every lane gets it).
+__device__ __forceinline__ double fj_wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Boys function F_m(T), m = 0..M, FP64.
+//  T < 40: F_M by a 9-term Taylor expansion about the nearest tabulated
+//          T_i = i/8 (|T - T_i| <= 1/16, truncation < 1e-16 relative),
+//          e^-T = e^-T_i e^{T_i - T} by the same 9-term series, then the
+//          downward recursion F_{m-1} = (2T F_m + e^-T) / (2m - 1), stable.
+//  T >= 40: F_0 = (1/2) sqrt(pi/T) (erfc(sqrt 40) < 2e-19) and the upward
+//          recursion F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable for T > m.
+// The table holds correctly rounded values of the Kummer series.
+template <int M>
+__device__ __forceinline__ void fj_boys(double T, double (&F)[M + 1]) {
+    static_assert(M + 8 <= 20, "Boys table holds F_0..F_20");
+    if (T < FJ_BOYS_TMAX) {
+        const int i = (int)(T * 8.0 + 0.5);
+        const double x = (double)i * 0.125 - T;
+        const double* __restrict__ r = fj_boys_tab + FJ_BOYS_ROW * i + 1 + M;
+        double f = r[8];
+        f = fma(f, x * (1.0 / 8.0), r[7]);
+        f = fma(f, x * (1.0 / 7.0), r[6]);
+        f = fma(f, x * (1.0 / 6.0), r[5]);
+        f = fma(f, x * (1.0 / 5.0), r[4]);
+        f = fma(f, x * (1.0 / 4.0), r[3]);
+        f = fma(f, x * (1.0 / 3.0), r[2]);
+        f = fma(f, x * (1.0 / 2.0), r[1]);
+        f = fma(f, x, r[0]);
+        if constexpr (M == 0) {
+            F[0] = f;
+        } else {
+            double ex = 1.0 / 40320.0;
+            ex = fma(ex, x, 1.0 / 5040.0);
+            ex = fma(ex, x, 1.0 / 720.0);
+            ex = fma(ex, x, 1.0 / 120.0);
+            ex = fma(ex, x, 1.0 / 24.0);
+            ex = fma(ex, x, 1.0 / 6.0);
+            ex = fma(ex, x, 0.5);
+            ex = fma(ex, x, 1.0);
+            ex = fma(ex, x, 1.0);
+            const double e = fj_boys_tab[FJ_BOYS_ROW * i] * ex;
+            const double t2 = 2.0 * T;
+            F[M] = f;
+#pragma unroll
+            for (int m = M; m > 0; --m) {
+                f = (t2 * f + e) * (1.0 / (double)(2 * m - 1));
+                F[m - 1] = f;
+            }
+        }
+    } else {
+        double f = 0.5 * sqrt(FJ_PI / T);
+        F[0] = f;
+        if constexpr (M > 0) {
+            const double e = T < 80.0 ? exp(-T) : 0.0;
+            const double h = 0.5 / T;
+#pragma unroll
+            for (int m = 1; m <= M; ++m) {
+                f = ((double)(2 * m - 1) * f - e) * h;
+                F[m] = f;
+            }
+        }
+    }
+}

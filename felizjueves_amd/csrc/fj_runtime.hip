@@ -1,0 +1,83 @@
+// fj_runtime.hip -- the library's small non-class entries: version, the
+// closing transpose, the density block maxima, the Boys test entry.
+#include "fj_device.h"
+
+#define FJ_VERSION 1
+
+extern "C" int fj_version(void) { return FJ_VERSION; }
+
+// F <- F + F^T: each thread owns one (i, j), i < j, pair and the diagonal.
+__global__ void fj_close_k(int n, double* __restrict__ F) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)n * (n + 1) / 2;
+    if (t >= total) return;
+    // row-major upper triangle index -> (i, j), i <= j
+    const double nn = 2.0 * n + 1.0;
+    int i = (int)((nn - sqrt(nn * nn - 8.0 * (double)t)) * 0.5);
+    long long start = (long long)i * n - (long long)i * (i - 1) / 2;
+    while (start > t) { --i; start = (long long)i * n - (long long)i * (i - 1) / 2; }
+    while (start + (n - i) <= t) { start += n - i; ++i; }
+    const int j = i + (int)(t - start);
+    if (i == j) {
+        F[(size_t)i * n + i] *= 2.0;
+    } else {
+        const double s = F[(size_t)i * n + j] + F[(size_t)j * n + i];
+        F[(size_t)i * n + j] = s;
+        F[(size_t)j * n + i] = s;
+    }
+}
+
+extern "C" int fj_close(hipStream_t s, int n, double* F) {
+    if (n <= 0 || !F) return FJ_ERR_ARG;
+    const long long total = (long long)n * (n + 1) / 2;
+    const int threads = 256;
+    hipLaunchKernelGGL(fj_close_k, dim3((unsigned)((total + threads - 1) / threads)), dim3(threads), 0, s,
+                       n, F);
+    return (int)hipGetLastError();
+}
+
+// Dsh[s][t] = max |D| over the AO block of shells (s, t): one thread per block.
+__global__ void fj_density_blocks_k(int n_sh, int n_ao, const int* __restrict__ ao,
+                                    const int* __restrict__ sz, const double* __restrict__ D,
+                                    double* __restrict__ Dsh) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_sh * n_sh) return;
+    const int s = t / n_sh, u = t % n_sh;
+    double m = 0.0;
+    for (int i = 0; i < sz[s]; ++i)
+        for (int j = 0; j < sz[u]; ++j) m = fmax(m, fabs(D[(size_t)(ao[s] + i) * n_ao + ao[u] + j]));
+    Dsh[t] = m;
+}
+
+extern "C" int fj_density_blocks(hipStream_t s, int n_sh, int n_ao, const int* shell_ao,
+                                 const int* shell_size, const double* D, double* Dsh) {
+    if (n_sh <= 0 || n_ao <= 0) return FJ_ERR_ARG;
+    const int threads = 256, total = n_sh * n_sh;
+    hipLaunchKernelGGL(fj_density_blocks_k, dim3((total + threads - 1) / threads), dim3(threads), 0, s,
+                       n_sh, n_ao, shell_ao, shell_size, D, Dsh);
+    return (int)hipGetLastError();
+}
+
+template <int M>
+__global__ void fj_boys_k(int n, const double* __restrict__ T, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double F[M + 1];
+    fj_boys<M>(T[i], F);
+#pragma unroll
+    for (int m = 0; m <= M; ++m) out[(size_t)i * (M + 1) + m] = F[m];
+}
+
+extern "C" int fj_boys_eval(hipStream_t s, int M, int n, const double* T, double* out) {
+    if (n < 0) return FJ_ERR_ARG;
+    if (n == 0) return 0;
+    dim3 g((n + 255) / 256), b(256);
+    switch (M) {
+#define FJ_B(m) case m: hipLaunchKernelGGL(fj_boys_k<m>, g, b, 0, s, n, T, out); break;
+        FJ_B(0) FJ_B(1) FJ_B(2) FJ_B(3) FJ_B(4) FJ_B(5) FJ_B(6) FJ_B(7) FJ_B(8) FJ_B(9) FJ_B(10)
+        FJ_B(11) FJ_B(12)
+#undef FJ_B
+        default: return FJ_ERR_ARG;
+    }
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,444 @@
+"""HIP generator for the fused Fock kernels, gfx950 only (wave64).
+
+One translation unit per class ``(la lb|lc ld)`` (AO order, la <= lb,
+lc <= ld, bra pair <= ket pair), written to ``csrc/gen/f_<la><lb><lc><ld>.hip``.
+Each unit holds
+
+* ``fj_q_<cls><R>``: the body for one contracted quartet -- the K_CD x K_AB
+  primitive nest (quartet scalars, Boys, HGP VRR, boundary accumulation),
+  the HRR on the contracted boundary and the six-block digestion -- emitted
+  straight-line from the DAGs of ``hgp.py``, templated on the VRR/HRR real
+  type R (double, or float for the FP32 tier; Boys, the seed, the weighted
+  densities and every F accumulation stay double);
+* the kernel ``fj_f_<cls>_k<R>``: one workgroup per bra pair (its primitive
+  records staged once in LDS with 1/p precomputed), one lane per ket pair,
+  the canonical half and the six-block screen as per-lane predicates, the
+  bra block ``F_ab`` reduced across the wavefront before its atomics;
+* the C-ABI: ``fj_f_<cls>`` with the upstream per-class parameter list
+  (upstream `eri/fock/levers/seams.py:228-235,739-770`) and ``fj_fx_<cls>``,
+  the ragged-degree, precision-selectable entry the native Fock builder
+  uses.
+
+The semantics (what is computed, weights, degeneracy, screen) are upstream
+`eri/fock/kernel_f.py:1-165`; the code layout is this generator's own.
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+from .basis import cartesians, n_cart
+from .hgp import ZERO, hrr_dag, vrr_dag
+
+HERE = Path(__file__).resolve().parent
+GEN = HERE / "csrc" / "gen"
+
+MAX_PRIM_PAIRS = 128   # staged bra records per workgroup (upstream cap)
+BOYS_H = 0.125         # Boys table spacing (exact in binary)
+BOYS_TMAX = 40.0       # table range; above it, the asymptotic branch
+BOYS_MTAB = 20         # orders tabulated: L <= 12 plus 8 more for the 9-term Taylor
+BOYS_ROW = 24          # doubles per row: e^-T, F_0..F_MTAB, padding
+
+
+def cls_name(cls) -> str:
+    return "".join(str(x) for x in cls)
+
+
+def _fidx(c) -> int:
+    """index of a Cartesian vector in the table of every |v| <= n"""
+    n = sum(c)
+    return n * (n + 1) * (n + 2) // 6 + cartesians(n).index(tuple(c))
+
+
+class _Emit:
+    def __init__(self):
+        self.lines: list[str] = []
+
+    def __call__(self, s: str = "", ind: int = 1):
+        self.lines.append("    " * ind + s)
+
+    def text(self) -> str:
+        return "\n".join(self.lines) + "\n"
+
+
+def _vname(node) -> str:
+    e, f, m = node
+    return f"v{_fidx(e)}_{_fidx(f)}_{m}"
+
+
+def _hname(node) -> str:
+    a, b, c, d = node
+    return f"h{_fidx(a)}_{_fidx(b)}_{_fidx(c)}_{_fidx(d)}"
+
+
+AX = "xyz"
+
+
+def emit_quartet(cls) -> str:
+    """The device function for one contracted quartet of class `cls`."""
+    la, lb, lc, ld = cls
+    L = sum(cls)
+    v = vrr_dag(tuple(cls))
+    order, terms, targets, boundary = hrr_dag(tuple(cls))
+    name = cls_name(cls)
+    na, nb, nc, nd = (n_cart(x) for x in cls)
+    # which quartet factors the VRR reads
+    used = set()
+    for n in v.order:
+        r = v.terms[n]
+        if r:
+            d, ts = r
+            for t in ts:
+                used.add((t.coef, d if t.coef in ("PA", "WP", "QC", "WQ") else None))
+    need = lambda c: any(u[0] == c for u in used)  # noqa: E731
+    hrr_bra = any(r and r[0] == "bra" for r in terms.values())
+    hrr_ket = any(r and r[0] == "ket" for r in terms.values())
+    acc = {ef: f"c{i}" for i, ef in enumerate(v.targets)}
+
+    E = _Emit()
+    E(f"// ({la}{lb}|{lc}{ld}): {v.n_nodes} VRR nodes, {len(v.targets)} boundary "
+      f"values, {len(order)} HRR statements, {len(targets)} ERIs", 0)
+    E("template <typename R>", 0)
+    E(f"__device__ __forceinline__ void fj_q_{name}(", 0)
+    E("const double* __restrict__ bp, int k_ab,       // bra records, LDS, FJ_SREC doubles each")
+    E("const double* __restrict__ kp, int k_cd,       // ket records, global, 8 doubles each")
+    E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
+    E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
+    E("double s_j, double s_k, const double* __restrict__ wab,")
+    E("double* __restrict__ fab, double* __restrict__ F) {", 1)
+    for i in range(len(v.targets)):
+        E(f"R c{i} = R(0);")
+    E("for (int kc = 0; kc < k_cd; ++kc) {")
+    E("const double* __restrict__ q_ = kp + 8 * kc;", 2)
+    E("const double Qx = q_[0], Qy = q_[1], Qz = q_[2], q = q_[6], Uq = q_[7];", 2)
+    for d in range(3):
+        if ("QC", d) in used:
+            E(f"const R QC{AX[d]} = R(q_[{3 + d}]);", 2)
+    if need("i2q") or need("i2q_r"):
+        E("const double iq = 1.0 / q;", 2)
+    E("for (int kb = 0; kb < k_ab; ++kb) {", 2)
+    E("const double* __restrict__ p_ = bp + FJ_SREC * kb;", 3)
+    E("const double PQx = p_[0] - Qx, PQy = p_[1] - Qy, PQz = p_[2] - Qz;", 3)
+    E("const double p = p_[6];", 3)
+    E("const double ippq = 1.0 / (p + q);", 3)
+    E("const double rho = p * q * ippq;", 3)
+    E("const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);", 3)
+    E(f"double Fm[{L + 1}];", 3)
+    E(f"fj_boys<{L}>(T, Fm);", 3)
+    E("const double S = p_[7] * Uq * (FJ_TWO_OVER_SQRT_PI * sqrt(rho));", 3)
+    for d in range(3):
+        if ("PA", d) in used:
+            E(f"const R PA{AX[d]} = R(p_[{3 + d}]);", 3)
+        if ("WP", d) in used:
+            E(f"const R WP{AX[d]} = R(-q * ippq * PQ{AX[d]});", 3)
+        if ("WQ", d) in used:
+            E(f"const R WQ{AX[d]} = R(p * ippq * PQ{AX[d]});", 3)
+    if need("i2p"):
+        E("const R i2p = R(0.5 * p_[8]);", 3)
+    if need("i2p_r"):
+        E("const R i2p_r = R(0.5 * p_[8] * rho * p_[8]);", 3)
+    if need("i2q"):
+        E("const R i2q = R(0.5 * iq);", 3)
+    if need("i2q_r"):
+        E("const R i2q_r = R(0.5 * iq * rho * iq);", 3)
+    if need("i2pq"):
+        E("const R i2pq = R(0.5 * ippq);", 3)
+    # VRR, in build order
+    for n in v.order:
+        r = v.terms[n]
+        if not r:
+            E(f"const R {_vname(n)} = R(S * Fm[{n[2]}]);", 3)
+            continue
+        d, ts = r
+        parts = []
+        for t in ts:
+            c = t.coef
+            ch = _vname(t.child)
+            if c in ("PA", "WP", "QC", "WQ"):
+                parts.append(f"{c}{AX[d]} * {ch}")
+            else:
+                m = t.mult
+                sign = "-" if m < 0 else "+"
+                mag = abs(m)
+                f = c if mag == 1 else f"(R({mag}) * {c})"
+                parts.append(f"{sign} {f} * {ch}")
+        expr = parts[0]
+        for p in parts[1:]:
+            expr += (" " + p) if p[0] in "+-" else (" + " + p)
+        E(f"const R {_vname(n)} = {expr};", 3)
+    for ef, cname in acc.items():
+        E(f"{cname} += {_vname((ef[0], ef[1], 0))};", 3)
+    E("}", 2)
+    E("}")
+    # HRR (bra first in build order, then ket)
+    for n in order:
+        r = terms[n]
+        if not r:
+            E(f"const R {_hname(n)} = {acc[(n[0], n[2])]};")
+            continue
+        side, k, moved, lowered = r
+        sep = ("AB" if side == "bra" else "CD") + AX[k]
+        E(f"const R {_hname(n)} = {_hname(moved)} + R({sep}) * {_hname(lowered)};")
+    # digestion: five ket-dependent blocks here, f_ab into the caller's buffer
+    ca, cb, cc, cd = (cartesians(x) for x in cls)
+    blocks = {"cd": (2, 3), "ac": (0, 2), "ad": (0, 3), "bc": (1, 2), "bd": (1, 3)}
+    comp = {"ab": (2, 3), "cd": (0, 1), "ac": (1, 3), "ad": (1, 2), "bc": (0, 3), "bd": (0, 2)}
+    sizes = (na, nb, nc, nd)
+    aov = ("oa", "ob", "oc", "od")
+    # weighted densities: w_cd (for F_ab) and the four exchange ones; w_ab comes in
+    E("// the weighted densities (weight x degeneracy folded in once)")
+    for blk in ("cd", "ac", "ad", "bc", "bd"):
+        p0, p1 = blocks[blk]
+        sc = "s_j" if blk == "cd" else "s_k"
+        for i in range(sizes[p0]):
+            for j in range(sizes[p1]):
+                E(f"const double w{blk}{i}_{j} = {sc} * D[({aov[p0]} + {i}) * n_ao + {aov[p1]} + {j}];")
+    for blk in ("cd", "ac", "ad", "bc", "bd"):
+        p0, p1 = blocks[blk]
+        for i in range(sizes[p0]):
+            for j in range(sizes[p1]):
+                E(f"double f{blk}{i}_{j} = 0.0;")
+    for t in targets:
+        idx = tuple(cs.index(x) for cs, x in zip((ca, cb, cc, cd), t))
+        E(f"{{ const double x = double({_hname(t)});")
+        for blk in ("ab", "cd", "ac", "ad", "bc", "bd"):
+            p0, p1 = (0, 1) if blk == "ab" else blocks[blk]
+            q0, q1 = comp[blk]
+            if (q0, q1) == (0, 1):
+                w = f"wab[{idx[0] * nb + idx[1]}]"
+            else:
+                w = f"w{_blkname(q0, q1)}{idx[q0]}_{idx[q1]}"
+            if blk == "ab":
+                E(f"  fab[{idx[0] * nb + idx[1]}] += {w} * x;")
+            else:
+                E(f"  f{blk}{idx[p0]}_{idx[p1]} += {w} * x;")
+        E("}")
+    for blk in ("cd", "ac", "ad", "bc", "bd"):
+        p0, p1 = blocks[blk]
+        for i in range(sizes[p0]):
+            for j in range(sizes[p1]):
+                E(f"unsafeAtomicAdd(F + ({aov[p0]} + {i}) * n_ao + {aov[p1]} + {j}, f{blk}{i}_{j});")
+    E("}", 0)
+    return E.text()
+
+
+def _blkname(p0: int, p1: int) -> str:
+    return "abcd"[p0] + "abcd"[p1]
+
+
+def emit_class(cls) -> str:
+    la, lb, lc, ld = cls
+    name = cls_name(cls)
+    na, nb = n_cart(la), n_cart(lb)
+    nab = na * nb
+    out = [f"// generated by felizjueves_amd/gen_hip.py -- class ({la}{lb}|{lc}{ld})",
+           '#include "../fj_device.h"', ""]
+    out.append(emit_quartet(cls))
+    out.append(f"""
+template <typename R>
+__global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
+    __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
+    const int bra = a.bra_list ? a.bra_list[blockIdx.x] : (int)blockIdx.x;
+    const int kmin = a.diag * bra;
+    const int k0 = (int)(blockIdx.y * blockDim.x);
+    if (k0 + (int)blockDim.x <= kmin || k0 >= a.ket.n) return;   // whole block below the canonical half
+    int kb0, kab;
+    fj_side_range(a.bra, bra, kb0, kab);
+    for (int i = threadIdx.x; i < kab; i += blockDim.x) {{
+        const double* __restrict__ r = a.bra.prim + 8 * (size_t)(kb0 + i);
+        double* s = bsh + FJ_SREC * i;
+#pragma unroll
+        for (int f = 0; f < 8; ++f) s[f] = r[f];
+        s[8] = 1.0 / r[6];
+    }}
+    __syncthreads();
+    const int sa = a.bra.sh[2 * bra], sb = a.bra.sh[2 * bra + 1];
+    const int oa = a.bra.ao[2 * bra], ob = a.bra.ao[2 * bra + 1];
+    const double Gb = a.bra.G[bra];
+    const double ABx = {"a.bra.sep[3 * bra]" if lb else "0.0"}, ABy = {"a.bra.sep[3 * bra + 1]" if lb else "0.0"}, ABz = {"a.bra.sep[3 * bra + 2]" if lb else "0.0"};
+    const int k = k0 + (int)threadIdx.x;
+    double fab[{nab}];
+    double wab[{nab}];
+#pragma unroll
+    for (int i = 0; i < {nab}; ++i) fab[i] = 0.0;
+    bool live = k < a.ket.n && k >= kmin;
+    int sc = 0, sd = 0;
+    if (live) {{
+        sc = a.ket.sh[2 * k]; sd = a.ket.sh[2 * k + 1];
+        const double* __restrict__ Dsh = a.Dsh;
+        const int ns = a.n_sh;
+        double dmax = 2.0 * Dsh[sa * ns + sb];
+        dmax = fmax(dmax, 2.0 * Dsh[sc * ns + sd]);
+        dmax = fmax(dmax, 0.5 * Dsh[sa * ns + sc]);
+        dmax = fmax(dmax, 0.5 * Dsh[sa * ns + sd]);
+        dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sc]);
+        dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sd]);
+        live = !(Gb * (a.ket.G[k] * dmax) < a.tau);
+    }}
+    if (live) {{
+        double deg = 1.0;
+        if (sa == sb) deg *= 0.5;
+        if (sc == sd) deg *= 0.5;
+        if (a.diag && k == bra) deg *= 0.5;
+        const double s_j = 2.0 * deg, s_k = -0.5 * deg;
+        const int oc = a.ket.ao[2 * k], od = a.ket.ao[2 * k + 1];
+#pragma unroll
+        for (int i = 0; i < {na}; ++i)
+#pragma unroll
+            for (int j = 0; j < {nb}; ++j) wab[i * {nb} + j] = s_j * a.D[(oa + i) * a.n_ao + ob + j];
+        int kc0, kcd;
+        fj_side_range(a.ket, k, kc0, kcd);
+        const double CDx = {"a.ket.sep[3 * k]" if ld else "0.0"}, CDy = {"a.ket.sep[3 * k + 1]" if ld else "0.0"}, CDz = {"a.ket.sep[3 * k + 2]" if ld else "0.0"};
+        fj_q_{name}<R>(bsh, kab, a.ket.prim + 8 * (size_t)kc0, kcd, ABx, ABy, ABz, CDx, CDy, CDz,
+                        a.D, a.n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, a.F);
+    }}
+    // F_ab: one bra per workgroup, so every lane of a wavefront adds into the
+    // same block; reduce across the wavefront, then one atomic per element.
+#pragma unroll
+    for (int i = 0; i < {nab}; ++i) {{
+        const double s = fj_wave_sum(fab[i]);
+        if ((threadIdx.x & 63) == 0 && s != 0.0)
+            unsafeAtomicAdd(a.F + (oa + i / {nb}) * a.n_ao + ob + i % {nb}, s);
+    }}
+}}
+
+static int fj_launch_{name}(hipStream_t s, int threads, int precision, const FjFArgs& a, int n_bra_blocks) {{
+    if (threads <= 0) threads = FJ_DEFAULT_THREADS;
+    if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
+    if (n_bra_blocks <= 0 || a.ket.n <= 0) return 0;
+    dim3 grid((unsigned)n_bra_blocks, (unsigned)((a.ket.n + threads - 1) / threads));
+    if (precision == 0)
+        hipLaunchKernelGGL(fj_f_{name}_k<double>, grid, dim3(threads), 0, s, a);
+    else
+        hipLaunchKernelGGL(fj_f_{name}_k<float>, grid, dim3(threads), 0, s, a);
+    return (int)hipGetLastError();
+}}
+
+extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+                        int n_sh, int n_ao, int diag, double tau,
+                        const double* bra_prim,{" const double* AB," if lb else ""} const double* G_ab, const int* bra_sh, const int* bra_ao,
+                        const double* ket_prim,{" const double* CD," if ld else ""} const double* G_cd, const int* ket_sh, const int* ket_ao,
+                        const double* Dsh, const double* D, double* F) {{
+    if (k_ab < 1 || k_ab > FJ_MAX_PRIM_PAIRS || k_cd < 1 || k_cd > FJ_MAX_PRIM_PAIRS) return FJ_ERR_DEGREE;
+    if (n_bra < 0 || n_ket < 0 || (diag && n_bra != n_ket)) return FJ_ERR_SHAPE;
+    FjFArgs a;
+    a.bra = FjSide{{bra_prim, nullptr, k_ab, {"AB" if lb else "nullptr"}, G_ab, bra_sh, bra_ao, n_bra}};
+    a.ket = FjSide{{ket_prim, nullptr, k_cd, {"CD" if ld else "nullptr"}, G_cd, ket_sh, ket_ao, n_ket}};
+    a.bra_list = nullptr;
+    a.n_sh = n_sh; a.n_ao = n_ao; a.diag = diag; a.tau = tau;
+    a.Dsh = Dsh; a.D = D; a.F = F;
+    return fj_launch_{name}(s, threads, 0, a, n_bra);
+}}
+
+extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
+                         const FjSide* bra, const FjSide* ket, const int* bra_list, int n_bra_blocks,
+                         int diag, int n_sh, int n_ao, double tau,
+                         const double* Dsh, const double* D, double* F) {{
+    FjFArgs a;
+    a.bra = *bra; a.ket = *ket; a.bra_list = bra_list;
+    a.n_sh = n_sh; a.n_ao = n_ao; a.diag = diag; a.tau = tau;
+    a.Dsh = Dsh; a.D = D; a.F = F;
+    return fj_launch_{name}(s, threads, precision, a, n_bra_blocks);
+}}
+""")
+    return "\n".join(out)
+
+
+# ---------------------------------------------------------------------------
+# Boys table: rows T_i = i * BOYS_H, i = 0..N; row = (e^-T_i, F_0(T_i) ..
+# F_MTAB(T_i), 0 padding), every value the correctly rounded double of the
+# Kummer series summed in 40-digit decimal arithmetic:
+#   F_m(T) = e^-T sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)).
+
+def boys_table() -> list[list[float]]:
+    from decimal import Decimal, getcontext
+    getcontext().prec = 40
+    n_rows = int(round(BOYS_TMAX / BOYS_H)) + 1
+    rows = []
+    for i in range(n_rows):
+        T = Decimal(i) / Decimal(8)
+        eT = (-T).exp()
+        row = [float(eT)]
+        M = BOYS_MTAB
+        term = Decimal(1) / Decimal(2 * M + 1)
+        s = term
+        k = 1
+        while True:
+            term = term * 2 * T / Decimal(2 * M + 2 * k + 1)
+            s += term
+            if term < s * Decimal("1e-38"):
+                break
+            k += 1
+        f = eT * s
+        vals = [f]
+        for m in range(M, 0, -1):
+            f = (2 * T * f + eT) / Decimal(2 * m - 1)
+            vals.append(f)
+        vals = vals[::-1]  # F_0 .. F_M
+        row += [float(x) for x in vals]
+        row += [0.0] * (BOYS_ROW - len(row))
+        rows.append(row)
+    return rows
+
+
+def emit_boys_table() -> str:
+    rows = boys_table()
+    body = ",\n".join("    " + ", ".join(float.hex(x) for x in r) for r in rows)
+    return (f"// generated by felizjueves_amd/gen_hip.py: Boys table, {len(rows)} rows "
+            f"x {BOYS_ROW}\n#pragma once\n"
+            f"#define FJ_BOYS_NROW {len(rows)}\n"
+            f"__device__ const double fj_boys_tab[{len(rows)} * {BOYS_ROW}] = {{\n{body}\n}};\n")
+
+
+def generate(classes, out: Path = GEN) -> list[Path]:
+    out.mkdir(parents=True, exist_ok=True)
+    files = []
+    tab = out / "fj_boys_table.h"
+    text = emit_boys_table()
+    if not tab.exists() or tab.read_text() != text:
+        tab.write_text(text)
+    for cls in classes:
+        p = out / f"f_{cls_name(cls)}.hip"
+        text = emit_class(cls)
+        if not p.exists() or p.read_text() != text:
+            p.write_text(text)
+        files.append(p)
+    return files
+
+
+def header_decls(classes) -> str:
+    """The F-class declarations of include/fj_mi355x.h."""
+    out = []
+    for cls in classes:
+        la, lb, lc, ld = cls
+        n = cls_name(cls)
+        out.append(
+            f"int fj_f_{n}(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,\n"
+            f"    int n_sh, int n_ao, int diag, double tau,\n"
+            f"    const double* bra_prim,{' const double* AB,' if lb else ''} const double* G_ab,"
+            f" const int* bra_sh, const int* bra_ao,\n"
+            f"    const double* ket_prim,{' const double* CD,' if ld else ''} const double* G_cd,"
+            f" const int* ket_sh, const int* ket_ao,\n"
+            f"    const double* Dsh, const double* D, double* F);\n"
+            f"int fj_fx_{n}(fj_stream_t stream, int threads, int precision, const FjSide* bra,"
+            f" const FjSide* ket,\n"
+            f"    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,\n"
+            f"    const double* Dsh, const double* D, double* F);")
+    return "\n".join(out) + "\n"
+
+
+HEADER = HERE.parent / "include" / "fj_mi355x.h"
+BEGIN, END = ("/* BEGIN F CLASSES (written by felizjueves_amd/gen_hip.py) */",
+              "/* END F CLASSES */")
+
+
+def header_text(classes, current: str) -> str:
+    i, j = current.index(BEGIN), current.index(END)
+    return current[:i + len(BEGIN)] + "\n" + header_decls(classes) + current[j:]
+
+
+def write_header(classes) -> None:
+    cur = HEADER.read_text()
+    new = header_text(classes, cur)
+    if new != cur:
+        HEADER.write_text(new)

@@ -1,0 +1,147 @@
+"""ctypes binding of libfj_mi355x.so (the gfx950 HIP library).
+
+The product path runs only through this library: if it is missing, or no
+GPU is present, every call raises -- there is no CPU fallback.
+Device buffers are torch tensors on ``cuda:0`` (PyTorch-ROCm is used for
+device memory and streams only).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libfj_mi355x.so"
+
+_lib = None
+
+
+class FjSide(C.Structure):
+    """Mirror of ``FjSide`` in include/fj_mi355x.h."""
+
+    _fields_ = [("prim", C.c_void_p), ("off", C.c_void_p), ("k", C.c_int),
+                ("sep", C.c_void_p), ("G", C.c_void_p), ("sh", C.c_void_p),
+                ("ao", C.c_void_p), ("n", C.c_int)]
+
+
+class FjError(RuntimeError):
+    pass
+
+
+ERRORS = {9001: "FJ_ERR_THREADS", 9002: "FJ_ERR_DEGREE", 9003: "FJ_ERR_SHAPE",
+          9004: "FJ_ERR_CLASS", 9005: "FJ_ERR_ARG"}
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise FjError(f"{what} returned {rc} ({ERRORS.get(rc, 'hipError_t')})")
+
+
+def lib():
+    """Load the library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FjError(f"{LIB_PATH} is missing: run `python -m felizjueves_amd.build` "
+                          "(the HIP library is the only implementation)")
+        L = C.CDLL(str(LIB_PATH))
+        L.fj_version.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def has(symbol: str) -> bool:
+    return hasattr(lib(), symbol)
+
+
+def stream_handle(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def ptr(t) -> int | None:
+    if t is None:
+        return None
+    return int(t.data_ptr())
+
+
+def device():
+    import torch
+    if not torch.cuda.is_available():
+        raise FjError("no GPU: the Fock build runs only on the gfx950 device")
+    return torch.device("cuda:0")
+
+
+def to_dev(a, dtype=None):
+    import torch
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device()).contiguous()
+
+
+def call_f(cls, threads: int, args: dict, F, stream=None) -> None:
+    """One launch of the upstream-ABI entry fj_f_<cls> on device tensors
+    (`args` by upstream parameter name)."""
+    la, lb, lc, ld = cls
+    fn = getattr(lib(), "fj_f_" + "".join(map(str, cls)))
+    a = args
+    vals = [C.c_void_p(stream_handle(stream)), C.c_int(threads), C.c_int(a["n_bra"]),
+            C.c_int(a["n_ket"]), C.c_int(a["k_ab"]), C.c_int(a["k_cd"]), C.c_int(a["n_sh"]),
+            C.c_int(a["n_ao"]), C.c_int(a["diag"]), C.c_double(a["tau"]),
+            C.c_void_p(ptr(a["bra_prim"]))]
+    if lb:
+        vals.append(C.c_void_p(ptr(a["AB"])))
+    vals += [C.c_void_p(ptr(a["G_ab"])), C.c_void_p(ptr(a["bra_sh"])),
+             C.c_void_p(ptr(a["bra_ao"])), C.c_void_p(ptr(a["ket_prim"]))]
+    if ld:
+        vals.append(C.c_void_p(ptr(a["CD"])))
+    vals += [C.c_void_p(ptr(a["G_cd"])), C.c_void_p(ptr(a["ket_sh"])),
+             C.c_void_p(ptr(a["ket_ao"])), C.c_void_p(ptr(a["Dsh"])),
+             C.c_void_p(ptr(a["D"])), C.c_void_p(ptr(F))]
+    fn.restype = C.c_int
+    check(fn(*vals), fn.__name__)
+
+
+def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, bra_list, n_blocks: int,
+            diag: int, n_sh: int, n_ao: int, tau: float, Dsh, D, F, stream=None) -> None:
+    fn = getattr(lib(), "fj_fx_" + "".join(map(str, cls)))
+    fn.restype = C.c_int
+    rc = fn(C.c_void_p(stream_handle(stream)), C.c_int(threads), C.c_int(precision),
+            C.byref(bra), C.byref(ket), C.c_void_p(ptr(bra_list)), C.c_int(n_blocks),
+            C.c_int(diag), C.c_int(n_sh), C.c_int(n_ao), C.c_double(tau),
+            C.c_void_p(ptr(Dsh)), C.c_void_p(ptr(D)), C.c_void_p(ptr(F)))
+    check(rc, fn.__name__)
+
+
+def close(F, stream=None) -> None:
+    L = lib()
+    L.fj_close.restype = C.c_int
+    check(L.fj_close(C.c_void_p(stream_handle(stream)), C.c_int(F.shape[0]),
+                     C.c_void_p(ptr(F))), "fj_close")
+
+
+def density_blocks(shell_ao, shell_size, D, Dsh, stream=None) -> None:
+    L = lib()
+    L.fj_density_blocks.restype = C.c_int
+    check(L.fj_density_blocks(C.c_void_p(stream_handle(stream)), C.c_int(Dsh.shape[0]),
+                              C.c_int(D.shape[0]), C.c_void_p(ptr(shell_ao)),
+                              C.c_void_p(ptr(shell_size)), C.c_void_p(ptr(D)),
+                              C.c_void_p(ptr(Dsh))), "fj_density_blocks")
+
+
+def boys(M: int, T) -> np.ndarray:
+    """Evaluate the device Boys function at host values T (test entry)."""
+    import torch
+    Td = to_dev(np.asarray(T, dtype=np.float64))
+    out = torch.zeros((Td.shape[0], M + 1), dtype=torch.float64, device=device())
+    L = lib()
+    L.fj_boys_eval.restype = C.c_int
+    check(L.fj_boys_eval(C.c_void_p(stream_handle()), C.c_int(M), C.c_int(Td.shape[0]),
+                         C.c_void_p(ptr(Td)), C.c_void_p(ptr(out))), "fj_boys_eval")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()

@@ -1,0 +1,254 @@
+/*
+ * fj_mi355x.h -- C-ABI of libfj_mi355x.so, the gfx950 (MI355X) two-electron
+ * Fock-build library.  Plain pointers and sizes; every pointer argument is
+ * DEVICE memory unless stated; every entry is asynchronous on `stream` and
+ * returns 0 or an FJ_ERR_* / hipError_t code (the library validates shapes
+ * before launching; the device has no error channel).
+ *
+ * Upstream interfaces replaced (paths relative to the FelizJueves checkout):
+ *   fj_f_<la><lb><lc><ld>   the emitted fused-F unit f_<la><lb><lc><ld>
+ *                           (eri/fock/kernel_f.py:655-692; parameter list
+ *                           eri/fock/levers/seams.py:228-235,739-770), one
+ *                           launch of walk_f's run(kernel, grid, block, args)
+ *                           (eri/fock/walk_f.py:385-401)
+ *   fj_fx_<la><lb><lc><ld>  the same kernel over ragged-degree pair tables
+ *                           and a precision tier (no upstream counterpart:
+ *                           the native builder's entry; the FP32 tier plays
+ *                           the role of the fmixed lever,
+ *                           eri/fock/levers/mixed.py:117-255)
+ *
+ * Ownership (upstream eri/j_coulomb/kernel_jab.py:38-43, walk_f.py:412-413):
+ * the caller owns every buffer; F is ACCUMULATED (+=) and not closed -- the
+ * caller zeroes it before the first launch of a build and closes
+ * F <- F + F^T after the last (fj_close).
+ */
+#ifndef FJ_MI355X_H
+#define FJ_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* fj_stream_t; /* a hipStream_t; NULL = default stream */
+
+/* error codes (hipError_t values are passed through unchanged) */
+#define FJ_OK 0
+#define FJ_ERR_THREADS 9001  /* threads not a positive multiple of 64, <= 256 */
+#define FJ_ERR_DEGREE 9002   /* primitive-pair degree outside 1..128 */
+#define FJ_ERR_SHAPE 9003    /* inconsistent sizes (e.g. diag with n_bra != n_ket) */
+#define FJ_ERR_CLASS 9004    /* no kernel for the requested class */
+#define FJ_ERR_ARG 9005      /* another invalid argument */
+
+/* One side of an F launch, as device pointers.  Pair i's primitive records
+ * are prim[off[i] .. off[i+1]) (off == NULL: uniform degree k, records
+ * prim[i*k .. (i+1)*k)); a record is 8 doubles (P_x,P_y,P_z, X_x,X_y,X_z,
+ * zeta, U) with X = P - A (bra) or Q - C (ket) and
+ * U = c_a c_b (pi/zeta)^{3/2} exp(-alpha beta/zeta |AB|^2)
+ * (upstream eri/fock/kernel_f.py:529-606, eri/fock/walk_f.py:188-210).
+ * sep[i] = A - B (3 doubles; may be NULL when the side's second shell is s),
+ * G[i] the contracted Schwarz-type bound, sh[i] = (a, b) shell indices,
+ * ao[i] = first AOs of the two shells. */
+typedef struct FjSide {
+    const double* prim;
+    const int* off;
+    int k;
+    const double* sep;
+    const double* G;
+    const int* sh;
+    const int* ao;
+    int n;
+} FjSide;
+
+/* Library version (also checks that the code object loaded). */
+int fj_version(void);
+
+/* F <- F + F^T for an n x n row-major device matrix (upstream
+ * eri/fock/kernel_f.py:621-628, close). */
+int fj_close(fj_stream_t stream, int n, double* F);
+
+/* Dsh[s][t] = max |D| over shell block (s, t) (upstream
+ * eri/fock/kernel_f.py:608-618, density_blocks).  shell_ao / shell_size
+ * are device int arrays of n_sh entries. */
+int fj_density_blocks(fj_stream_t stream, int n_sh, int n_ao, const int* shell_ao,
+                      const int* shell_size, const double* D, double* Dsh);
+
+/* Boys function F_m(T), m = 0..M (M <= 12), for n values of T: out[i][m]
+ * (upstream emit/_c.py:144-208, fj_boys<M>; host truth
+ * reference/boys.py:184-218).  Test entry for the device function. */
+int fj_boys_eval(fj_stream_t stream, int M, int n, const double* T, double* out);
+
+/* BEGIN F CLASSES (written by felizjueves_amd/gen_hip.py) */
+int fj_f_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0000(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0001(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0011(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0101(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0211(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const double* Dsh, const double* D, double* F);
+/* END F CLASSES */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,99 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product package never
+imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build() -> Path:
+    """Compile liboracle.so from fj_oracle.c (make)."""
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        L.oracle_boys.argtypes = [C.c_int, C.c_double, f64p]
+        basis_args = [i32p, f64p, i32p, i32p, f64p, f64p]
+        for fn in ("oracle_eri_mmd", "oracle_eri_hgp"):
+            getattr(L, fn).argtypes = [C.c_int] * 4 + basis_args + [f64p]
+        L.oracle_fock_ordered.argtypes = ([C.c_int] + basis_args +
+                                          [i32p, C.c_int, f64p, f64p, f64p])
+        L.oracle_schwarz.argtypes = [C.c_int] + basis_args + [i32p, C.c_int, f64p]
+        vp = C.c_void_p
+        L.oracle_f_launch.argtypes = ([C.c_int] * 11 + [C.c_double] +
+                                      [vp] * 13)
+        _lib = L
+    return _lib
+
+
+def _b(basis):
+    return (basis.shell_l, np.ascontiguousarray(basis.shell_centre),
+            basis.shell_nprim, basis.shell_pstart, basis.exps, basis.coefs)
+
+
+def boys(M: int, T: float) -> np.ndarray:
+    out = np.zeros(M + 1)
+    lib().oracle_boys(M, float(T), out)
+    return out
+
+
+def eri(basis, q, method: str = "hgp") -> np.ndarray:
+    shape = tuple(basis.shells[s].n_cart for s in q)
+    out = np.zeros(int(np.prod(shape)))
+    fn = lib().oracle_eri_hgp if method == "hgp" else lib().oracle_eri_mmd
+    fn(*[int(s) for s in q], *_b(basis), out)
+    return out.reshape(shape)
+
+
+def fock_ordered(basis, D) -> tuple[np.ndarray, np.ndarray]:
+    n = basis.n_ao
+    J, K = np.zeros((n, n)), np.zeros((n, n))
+    lib().oracle_fock_ordered(basis.n_sh, *_b(basis), basis.shell_ao, n,
+                              np.ascontiguousarray(D, dtype=np.float64), J, K)
+    return J, K
+
+
+def schwarz(basis) -> np.ndarray:
+    n = basis.n_ao
+    Q = np.zeros((n, n))
+    lib().oracle_schwarz(basis.n_sh, *_b(basis), basis.shell_ao, n, Q)
+    return Q
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def f_launch(cls, args: dict, F: np.ndarray) -> None:
+    """Run one fused-F launch on the CPU, adding into F (n_ao x n_ao).
+    `args` holds the device entry's arguments by name (numpy arrays)."""
+    la, lb, lc, ld = cls
+    a = args
+    lib().oracle_f_launch(
+        la, lb, lc, ld, a["n_bra"], a["n_ket"], a["k_ab"], a["k_cd"], a["n_sh"],
+        a["n_ao"], a["diag"], float(a["tau"]), _ptr(a["bra_prim"]), _ptr(a.get("AB")),
+        _ptr(a["G_ab"]), _ptr(a["bra_sh"]), _ptr(a["bra_ao"]), _ptr(a["ket_prim"]),
+        _ptr(a.get("CD")), _ptr(a["G_cd"]), _ptr(a["ket_sh"]), _ptr(a["ket_ao"]),
+        _ptr(a["Dsh"]), _ptr(a["D"]), _ptr(F))

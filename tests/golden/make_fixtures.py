@@ -1,0 +1,102 @@
+"""Write the libcint golden vectors this repository's tests read, as .npz.
+
+Run once, in a container that has the upstream FelizJueves checkout at
+/root/reference (it is not present where the tests run):
+
+    python tests/golden/make_fixtures.py
+
+Source: the upstream tracked fixtures `test/oracle/fixtures/*.json`, produced
+there by PySCF 2.14.0 / libcint 6.1.3 (each file's `provenance`). They are
+DATA -- the molecule, the basis already converted to the unnormalised-
+Cartesian-primitive convention (`converted.shells`), the recorded density and
+libcint's arrays -- read here with `json` only; nothing of the upstream code
+is imported or executed.
+
+Each output `<name>.npz` holds:
+
+    shell_l [n_sh] i4, shell_centre [n_sh,3], shell_nprim [n_sh] i4,
+    exps [sum nprim], coefs [sum nprim]        the converted basis, AO order
+    density [nao,nao]                          the fixture's D (Hilbert matrix)
+    vj, vk [nao,nao]          (if present)     PySCF get_jk J[D], K[D]
+    int2e_diag [nao,nao]      (if present)     libcint (mn|mn)
+    eri_keys [nblk,4] i4, eri_off [nblk+1] i8, eri_vals   int2e blocks, one
+                                               shell quartet each, row-major
+                                               (n_i, n_j, n_k, n_l)
+"""
+
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+SRC = Path("/root/reference/test/oracle/fixtures")
+OUT = Path(__file__).resolve().parent
+
+# output name -> source fixture files merged into it
+SETS = {
+    "h2_sto3g": ["h2_sto3g_j", "h2_sto3g_k", "h2_sto3g_schwarz"],
+    "h2o_sto3g": ["h2o_sto3g_j", "h2o_sto3g_k", "h2o_sto3g_schwarz"],
+    "h2o_631g": ["h2o_631g_j", "h2o_631g_k", "h2o_631g_schwarz", "h2o_631g_eri"],
+    "h2o_ccpvdz": ["h2o_ccpvdz_j", "h2o_ccpvdz_k"],
+    "h10_chain_ccpvdz": ["h10_chain_ccpvdz_j", "h10_chain_ccpvdz_k"],
+    "co_ccpvtz": ["co_ccpvtz_j", "co_ccpvtz_k"],
+    "hf_ccpvtz": ["hf_ccpvtz_eri", "hf_ccpvtz_eri_l9", "hf_ccpvtz_eri_l10",
+                  "hf_ccpvtz_eri_l11", "hf_ccpvtz_eri_l12"],
+}
+
+
+def _arr(a: dict) -> np.ndarray:
+    return np.asarray(a["values"], dtype=np.float64).reshape(a["shape"])
+
+
+def _basis(conv: dict) -> dict[str, np.ndarray]:
+    sh = conv["shells"]
+    return {
+        "shell_l": np.array([s["l"] for s in sh], dtype=np.int32),
+        "shell_centre": np.array([s["centre"] for s in sh], dtype=np.float64),
+        "shell_nprim": np.array([len(s["exponents"]) for s in sh], dtype=np.int32),
+        "exps": np.array([x for s in sh for x in s["exponents"]], dtype=np.float64),
+        "coefs": np.array([x for s in sh for x in s["coefficients"]], dtype=np.float64),
+    }
+
+
+def convert(name: str, sources: list[str]) -> dict[str, np.ndarray]:
+    out: dict[str, np.ndarray] = {}
+    blocks: dict[tuple[int, ...], np.ndarray] = {}
+    for src in sources:
+        fx = json.loads((SRC / f"{src}.json").read_text())
+        basis = _basis(fx["converted"])
+        for k, v in basis.items():
+            if k in out and not np.array_equal(out[k], v):
+                raise ValueError(f"{src}: basis differs from the set's")
+            out[k] = v
+        arrays = fx["arrays"]
+        for key in ("vj", "vk", "int2e_diag"):
+            if key in arrays:
+                out[key] = _arr(arrays[key])
+        if "vj" in arrays or "vk" in arrays or "density" not in out:
+            out["density"] = _arr(arrays["density"])
+        for k, a in arrays.items():
+            if k.startswith("int2e:"):
+                q = tuple(int(x) for x in k[len("int2e:"):].split(","))
+                blocks[q] = _arr(a)
+    if blocks:
+        keys = sorted(blocks)
+        out["eri_keys"] = np.array(keys, dtype=np.int32)
+        off = np.cumsum([0] + [blocks[q].size for q in keys]).astype(np.int64)
+        out["eri_off"] = off
+        out["eri_vals"] = np.concatenate([blocks[q].ravel() for q in keys])
+    return out
+
+
+def main() -> None:
+    for name, sources in SETS.items():
+        data = convert(name, sources)
+        np.savez_compressed(OUT / f"{name}.npz", **data)
+        print(name, {k: v.shape for k, v in data.items()})
+
+
+if __name__ == "__main__":
+    main()
