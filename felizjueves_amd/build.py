@@ -44,6 +44,9 @@ def _compile(src: Path, extra: list[str]) -> tuple[Path, float, str]:
     obj = OBJ / (src.stem + ".o")
     deps = [src, CSRC / "fj_device.h", HERE.parent / "include" / "fj_mi355x.h",
             CSRC / "gen" / "fj_boys_table.h"]
+    hdr = src.parent / (src.stem.replace("_r32", "") + ".h")
+    if hdr.exists():
+        deps.append(hdr)
     if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps if d.exists()):
         return obj, 0.0, ""
     t = time.time()

@@ -20,7 +20,7 @@
 // Launch arguments shared by the per-class kernels (both C-ABI entries).
 struct FjFArgs {
     FjSide bra, ket;
-    const int* bra_list;   // optional: workgroup x -> bra pair index
+    const int* work;       // optional [n][2]: workgroup -> (bra pair, first ket)
     int n_sh, n_ao, diag;
     double tau;
     const double* Dsh;

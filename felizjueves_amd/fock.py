@@ -1,0 +1,210 @@
+"""The native Fock builder: ``G = J - K/2`` on one MI355X.
+
+Host planning, once per basis (and density screen):
+
+* every canonical shell pair, its primitive-pair bounds, pruning and staged
+  records (`pairs.kept_pairs`, the upstream rules);
+* per pair class (la, lb) ONE ragged pair table on the device -- pairs sorted
+  by (degree, G) descending, each pair's records contiguous, an offset
+  array, no degree-uniform batching (so a class pair is one launch, not one
+  per (degree, magnitude) bin pair);
+* per class pair (X <= Y) a work list of tiles (bra pair, first ket): one
+  workgroup each, ``threads`` consecutive kets; tiles whose every quartet
+  fails the six-block screen ``G_ab G_cd dmax < tau`` are dropped here, and
+  (mixed mode) a tile whose largest bound is below the FP32 threshold goes
+  to the FP32-tier kernel instead of the FP64 one.
+
+Per build (device only): zero F, the density block maxima, one launch per
+(class pair, tier) on one stream, close F <- F + F^T.
+
+The screen, weights and degeneracy are the upstream ones
+(`eri/fock/kernel_f.py:1-165`), so with the same pruning the result equals
+the upstream pair-stream walk up to the eq. 29 pre-launch screen, which
+only removes quartets with G_ab G_cd < tau.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import native
+from . import pairs as P
+from .basis import Basis
+from .hgp import cost
+
+FP64, FP32 = 0, 1
+
+
+@dataclass
+class SideTable:
+    cls: tuple
+    pairs: list               # FPair, in table order
+    G: np.ndarray
+    deg: np.ndarray
+    sh: np.ndarray            # [n, 2]
+    dev: dict = field(default_factory=dict)
+    side: native.FjSide = None
+
+    @property
+    def n(self) -> int:
+        return len(self.pairs)
+
+
+@dataclass
+class Launch:
+    cls: tuple
+    bra: SideTable
+    ket: SideTable
+    diag: int
+    precision: int
+    work: np.ndarray          # [n, 2] int32
+    work_dev: object = None
+    n_quartets: int = 0       # computed (post-screen) contracted quartets
+    n_prim: int = 0           # primitive quartets of those
+    flops: float = 0.0        # algorithmic FLOPs of those
+
+
+class FockBuilder:
+    def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads: int = 64,
+                 mixed: bool = False, fp32_threshold: float = 1e-6, tile_precision=None):
+        self.basis = basis
+        self.tau = float(tau)
+        self.threads = int(threads)
+        self.mixed = bool(mixed)
+        self.fp32_threshold = float(fp32_threshold)
+        t0 = time.time()
+        kept, self.deleted, self.eps, self.g_max, self.n_prim_total, _ = P.kept_pairs(
+            basis, D, self.tau, 1)
+        by_cls: dict = {}
+        for p in kept:
+            by_cls.setdefault((p.la, p.lb), []).append(p)
+        self.tables = {}
+        for c, ps in sorted(by_cls.items()):
+            ps = sorted(ps, key=lambda p: (-p.degree, -p.G))
+            self.tables[c] = SideTable(
+                c, ps, np.array([p.G for p in ps]), np.array([p.degree for p in ps]),
+                np.array([(p.a, p.b) for p in ps], dtype=np.int64))
+        self.Dsh_plan = P.density_blocks(basis, D)
+        self.tile_precision = tile_precision
+        self.launches = self._plan()
+        self.plan_seconds = time.time() - t0
+        self._dev = False
+
+    # -- planning ---------------------------------------------------------
+    def _bounds(self, bra: SideTable, ket: SideTable, b: int) -> np.ndarray:
+        Dsh = self.Dsh_plan
+        sa, sb = bra.sh[b]
+        sc, sd = ket.sh[:, 0], ket.sh[:, 1]
+        dmax = np.maximum.reduce([
+            np.full(ket.n, 2.0 * Dsh[sa, sb]), 2.0 * Dsh[sc, sd], 0.5 * Dsh[sa, sc],
+            0.5 * Dsh[sa, sd], 0.5 * Dsh[sb, sc], 0.5 * Dsh[sb, sd]])
+        return bra.G[b] * (ket.G * dmax)
+
+    def _plan(self) -> list:
+        T = self.threads
+        out = []
+        classes = sorted(self.tables)
+        for i, X in enumerate(classes):
+            for Y in classes[i:]:
+                bra, ket = self.tables[X], self.tables[Y]
+                cls = (*X, *Y)
+                c = cost(cls)
+                diag = int(X == Y)
+                work = {FP64: [], FP32: []}
+                stats = {FP64: [0, 0, 0.0], FP32: [0, 0, 0.0]}
+                for b in range(bra.n):
+                    bound = self._bounds(bra, ket, b)
+                    k_start = b if diag else 0
+                    for k0 in range(k_start, ket.n, T):
+                        seg = bound[k0:k0 + T]
+                        live = seg >= self.tau
+                        if not live.any():
+                            continue
+                        tier = FP64
+                        if self.mixed:
+                            if self.tile_precision is not None:
+                                tier = self.tile_precision(cls, bra, ket, b, k0, seg)
+                            elif seg.max() < self.fp32_threshold:
+                                tier = FP32
+                        work[tier].append((b, k0))
+                        kab = int(bra.deg[b])
+                        kcd = ket.deg[k0:k0 + T][live]
+                        st = stats[tier]
+                        st[0] += int(live.sum())
+                        st[1] += int(kab * kcd.sum())
+                        st[2] += float(c.per_prim * kab * kcd.sum() + c.per_quartet * live.sum())
+                for tier in (FP64, FP32):
+                    if work[tier]:
+                        L = Launch(cls, bra, ket, diag, tier, np.array(work[tier], dtype=np.int32))
+                        L.n_quartets, L.n_prim, L.flops = stats[tier]
+                        out.append(L)
+        # biggest launches first: their tails overlap the small ones'
+        out.sort(key=lambda L: -L.flops)
+        return out
+
+    # -- device -------------------------------------------------------------
+    def to_device(self) -> None:
+        import torch
+        dev = native.device()
+        b = self.basis
+        for t in self.tables.values():
+            recs = np.concatenate([p.rec for p in t.pairs], axis=0)
+            off = np.concatenate([[0], np.cumsum(t.deg)]).astype(np.int32)
+            sep = np.array([b.shell_centre[p.a] - b.shell_centre[p.b] for p in t.pairs])
+            ao = np.array([(b.shell_ao[p.a], b.shell_ao[p.b]) for p in t.pairs], dtype=np.int32)
+            t.dev = {"prim": native.to_dev(recs), "off": native.to_dev(off),
+                     "sep": native.to_dev(sep), "G": native.to_dev(t.G),
+                     "sh": native.to_dev(t.sh.astype(np.int32)), "ao": native.to_dev(ao)}
+            d = t.dev
+            t.side = native.FjSide(d["prim"].data_ptr(), d["off"].data_ptr(), 0,
+                                   d["sep"].data_ptr(), d["G"].data_ptr(), d["sh"].data_ptr(),
+                                   d["ao"].data_ptr(), t.n)
+        for L in self.launches:
+            L.work_dev = native.to_dev(L.work)
+        self.shell_ao = native.to_dev(b.shell_ao.astype(np.int32))
+        self.shell_size = native.to_dev(b.shell_size.astype(np.int32))
+        self.Dsh = torch.zeros((b.n_sh, b.n_sh), dtype=torch.float64, device=dev)
+        self.F = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
+        self._dev = True
+
+    def build(self, D_dev, stream=None, close: bool = True):
+        """G for the device density D_dev (n_ao x n_ao float64), on `stream`.
+        Returns the builder's F tensor (overwritten by the next build)."""
+        if not self._dev:
+            self.to_device()
+        b = self.basis
+        self.F.zero_()
+        native.density_blocks(self.shell_ao, self.shell_size, D_dev, self.Dsh, stream)
+        for L in self.launches:
+            native.call_fx(L.cls, self.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
+                           L.work.shape[0], L.diag, b.n_sh, b.n_ao, self.tau, self.Dsh, D_dev,
+                           self.F, stream)
+        if close:
+            native.close(self.F, stream)
+        return self.F
+
+    # -- counts -------------------------------------------------------------
+    @property
+    def n_quartets(self) -> int:
+        return sum(L.n_quartets for L in self.launches)
+
+    @property
+    def n_prim_quartets(self) -> int:
+        return sum(L.n_prim for L in self.launches)
+
+    @property
+    def flops(self) -> float:
+        return sum(L.flops for L in self.launches)
+
+    def fp32_share(self) -> float:
+        q = self.n_quartets
+        return sum(L.n_quartets for L in self.launches if L.precision == FP32) / q if q else 0.0
+
+
+def energy(D: np.ndarray, G: np.ndarray) -> float:
+    """E_2e = 1/2 sum_mn D_mn G_mn."""
+    return 0.5 * float(np.sum(D * G))

@@ -232,15 +232,22 @@ def emit_class(cls) -> str:
     na, nb = n_cart(la), n_cart(lb)
     nab = na * nb
     out = [f"// generated by felizjueves_amd/gen_hip.py -- class ({la}{lb}|{lc}{ld})",
-           '#include "../fj_device.h"', ""]
+           "#pragma once", '#include "../fj_device.h"', ""]
     out.append(emit_quartet(cls))
     out.append(f"""
 template <typename R>
 __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
-    const int bra = a.bra_list ? a.bra_list[blockIdx.x] : (int)blockIdx.x;
+    // work item: (bra pair, first ket) from the list, else (x, y * blockDim)
+    int bra, k0;
+    if (a.work) {{
+        bra = a.work[2 * blockIdx.x];
+        k0 = a.work[2 * blockIdx.x + 1];
+    }} else {{
+        bra = (int)blockIdx.x;
+        k0 = (int)(blockIdx.y * blockDim.x);
+    }}
     const int kmin = a.diag * bra;
-    const int k0 = (int)(blockIdx.y * blockDim.x);
     if (k0 + (int)blockDim.x <= kmin || k0 >= a.ket.n) return;   // whole block below the canonical half
     int kb0, kab;
     fj_side_range(a.bra, bra, kb0, kab);
@@ -302,16 +309,20 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     }}
 }}
 
-static int fj_launch_{name}(hipStream_t s, int threads, int precision, const FjFArgs& a, int n_bra_blocks) {{
+// the FP32-tier instantiation lives in its own translation unit (f_{name}_r32.hip)
+int fj_launch_{name}_r32(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a);
+//@@SPLIT@@
+static int fj_launch_{name}(hipStream_t s, int threads, int precision, const FjFArgs& a, int n_blocks) {{
     if (threads <= 0) threads = FJ_DEFAULT_THREADS;
     if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
-    if (n_bra_blocks <= 0 || a.ket.n <= 0) return 0;
-    dim3 grid((unsigned)n_bra_blocks, (unsigned)((a.ket.n + threads - 1) / threads));
-    if (precision == 0)
+    if (precision != 0 && precision != 1) return FJ_ERR_ARG;
+    if (n_blocks <= 0 || a.ket.n <= 0) return 0;
+    dim3 grid((unsigned)n_blocks, a.work ? 1u : (unsigned)((a.ket.n + threads - 1) / threads));
+    if (precision == 0) {{
         hipLaunchKernelGGL(fj_f_{name}_k<double>, grid, dim3(threads), 0, s, a);
-    else
-        hipLaunchKernelGGL(fj_f_{name}_k<float>, grid, dim3(threads), 0, s, a);
-    return (int)hipGetLastError();
+        return (int)hipGetLastError();
+    }}
+    return fj_launch_{name}_r32(s, grid, dim3(threads), a);
 }}
 
 extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -324,21 +335,22 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
     FjFArgs a;
     a.bra = FjSide{{bra_prim, nullptr, k_ab, {"AB" if lb else "nullptr"}, G_ab, bra_sh, bra_ao, n_bra}};
     a.ket = FjSide{{ket_prim, nullptr, k_cd, {"CD" if ld else "nullptr"}, G_cd, ket_sh, ket_ao, n_ket}};
-    a.bra_list = nullptr;
+    a.work = nullptr;
     a.n_sh = n_sh; a.n_ao = n_ao; a.diag = diag; a.tau = tau;
     a.Dsh = Dsh; a.D = D; a.F = F;
     return fj_launch_{name}(s, threads, 0, a, n_bra);
 }}
 
 extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
-                         const FjSide* bra, const FjSide* ket, const int* bra_list, int n_bra_blocks,
+                         const FjSide* bra, const FjSide* ket, const int* work, int n_work,
                          int diag, int n_sh, int n_ao, double tau,
                          const double* Dsh, const double* D, double* F) {{
+    if (!bra || !ket || !work) return FJ_ERR_ARG;
     FjFArgs a;
-    a.bra = *bra; a.ket = *ket; a.bra_list = bra_list;
+    a.bra = *bra; a.ket = *ket; a.work = work;
     a.n_sh = n_sh; a.n_ao = n_ao; a.diag = diag; a.tau = tau;
     a.Dsh = Dsh; a.D = D; a.F = F;
-    return fj_launch_{name}(s, threads, precision, a, n_bra_blocks);
+    return fj_launch_{name}(s, threads, precision, a, n_work);
 }}
 """)
     return "\n".join(out)
@@ -398,11 +410,21 @@ def generate(classes, out: Path = GEN) -> list[Path]:
     if not tab.exists() or tab.read_text() != text:
         tab.write_text(text)
     for cls in classes:
-        p = out / f"f_{cls_name(cls)}.hip"
-        text = emit_class(cls)
-        if not p.exists() or p.read_text() != text:
-            p.write_text(text)
-        files.append(p)
+        n = cls_name(cls)
+        head, main = emit_class(cls).split("//@@SPLIT@@\n")
+        r32 = (f"// generated by felizjueves_amd/gen_hip.py -- FP32-tier kernel of ({n})\n"
+               f'#include "f_{n}.h"\n\n'
+               f"int fj_launch_{n}_r32(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a) {{\n"
+               f"    hipLaunchKernelGGL(fj_f_{n}_k<float>, grid, block, 0, s, a);\n"
+               f"    return (int)hipGetLastError();\n}}\n")
+        texts = {f"f_{n}.h": head,
+                 f"f_{n}.hip": f'// generated by felizjueves_amd/gen_hip.py\n#include "f_{n}.h"\n' + main,
+                 f"f_{n}_r32.hip": r32}
+        for fname, text in texts.items():
+            p = out / fname
+            if not p.exists() or p.read_text() != text:
+                p.write_text(text)
+        files += [out / f"f_{n}.hip", out / f"f_{n}_r32.hip"]
     return files
 
 
@@ -422,7 +444,7 @@ def header_decls(classes) -> str:
             f"    const double* Dsh, const double* D, double* F);\n"
             f"int fj_fx_{n}(fj_stream_t stream, int threads, int precision, const FjSide* bra,"
             f" const FjSide* ket,\n"
-            f"    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,\n"
+            f"    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,\n"
             f"    const double* Dsh, const double* D, double* F);")
     return "\n".join(out) + "\n"
 

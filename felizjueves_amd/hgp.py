@@ -188,13 +188,22 @@ class Cost:
     hrr_flops: int       # per contracted quartet
     boys_order: int
 
+    @property
+    def per_prim(self) -> int:
+        """FLOPs per primitive quartet: the VRR steps and seed multiplies, one
+        add per boundary value accumulated, and the quartet geometry (16 for
+        (ss|ss), 26 otherwise: the upstream table's counts)."""
+        return self.vrr_flops + self.boundary + (16 if self.boys_order == 0 else 26)
+
+    @property
+    def per_quartet(self) -> int:
+        """FLOPs per contracted quartet: the HRR and the 12-FLOP-per-ERI
+        six-block predigest."""
+        return self.hrr_flops + 12 * self.n_eri
+
     def flops(self, k_ab: int, k_cd: int) -> int:
-        """Algorithmic FLOPs of one contracted quartet: per primitive
-        quartet the VRR, the seed and the boundary accumulation plus 30 for
-        the quartet geometry and scalars; once per contracted quartet the
-        HRR and the 12-FLOP-per-ERI six-block predigest."""
-        per_prim = self.vrr_flops + self.boundary + 30
-        return per_prim * k_ab * k_cd + self.hrr_flops + 12 * self.n_eri
+        """Algorithmic FLOPs of one contracted quartet of degrees k_ab, k_cd."""
+        return self.per_prim * k_ab * k_cd + self.per_quartet
 
 
 @lru_cache(maxsize=None)

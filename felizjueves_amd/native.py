@@ -107,12 +107,12 @@ def call_f(cls, threads: int, args: dict, F, stream=None) -> None:
     check(fn(*vals), fn.__name__)
 
 
-def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, bra_list, n_blocks: int,
+def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, work, n_work: int,
             diag: int, n_sh: int, n_ao: int, tau: float, Dsh, D, F, stream=None) -> None:
     fn = getattr(lib(), "fj_fx_" + "".join(map(str, cls)))
     fn.restype = C.c_int
     rc = fn(C.c_void_p(stream_handle(stream)), C.c_int(threads), C.c_int(precision),
-            C.byref(bra), C.byref(ket), C.c_void_p(ptr(bra_list)), C.c_int(n_blocks),
+            C.byref(bra), C.byref(ket), C.c_void_p(ptr(work)), C.c_int(n_work),
             C.c_int(diag), C.c_int(n_sh), C.c_int(n_ao), C.c_double(tau),
             C.c_void_p(ptr(Dsh)), C.c_void_p(ptr(D)), C.c_void_p(ptr(F)))
     check(rc, fn.__name__)

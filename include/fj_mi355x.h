@@ -84,7 +84,7 @@ int fj_f_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0000(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -92,7 +92,7 @@ int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0001(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -100,7 +100,7 @@ int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -108,7 +108,7 @@ int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0011(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -116,7 +116,7 @@ int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -124,7 +124,7 @@ int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -132,7 +132,7 @@ int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0101(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -140,7 +140,7 @@ int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -148,7 +148,7 @@ int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -156,7 +156,7 @@ int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -164,7 +164,7 @@ int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -172,7 +172,7 @@ int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -180,7 +180,7 @@ int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0211(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -188,7 +188,7 @@ int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -196,7 +196,7 @@ int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -204,7 +204,7 @@ int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -212,7 +212,7 @@ int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -220,7 +220,7 @@ int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -228,7 +228,7 @@ int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -236,7 +236,7 @@ int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -244,7 +244,7 @@ int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* bra_list, int n_bra_blocks, int diag, int n_sh, int n_ao, double tau,
+    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,
     const double* Dsh, const double* D, double* F);
 /* END F CLASSES */
 

@@ -469,12 +469,13 @@ void oracle_eri_hgp(int sa, int sb, int sc, int sd, const int *shell_l,
  * deg = (1/2)^{[a=b]+[c=d]+[P=Q]} (kernel_f.py:439-487), ADDED into F
  * (the caller closes F <- F + F^T).  AB / CD may be NULL when lb / ld is 0.
  * Rows of bra_prim / ket_prim: (Px,Py,Pz, Xx,Xy,Xz, zeta, U).               */
-void oracle_f_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
-                     int k_ab, int k_cd, int n_sh, int n_ao, int diag, double tau,
-                     const double *bra_prim, const double *AB, const double *G_ab,
-                     const int *bra_sh, const int *bra_ao, const double *ket_prim,
-                     const double *CD, const double *G_cd, const int *ket_sh,
-                     const int *ket_ao, const double *Dsh, const double *D, double *F) {
+void oracle_fx_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
+                      int k_ab, int k_cd, const int *bra_off, const int *ket_off,
+                      int n_sh, int n_ao, int diag, double tau,
+                      const double *bra_prim, const double *AB, const double *G_ab,
+                      const int *bra_sh, const int *bra_ao, const double *ket_prim,
+                      const double *CD, const double *G_cd, const int *ket_sh,
+                      const int *ket_ao, const double *Dsh, const double *D, double *F) {
     int E = la + lb, Fk = lc + ld, L = E + Fk;
     int ne = ncart_upto(E + 1), nf = ncart_upto(Fk + 1);
     int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
@@ -510,11 +511,15 @@ void oracle_f_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
                 if ((t = 0.5 * Dsh[sb * n_sh + sd]) > dm) dm = t;
                 if (G_ab[bra] * (G_cd[k] * dm) < tau) continue;
                 memset(C, 0, sizeof(double) * ne * nf);
-                for (int kc = 0; kc < k_cd; ++kc) {
-                    const double *r = ket_prim + ((size_t)k * k_cd + kc) * 8;
+                int kc0 = ket_off ? ket_off[k] : k * k_cd;
+                int nkc = ket_off ? ket_off[k + 1] - kc0 : k_cd;
+                int kb0 = bra_off ? bra_off[bra] : bra * k_ab;
+                int nkb = bra_off ? bra_off[bra + 1] - kb0 : k_ab;
+                for (int kc = 0; kc < nkc; ++kc) {
+                    const double *r = ket_prim + ((size_t)kc0 + kc) * 8;
                     prim_t kp = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}, r[6], r[7]};
-                    for (int kb = 0; kb < k_ab; ++kb) {
-                        const double *s = bra_prim + ((size_t)bra * k_ab + kb) * 8;
+                    for (int kb = 0; kb < nkb; ++kb) {
+                        const double *s = bra_prim + ((size_t)kb0 + kb) * 8;
                         prim_t bp = {{s[0], s[1], s[2]}, {s[3], s[4], s[5]}, s[6], s[7]};
                         hgp_vrr_add(la, E, lc, Fk, &bp, &kp, V, C);
                     }
@@ -548,6 +553,18 @@ void oracle_f_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
     for (int t = 0; t < nthr; ++t)
         for (size_t i = 0; i < (size_t)n_ao * n_ao; ++i) F[i] += Fp[(size_t)t * n_ao * n_ao + i];
     free(Fp);
+}
+
+/* The upstream per-class argument list: uniform degrees k_ab, k_cd. */
+void oracle_f_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
+                     int k_ab, int k_cd, int n_sh, int n_ao, int diag, double tau,
+                     const double *bra_prim, const double *AB, const double *G_ab,
+                     const int *bra_sh, const int *bra_ao, const double *ket_prim,
+                     const double *CD, const double *G_cd, const int *ket_sh,
+                     const int *ket_ao, const double *Dsh, const double *D, double *F) {
+    oracle_fx_launch(la, lb, lc, ld, n_bra, n_ket, k_ab, k_cd, NULL, NULL, n_sh, n_ao, diag,
+                     tau, bra_prim, AB, G_ab, bra_sh, bra_ao, ket_prim, CD, G_cd, ket_sh,
+                     ket_ao, Dsh, D, F);
 }
 
 /* ---------------------------------------------------------------------- */

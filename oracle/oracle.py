@@ -44,6 +44,8 @@ def lib():
         vp = C.c_void_p
         L.oracle_f_launch.argtypes = ([C.c_int] * 11 + [C.c_double] +
                                       [vp] * 13)
+        L.oracle_fx_launch.argtypes = ([C.c_int] * 8 + [vp] * 2 + [C.c_int] * 3 +
+                                       [C.c_double] + [vp] * 13)
         _lib = L
     return _lib
 
@@ -97,3 +99,17 @@ def f_launch(cls, args: dict, F: np.ndarray) -> None:
         _ptr(a["G_ab"]), _ptr(a["bra_sh"]), _ptr(a["bra_ao"]), _ptr(a["ket_prim"]),
         _ptr(a.get("CD")), _ptr(a["G_cd"]), _ptr(a["ket_sh"]), _ptr(a["ket_ao"]),
         _ptr(a["Dsh"]), _ptr(a["D"]), _ptr(F))
+
+
+def fx_launch(cls, bra: dict, ket: dict, diag: int, n_sh: int, n_ao: int, tau: float,
+              Dsh, D, F) -> None:
+    """Ragged-degree launch (pair i's records prim[off[i]:off[i+1]]), adding
+    into F. Each side: prim [n_rec, 8], off [n+1], sep [n, 3], G, sh, ao."""
+    la, lb, lc, ld = cls
+    c = lambda a: np.ascontiguousarray(a)  # noqa: E731
+    b, k = {x: c(v) for x, v in bra.items()}, {x: c(v) for x, v in ket.items()}
+    lib().oracle_fx_launch(
+        la, lb, lc, ld, len(b["G"]), len(k["G"]), 0, 0, _ptr(b["off"]), _ptr(k["off"]),
+        n_sh, n_ao, diag, float(tau), _ptr(b["prim"]), _ptr(b["sep"]), _ptr(b["G"]),
+        _ptr(b["sh"]), _ptr(b["ao"]), _ptr(k["prim"]), _ptr(k["sep"]), _ptr(k["G"]),
+        _ptr(k["sh"]), _ptr(k["ao"]), _ptr(c(Dsh)), _ptr(c(D)), _ptr(F))

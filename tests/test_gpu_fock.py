@@ -1,0 +1,128 @@
+"""GPU parity: the gfx950 library against the CPU oracle and libcint.
+
+Every test here launches the HIP library on cuda:0 and is marked `gpu`.
+Bars: FP64 max|dG| <= 1e-10 and |dE| <= 1e-9 Eh against the oracle; the
+FP32 tier |dE| <= 1e-6 Eh and max|dG| <= 1e-8."""
+
+import numpy as np
+import pytest
+
+from felizjueves_amd import native
+from felizjueves_amd import pairs as P
+from felizjueves_amd.basis import load_fixture
+from felizjueves_amd.fock import FockBuilder, energy
+from felizjueves_amd.systems import config1, config2
+from felizjueves_amd.walk import GpuRunner, walk_f
+from helpers import OracleRunner
+from oracle import oracle as O
+from test_builder_oracle import oracle_build
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def test_library_loads():
+    assert native.lib().fj_version() >= 1
+
+
+def test_boys_device_vs_oracle():
+    rng = np.random.default_rng(7)
+    T = np.concatenate([
+        [0.0, 1e-14, 1e-8, 1e-4, 0.0625, 0.1, 1.0, 39.9375, 39.99, 40.0, 40.0001, 79.9, 80.0, 1e3, 1e6],
+        rng.uniform(0, 45, 2000), np.exp(rng.uniform(np.log(1e-6), np.log(1e6), 2000)),
+        np.arange(0, 40, 0.125) + 1.0 / 16 - 1e-12])
+    for M in (0, 1, 4, 8, 12):
+        got = native.boys(M, T)
+        ref = np.array([O.boys(M, t) for t in T])
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+        _print("boys M", M, "max rel", rel.max())
+        assert rel.max() < 2e-14, (M, T[np.argmax(rel.max(axis=1))])
+
+
+def _launch_errors(system_basis, D, tau, n):
+    """Every launch of the upstream stream, device vs oracle, per class."""
+    plan = P.f_launches(system_basis, D, tau=tau, n=n)
+    Dsh = P.density_blocks(system_basis, D)
+    gpu = GpuRunner()
+    worst = {}
+    for L in plan.launches:
+        args = P.launch_args(system_basis, plan, L, D, Dsh)
+        Fo = np.zeros((system_basis.n_ao,) * 2)
+        O.f_launch(L.cls, args, Fo)
+        Fg = gpu.zeros(system_basis.n_ao)
+        gpu(L.cls, args["n_bra"], (64, 1), gpu.put(args), Fg)
+        err = np.abs(gpu.host(Fg) - Fo).max()
+        worst[L.cls] = max(worst.get(L.cls, 0.0), err)
+    return worst
+
+
+def test_every_class_config1_launches():
+    s = config1()
+    worst = _launch_errors(s.basis, s.D, 0.0, 1)
+    _print("per-class max |dF|:", {k: f"{v:.1e}" for k, v in sorted(worst.items())})
+    assert len(worst) == 21
+    assert max(worst.values()) < 1e-11
+
+
+@pytest.mark.parametrize("name", ["h2o_sto3g", "h2o_631g", "h2o_ccpvdz", "h10_chain_ccpvdz"])
+def test_walk_gpu_vs_libcint(name):
+    fx = load_fixture(name)
+    plan = P.f_launches(fx.basis, fx.density, tau=1e-12, n=3)
+    w = walk_f(plan, fx.basis, fx.density, run=GpuRunner())
+    ref = fx.arrays["vj"] - 0.5 * fx.arrays["vk"]
+    wo = walk_f(plan, fx.basis, fx.density, run=OracleRunner())
+    _print(name, "gpu-libcint", np.abs(w.F - ref).max(), "gpu-oracle", np.abs(w.F - wo.F).max())
+    assert np.abs(w.F - wo.F).max() < 1e-12 * max(1.0, np.abs(ref).max())
+    assert np.abs(w.F - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
+
+
+def _builder_vs_oracle(s, mixed: bool):
+    import torch
+    fb = FockBuilder(s.basis, s.D, tau=1e-12, mixed=mixed)
+    Dd = native.to_dev(s.D)
+    G = fb.build(Dd)
+    torch.cuda.synchronize()
+    Gg = G.cpu().numpy()
+    Go = oracle_build(fb if not mixed else FockBuilder(s.basis, s.D, tau=1e-12), s.D)
+    dG = np.abs(Gg - Go).max()
+    dE = abs(energy(s.D, Gg) - energy(s.D, Go))
+    return fb, dG, dE, Gg
+
+
+def test_builder_config1_fp64():
+    s = config1()
+    fb, dG, dE, G = _builder_vs_oracle(s, False)
+    _print("c1 fp64 max|dG|", dG, "|dE|", dE, "quartets", fb.n_quartets)
+    assert dG <= 1e-10 and dE <= 1e-9
+    assert np.abs(G - G.T).max() == 0.0
+
+
+def test_builder_config1_mixed():
+    s = config1()
+    fb, dG, dE, _ = _builder_vs_oracle(s, True)
+    _print("c1 mixed max|dG|", dG, "|dE|", dE, "fp32 share", fb.fp32_share())
+    assert dG <= 1e-8 and dE <= 1e-6
+
+
+def test_builder_repeatable():
+    """Two builds of the same density agree to the atomics' reordering."""
+    import torch
+    s = config1()
+    fb = FockBuilder(s.basis, s.D, tau=1e-12)
+    Dd = native.to_dev(s.D)
+    G1 = fb.build(Dd).clone()
+    G2 = fb.build(Dd).clone()
+    torch.cuda.synchronize()
+    assert (G1 - G2).abs().max().item() < 1e-12
+
+
+def test_builder_small_cluster_vs_oracle():
+    """A 10-atom cluster of the working-size recipe (the oracle finishes it
+    in seconds): FP64 bars."""
+    s = config2(seed=11, n_atoms=10)
+    fb, dG, dE, G = _builder_vs_oracle(s, False)
+    _print("c2-10 fp64 max|dG|", dG, "|dE|", dE, "quartets", fb.n_quartets)
+    assert dG <= 1e-10 and dE <= 1e-9
