@@ -45,15 +45,14 @@ def timed_builds(fb, Dd, steps, warmup, probe=None):
             fb.build(Dd)
             continue
         # the same launches as FockBuilder.build, with events around `probe`
-        b = fb.basis
+        sys = fb._sys(Dd)
         fb.F.zero_()
         native.density_blocks(fb.shell_ao, fb.shell_size, Dd, fb.Dsh)
         for i, L in enumerate(fb.launches):
             if i == probe:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
-            native.call_fx(L.cls, fb.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
-                           L.work.shape[0], L.diag, b.n_sh, b.n_ao, fb.tau, fb.Dsh, Dd, fb.F)
+            fb.launch(L, sys)
             if i == probe:
                 e1.record()
                 evs.append((e0, e1))
@@ -68,15 +67,14 @@ def launch_times(fb, Dd):
     """Per-launch durations (ms) of one build, HIP events on the stream."""
     import torch
     from felizjueves_amd import native
-    b = fb.basis
+    sys = fb._sys(Dd)
     fb.F.zero_()
     native.density_blocks(fb.shell_ao, fb.shell_size, Dd, fb.Dsh)
     evs = []
     for L in fb.launches:
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        native.call_fx(L.cls, fb.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
-                       L.work.shape[0], L.diag, b.n_sh, b.n_ao, fb.tau, fb.Dsh, Dd, fb.F)
+        fb.launch(L, sys)
         e1.record()
         evs.append((e0, e1))
     native.close(fb.F)

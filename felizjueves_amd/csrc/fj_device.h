@@ -20,12 +20,11 @@
 // Launch arguments shared by the per-class kernels (both C-ABI entries).
 struct FjFArgs {
     FjSide bra, ket;
-    const int* work;       // optional [n][2]: workgroup -> (bra pair, first ket)
-    int n_sh, n_ao, diag;
+    const int4* work;      // optional: workgroup -> (bra pair, first ket, system, ket end)
+    const FjSys* sys;      // the system table the work items index (device)
+    FjSys one;             // the single system of a launch without a work list
+    int diag;
     double tau;
-    const double* Dsh;
-    const double* D;
-    double* F;
 };
 
 // First primitive record and degree of pair i of one side.

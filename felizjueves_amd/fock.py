@@ -79,6 +79,10 @@ class FockBuilder:
         t0 = time.time()
         kept, self.deleted, self.eps, self.g_max, self.n_prim_total, _ = P.kept_pairs(
             basis, D, self.tau, 1)
+        big = [p for p in kept if p.degree > P.MAX_PRIM_PAIRS]
+        if big:
+            raise ValueError(f"{len(big)} pairs keep more than {P.MAX_PRIM_PAIRS} primitive "
+                             "pairs, the staging cap of the F kernels")
         by_cls: dict = {}
         for p in kept:
             by_cls.setdefault((p.la, p.lb), []).append(p)
@@ -130,7 +134,7 @@ class FockBuilder:
                                 tier = self.tile_precision(cls, bra, ket, b, k0, seg)
                             elif seg.max() < self.fp32_threshold:
                                 tier = FP32
-                        work[tier].append((b, k0))
+                        work[tier].append((b, k0, 0, ket.n))
                         kab = int(bra.deg[b])
                         kcd = ket.deg[k0:k0 + T][live]
                         st = stats[tier]
@@ -169,20 +173,33 @@ class FockBuilder:
         self.shell_size = native.to_dev(b.shell_size.astype(np.int32))
         self.Dsh = torch.zeros((b.n_sh, b.n_sh), dtype=torch.float64, device=dev)
         self.F = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
+        self._sys_for = None
         self._dev = True
+
+    def _sys(self, D_dev):
+        """The one-entry system table for density D_dev (rebuilt when the
+        density buffer changes)."""
+        key = D_dev.data_ptr()
+        if self._sys_for != key:
+            self._D_ref = D_dev
+            self.sys = native.sys_table([(D_dev, self.Dsh, self.F)])
+            self._sys_for = key
+        return self.sys
+
+    def launch(self, L: "Launch", sys, stream=None) -> None:
+        native.call_fx(L.cls, self.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
+                       L.work.shape[0], sys, L.diag, self.tau, stream)
 
     def build(self, D_dev, stream=None, close: bool = True):
         """G for the device density D_dev (n_ao x n_ao float64), on `stream`.
         Returns the builder's F tensor (overwritten by the next build)."""
         if not self._dev:
             self.to_device()
-        b = self.basis
+        sys = self._sys(D_dev)
         self.F.zero_()
         native.density_blocks(self.shell_ao, self.shell_size, D_dev, self.Dsh, stream)
         for L in self.launches:
-            native.call_fx(L.cls, self.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
-                           L.work.shape[0], L.diag, b.n_sh, b.n_ao, self.tau, self.Dsh, D_dev,
-                           self.F, stream)
+            self.launch(L, sys, stream)
         if close:
             native.close(self.F, stream)
         return self.F

@@ -174,11 +174,11 @@ def emit_quartet(cls) -> str:
     for n in order:
         r = terms[n]
         if not r:
-            E(f"const R {_hname(n)} = {acc[(n[0], n[2])]};")
+            E(f"const double {_hname(n)} = double({acc[(n[0], n[2])]});")
             continue
         side, k, moved, lowered = r
         sep = ("AB" if side == "bra" else "CD") + AX[k]
-        E(f"const R {_hname(n)} = {_hname(moved)} + R({sep}) * {_hname(lowered)};")
+        E(f"const double {_hname(n)} = {_hname(moved)} + {sep} * {_hname(lowered)};")
     # digestion: five ket-dependent blocks here, f_ab into the caller's buffer
     ca, cb, cc, cd = (cartesians(x) for x in cls)
     blocks = {"cd": (2, 3), "ac": (0, 2), "ad": (0, 3), "bc": (1, 2), "bd": (1, 3)}
@@ -200,7 +200,7 @@ def emit_quartet(cls) -> str:
                 E(f"double f{blk}{i}_{j} = 0.0;")
     for t in targets:
         idx = tuple(cs.index(x) for cs, x in zip((ca, cb, cc, cd), t))
-        E(f"{{ const double x = double({_hname(t)});")
+        E(f"{{ const double x = {_hname(t)};")
         for blk in ("ab", "cd", "ac", "ad", "bc", "bd"):
             p0, p1 = (0, 1) if blk == "ab" else blocks[blk]
             q0, q1 = comp[blk]
@@ -234,23 +234,32 @@ def emit_class(cls) -> str:
     out = [f"// generated by felizjueves_amd/gen_hip.py -- class ({la}{lb}|{lc}{ld})",
            "#pragma once", '#include "../fj_device.h"', ""]
     out.append(emit_quartet(cls))
+    abx, aby, abz = (("a.bra.sep[3 * bra]", "a.bra.sep[3 * bra + 1]", "a.bra.sep[3 * bra + 2]")
+                     if lb else ("0.0",) * 3)
+    cdx, cdy, cdz = (("a.ket.sep[3 * k]", "a.ket.sep[3 * k + 1]", "a.ket.sep[3 * k + 2]")
+                     if ld else ("0.0",) * 3)
     out.append(f"""
 template <typename R>
 __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
-    // work item: (bra pair, first ket) from the list, else (x, y * blockDim)
-    int bra, k0;
+    // work item (bra pair, first ket, system, ket end) from the list, else
+    // (x, y * blockDim, the single system, n_ket)
+    int bra, k0, k1;
+    FjSys S;
     if (a.work) {{
-        bra = a.work[2 * blockIdx.x];
-        k0 = a.work[2 * blockIdx.x + 1];
+        const int4 w = a.work[blockIdx.x];
+        bra = w.x; k0 = w.y; S = a.sys[w.z]; k1 = w.w;
     }} else {{
         bra = (int)blockIdx.x;
         k0 = (int)(blockIdx.y * blockDim.x);
+        S = a.one;
+        k1 = a.ket.n;
     }}
     const int kmin = a.diag * bra;
-    if (k0 + (int)blockDim.x <= kmin || k0 >= a.ket.n) return;   // whole block below the canonical half
+    if (k0 + (int)blockDim.x <= kmin || k0 >= k1) return;   // whole block below the canonical half
     int kb0, kab;
     fj_side_range(a.bra, bra, kb0, kab);
+    if (kab > FJ_MAX_PRIM_PAIRS) return;   // the host refuses such a pair; never overrun the staging
     for (int i = threadIdx.x; i < kab; i += blockDim.x) {{
         const double* __restrict__ r = a.bra.prim + 8 * (size_t)(kb0 + i);
         double* s = bsh + FJ_SREC * i;
@@ -262,18 +271,19 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     const int sa = a.bra.sh[2 * bra], sb = a.bra.sh[2 * bra + 1];
     const int oa = a.bra.ao[2 * bra], ob = a.bra.ao[2 * bra + 1];
     const double Gb = a.bra.G[bra];
-    const double ABx = {"a.bra.sep[3 * bra]" if lb else "0.0"}, ABy = {"a.bra.sep[3 * bra + 1]" if lb else "0.0"}, ABz = {"a.bra.sep[3 * bra + 2]" if lb else "0.0"};
+    const double ABx = {abx}, ABy = {aby}, ABz = {abz};
+    const int n_ao = S.n_ao;
     const int k = k0 + (int)threadIdx.x;
     double fab[{nab}];
     double wab[{nab}];
 #pragma unroll
     for (int i = 0; i < {nab}; ++i) fab[i] = 0.0;
-    bool live = k < a.ket.n && k >= kmin;
+    bool live = k < k1 && k >= kmin;
     int sc = 0, sd = 0;
     if (live) {{
         sc = a.ket.sh[2 * k]; sd = a.ket.sh[2 * k + 1];
-        const double* __restrict__ Dsh = a.Dsh;
-        const int ns = a.n_sh;
+        const double* __restrict__ Dsh = S.Dsh;
+        const int ns = S.n_sh;
         double dmax = 2.0 * Dsh[sa * ns + sb];
         dmax = fmax(dmax, 2.0 * Dsh[sc * ns + sd]);
         dmax = fmax(dmax, 0.5 * Dsh[sa * ns + sc]);
@@ -292,12 +302,12 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
 #pragma unroll
         for (int i = 0; i < {na}; ++i)
 #pragma unroll
-            for (int j = 0; j < {nb}; ++j) wab[i * {nb} + j] = s_j * a.D[(oa + i) * a.n_ao + ob + j];
+            for (int j = 0; j < {nb}; ++j) wab[i * {nb} + j] = s_j * S.D[(oa + i) * n_ao + ob + j];
         int kc0, kcd;
         fj_side_range(a.ket, k, kc0, kcd);
-        const double CDx = {"a.ket.sep[3 * k]" if ld else "0.0"}, CDy = {"a.ket.sep[3 * k + 1]" if ld else "0.0"}, CDz = {"a.ket.sep[3 * k + 2]" if ld else "0.0"};
+        const double CDx = {cdx}, CDy = {cdy}, CDz = {cdz};
         fj_q_{name}<R>(bsh, kab, a.ket.prim + 8 * (size_t)kc0, kcd, ABx, ABy, ABz, CDx, CDy, CDz,
-                        a.D, a.n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, a.F);
+                        S.D, n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, S.F);
     }}
     // F_ab: one bra per workgroup, so every lane of a wavefront adds into the
     // same block; reduce across the wavefront, then one atomic per element.
@@ -305,7 +315,7 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     for (int i = 0; i < {nab}; ++i) {{
         const double s = fj_wave_sum(fab[i]);
         if ((threadIdx.x & 63) == 0 && s != 0.0)
-            unsafeAtomicAdd(a.F + (oa + i / {nb}) * a.n_ao + ob + i % {nb}, s);
+            unsafeAtomicAdd(S.F + (oa + i / {nb}) * n_ao + ob + i % {nb}, s);
     }}
 }}
 
@@ -336,20 +346,22 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
     a.bra = FjSide{{bra_prim, nullptr, k_ab, {"AB" if lb else "nullptr"}, G_ab, bra_sh, bra_ao, n_bra}};
     a.ket = FjSide{{ket_prim, nullptr, k_cd, {"CD" if ld else "nullptr"}, G_cd, ket_sh, ket_ao, n_ket}};
     a.work = nullptr;
-    a.n_sh = n_sh; a.n_ao = n_ao; a.diag = diag; a.tau = tau;
-    a.Dsh = Dsh; a.D = D; a.F = F;
+    a.sys = nullptr;
+    a.one = FjSys{{D, Dsh, F, n_ao, n_sh}};
+    a.diag = diag; a.tau = tau;
     return fj_launch_{name}(s, threads, 0, a, n_bra);
 }}
 
 extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
                          const FjSide* bra, const FjSide* ket, const int* work, int n_work,
-                         int diag, int n_sh, int n_ao, double tau,
-                         const double* Dsh, const double* D, double* F) {{
-    if (!bra || !ket || !work) return FJ_ERR_ARG;
+                         const FjSys* sys, int diag, double tau) {{
+    if (!bra || !ket || !work || !sys) return FJ_ERR_ARG;
     FjFArgs a;
-    a.bra = *bra; a.ket = *ket; a.work = work;
-    a.n_sh = n_sh; a.n_ao = n_ao; a.diag = diag; a.tau = tau;
-    a.Dsh = Dsh; a.D = D; a.F = F;
+    a.bra = *bra; a.ket = *ket;
+    a.work = reinterpret_cast<const int4*>(work);
+    a.sys = sys;
+    a.one = FjSys{{nullptr, nullptr, nullptr, 0, 0}};
+    a.diag = diag; a.tau = tau;
     return fj_launch_{name}(s, threads, precision, a, n_work);
 }}
 """)
@@ -444,8 +456,7 @@ def header_decls(classes) -> str:
             f"    const double* Dsh, const double* D, double* F);\n"
             f"int fj_fx_{n}(fj_stream_t stream, int threads, int precision, const FjSide* bra,"
             f" const FjSide* ket,\n"
-            f"    const int* work, int n_work, int diag, int n_sh, int n_ao, double tau,\n"
-            f"    const double* Dsh, const double* D, double* F);")
+            f"    const int* work, int n_work, const FjSys* sys, int diag, double tau);")
     return "\n".join(out) + "\n"
 
 

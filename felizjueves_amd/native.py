@@ -27,6 +27,22 @@ class FjSide(C.Structure):
                 ("ao", C.c_void_p), ("n", C.c_int)]
 
 
+class FjSys(C.Structure):
+    """Mirror of ``FjSys`` in include/fj_mi355x.h (device pointers)."""
+
+    _fields_ = [("D", C.c_void_p), ("Dsh", C.c_void_p), ("F", C.c_void_p),
+                ("n_ao", C.c_int), ("n_sh", C.c_int)]
+
+
+def sys_table(systems) -> "torch.Tensor":
+    """A device array of FjSys from (D, Dsh, F) device tensors (square)."""
+    arr = (FjSys * len(systems))()
+    for i, (D, Dsh, F) in enumerate(systems):
+        arr[i] = FjSys(D.data_ptr(), Dsh.data_ptr(), F.data_ptr(), D.shape[0], Dsh.shape[0])
+    raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+    return to_dev(raw)
+
+
 class FjError(RuntimeError):
     pass
 
@@ -47,6 +63,10 @@ def lib():
         if not LIB_PATH.exists():
             raise FjError(f"{LIB_PATH} is missing: run `python -m felizjueves_amd.build` "
                           "(the HIP library is the only implementation)")
+        # torch ships its own HIP runtime (SONAME libamdhip64.so.7); load it
+        # first so this library binds to that one instance instead of pulling
+        # a second HIP/HSA runtime into the process.
+        import torch  # noqa: F401
         L = C.CDLL(str(LIB_PATH))
         L.fj_version.restype = C.c_int
         _lib = L
@@ -108,13 +128,14 @@ def call_f(cls, threads: int, args: dict, F, stream=None) -> None:
 
 
 def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, work, n_work: int,
-            diag: int, n_sh: int, n_ao: int, tau: float, Dsh, D, F, stream=None) -> None:
+            sys, diag: int, tau: float, stream=None) -> None:
+    """One launch of fj_fx_<cls>: `work` a device int32 [n_work, 4] tensor of
+    (bra, first ket, system, ket end), `sys` a device FjSys table."""
     fn = getattr(lib(), "fj_fx_" + "".join(map(str, cls)))
     fn.restype = C.c_int
     rc = fn(C.c_void_p(stream_handle(stream)), C.c_int(threads), C.c_int(precision),
             C.byref(bra), C.byref(ket), C.c_void_p(ptr(work)), C.c_int(n_work),
-            C.c_int(diag), C.c_int(n_sh), C.c_int(n_ao), C.c_double(tau),
-            C.c_void_p(ptr(Dsh)), C.c_void_p(ptr(D)), C.c_void_p(ptr(F)))
+            C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau))
     check(rc, fn.__name__)
 
 
