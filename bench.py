@@ -118,14 +118,16 @@ def cpu_baseline(fb, D, budget_s=15.0):
         O.fx_launch(L.cls, sub, host[Y], 0, fb.basis.n_sh, fb.basis.n_ao, fb.tau, Dsh, D, F)
         return time.perf_counter() - t, fl, nq, len(idx)
 
-    # calibrate on a thin sample, then size the real one to the budget
-    stride = max(1, bra.n // 4)
+    # grow the sample (halving the stride) until it takes a third of the
+    # budget, then size the measured one to the budget
+    stride = max(1, bra.n // 8)
     t, fl, nq, nb = sample(stride)
-    rate = fl / max(t, 1e-3)
-    want = budget_s * rate
-    per_bra = fl / max(nb, 1)
-    stride = int(max(1, min(bra.n, round(bra.n * per_bra / max(want, 1.0)))))
-    t, fl, nq, nb = sample(stride)
+    while t < budget_s / 3 and stride > 1:
+        stride = max(1, stride // 2)
+        t, fl, nq, nb = sample(stride)
+    if t < budget_s and stride > 1:
+        stride = max(1, int(stride * t / budget_s))
+        t, fl, nq, nb = sample(stride)
     gflops = fl / t / 1e9
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     return {

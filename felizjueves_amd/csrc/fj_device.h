@@ -21,6 +21,7 @@
 struct FjFArgs {
     FjSide bra, ket;
     const int4* work;      // optional: workgroup -> (bra pair, first ket, system, ket end)
+    const int* ket_list;   // optional: work item kets are ket_list[first .. end)
     const FjSys* sys;      // the system table the work items index (device)
     FjSys one;             // the single system of a launch without a work list
     int diag;

@@ -63,6 +63,8 @@ class Launch:
     precision: int
     work: np.ndarray          # [n, 2] int32
     work_dev: object = None
+    ket_list: np.ndarray = None
+    ket_list_dev: object = None
     n_quartets: int = 0       # computed (post-screen) contracted quartets
     n_prim: int = 0           # primitive quartets of those
     flops: float = 0.0        # algorithmic FLOPs of those
@@ -70,12 +72,14 @@ class Launch:
 
 class FockBuilder:
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads: int = 64,
-                 mixed: bool = False, fp32_threshold: float = 1e-6, tile_precision=None):
+                 mixed: bool = False, fp32_threshold: float = 1e-6, tile_precision=None,
+                 lane_flops: float = 1.0e6):
         self.basis = basis
         self.tau = float(tau)
         self.threads = int(threads)
         self.mixed = bool(mixed)
         self.fp32_threshold = float(fp32_threshold)
+        self.lane_flops = float(lane_flops)
         t0 = time.time()
         kept, self.deleted, self.eps, self.g_max, self.n_prim_total, _ = P.kept_pairs(
             basis, D, self.tau, 1)
@@ -109,6 +113,9 @@ class FockBuilder:
         return bra.G[b] * (ket.G * dmax)
 
     def _plan(self) -> list:
+        """Per class pair and tier: the live (bra, ket) quartets of each bra,
+        compacted into a ket list in table order (degree, then G, descending)
+        and cut into tiles of `threads` kets; one work item per tile."""
         T = self.threads
         out = []
         classes = sorted(self.tables)
@@ -119,31 +126,56 @@ class FockBuilder:
                 c = cost(cls)
                 diag = int(X == Y)
                 work = {FP64: [], FP32: []}
+                cost_w = {FP64: [], FP32: []}
+                lists = {FP64: [], FP32: []}
+                n_list = {FP64: 0, FP32: 0}
                 stats = {FP64: [0, 0, 0.0], FP32: [0, 0, 0.0]}
                 for b in range(bra.n):
                     bound = self._bounds(bra, ket, b)
-                    k_start = b if diag else 0
-                    for k0 in range(k_start, ket.n, T):
-                        seg = bound[k0:k0 + T]
-                        live = seg >= self.tau
-                        if not live.any():
-                            continue
-                        tier = FP64
-                        if self.mixed:
-                            if self.tile_precision is not None:
-                                tier = self.tile_precision(cls, bra, ket, b, k0, seg)
-                            elif seg.max() < self.fp32_threshold:
-                                tier = FP32
-                        work[tier].append((b, k0, 0, ket.n))
-                        kab = int(bra.deg[b])
-                        kcd = ket.deg[k0:k0 + T][live]
-                        st = stats[tier]
-                        st[0] += int(live.sum())
-                        st[1] += int(kab * kcd.sum())
-                        st[2] += float(c.per_prim * kab * kcd.sum() + c.per_quartet * live.sum())
+                    live = bound >= self.tau
+                    if diag:
+                        live[:b] = False
+                    ks = np.nonzero(live)[0]
+                    if not len(ks):
+                        continue
+                    if self.mixed and self.tile_precision is None:
+                        lo = bound[ks] < self.fp32_threshold
+                        parts = {FP64: ks[~lo], FP32: ks[lo]}
+                    else:
+                        parts = {FP64: ks, FP32: ks[:0]}
+                    kab = int(bra.deg[b])
+                    # a lane's primitive loop is cut into chunks of at most
+                    # lane_flops / per_prim primitive quartets, so no work item
+                    # runs much longer than the others
+                    chunk = max(1, min(255, max(8, int(self.lane_flops // c.per_prim)) // kab))
+                    for tier, kt in parts.items():
+                        for t0 in range(0, len(kt), T):
+                            seg = kt[t0:t0 + T]
+                            tr = tier
+                            if self.mixed and self.tile_precision is not None:
+                                tr = self.tile_precision(cls, bra, ket, b, seg, bound[seg])
+                            st = n_list[tr]
+                            lists[tr].append(seg)
+                            n_list[tr] += len(seg)
+                            kmax = int(ket.deg[seg].max())
+                            if kmax <= chunk:
+                                work[tr].append((b, st, 0, st + len(seg)))
+                                cost_w[tr].append(kab * kmax)
+                            else:
+                                for kc in range(0, kmax, chunk):
+                                    work[tr].append((b, st, (kc << 16) | (chunk << 24), st + len(seg)))
+                                    cost_w[tr].append(kab * min(chunk, kmax - kc))
+                            s_ = stats[tr]
+                            s_[0] += len(seg)
+                            s_[1] += int(kab * ket.deg[seg].sum())
+                            s_[2] += float(c.per_prim * kab * ket.deg[seg].sum() + c.per_quartet * len(seg))
                 for tier in (FP64, FP32):
                     if work[tier]:
-                        L = Launch(cls, bra, ket, diag, tier, np.array(work[tier], dtype=np.int32))
+                        # costliest work items first: the dispatcher starts them early
+                        order = np.argsort(-np.array(cost_w[tier]), kind="stable")
+                        L = Launch(cls, bra, ket, diag, tier,
+                                   np.array(work[tier], dtype=np.int32)[order])
+                        L.ket_list = np.concatenate(lists[tier]).astype(np.int32)
                         L.n_quartets, L.n_prim, L.flops = stats[tier]
                         out.append(L)
         # biggest launches first: their tails overlap the small ones'
@@ -169,6 +201,7 @@ class FockBuilder:
                                    d["ao"].data_ptr(), t.n)
         for L in self.launches:
             L.work_dev = native.to_dev(L.work)
+            L.ket_list_dev = native.to_dev(L.ket_list)
         self.shell_ao = native.to_dev(b.shell_ao.astype(np.int32))
         self.shell_size = native.to_dev(b.shell_size.astype(np.int32))
         self.Dsh = torch.zeros((b.n_sh, b.n_sh), dtype=torch.float64, device=dev)
@@ -188,7 +221,7 @@ class FockBuilder:
 
     def launch(self, L: "Launch", sys, stream=None) -> None:
         native.call_fx(L.cls, self.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
-                       L.work.shape[0], sys, L.diag, self.tau, stream)
+                       L.work.shape[0], sys, L.diag, self.tau, stream, L.ket_list_dev)
 
     def build(self, D_dev, stream=None, close: bool = True):
         """G for the device density D_dev (n_ao x n_ao float64), on `stream`.

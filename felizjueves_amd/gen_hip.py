@@ -233,7 +233,7 @@ def emit_class(cls) -> str:
     nab = na * nb
     out = [f"// generated by felizjueves_amd/gen_hip.py -- class ({la}{lb}|{lc}{ld})",
            "#pragma once", '#include "../fj_device.h"', ""]
-    out.append(emit_quartet(cls))
+    out.append(emit_quartet_staged(cls) if staged(cls) else emit_quartet(cls))
     abx, aby, abz = (("a.bra.sep[3 * bra]", "a.bra.sep[3 * bra + 1]", "a.bra.sep[3 * bra + 2]")
                      if lb else ("0.0",) * 3)
     cdx, cdy, cdz = (("a.ket.sep[3 * k]", "a.ket.sep[3 * k + 1]", "a.ket.sep[3 * k + 2]")
@@ -244,11 +244,18 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
     // work item (bra pair, first ket, system, ket end) from the list, else
     // (x, y * blockDim, the single system, n_ket)
-    int bra, k0, k1;
+    // The third field packs the system (bits 0-15) and a chunk of the ket
+    // primitive loop: first kc (bits 16-23) and count (bits 24-31, 0 = all).
+    // The HRR and the digestion are linear in the contracted boundary, so a
+    // quartet split over kc chunks sums exactly through F's atomics.
+    int bra, k0, k1, kc_lo = 0, kc_n = 0;
     FjSys S;
     if (a.work) {{
         const int4 w = a.work[blockIdx.x];
-        bra = w.x; k0 = w.y; S = a.sys[w.z]; k1 = w.w;
+        bra = w.x; k0 = w.y; k1 = w.w;
+        S = a.sys[w.z & 0xffff];
+        kc_lo = (w.z >> 16) & 0xff;
+        kc_n = (w.z >> 24) & 0xff;
     }} else {{
         bra = (int)blockIdx.x;
         k0 = (int)(blockIdx.y * blockDim.x);
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
         k1 = a.ket.n;
     }}
     const int kmin = a.diag * bra;
-    if (k0 + (int)blockDim.x <= kmin || k0 >= k1) return;   // whole block below the canonical half
+    if (k0 >= k1 || (!a.ket_list && k0 + (int)blockDim.x <= kmin)) return;   // nothing for this block
     int kb0, kab;
     fj_side_range(a.bra, bra, kb0, kab);
     if (kab > FJ_MAX_PRIM_PAIRS) return;   // the host refuses such a pair; never overrun the staging
@@ -273,12 +280,14 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     const double Gb = a.bra.G[bra];
     const double ABx = {abx}, ABy = {aby}, ABz = {abz};
     const int n_ao = S.n_ao;
-    const int k = k0 + (int)threadIdx.x;
+    // with a ket list the work item names list[k0 .. k1), the live kets only
+    const int kk = k0 + (int)threadIdx.x;
+    const int k = a.ket_list ? (kk < k1 ? a.ket_list[kk] : 0) : kk;
     double fab[{nab}];
     double wab[{nab}];
 #pragma unroll
     for (int i = 0; i < {nab}; ++i) fab[i] = 0.0;
-    bool live = k < k1 && k >= kmin;
+    bool live = kk < k1 && k >= kmin;
     int sc = 0, sd = 0;
     if (live) {{
         sc = a.ket.sh[2 * k]; sd = a.ket.sh[2 * k + 1];
@@ -291,6 +300,9 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
         dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sc]);
         dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sd]);
         live = !(Gb * (a.ket.G[k] * dmax) < a.tau);
+        int kc0_, kcd_;
+        fj_side_range(a.ket, k, kc0_, kcd_);
+        if (kcd_ <= kc_lo) live = false;   // this lane's ket has no primitives in the chunk
     }}
     if (live) {{
         double deg = 1.0;
@@ -305,6 +317,8 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
             for (int j = 0; j < {nb}; ++j) wab[i * {nb} + j] = s_j * S.D[(oa + i) * n_ao + ob + j];
         int kc0, kcd;
         fj_side_range(a.ket, k, kc0, kcd);
+        kc0 += kc_lo;
+        kcd = kc_n ? min(kcd - kc_lo, kc_n) : kcd - kc_lo;
         const double CDx = {cdx}, CDy = {cdy}, CDz = {cdz};
         fj_q_{name}<R>(bsh, kab, a.ket.prim + 8 * (size_t)kc0, kcd, ABx, ABy, ABz, CDx, CDy, CDz,
                         S.D, n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, S.F);
@@ -346,6 +360,7 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
     a.bra = FjSide{{bra_prim, nullptr, k_ab, {"AB" if lb else "nullptr"}, G_ab, bra_sh, bra_ao, n_bra}};
     a.ket = FjSide{{ket_prim, nullptr, k_cd, {"CD" if ld else "nullptr"}, G_cd, ket_sh, ket_ao, n_ket}};
     a.work = nullptr;
+    a.ket_list = nullptr;
     a.sys = nullptr;
     a.one = FjSys{{D, Dsh, F, n_ao, n_sh}};
     a.diag = diag; a.tau = tau;
@@ -354,11 +369,12 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
 
 extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
                          const FjSide* bra, const FjSide* ket, const int* work, int n_work,
-                         const FjSys* sys, int diag, double tau) {{
+                         const int* ket_list, const FjSys* sys, int diag, double tau) {{
     if (!bra || !ket || !work || !sys) return FJ_ERR_ARG;
     FjFArgs a;
     a.bra = *bra; a.ket = *ket;
     a.work = reinterpret_cast<const int4*>(work);
+    a.ket_list = ket_list;
     a.sys = sys;
     a.one = FjSys{{nullptr, nullptr, nullptr, 0, 0}};
     a.diag = diag; a.tau = tau;
@@ -456,7 +472,8 @@ def header_decls(classes) -> str:
             f"    const double* Dsh, const double* D, double* F);\n"
             f"int fj_fx_{n}(fj_stream_t stream, int threads, int precision, const FjSide* bra,"
             f" const FjSide* ket,\n"
-            f"    const int* work, int n_work, const FjSys* sys, int diag, double tau);")
+            f"    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,\n"
+            f"    double tau);")
     return "\n".join(out) + "\n"
 
 
@@ -475,3 +492,233 @@ def write_header(classes) -> None:
     new = header_text(classes, cur)
     if new != cur:
         HEADER.write_text(new)
+
+
+# ---------------------------------------------------------------------------
+# The staged body, for classes whose fused body would not fit the register
+# file (FUSED_LIVE_MAX): phase A runs the primitive nest once per group of
+# bra components e (hgp.e_groups), each group holding only its own
+# accumulators and the closure of its targets, and stores the contracted
+# boundary (e0|f0) to a private array; phase B is the bra HRR, one loop trip
+# per ket component f; phase C is the ket HRR per bra pair component ab,
+# with that row's digestion (F_ab and F_cd in registers, the four exchange
+# blocks flushed per ab).
+
+FUSED_LIVE_MAX = 120
+
+
+def staged(cls) -> bool:
+    from .hgp import peak_live
+    v = vrr_dag(tuple(cls))
+    return peak_live(v, set(v.order)) + len(v.targets) > FUSED_LIVE_MAX
+
+
+def _hrr_side(la: int, lb: int):
+    """One side's HRR as a DAG over (a, b) with the other side fixed:
+    (a, b) = (a + 1_k, b - 1_k) + sep_k (a, b - 1_k), k the first nonzero
+    axis of b; leaves (e, 0), la <= |e| <= la + lb. Returns (order, terms,
+    targets, leaves) with targets in cartesians(la) x cartesians(lb) order."""
+    from .hgp import _add, _sub
+    targets = [(a, b) for a in cartesians(la) for b in cartesians(lb)]
+    terms: dict = {}
+    todo = list(targets)
+    while todo:
+        n = todo.pop()
+        if n in terms:
+            continue
+        a, b = n
+        if b != ZERO:
+            k = next(i for i, x in enumerate(b) if x)
+            r = (k, (_add(a, k), _sub(b, k)), (a, _sub(b, k)))
+            terms[n] = r
+            todo += [r[1], r[2]]
+        else:
+            terms[n] = ()
+    order = sorted(terms, key=lambda n: (sum(n[1]), -sum(n[0]), tuple(-x for x in n[0]),
+                                         tuple(-x for x in n[1])))
+    leaves = [e for n in range(la, la + lb + 1) for e in cartesians(n)]
+    return order, terms, targets, leaves
+
+
+def _emit_side_hrr(E: _Emit, la: int, lb: int, sep: str, src, dst, ind: int) -> None:
+    """Emit one side's HRR: leaves read by src(i) (i over the e range),
+    targets written by dst(j, expr) (j over cartesians(la) x cartesians(lb))."""
+    order, terms, targets, leaves = _hrr_side(la, lb)
+    li = {e: i for i, e in enumerate(leaves)}
+    name = {}
+    for n in order:
+        nm = f"t{_fidx(n[0])}_{_fidx(n[1])}"
+        name[n] = nm
+        r = terms[n]
+        if not r:
+            E(f"const double {nm} = {src(li[n[0]])};", ind)
+        else:
+            k, moved, lowered = r
+            E(f"const double {nm} = {name[moved]} + {sep}{AX[k]} * {name[lowered]};", ind)
+    for j, t in enumerate(targets):
+        dst(j, name[t])
+
+
+def emit_quartet_staged(cls) -> str:
+    from .hgp import closure, e_groups
+    la, lb, lc, ld = cls
+    L = sum(cls)
+    v = vrr_dag(tuple(cls))
+    name = cls_name(cls)
+    na, nb, nc, nd = (n_cart(x) for x in cls)
+    eR = [e for n in range(la, la + lb + 1) for e in cartesians(n)]
+    fR = [f for n in range(lc, lc + ld + 1) for f in cartesians(n)]
+    NE, NF = len(eR), len(fR)
+    NAB, NCD = na * nb, nc * nd
+    ei = {e: i for i, e in enumerate(eR)}
+    fi = {f: i for i, f in enumerate(fR)}
+    groups = e_groups(tuple(cls))
+
+    E = _Emit()
+    E(f"// ({la}{lb}|{lc}{ld}), staged: {len(groups)} VRR groups over {NE} bra components, "
+      f"{len(v.targets)} boundary values", 0)
+    E("template <typename R>", 0)
+    E(f"__device__ __forceinline__ void fj_q_{name}(", 0)
+    E("const double* __restrict__ bp, int k_ab,")
+    E("const double* __restrict__ kp, int k_cd,")
+    E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
+    E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
+    E("double s_j, double s_k, const double* __restrict__ wab,")
+    E("double* __restrict__ fab, double* __restrict__ F) {", 1)
+    E(f"double bnd[{NF * NE}];   // contracted (e0|f0), [f][e]")
+    for gi, g in enumerate(groups):
+        tg = [(e, f) for e, f in v.targets if e in g]
+        nodes = closure(v, [(e, f, 0) for e, f in tg])
+        used = set()
+        for n in nodes:
+            r = v.terms[n]
+            if r:
+                d, ts = r
+                for t in ts:
+                    used.add((t.coef, d if t.coef in ("PA", "WP", "QC", "WQ") else None))
+        need = lambda c: any(u[0] == c for u in used)  # noqa: E731
+        E(f"{{   // group {gi}: e = {list(g)}")
+        for i in range(len(tg)):
+            E(f"R c{i} = R(0);", 2)
+        E("for (int kc = 0; kc < k_cd; ++kc) {", 2)
+        E("const double* __restrict__ q_ = kp + 8 * kc;", 3)
+        E("const double Qx = q_[0], Qy = q_[1], Qz = q_[2], q = q_[6], Uq = q_[7];", 3)
+        for d in range(3):
+            if ("QC", d) in used:
+                E(f"const R QC{AX[d]} = R(q_[{3 + d}]);", 3)
+        if need("i2q") or need("i2q_r"):
+            E("const double iq = 1.0 / q;", 3)
+        E("for (int kb = 0; kb < k_ab; ++kb) {", 3)
+        E("const double* __restrict__ p_ = bp + FJ_SREC * kb;", 4)
+        E("const double PQx = p_[0] - Qx, PQy = p_[1] - Qy, PQz = p_[2] - Qz;", 4)
+        E("const double p = p_[6];", 4)
+        E("const double ippq = 1.0 / (p + q);", 4)
+        E("const double rho = p * q * ippq;", 4)
+        E("const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);", 4)
+        E(f"double Fm[{L + 1}];", 4)
+        E(f"fj_boys<{L}>(T, Fm);", 4)
+        E("const double S = p_[7] * Uq * (FJ_TWO_OVER_SQRT_PI * sqrt(rho));", 4)
+        for d in range(3):
+            if ("PA", d) in used:
+                E(f"const R PA{AX[d]} = R(p_[{3 + d}]);", 4)
+            if ("WP", d) in used:
+                E(f"const R WP{AX[d]} = R(-q * ippq * PQ{AX[d]});", 4)
+            if ("WQ", d) in used:
+                E(f"const R WQ{AX[d]} = R(p * ippq * PQ{AX[d]});", 4)
+        if need("i2p"):
+            E("const R i2p = R(0.5 * p_[8]);", 4)
+        if need("i2p_r"):
+            E("const R i2p_r = R(0.5 * p_[8] * rho * p_[8]);", 4)
+        if need("i2q"):
+            E("const R i2q = R(0.5 * iq);", 4)
+        if need("i2q_r"):
+            E("const R i2q_r = R(0.5 * iq * rho * iq);", 4)
+        if need("i2pq"):
+            E("const R i2pq = R(0.5 * ippq);", 4)
+        for n in v.order:
+            if n not in nodes:
+                continue
+            r = v.terms[n]
+            if not r:
+                E(f"const R {_vname(n)} = R(S * Fm[{n[2]}]);", 4)
+                continue
+            d, ts = r
+            parts = []
+            for t in ts:
+                c = t.coef
+                ch = _vname(t.child)
+                if c in ("PA", "WP", "QC", "WQ"):
+                    parts.append(f"{c}{AX[d]} * {ch}")
+                else:
+                    m = t.mult
+                    sign = "-" if m < 0 else "+"
+                    f = c if abs(m) == 1 else f"(R({abs(m)}) * {c})"
+                    parts.append(f"{sign} {f} * {ch}")
+            expr = parts[0]
+            for p in parts[1:]:
+                expr += (" " + p) if p[0] in "+-" else (" + " + p)
+            E(f"const R {_vname(n)} = {expr};", 4)
+        for i, (e, f) in enumerate(tg):
+            E(f"c{i} += {_vname((e, f, 0))};", 4)
+        E("}", 3)
+        E("}", 2)
+        for i, (e, f) in enumerate(tg):
+            E(f"bnd[{fi[f] * NE + ei[e]}] = double(c{i});", 2)
+        E("}")
+    # phase B: bra HRR per f
+    E(f"double mid[{NAB * NF}];   // (ab|f0), [ab][f]")
+    if lb == 0:
+        E("#pragma unroll 1", 0)
+        E(f"for (int i = 0; i < {NAB * NF}; ++i) mid[(i % {NE}) * {NF} + i / {NE}] = bnd[i];")
+    else:
+        E("#pragma unroll 1", 0)
+        E(f"for (int f = 0; f < {NF}; ++f) {{")
+        E(f"const double* __restrict__ x = bnd + f * {NE};", 2)
+        _emit_side_hrr(E, la, lb, "AB", lambda i: f"x[{i}]",
+                       lambda j, nm: E(f"mid[{j} * {NF} + f] = {nm};", 2), 2)
+        E("}")
+    # phase C: ket HRR per ab + digestion
+    for c in range(nc):
+        for d in range(nd):
+            E(f"const double wcd{c}_{d} = s_j * D[(oc + {c}) * n_ao + od + {d}];")
+            E(f"double fcd{c}_{d} = 0.0;")
+    E("#pragma unroll 1", 0)
+    E(f"for (int ab = 0; ab < {NAB}; ++ab) {{")
+    E(f"const int a = ab / {nb}, b = ab % {nb};", 2)
+    E(f"const double* __restrict__ y = mid + ab * {NF};", 2)
+    xs = {}
+
+    def keep(j, nm):
+        xs[j] = nm
+    _emit_side_hrr(E, lc, ld, "CD", lambda i: f"y[{i}]", keep, 2)
+    E("const double w_ab = s_j * D[(oa + a) * n_ao + ob + b];", 2)
+    E("const double* __restrict__ Db = D + (size_t)(ob + b) * n_ao;", 2)
+    E("const double* __restrict__ Da = D + (size_t)(oa + a) * n_ao;", 2)
+    for c in range(nc):
+        E(f"const double dbc{c} = s_k * Db[oc + {c}], dac{c} = s_k * Da[oc + {c}];", 2)
+    for d in range(nd):
+        E(f"const double dbd{d} = s_k * Db[od + {d}], dad{d} = s_k * Da[od + {d}];", 2)
+    E("double sab = 0.0;", 2)
+    for c in range(nc):
+        E(f"double fac{c} = 0.0, fbc{c} = 0.0;", 2)
+    for d in range(nd):
+        E(f"double fad{d} = 0.0, fbd{d} = 0.0;", 2)
+    for c in range(nc):
+        for d in range(nd):
+            x = xs[c * nd + d]
+            E(f"sab += wcd{c}_{d} * {x}; fcd{c}_{d} += w_ab * {x};", 2)
+            E(f"fac{c} += dbd{d} * {x}; fad{d} += dbc{c} * {x}; "
+              f"fbc{c} += dad{d} * {x}; fbd{d} += dac{c} * {x};", 2)
+    E("fab[ab] += sab;", 2)
+    for c in range(nc):
+        E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + oc + {c}, fac{c});", 2)
+        E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + oc + {c}, fbc{c});", 2)
+    for d in range(nd):
+        E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + od + {d}, fad{d});", 2)
+        E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + od + {d}, fbd{d});", 2)
+    E("}")
+    for c in range(nc):
+        for d in range(nd):
+            E(f"unsafeAtomicAdd(F + (size_t)(oc + {c}) * n_ao + od + {d}, fcd{c}_{d});")
+    E("}", 0)
+    return E.text()

@@ -228,3 +228,76 @@ def fock_classes(lmax: int) -> tuple[tuple[int, int, int, int], ...]:
     ket pair (upstream `hgp/vrr.py:89-91` for lmax = 2: 21 classes)."""
     pairs = [(a, b) for a in range(lmax + 1) for b in range(a, lmax + 1)]
     return tuple(p + q for i, p in enumerate(pairs) for q in pairs[i:])
+
+
+# ---------------------------------------------------------------------------
+# Target groups for the staged kernels: the boundary split by bra component
+# e, each group's VRR being the closure of its targets (nodes shared between
+# groups are recomputed). Chosen so that each group's live set fits the
+# register file.
+
+def closure(dag: VrrDag, targets) -> frozenset:
+    seen: set = set()
+    todo = list(targets)
+    while todo:
+        n = todo.pop()
+        if n in seen:
+            continue
+        seen.add(n)
+        r = dag.terms[n]
+        if r:
+            todo.extend(t.child for t in r[1])
+    return frozenset(seen)
+
+
+def peak_live(dag: VrrDag, nodes) -> int:
+    """Most values live at once executing `nodes` in build order, each
+    target consumed (accumulated) right after it is formed, each other node
+    dead after its last reader."""
+    order = [n for n in dag.order if n in nodes]
+    pos = {n: i for i, n in enumerate(order)}
+    last: dict = {}
+    for n in order:
+        r = dag.terms[n]
+        if r:
+            for t in r[1]:
+                last[t.child] = pos[n]
+    alive: set = set()
+    peak = 0
+    for i, n in enumerate(order):
+        alive.add(n)
+        r = dag.terms[n]
+        if r:
+            for t in r[1]:
+                if last.get(t.child) == i:
+                    alive.discard(t.child)
+        peak = max(peak, len(alive))
+        if n not in last:
+            alive.discard(n)
+    return peak
+
+
+@lru_cache(maxsize=None)
+def e_groups(cls, live_max: int = 100):
+    """Split the boundary's bra components into contiguous groups whose
+    closures (plus their accumulators) keep at most `live_max` values live.
+    Returns a tuple of tuples of e; one group means the fused kernel."""
+    dag = vrr_dag(tuple(cls))
+    es = []
+    for e, _ in dag.targets:
+        if e not in es:
+            es.append(e)
+    nF = len(dag.targets) // len(es)
+
+    def live(group) -> int:
+        tg = [(e, f, 0) for e, f in dag.targets if e in group]
+        return peak_live(dag, closure(dag, tg)) + len(tg)
+
+    if live(es) <= live_max:
+        return (tuple(es),)
+    for size in range(len(es) - 1, 0, -1):
+        groups = [tuple(es[i:i + size]) for i in range(0, len(es), size)]
+        if all(live(g) <= live_max for g in groups):
+            return tuple(groups)
+    del nF
+    return tuple((e,) for e in es)

@@ -128,14 +128,15 @@ def call_f(cls, threads: int, args: dict, F, stream=None) -> None:
 
 
 def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, work, n_work: int,
-            sys, diag: int, tau: float, stream=None) -> None:
+            sys, diag: int, tau: float, stream=None, ket_list=None) -> None:
     """One launch of fj_fx_<cls>: `work` a device int32 [n_work, 4] tensor of
-    (bra, first ket, system, ket end), `sys` a device FjSys table."""
+    (bra, first ket, system, ket end), `sys` a device FjSys table; with
+    `ket_list` the (first, end) range indexes that device int32 list."""
     fn = getattr(lib(), "fj_fx_" + "".join(map(str, cls)))
     fn.restype = C.c_int
     rc = fn(C.c_void_p(stream_handle(stream)), C.c_int(threads), C.c_int(precision),
             C.byref(bra), C.byref(ket), C.c_void_p(ptr(work)), C.c_int(n_work),
-            C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau))
+            C.c_void_p(ptr(ket_list)), C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau))
     check(rc, fn.__name__)
 
 

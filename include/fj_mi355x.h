@@ -92,7 +92,12 @@ int fj_boys_eval(fj_stream_t stream, int M, int n, const double* T, double* out)
 /* fj_fx_<cls>: `bra`, `ket` are HOST pointers to FjSide records of device
  * pointers; `work` is a device int array [n_work][4] of (bra pair, first
  * ket, system index into the device array `sys`, ket end exclusive), one
- * workgroup each, `threads` consecutive kets from the first; `precision` 0
+ * workgroup each, `threads` consecutive kets from the first (the system
+ * field packs the system index in bits 0-15 and, optionally, a chunk of each
+ * ket's primitive pairs: first in bits 16-23, count in bits 24-31, 0 = all;
+ * chunks of one quartet add up exactly in F); with a device
+ * `ket_list` (may be NULL) the (first, end) range indexes that list instead,
+ * so a work item can name a compacted set of kets; `precision` 0
  * is the FP64 kernel, 1 the FP32 tier (VRR and boundary in FP32; Boys, the
  * seed, the HRR and every F update in FP64). */
 /* BEGIN F CLASSES (written by felizjueves_amd/gen_hip.py) */
@@ -102,147 +107,168 @@ int fj_f_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0000(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0001(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0011(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0101(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0211(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const FjSys* sys, int diag, double tau);
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau);
 /* END F CLASSES */
 
 #ifdef __cplusplus
