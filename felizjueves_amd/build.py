@@ -13,7 +13,7 @@ import time
 from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
-from . import gen_hip
+from . import gen_hip, gen_mmd
 from .hgp import fock_classes
 
 HERE = Path(__file__).resolve().parent
@@ -22,6 +22,7 @@ OBJ = HERE / "_build"
 LIB = HERE / "libfj_mi355x.so"
 ARCH = "gfx950"
 LMAX = 2
+MMD_LMAX = 2
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -36,7 +37,9 @@ def sources(lmax: int = LMAX) -> list[Path]:
     classes = fock_classes(lmax)
     files = gen_hip.generate(classes)
     gen_hip.write_header(classes)
-    return [CSRC / "fj_runtime.hip", *files]
+    mmd = gen_mmd.generate(MMD_LMAX)
+    gen_mmd.write_header(MMD_LMAX)
+    return [CSRC / "fj_runtime.hip", *files, *mmd]
 
 
 def _compile(src: Path, extra: list[str]) -> tuple[Path, float, str]:

@@ -81,3 +81,24 @@ extern "C" int fj_boys_eval(hipStream_t s, int M, int n, const double* T, double
     }
     return (int)hipGetLastError();
 }
+
+// K[oa[r] + i, ob + j] += Kr[r, i * nb + j]: the K kernel's per-bra-record
+// blocks added into the exchange matrix (several records share a block).
+__global__ void fj_k_scatter_k(int n_rec, int na, int nb, int n_ao, const int* __restrict__ oa, int ob,
+                               const double* __restrict__ Kr, double* __restrict__ K) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nab = na * nb;
+    if (t >= n_rec * nab) return;
+    const int r = t / nab, e = t % nab;
+    unsafeAtomicAdd(K + (size_t)(oa[r] + e / nb) * n_ao + ob + e % nb, Kr[t]);
+}
+
+extern "C" int fj_k_scatter(hipStream_t s, int n_rec, int na, int nb, int n_ao, const int* oa, int ob,
+                            const double* Kr, double* K) {
+    if (n_rec < 0 || na <= 0 || nb <= 0 || n_ao <= 0) return FJ_ERR_ARG;
+    const long long total = (long long)n_rec * na * nb;
+    if (total == 0) return 0;
+    hipLaunchKernelGGL(fj_k_scatter_k, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       n_rec, na, nb, n_ao, oa, ob, Kr, K);
+    return (int)hipGetLastError();
+}

@@ -37,6 +37,7 @@ from .basis import Basis
 from .hgp import cost
 
 FP64, FP32 = 0, 1
+FOCK_LMAX = 2  # the top class built: (dd|dd)
 
 
 @dataclass
@@ -74,6 +75,9 @@ class FockBuilder:
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads: int = 64,
                  mixed: bool = False, fp32_threshold: float = 1e-6, tile_precision=None,
                  lane_flops: float = 1.0e6):
+        if basis.l_max > FOCK_LMAX:
+            raise ValueError(f"shells up to l = {basis.l_max}: the fused F kernels are built through "
+                             f"l = {FOCK_LMAX} ((dd|dd)); f classes are not built yet")
         self.basis = basis
         self.tau = float(tau)
         self.threads = int(threads)

@@ -1,0 +1,287 @@
+"""Host drivers of the MMD families: the Schwarz matrix, the J chain
+(DQ -> [p|q] -> JAB) and the K exchange build, all on the device.
+
+Records (upstream `ao/density.py:151`, `eri/schwarz.py:152-176`): per
+primitive pair of an ordered shell pair (a, b): ``zeta = alpha + beta``,
+``P = (alpha A + beta B)/zeta``, ``X_PA = P - A``, ``X_PB = P - B`` and the
+dressing ``s = c_a c_b (pi/zeta)^{3/2}``; the Hermite coefficients (with
+their ``exp(-alpha beta/zeta |AB|^2)``) are formed in the kernels.
+
+J chain (upstream `eri/j_coulomb/j.py:1-60,254-326`, `drive.py:367-594`):
+unique ket pairs with weight 2 off the diagonal, 1 on it, folded into D_q;
+kets sorted descending by ``Iq Dmax`` (Dmax read back, as upstream's one
+round trip); one [p|q] launch per (bra class, ket class) into its own J_p
+buffer; one JAB launch per buffer, atomically added into J with the mirror
+and diagonal rules.
+
+K (upstream `eri/k_exchange/kernel_k.py:1-110`, `launch_k.py:308-479`):
+``K_ab = sum_cd D_cd (ac|bd)``; bra records of ordered pairs (a, c), ket
+records of ordered pairs (b, d) of ONE b shell per launch, sorted by Iq; the
+kernel's per-bra-record blocks K[rec, a, b] are scattered into K by a small
+kernel.
+
+Bounds (upstream `ao/pairs.py:1-120`): ``I_p(zeta)`` closed form as in
+`pairs.hermite_bound_sq`; a record's ``I = sum_p I_p`` over
+``hermite_order(L)``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import native
+from .basis import Basis, n_cart
+from .pairs import hermite_bound_sq, hermite_order
+
+
+def _records(basis: Basis, a: int, b: int):
+    """(P [n,3], zeta [n], s [n], XPA [n,3], XPB [n,3]) of ordered pair (a, b)."""
+    A, B = basis.shell_centre[a], basis.shell_centre[b]
+    ia = slice(basis.shell_pstart[a], basis.shell_pstart[a] + basis.shell_nprim[a])
+    ib = slice(basis.shell_pstart[b], basis.shell_pstart[b] + basis.shell_nprim[b])
+    al, be = basis.exps[ia], basis.exps[ib]
+    ca, cb = basis.coefs[ia], basis.coefs[ib]
+    z = (al[:, None] + be[None, :]).ravel()
+    P = ((al[:, None, None] * A + be[None, :, None] * B).reshape(-1, 3)) / z[:, None]
+    s = (ca[:, None] * cb[None, :]).ravel() * (math.pi / z) ** 1.5
+    return P, z, s, P - A, P - B
+
+
+def record_bound(zeta: np.ndarray, L: int) -> np.ndarray:
+    return sum(np.sqrt(hermite_bound_sq(zeta, t)) for t in hermite_order(L))
+
+
+def _c(x):
+    return C.c_void_p(native.ptr(x))
+
+
+def _marshal(args):
+    """ctypes values for `args`: a device tensor becomes its pointer, a
+    ctypes value passes through. The caller keeps `args` (and so every
+    tensor) alive until the launch is enqueued: a tensor freed earlier
+    would hand its block to the next allocation on the stream, whose copy
+    lands before the kernel reads it."""
+    out = []
+    for a in args:
+        if hasattr(a, "data_ptr"):
+            out.append(C.c_void_p(a.data_ptr()))
+        else:
+            out.append(a)
+    return out
+
+
+def _call(name: str, *args) -> None:
+    """Launch entry `name` on the current stream. Device arguments are passed
+    as tensors and stay referenced by `args` through the call."""
+    fn = getattr(native.lib(), name)
+    fn.restype = C.c_int
+    native.check(fn(C.c_void_p(native.stream_handle()), *_marshal(args)), name)
+
+
+def _canon(basis: Basis, i: int, j: int):
+    li, lj = int(basis.shell_l[i]), int(basis.shell_l[j])
+    return (i, j) if (li, i) <= (lj, j) else (j, i)
+
+
+def _pair_classes(basis: Basis):
+    out: dict = {}
+    for i in range(basis.n_sh):
+        for j in range(i, basis.n_sh):
+            a, b = _canon(basis, i, j)
+            out.setdefault((int(basis.shell_l[a]), int(basis.shell_l[b])), []).append((a, b))
+    return dict(sorted(out.items()))
+
+
+# ---------------------------------------------------------------------------
+
+
+def schwarz_matrix(basis: Basis, threads: int = 64):
+    """Q[m, n] = sqrt(max((mn|mn), 0)) on the device (torch float64)."""
+    import torch
+    dev = native.device()
+    Q = torch.zeros((basis.n_ao, basis.n_ao), dtype=torch.float64, device=dev)
+    for (la, lb), prs in _pair_classes(basis).items():
+        recs = [_records(basis, a, b) for a, b in prs]
+        n_rec = np.cumsum([0] + [len(r[1]) for r in recs])
+        cat = lambda k: native.to_dev(np.concatenate([r[k] for r in recs]))  # noqa: E731
+        P, z, s, XA, XB = cat(0), cat(1), cat(2), cat(3), cat(4)
+        rec0 = native.to_dev(n_rec[:-1].astype(np.int32))
+        rec1 = native.to_dev(n_rec[1:].astype(np.int32))
+        oa = native.to_dev(np.array([basis.shell_ao[a] for a, _ in prs], dtype=np.int32))
+        ob = native.to_dev(np.array([basis.shell_ao[b] for _, b in prs], dtype=np.int32))
+        _call(f"fj_schwarz_{la}{lb}", C.c_int(threads), C.c_int(len(prs)), C.c_int(int(n_rec[-1])),
+              C.c_int(basis.n_ao), _c(z), _c(P), _c(XA), _c(XB), _c(s), _c(rec0), _c(rec1),
+              _c(oa), _c(ob), _c(Q))
+    return Q
+
+
+# ---------------------------------------------------------------------------
+
+
+class CoulombChain:
+    """The J chain for one basis; `run(D)` returns J on the device."""
+
+    def __init__(self, basis: Basis, tau: float = 0.0):
+        self.basis = basis
+        self.tau = float(tau)
+        self.classes = _pair_classes(basis)
+
+    def run(self, D: np.ndarray):
+        import torch
+        b = self.basis
+        dev = native.device()
+        J = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
+        # the ket side: DQ per class, then Dmax read back and the sort
+        kets = {}
+        for (lc, ld), prs in self.classes.items():
+            recs = [_records(b, c, d) for c, d in prs]
+            pair = np.concatenate([np.full(len(r[1]), k) for k, r in enumerate(recs)]).astype(np.int32)
+            w = np.concatenate([np.full(len(r[1]), 1.0 if c == d else 2.0)
+                                for (c, d), r in zip(prs, recs)])
+            ncd = n_cart(lc) * n_cart(ld)
+            Dcd = np.array([D[b.shell_ao[c]:b.shell_ao[c] + n_cart(lc),
+                              b.shell_ao[d]:b.shell_ao[d] + n_cart(ld)].ravel() for c, d in prs])
+            P = np.concatenate([r[0] for r in recs])
+            z = np.concatenate([r[1] for r in recs])
+            s = np.concatenate([r[2] for r in recs])
+            XA = np.concatenate([r[3] for r in recs])
+            XB = np.concatenate([r[4] for r in recs])
+            nq = len(hermite_order(lc + ld))
+            Dq = torch.zeros((len(z), nq), dtype=torch.float64, device=dev)
+            Dm = torch.zeros(len(z), dtype=torch.float64, device=dev)
+            _call(f"fj_dq_{lc}_{ld}", C.c_int(64), C.c_int(len(z)), C.c_int(len(prs)),
+                  native.to_dev(z), native.to_dev(XA), native.to_dev(XB), native.to_dev(s),
+                  native.to_dev(w), native.to_dev(pair), native.to_dev(Dcd.reshape(len(prs), ncd)),
+                  Dq, Dm)
+            torch.cuda.synchronize()
+            Iq = record_bound(z, lc + ld)
+            key = Iq * Dm.cpu().numpy()
+            order = np.argsort(-key, kind="stable")
+            kets[(lc, ld)] = dict(Q=native.to_dev(P[order]), eta=native.to_dev(z[order]),
+                                  Iq=native.to_dev(Iq[order]), Dmax=Dm[order].contiguous(),
+                                  D=Dq[order].contiguous(), n=len(z))
+        # the bra side: [p|q] per ket class, then JAB per buffer
+        for (la, lb), prs in self.classes.items():
+            recs = [_records(b, a, c) for a, c in prs]
+            P = np.concatenate([r[0] for r in recs])
+            z = np.concatenate([r[1] for r in recs])
+            s = np.concatenate([r[2] for r in recs])
+            XA = np.concatenate([r[3] for r in recs])
+            XB = np.concatenate([r[4] for r in recs])
+            oa = np.concatenate([np.full(len(r[1]), b.shell_ao[a]) for (a, _), r in zip(prs, recs)])
+            ob = np.concatenate([np.full(len(r[1]), b.shell_ao[c]) for (_, c), r in zip(prs, recs)])
+            Ip = record_bound(z, la + lb)
+            dP, dz, dIp = native.to_dev(P), native.to_dev(z), native.to_dev(Ip)
+            rows = native.to_dev(np.arange(len(z), dtype=np.int32))
+            da, db = native.to_dev(oa.astype(np.int32)), native.to_dev(ob.astype(np.int32))
+            ds, dXA, dXB = native.to_dev(s), native.to_dev(XA), native.to_dev(XB)
+            lp = la + lb
+            npp = len(hermite_order(lp))
+            for (lc, ld), k in kets.items():
+                lq = lc + ld
+                Jp = torch.zeros((len(z), npp), dtype=torch.float64, device=dev)
+                _call(f"fj_pq_L{lp + lq}_p{lp}_q{lq}", C.c_int(256), C.c_int(len(z)), C.c_int(k["n"]),
+                      C.c_double(self.tau), _c(dP), _c(dz), _c(dIp), _c(k["Q"]), _c(k["eta"]),
+                      _c(k["Iq"]), _c(k["Dmax"]), _c(k["D"]), _c(Jp))
+                _call(f"fj_jab_{la}{lb}", C.c_int(64), C.c_int(len(z)), C.c_int(b.n_ao),
+                      C.c_int(len(z)), _c(dz), _c(dXA), _c(dXB), _c(ds), _c(da), _c(db),
+                      _c(rows), _c(Jp), _c(J))
+        return J
+
+
+# ---------------------------------------------------------------------------
+
+
+class ExchangeBuild:
+    """K_ab = sum_cd D_cd (ac|bd) through the K kernel; `run(D)` returns K
+    on the device."""
+
+    def __init__(self, basis: Basis, tau: float = 0.0):
+        self.basis = basis
+        self.tau = float(tau)
+        b = basis
+        self.by_l = {}
+        for s in range(b.n_sh):
+            self.by_l.setdefault(int(b.shell_l[s]), []).append(s)
+        self.lmax = max(self.by_l)
+
+    def _side(self, l1: int, l2: int, fixed: int | None = None):
+        """Records of every ordered pair (s1, s2), l(s1) = l1, l(s2) = l2
+        (s1 == fixed if given), with each record's s1 index, s2 index and
+        s2's position among the shells of momentum l2."""
+        b = self.basis
+        firsts = [fixed] if fixed is not None else self.by_l.get(l1, [])
+        recs, s1, s2, pos2 = [], [], [], []
+        for u in firsts:
+            for j, v in enumerate(self.by_l.get(l2, [])):
+                r = _records(b, u, v)
+                recs.append(r)
+                n = len(r[1])
+                s1.append(np.full(n, u))
+                s2.append(np.full(n, v))
+                pos2.append(np.full(n, j))
+        if not recs:
+            return None
+        cat = lambda k: np.concatenate([r[k] for r in recs])  # noqa: E731
+        return dict(P=cat(0), z=cat(1), s=cat(2), X1=cat(3), X2=cat(4),
+                    s1=np.concatenate(s1), s2=np.concatenate(s2), pos2=np.concatenate(pos2))
+
+    def run(self, D: np.ndarray):
+        import torch
+        b = self.basis
+        dev = native.device()
+        K = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
+        maxD = float(np.abs(D).max())
+        L = sorted(self.by_l)
+        for la in L:
+            for lc in L:
+                bra = self._side(la, lc)
+                if bra is None:
+                    continue
+                Ip = record_bound(bra["z"], la + lc)
+                dbra = {k: native.to_dev(v) for k, v in
+                        dict(P=bra["P"], z=bra["z"], s=bra["s"], XA=bra["X1"], XC=bra["X2"],
+                             Ip=Ip).items()}
+                oa = native.to_dev(b.shell_ao[bra["s1"]].astype(np.int32))
+                for ld in L:
+                    nc_, nd_ = n_cart(lc), n_cart(ld)
+                    cs, ds = self.by_l[lc], self.by_l[ld]
+                    # Dcd rows: (c position, d position) blocks, (c, d) components
+                    Dcd = np.array([D[b.shell_ao[c]:b.shell_ao[c] + nc_,
+                                      b.shell_ao[d]:b.shell_ao[d] + nd_].ravel()
+                                    for c in cs for d in ds])
+                    dc = native.to_dev((np.searchsorted(cs, bra["s2"]) * len(ds)).astype(np.int32))
+                    dDcd = native.to_dev(Dcd)
+                    for lb in L:
+                        for bs in self.by_l[lb]:
+                            ket = self._side(lb, ld, fixed=bs)
+                            if ket is None:
+                                continue
+                            Iq = record_bound(ket["z"], lb + ld)
+                            order = np.argsort(-Iq, kind="stable")
+                            dk = {k: native.to_dev(v[order]) for k, v in
+                                  dict(Q=ket["P"], eta=ket["z"], s=ket["s"], XB=ket["X1"],
+                                       XD=ket["X2"], Iq=Iq).items()}
+                            dd = native.to_dev(ket["pos2"][order].astype(np.int32))
+                            nab = n_cart(la) * n_cart(lb)
+                            Kr = torch.zeros((len(bra["z"]), nab), dtype=torch.float64, device=dev)
+                            _call(f"fj_k_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(len(bra["z"])),
+                                  C.c_int(len(ket["z"])), C.c_int(Dcd.shape[0]), C.c_double(self.tau),
+                                  C.c_double(maxD), _c(dbra["P"]), _c(dbra["z"]), _c(dbra["XA"]),
+                                  _c(dbra["XC"]), _c(dbra["s"]), _c(dbra["Ip"]), _c(dc),
+                                  _c(dk["Q"]), _c(dk["eta"]), _c(dk["XB"]), _c(dk["XD"]),
+                                  _c(dk["s"]), _c(dk["Iq"]), _c(dd), _c(dDcd), _c(Kr))
+                            _call("fj_k_scatter", C.c_int(len(bra["z"])), C.c_int(n_cart(la)),
+                                  C.c_int(n_cart(lb)), C.c_int(b.n_ao), _c(oa),
+                                  C.c_int(int(b.shell_ao[bs])), _c(Kr), _c(K))
+        return K
+
+
+def fock_jk(basis: Basis, D: np.ndarray, tau: float = 0.0):
+    """G = J - K/2 through the MMD families (device tensors J, K, G)."""
+    J = CoulombChain(basis, tau).run(D)
+    K = ExchangeBuild(basis, tau).run(D)
+    return J, K, J - 0.5 * K

@@ -271,6 +271,568 @@ int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra
     double tau);
 /* END F CLASSES */
 
+/* The MMD families, one entry per class, each with the upstream kernel's
+ * parameter list after (stream, threads):
+ *   fj_schwarz_<la><lb>  Q_mn = sqrt(max((mn|mn), 0)), one thread per shell
+ *                        pair (upstream eri/kernel_schwarz.py:76-101)
+ *   fj_dq_<lc>_<ld>      D_q and Dmax per ket record
+ *                        (upstream eri/j_coulomb/kernel_dq.py:10-60)
+ *   fj_pq_L<L>_p<lp>_q<lq>  J_p = sum_k sum_q [p|q] D_q, kets sorted by
+ *                        Iq Dmax, one wavefront per bra record
+ *                        (upstream mmd/kernel.py:16-60,476-486)
+ *   fj_jab_<la><lb>      J_ab = sum_p (s E_p) J_p scattered into J, atomics
+ *                        (upstream eri/j_coulomb/kernel_jab.py:10-60)
+ *   fj_k_<la><lc><lb><ld>  K[bra, a, b] = sum_ket D_cd (ac|bd), kets sorted
+ *                        by Iq (upstream eri/k_exchange/kernel_k.py:1-110) */
+/* K[oa[r] + i][ob + j] += Kr[r][i * nb + j] for n_rec records: the K
+ * kernel's per-record blocks summed into K (the host-side sum and scatter
+ * of upstream eri/k_exchange/launch_k.py:401-479, on the device). */
+int fj_k_scatter(fj_stream_t stream, int n_rec, int na, int nb, int n_ao, const int* oa, int ob,
+                 const double* Kr, double* K);
+
+/* BEGIN MMD CLASSES (written by felizjueves_amd/gen_mmd.py) */
+int fj_schwarz_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_0_0(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_00(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_01(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_0_1(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_01(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_02(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_0_2(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_02(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_11(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_1_1(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_11(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_12(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_1_2(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_12(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_22(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_2_2(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_22(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_pq_L0_p0_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L1_p0_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L2_p0_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L3_p0_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L4_p0_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L1_p1_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L2_p1_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L3_p1_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L4_p1_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L5_p1_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L2_p2_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L3_p2_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L4_p2_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L5_p2_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L6_p2_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L3_p3_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L4_p3_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L5_p3_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L6_p3_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L7_p3_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L4_p4_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L5_p4_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L6_p4_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L7_p4_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L8_p4_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_k_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+/* END MMD CLASSES */
+
 #ifdef __cplusplus
 }
 #endif

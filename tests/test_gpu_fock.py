@@ -126,3 +126,43 @@ def test_builder_small_cluster_vs_oracle():
     fb, dG, dE, G = _builder_vs_oracle(s, False)
     _print("c2-10 fp64 max|dG|", dG, "|dE|", dE, "quartets", fb.n_quartets)
     assert dG <= 1e-10 and dE <= 1e-9
+
+
+def _two_atom_spdf():
+    """Two atoms 1.9 bohr apart, shells s, s, p, d, f each (seeded): every
+    one of the 55 classes through (ff|ff) occurs."""
+    from felizjueves_amd.basis import Basis
+    from felizjueves_amd.systems import atom_shells, closed_shell_density
+    rng = np.random.default_rng(21)
+    shells = atom_shells(rng, (0.0, 0.0, 0.0), (0, 0, 1, 2, 3)) + \
+        atom_shells(rng, (0.9, -1.1, 1.2), (0, 0, 1, 2, 3))
+    basis = Basis(shells)
+    return basis, closed_shell_density(rng, basis.n_ao, 6)
+
+
+@pytest.mark.skip(reason="f classes are not built: their staged kernels asked for 105 KB of "
+                  "scratch per lane and the runtime refused it (HSA out of resources); "
+                  "kept off the GPU until the boundary no longer lives in private scratch")
+def test_every_class_through_f_launches():
+    basis, D = _two_atom_spdf()
+    worst = _launch_errors(basis, D, 0.0, 1)
+    _print("per-class max |dF| (f):", {k: f"{v:.1e}" for k, v in sorted(worst.items()) if 3 in k})
+    assert len(worst) == 55
+    assert max(worst.values()) < 1e-11 * max(1.0, np.abs(D).max())
+
+
+@pytest.mark.skip(reason="needs the f classes (see test_every_class_through_f_launches)")
+def test_builder_co_ccpvtz_vs_libcint():
+    """CO/cc-pVTZ (f on both atoms): the native builder against libcint's
+    vj - vk/2 and the oracle."""
+    import torch
+    fx = load_fixture("co_ccpvtz")
+    fb = FockBuilder(fx.basis, fx.density, tau=1e-12)
+    G = fb.build(native.to_dev(fx.density))
+    torch.cuda.synchronize()
+    G = G.cpu().numpy()
+    ref = fx.arrays["vj"] - 0.5 * fx.arrays["vk"]
+    Go = oracle_build(fb, fx.density)
+    _print("co_ccpvtz gpu-libcint", np.abs(G - ref).max(), "gpu-oracle", np.abs(G - Go).max())
+    assert np.abs(G - Go).max() < 1e-10
+    assert np.abs(G - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
