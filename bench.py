@@ -3,8 +3,9 @@
 Workload (default): the 40-atom random cluster (s,s,p,d per atom, n_ao 440,
 tau 1e-12, seed 2), one step = one closed two-electron Fock build
 G = J - K/2 from device-resident pair tables and density.  Timed: the plain
-FP64 build and the mixed build (FP32 tier for tiles whose largest bound
-G_ab G_cd dmax is below 1e-6), K builds each after W warm-up builds.
+FP64 build and the mixed build (FP32 tier for the kets whose estimated FP32
+error -- the six-block bound times the HRR amplification, fock.py -- is
+below 1e-11 per tile), K builds each after W warm-up builds.
 
 Prints ONE JSON line.  `value` is the mixed build's Fock builds/s; the FP64
 build's figure, the quartet rates, |dE| and max|dG| of mixed vs FP64, the
@@ -205,7 +206,7 @@ def main() -> None:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+f32-tier", "data": "synthetic",
         "config": {"workload": s.name, "n_atoms": a.atoms, "n_ao": s.basis.n_ao, "tau": 1e-12,
-                   "fp32_threshold": 1e-6, "threads_per_workgroup": a.threads},
+                   "fp32_error_per_tile": fbmx.fp32_error, "threads_per_workgroup": a.threads},
         "fp64_builds_per_s": a.steps / t64, "fp64_ms_per_build": 1e3 * t64 / a.steps,
         "quartets_per_s_mixed": fbmx.n_quartets * a.steps / tmx,
         "quartets_per_s_fp64": fb64.n_quartets * a.steps / t64,

@@ -1,0 +1,197 @@
+// fj_mlp.hip -- the precision selector's network: x[F] -> 64 -> 64 -> 1,
+// ReLU, BF16 weights, FP32 activations and accumulation.
+//
+//   forward:   h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), y = w3 . h2 + b3
+//   backward:  the loss L = mean_i (y_i - t_i)^2, gradients of the FP32
+//              master weights through the BF16 weights the forward used
+//   adam:      one Adam step on the FP32 master weights, then their BF16
+//              copies (round to nearest even)
+//
+// One lane per sample; the weights are staged in LDS per workgroup. The
+// weight gradients are reduced in LDS per workgroup (LDS atomics) and then
+// added to global memory once per workgroup.
+#include "fj_device.h"
+#include <hip/hip_bf16.h>
+
+#define FJ_MLP_H 64
+#define FJ_MLP_FMAX 32
+
+// parameter block layout (floats): W1[H][F] b1[H] W2[H][H] b2[H] w3[H] b3
+__host__ __device__ static inline int fj_mlp_nparam(int F) {
+    return FJ_MLP_H * F + FJ_MLP_H + FJ_MLP_H * FJ_MLP_H + FJ_MLP_H + FJ_MLP_H + 1;
+}
+
+__device__ __forceinline__ float fj_bf(const uint16_t v) {
+    return __uint_as_float(((uint32_t)v) << 16);
+}
+
+__device__ __forceinline__ uint16_t fj_to_bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+
+// weights: bf16 copies of W1, W2, w3 (same layout, biases kept FP32 in P)
+struct FjMlpW {
+    const uint16_t* w;  // bf16 [nparam] (bias slots unused)
+    const float* p;     // fp32 master [nparam] (biases read from here)
+};
+
+template <bool KEEP>
+__device__ __forceinline__ float fj_mlp_eval(int F, const float* __restrict__ x, const float* sW,
+                                             float* h1, float* h2) {
+    const float* W1 = sW;
+    const float* b1 = W1 + FJ_MLP_H * F;
+    const float* W2 = b1 + FJ_MLP_H;
+    const float* b2 = W2 + FJ_MLP_H * FJ_MLP_H;
+    const float* w3 = b2 + FJ_MLP_H;
+    const float b3 = w3[FJ_MLP_H];
+    for (int j = 0; j < FJ_MLP_H; ++j) {
+        float a = b1[j];
+        for (int k = 0; k < F; ++k) a = fmaf(W1[j * F + k], x[k], a);
+        h1[j] = fmaxf(a, 0.0f);
+    }
+    float y = b3;
+    for (int j = 0; j < FJ_MLP_H; ++j) {
+        float a = b2[j];
+#pragma unroll 8
+        for (int k = 0; k < FJ_MLP_H; ++k) a = fmaf(W2[j * FJ_MLP_H + k], h1[k], a);
+        a = fmaxf(a, 0.0f);
+        if (KEEP) h2[j] = a;
+        y = fmaf(w3[j], a, y);
+    }
+    return y;
+}
+
+// Stage the effective weights (bf16 values as floats, fp32 biases) in LDS.
+__device__ void fj_mlp_stage(int F, FjMlpW W, float* sW) {
+    const int n = fj_mlp_nparam(F);
+    const int b1 = FJ_MLP_H * F, b2 = b1 + FJ_MLP_H + FJ_MLP_H * FJ_MLP_H, w3 = b2 + FJ_MLP_H;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const bool bias = (i >= b1 && i < b1 + FJ_MLP_H) || (i >= b2 && i < w3) || i == n - 1;
+        sW[i] = bias ? W.p[i] : fj_bf(W.w[i]);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) fj_mlp_forward_k(int n, int F, const float* __restrict__ X,
+                                                        FjMlpW W, float* __restrict__ Y) {
+    extern __shared__ float sW[];
+    fj_mlp_stage(F, W, sW);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[FJ_MLP_FMAX], h1[FJ_MLP_H];
+    for (int k = 0; k < F; ++k) x[k] = X[(size_t)i * F + k];
+    Y[i] = fj_mlp_eval<false>(F, x, sW, h1, nullptr);
+}
+
+__global__ void __launch_bounds__(256) fj_mlp_backward_k(int n, int F, const float* __restrict__ X,
+                                                         const float* __restrict__ T, FjMlpW W,
+                                                         float* __restrict__ G, float* __restrict__ loss) {
+    extern __shared__ float smem[];
+    const int np = fj_mlp_nparam(F);
+    float* sW = smem;
+    float* sG = smem + np;
+    for (int k = threadIdx.x; k < np; k += blockDim.x) sG[k] = 0.0f;
+    fj_mlp_stage(F, W, sW);  // ends with a barrier, so sG is zeroed too
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (i < n) {
+        float x[FJ_MLP_FMAX], h1[FJ_MLP_H], h2[FJ_MLP_H];
+        for (int k = 0; k < F; ++k) x[k] = X[(size_t)i * F + k];
+        const float y = fj_mlp_eval<true>(F, x, sW, h1, h2);
+        const float r = y - T[i];
+        l = r * r / (float)n;
+        const float dy = 2.0f * r / (float)n;
+        const float* W2 = sW + FJ_MLP_H * F + FJ_MLP_H;
+        const float* w3 = W2 + FJ_MLP_H * FJ_MLP_H + FJ_MLP_H;
+        const int o_b1 = FJ_MLP_H * F, o_W2 = o_b1 + FJ_MLP_H, o_b2 = o_W2 + FJ_MLP_H * FJ_MLP_H;
+        const int o_w3 = o_b2 + FJ_MLP_H, o_b3 = o_w3 + FJ_MLP_H;
+        atomicAdd(&sG[o_b3], dy);
+        float d1[FJ_MLP_H];
+        for (int k = 0; k < FJ_MLP_H; ++k) d1[k] = 0.0f;
+        for (int j = 0; j < FJ_MLP_H; ++j) {
+            atomicAdd(&sG[o_w3 + j], dy * h2[j]);
+            const float d2 = h2[j] > 0.0f ? dy * w3[j] : 0.0f;
+            if (d2 != 0.0f) {
+                atomicAdd(&sG[o_b2 + j], d2);
+                for (int k = 0; k < FJ_MLP_H; ++k) {
+                    atomicAdd(&sG[o_W2 + j * FJ_MLP_H + k], d2 * h1[k]);
+                    d1[k] = fmaf(W2[j * FJ_MLP_H + k], d2, d1[k]);
+                }
+            }
+        }
+        for (int k = 0; k < FJ_MLP_H; ++k) {
+            const float g = h1[k] > 0.0f ? d1[k] : 0.0f;
+            if (g != 0.0f) {
+                atomicAdd(&sG[o_b1 + k], g);
+                for (int f = 0; f < F; ++f) atomicAdd(&sG[k * F + f], g * x[f]);
+            }
+        }
+    }
+    // the workgroup's loss and gradient, once per workgroup
+    l = fj_wave_sum(l);
+    if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(loss, l);
+    __syncthreads();
+    for (int k = threadIdx.x; k < np; k += blockDim.x)
+        if (sG[k] != 0.0f) atomicAdd(&G[k], sG[k]);
+}
+
+__global__ void fj_mlp_adam_k(int np, float* __restrict__ P, const float* __restrict__ G,
+                              float* __restrict__ M, float* __restrict__ V, uint16_t* __restrict__ Wb,
+                              float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float g = G[i];
+    const float m = b1 * M[i] + (1.0f - b1) * g;
+    const float v = b2 * V[i] + (1.0f - b2) * g * g;
+    M[i] = m;
+    V[i] = v;
+    const float p = P[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    P[i] = p;
+    Wb[i] = fj_to_bf(p);
+}
+
+__global__ void fj_mlp_round_k(int np, const float* __restrict__ P, uint16_t* __restrict__ Wb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) Wb[i] = fj_to_bf(P[i]);
+}
+
+extern "C" int fj_mlp_nparams(int F) { return fj_mlp_nparam(F); }
+
+extern "C" int fj_mlp_forward(hipStream_t s, int n, int F, const float* X, const uint16_t* Wb,
+                              const float* P, float* Y) {
+    if (n < 0 || F < 1 || F > FJ_MLP_FMAX) return FJ_ERR_ARG;
+    if (n == 0) return 0;
+    FjMlpW W{Wb, P};
+    const size_t lds = sizeof(float) * fj_mlp_nparam(F);
+    hipLaunchKernelGGL(fj_mlp_forward_k, dim3((n + 255) / 256), dim3(256), lds, s, n, F, X, W, Y);
+    return (int)hipGetLastError();
+}
+
+extern "C" int fj_mlp_backward(hipStream_t s, int n, int F, const float* X, const float* T,
+                               const uint16_t* Wb, const float* P, float* G, float* loss) {
+    if (n < 0 || F < 1 || F > FJ_MLP_FMAX) return FJ_ERR_ARG;
+    if (n == 0) return 0;
+    FjMlpW W{Wb, P};
+    const size_t lds = 2 * sizeof(float) * fj_mlp_nparam(F);
+    hipLaunchKernelGGL(fj_mlp_backward_k, dim3((n + 255) / 256), dim3(256), lds, s, n, F, X, T, W, G,
+                       loss);
+    return (int)hipGetLastError();
+}
+
+extern "C" int fj_mlp_adam(hipStream_t s, int np, float* P, const float* G, float* M, float* V,
+                           uint16_t* Wb, float lr, float b1, float b2, float eps, int t) {
+    if (np <= 0 || t < 1) return FJ_ERR_ARG;
+    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    hipLaunchKernelGGL(fj_mlp_adam_k, dim3((np + 255) / 256), dim3(256), 0, s, np, P, G, M, V, Wb, lr, b1,
+                       b2, eps, c1, c2);
+    return (int)hipGetLastError();
+}
+
+extern "C" int fj_mlp_round(hipStream_t s, int np, const float* P, uint16_t* Wb) {
+    if (np <= 0) return FJ_ERR_ARG;
+    hipLaunchKernelGGL(fj_mlp_round_k, dim3((np + 255) / 256), dim3(256), 0, s, np, P, Wb);
+    return (int)hipGetLastError();
+}

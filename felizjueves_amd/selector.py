@@ -10,45 +10,19 @@ sharing D and Dsh), and compares. The FP64 result is the judge.
 
 from __future__ import annotations
 
-import math
-
 import numpy as np
 
 from . import native
+from .features import N_FEATURES, tile_features
 from .fock import FP32, FP64, FockBuilder
 
-N_FEATURES = 14
-
-
 def item_features(fb: FockBuilder, L, w) -> np.ndarray:
-    """Features of work item w = (bra, first, sys|chunk, end) of launch L:
-    [la, lb, lc, ld, log10 min/max bra exponent sum, log10 min/max ket
-    exponent sum, |AB|, max |CD|, min |PQ| (pair centres), log10 G_ab,
-    log10 max G_cd, log10 max |D| over the tile's six blocks]."""
+    """The tile features (features.tile_features) of work item
+    w = (bra, first, sys|chunk, end) of launch L."""
     b = int(w[0])
     ks = L.ket_list[w[1]:w[3]]
-    bra, ket = L.bra, L.ket
-    pb = bra.pairs[b]
-    zb = pb.rec[:, 6]
-    zk = np.concatenate([ket.pairs[k].rec[:, 6] for k in ks])
-    basis = fb.basis
-    A, B = basis.shell_centre[pb.a], basis.shell_centre[pb.b]
-    ab = float(np.linalg.norm(A - B))
-    cd = max(float(np.linalg.norm(basis.shell_centre[ket.pairs[k].a] -
-                                  basis.shell_centre[ket.pairs[k].b])) for k in ks)
-    P = (pb.rec[:, :3] * zb[:, None]).sum(0) / zb.sum()
-    pq = min(float(np.linalg.norm(P - (ket.pairs[k].rec[:, :3] * ket.pairs[k].rec[:, 6:7]).sum(0)
-                                  / ket.pairs[k].rec[:, 6].sum())) for k in ks)
-    Gk = max(ket.G[k] for k in ks)
-    Dsh = fb.Dsh_plan
-    sa, sb = pb.a, pb.b
-    dm = 0.0
-    for k in ks:
-        sc, sd = ket.pairs[k].a, ket.pairs[k].b
-        dm = max(dm, Dsh[sa, sb], Dsh[sc, sd], Dsh[sa, sc], Dsh[sa, sd], Dsh[sb, sc], Dsh[sb, sd])
-    lg = lambda x: math.log10(max(x, 1e-300))  # noqa: E731
-    return np.array([*L.cls, lg(zb.min()), lg(zb.max()), lg(zk.min()), lg(zk.max()), ab, cd, pq,
-                     lg(bra.G[b]), lg(Gk), lg(dm)], dtype=np.float64)
+    _, dmax = fb._bounds_dmax(L.bra, L.ket, b)
+    return tile_features(L.cls, L.bra.feat, L.ket.feat, b, ks, np.array([0]), dmax[ks])[0]
 
 
 def probe_errors(fb: FockBuilder, D: np.ndarray, max_items: int = 4096, seed: int = 0):
@@ -96,3 +70,112 @@ def probe_errors(fb: FockBuilder, D: np.ndarray, max_items: int = 4096, seed: in
         errs.append(err)
         des.append(de)
     return np.array(feats), np.concatenate(errs), np.concatenate(des)
+
+
+def probe_builder(basis, D, tau: float = 1e-12, threads: int = 64) -> FockBuilder:
+    """An all-FP64 builder whose work items are whole tiles (no primitive
+    loop chunks), so each item's error is that of one tile."""
+    return FockBuilder(basis, D, tau=tau, threads=threads, lane_flops=1e30)
+
+
+def training_systems(n: int, seed0: int = 1000):
+    """Seeded training molecules: three-atom systems of the small recipe and
+    random clusters of 4-14 atoms of the working-size recipe (seeds from
+    seed0, disjoint from the evaluation systems' seeds)."""
+    from .systems import config1, config2
+    out = []
+    for i in range(n):
+        s = seed0 + i
+        if i % 2 == 0:
+            out.append(config1(seed=s))
+        else:
+            n_atoms = 4 + (i * 7919) % 11
+            out.append(config2(seed=s, n_atoms=n_atoms))
+    return out
+
+
+LOG_FLOOR = -30.0
+
+
+def log_error(err: np.ndarray) -> np.ndarray:
+    return np.log10(np.maximum(err, 10.0 ** LOG_FLOOR))
+
+
+class LearnedSelector:
+    """FP32 for a tile iff the network's predicted log10 error plus
+    `margin` is below log10(budget) and the tile's features lie inside the
+    training data's range (else the tile keeps the bound rule's FP64).
+
+    margin: the `quantile` of (measured - predicted) on held-out tiles, so
+    a tile the network sends to FP32 exceeds the budget with about that
+    probability."""
+
+    def __init__(self, net, budget: float, margin: float, lo: np.ndarray, hi: np.ndarray,
+                 classes: set):
+        self.net = net
+        self.budget = float(budget)
+        self.margin = float(margin)
+        self.lo, self.hi = np.asarray(lo), np.asarray(hi)
+        self.classes = set(classes)
+
+    def inside(self, X: np.ndarray) -> np.ndarray:
+        ok = np.all((X[:, 4:] >= self.lo[4:]) & (X[:, 4:] <= self.hi[4:]), axis=1)
+        cls = [tuple(int(v) for v in r) for r in X[:, :4]]
+        return ok & np.array([c in self.classes for c in cls], dtype=bool)
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        return self.net.predict(X)
+
+    def __call__(self, X: np.ndarray) -> np.ndarray:
+        if not len(X):
+            return np.zeros(0, dtype=bool)
+        return (self.predict(X) + self.margin < np.log10(self.budget)) & self.inside(X)
+
+    def save(self, path) -> None:
+        from pathlib import Path
+        path = Path(path)
+        self.net.save(path.with_suffix(".net.npz"))
+        np.savez(path.with_suffix(".sel.npz"), budget=self.budget, margin=self.margin, lo=self.lo,
+                 hi=self.hi, classes=np.array(sorted(self.classes), dtype=np.int32).reshape(-1, 4))
+
+    @classmethod
+    def load(cls, path) -> "LearnedSelector":
+        from pathlib import Path
+        from .mlp import MLP
+        path = Path(path)
+        net = MLP.load(path.with_suffix(".net.npz"))
+        with np.load(path.with_suffix(".sel.npz"), allow_pickle=False) as z:
+            return cls(net, float(z["budget"]), float(z["margin"]), z["lo"], z["hi"],
+                       {tuple(int(v) for v in r) for r in z["classes"]})
+
+
+def collect(systems, max_items: int = 20000, seed: int = 0, log=None):
+    """Probe every system's tiles (up to max_items each): (X, log10 err)."""
+    Xs, ys = [], []
+    for i, s in enumerate(systems):
+        fb = probe_builder(s.basis, s.D)
+        X, err, _ = probe_errors(fb, s.D, max_items=max_items, seed=seed + i)
+        Xs.append(X)
+        ys.append(log_error(err))
+        if log:
+            log(i, s, len(X))
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def train_selector(X: np.ndarray, y: np.ndarray, budget: float = 1e-11, quantile: float = 0.9999,
+                   epochs: int = 40, seed: int = 0, log=None) -> LearnedSelector:
+    """Train the network on the device (90/10 split), set the margin from
+    the held-out residuals, return the selector."""
+    from .mlp import MLP
+    rng = np.random.default_rng(seed)
+    idx = rng.permutation(len(X))
+    n_val = max(1, len(X) // 10)
+    va, tr = idx[:n_val], idx[n_val:]
+    net = MLP(N_FEATURES, seed=seed)
+    net.fit(X[tr], y[tr], epochs=epochs, batch=4096, lr=2e-3, seed=seed, log=log)
+    r = y[va] - net.predict(X[va])
+    margin = float(np.quantile(r, quantile))
+    classes = {tuple(int(v) for v in row) for row in np.unique(X[:, :4], axis=0)}
+    sel = LearnedSelector(net, budget, margin, X.min(0), X.max(0), classes)
+    sel.val_rmse = float(np.sqrt(np.mean(r * r)))
+    return sel

@@ -290,6 +290,25 @@ int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_k_scatter(fj_stream_t stream, int n_rec, int na, int nb, int n_ao, const int* oa, int ob,
                  const double* Kr, double* K);
 
+/* The precision selector's network (no upstream counterpart: the learned
+ * FP32/FP64 tier choice per work item).  x[F] (F <= 32, float32, already
+ * normalised) -> 64 -> 64 -> 1 with ReLU; parameters in one float32 block
+ * P[fj_mlp_nparams(F)] = W1[64][F] b1[64] W2[64][64] b2[64] w3[64] b3 and its
+ * BF16 copy Wb (uint16 bit patterns; the forward pass uses the BF16 weights
+ * and the FP32 biases, accumulating in FP32). */
+int fj_mlp_nparams(int F);
+int fj_mlp_forward(fj_stream_t stream, int n, int F, const float* X, const unsigned short* Wb,
+                   const float* P, float* Y);
+/* G += dL/dP and *loss += L for L = mean_i (y_i - t_i)^2 (G, loss zeroed by
+ * the caller). */
+int fj_mlp_backward(fj_stream_t stream, int n, int F, const float* X, const float* T,
+                    const unsigned short* Wb, const float* P, float* G, float* loss);
+/* One Adam step (step t >= 1) on P from G, moments M, V; refreshes Wb. */
+int fj_mlp_adam(fj_stream_t stream, int np, float* P, const float* G, float* M, float* V,
+                unsigned short* Wb, float lr, float beta1, float beta2, float eps, int t);
+/* Wb <- BF16(P), round to nearest even. */
+int fj_mlp_round(fj_stream_t stream, int np, const float* P, unsigned short* Wb);
+
 /* BEGIN MMD CLASSES (written by felizjueves_amd/gen_mmd.py) */
 int fj_schwarz_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
     const double* zeta_ab, const double* P, const double* XPA, const double* XPB,

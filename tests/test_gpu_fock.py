@@ -166,3 +166,23 @@ def test_builder_co_ccpvtz_vs_libcint():
     _print("co_ccpvtz gpu-libcint", np.abs(G - ref).max(), "gpu-oracle", np.abs(G - Go).max())
     assert np.abs(G - Go).max() < 1e-10
     assert np.abs(G - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+
+
+def test_graph_replays_match_eager():
+    """A captured build replayed several times, with eager builds and
+    synchronisations between the replays, equals the eager build every
+    time (no memset node is recorded; F is cleared by a fill kernel)."""
+    import torch
+    s = config1()
+    fb = FockBuilder(s.basis, s.D, tau=1e-12)
+    Dd = native.to_dev(s.D)
+    eager = fb.build(Dd).clone()
+    torch.cuda.synchronize()
+    fb.capture(Dd)
+    for i in range(4):
+        G = fb.replay().clone()
+        torch.cuda.synchronize()
+        assert (G - eager).abs().max().item() < 1e-12, i
+        e2 = fb.build(Dd).clone()          # eager work between replays
+        torch.cuda.synchronize()
+        assert (e2 - eager).abs().max().item() < 1e-12
