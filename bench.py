@@ -46,14 +46,14 @@ def timed_builds(fb, Dd, steps, warmup, probe=None):
             fb.build(Dd)
             continue
         # the same launches as FockBuilder.build, with events around `probe`
-        sys = fb._sys(Dd)
+        sys = fb._sys([Dd])
         fb.F.zero_()
-        native.density_blocks(fb.shell_ao, fb.shell_size, Dd, fb.Dsh)
+        native.density_blocks(fb.shell_ao[0], fb.shell_size[0], Dd, fb.Dsh)
         for i, L in enumerate(fb.launches):
             if i == probe:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
-            fb.launch(L, sys)
+            fb.launch(L, sys, None, fb._workspace(0))
             if i == probe:
                 e1.record()
                 evs.append((e0, e1))
@@ -68,14 +68,14 @@ def launch_times(fb, Dd):
     """Per-launch durations (ms) of one build, HIP events on the stream."""
     import torch
     from felizjueves_amd import native
-    sys = fb._sys(Dd)
+    sys = fb._sys([Dd])
     fb.F.zero_()
-    native.density_blocks(fb.shell_ao, fb.shell_size, Dd, fb.Dsh)
+    native.density_blocks(fb.shell_ao[0], fb.shell_size[0], Dd, fb.Dsh)
     evs = []
     for L in fb.launches:
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        fb.launch(L, sys)
+        fb.launch(L, sys, None, fb._workspace(0))
         e1.record()
         evs.append((e0, e1))
     native.close(fb.F)

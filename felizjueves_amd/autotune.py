@@ -1,0 +1,112 @@
+"""Autotuner of the batched Fock build: per class pair, the lanes per work
+item (threads) and the primitive-loop split (lane_flops), then the number
+of streams the launches are dealt to.
+
+Each candidate plan is timed on the device: HIP events around every launch
+of a whole build (median over repeats), summed per class pair over its
+tiers. The fastest candidate per class pair is kept; the tuned plan is then
+timed as whole captured builds for each stream count. The choice is a pure
+function of the measured times, written to / read from a JSON file so that
+a tuned configuration can be reused without measuring again.
+"""
+
+from __future__ import annotations
+
+import json
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import native
+from .fock import FockBuilder
+
+THREADS = (64, 128, 256)
+LANE_FLOPS = (3.0e4, 1.0e5, 3.0e5, 1.0e6)
+STREAMS = (1, 2, 4)
+
+
+def launch_times(fb: FockBuilder, D_devs, reps: int = 3) -> dict:
+    """{class pair: median over reps of the summed launch times (ms)} of
+    one eager build, each launch bracketed by events on the stream it runs
+    on."""
+    import torch
+    fb.build(D_devs)
+    torch.cuda.synchronize()
+    sys = fb._sys(D_devs)
+    ws = fb._workspace(0)
+    per: dict = {}
+    for _ in range(reps):
+        evs = []
+        for L in fb.launches:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fb.launch(L, sys, None, ws)
+            e1.record()
+            evs.append((L.cls, e0, e1))
+        torch.cuda.synchronize()
+        tot: dict = {}
+        for c, a, b in evs:
+            tot[c] = tot.get(c, 0.0) + a.elapsed_time(b)
+        for c, t in tot.items():
+            per.setdefault(c, []).append(t)
+    return {c: float(np.median(v)) for c, v in per.items()}
+
+
+def build_time(fb: FockBuilder, D_devs, n_streams: int, steps: int = 10) -> float:
+    """ms per captured-graph build with n_streams."""
+    import torch
+    fb.capture(D_devs, n_streams=n_streams)
+    fb.replay()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        fb.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / steps * 1e3
+
+
+def tune(systems, threads=THREADS, lane_flops=LANE_FLOPS, streams=STREAMS, log=None, **kw) -> dict:
+    """Tune the build of `systems` [(basis, D), ...] (other FockBuilder
+    keywords in kw). Returns {"threads": {cls: t}, "lane_flops": {cls: lf},
+    "n_streams": s, "ms": {...}} with class pairs as 'labcd' strings."""
+    D_devs = [native.to_dev(D) for _, D in systems]
+    best: dict = {}
+    table = {}
+    for t in threads:
+        for lf in lane_flops:
+            fb = FockBuilder.batch(systems, threads=t, lane_flops=lf, **kw)
+            ms = launch_times(fb, D_devs)
+            table[f"{t}/{lf:g}"] = {"".join(map(str, c)): v for c, v in ms.items()}
+            for c, v in ms.items():
+                if c not in best or v < best[c][0]:
+                    best[c] = (v, t, lf)
+            if log:
+                log(t, lf, sum(ms.values()))
+            del fb
+    th = {c: b[1] for c, b in best.items()}
+    lfs = {c: b[2] for c, b in best.items()}
+    fb = FockBuilder.batch(systems, threads=th, lane_flops=lfs, **kw)
+    st = {s: build_time(fb, D_devs, s) for s in streams}
+    n_streams = min(st, key=st.get)
+    if log:
+        log("streams", st)
+    return {"threads": {"".join(map(str, c)): v for c, v in th.items()},
+            "lane_flops": {"".join(map(str, c)): v for c, v in lfs.items()},
+            "n_streams": n_streams, "graph_ms": {str(k): v for k, v in st.items()},
+            "launch_ms": table}
+
+
+def save(cfg: dict, path: Path) -> None:
+    Path(path).write_text(json.dumps(cfg, indent=1, sort_keys=True))
+
+
+def load(path: Path) -> dict:
+    return json.loads(Path(path).read_text())
+
+
+def builder_kwargs(cfg: dict) -> dict:
+    """FockBuilder keywords of a tuned configuration."""
+    key = lambda s: tuple(int(ch) for ch in s)  # noqa: E731
+    return {"threads": {key(k): v for k, v in cfg["threads"].items()},
+            "lane_flops": {key(k): v for k, v in cfg["lane_flops"].items()}}

@@ -21,7 +21,7 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libfj_mi355x.so"
 ARCH = "gfx950"
-LMAX = 2
+LMAX = 3
 MMD_LMAX = 2
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -53,10 +53,11 @@ def _compile(src: Path, extra: list[str]) -> tuple[Path, float, str]:
     if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps if d.exists()):
         return obj, 0.0, ""
     t = time.time()
-    r = subprocess.run([HIPCC, *FLAGS, *extra, "-c", str(src), "-o", str(obj)],
-                       capture_output=True, text=True)
+    r = subprocess.run([HIPCC, *FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage", "-c",
+                        str(src), "-o", str(obj)], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-4000:]}")
+    obj.with_suffix(".res").write_text(r.stderr)
     return obj, time.time() - t, r.stderr
 
 
@@ -74,6 +75,7 @@ def build(lmax: int = LMAX, jobs: int | None = None, extra: list[str] | None = N
         print(f"[fj build] {len(res)} units in {time.time() - t0:.1f}s; slowest: " +
               ", ".join(f"{p.name} {dt:.1f}s" for p, dt, _ in slow), flush=True)
     objs = [str(p) for p, _, _ in res]
+    check_resources()
     tmp = LIB.with_suffix(".so.tmp")
     r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs],
                        capture_output=True, text=True)
@@ -81,6 +83,46 @@ def build(lmax: int = LMAX, jobs: int | None = None, extra: list[str] | None = N
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
     return LIB
+
+
+# Private scratch per lane that every built kernel stays within: the runtime
+# reserves it for every wavefront a queue can hold, on every queue in use.
+SCRATCH_MAX = 6 * 1024
+
+
+def resources() -> dict:
+    """{kernel: (VGPRs, scratch bytes per lane, occupancy)} of the last
+    build, from hipcc's kernel resource remarks."""
+    import re
+    out = {}
+    for f in sorted(OBJ.glob("*.res")):
+        name = None
+        cur: dict = {}
+        for line in f.read_text().splitlines():
+            m = re.search(r"Function Name: (\S+)", line)
+            if m:
+                name = m.group(1)
+                cur = out.setdefault(name, {})
+                continue
+            m = re.search(r"(VGPRs|ScratchSize \[bytes/lane\]|Occupancy \[waves/SIMD\]): (\d+)", line)
+            if m and name:
+                cur[m.group(1).split()[0]] = int(m.group(2))
+            m = re.search(r"Dynamic Stack: (True|False)", line)
+            if m and name:
+                cur["DynamicStack"] = m.group(1) == "True"
+    return out
+
+
+def check_resources() -> None:
+    """Refuse a build in which any kernel asks for more private scratch per
+    lane than SCRATCH_MAX (the runtime fails such launches)."""
+    res = resources()
+    big = {k: v["ScratchSize"] for k, v in res.items() if v.get("ScratchSize", 0) > SCRATCH_MAX}
+    if big:
+        raise RuntimeError(f"kernels over {SCRATCH_MAX} B of scratch per lane: {big}")
+    dyn = [k for k, v in res.items() if v.get("DynamicStack")]
+    if dyn:
+        raise RuntimeError(f"kernels with a dynamic stack (unknown scratch size): {dyn}")
 
 
 def main() -> None:

@@ -26,7 +26,21 @@ struct FjFArgs {
     FjSys one;             // the single system of a launch without a work list
     int diag;
     double tau;
+    double* ws;            // launch workspace (classes with ws_doubles > 0)
+    long long ws_doubles;  // its size
+    int n_bx;              // work items: n_items = n_bx * (grid y of a walk launch)
+    int n_items;
 };
+
+// Workgroups of a workspace class: at most FJ_WS_MAX_BLOCKS in one launch;
+// the upstream-ABI entries (no workspace argument) use a library-owned one
+// sized for FJ_LIB_WS_BLOCKS.
+#define FJ_WS_MAX_BLOCKS 1024
+#define FJ_LIB_WS_BLOCKS 256
+
+// The library-owned workspace of at least `bytes` on the current device
+// (allocated on first use, grown when asked for more; NULL on failure).
+double* fj_library_workspace(long long bytes);
 
 // First primitive record and degree of pair i of one side.
 __device__ __forceinline__ void fj_side_range(const FjSide& s, int i, int& k0, int& k) {

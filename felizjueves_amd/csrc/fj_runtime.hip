@@ -2,9 +2,36 @@
 // closing transpose, the density block maxima, the Boys test entry.
 #include "fj_device.h"
 
+#include <mutex>
+
 #define FJ_VERSION 1
 
 extern "C" int fj_version(void) { return FJ_VERSION; }
+
+// One grow-only workspace per device for the upstream-ABI entries of the
+// workspace classes. Callers of those entries on one device must not
+// overlap in time (they share it); the native builder passes its own.
+double* fj_library_workspace(long long bytes) {
+    static std::mutex mu;
+    static double* buf[64] = {};
+    static long long have[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || bytes <= 0) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (have[dev] < bytes) {
+        if (buf[dev]) {
+            if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+            (void)hipFree(buf[dev]);
+            buf[dev] = nullptr;
+            have[dev] = 0;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;
+        buf[dev] = static_cast<double*>(p);
+        have[dev] = bytes;
+    }
+    return buf[dev];
+}
 
 // F <- F + F^T: each thread owns one (i, j), i < j, pair and the diagonal.
 __global__ void fj_close_k(int n, double* __restrict__ F) {

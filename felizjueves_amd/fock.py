@@ -55,7 +55,25 @@ def fp32_amplification(basis, p) -> float:
     d = float(np.linalg.norm(basis.shell_centre[p.a] - basis.shell_centre[p.b]))
     z = p.rec[:, 6]
     return (1.0 + d) ** (p.la + p.lb) * math.sqrt(z.max() / z.min())
-FOCK_LMAX = 2  # the top class built: (dd|dd)
+FOCK_LMAX = 3  # the top class built: (ff|ff)
+
+# The primitive-loop split (see FockBuilder._lane_flops): a lane's share of
+# a quartet's primitive loop is at most LANE_FLOPS_MAX algorithmic FLOPs, and
+# less when the whole build would not give LANES_TARGET lanes (256 CUs x 4
+# SIMDs x 4 wavefronts x 64 lanes) that much work each.
+LANE_FLOPS_MAX = 1.0e6
+LANES_TARGET = 256 * 4 * 4 * 64
+# Splitting a quartet's primitive loop costs FP64 accuracy: every part goes
+# through its own HRR and its own F updates, and F then sums many more,
+# larger, partly cancelling terms (on a 10-atom cluster, single-primitive
+# parts move G by ~1e-10 where whole quartets agree with the oracle to
+# ~1e-11). So builds of at least SPLIT_MAX_FLOPS, which fill the chip
+# anyway, are split only down to LANE_FLOPS_MAX per lane.
+SPLIT_MAX_FLOPS = 4.0e9
+# Launch workspace per stream (the f classes keep their contracted boundary
+# there): at most this many bytes.
+WS_MAX_BYTES = 8 << 30
+KET_INDEX_LIMIT = 1 << 17  # ket-list entries: pair index in bits 0-16
 
 
 @dataclass
@@ -84,43 +102,98 @@ class Launch:
     work_dev: object = None
     ket_list: np.ndarray = None
     ket_list_dev: object = None
+    threads: int = 64         # lanes per work item
     n_quartets: int = 0       # computed (post-screen) contracted quartets
     n_prim: int = 0           # primitive quartets of those
     flops: float = 0.0        # algorithmic FLOPs of those
 
 
 class FockBuilder:
-    def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads: int = 64,
+    """G = J - K/2 for one system, or (FockBuilder.batch) for many
+    independent systems in the same launches: every class pair of every
+    system is one launch per tier, each work item naming its system."""
+
+    def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads=64,
                  mixed: bool = False, fp32_error: float = 1e-11, selector=None,
-                 lane_flops: float = 1.0e6):
-        if basis.l_max > FOCK_LMAX:
-            raise ValueError(f"shells up to l = {basis.l_max}: the fused F kernels are built through "
-                             f"l = {FOCK_LMAX} ((dd|dd)); f classes are not built yet")
-        self.basis = basis
+                 lane_flops=None):
+        self._setup([(basis, D)], tau=tau, threads=threads, mixed=mixed, fp32_error=fp32_error,
+                     selector=selector, lane_flops=lane_flops)
+
+    @classmethod
+    def batch(cls, systems, **kw) -> "FockBuilder":
+        """One builder for the systems [(basis, D), ...]: tables hold every
+        system's pairs (grouped by system), kets pair only with bras of
+        their own system, F/D/Dsh per system."""
+        self = cls.__new__(cls)
+        self._setup(list(systems), **kw)
+        return self
+
+    def _setup(self, systems, *, tau: float = 1e-12, threads=64, mixed: bool = False,
+               fp32_error: float = 1e-11, selector=None, lane_flops=None):
+        """threads, lane_flops: one value for every class pair, or a dict
+        {(la, lb, lc, ld): value} (the autotuner's choice per class pair;
+        missing ones take the default 64 and the automatic split)."""
+        if not systems:
+            raise ValueError("no systems")
+        if len(systems) > 0xFFFF:
+            raise ValueError("a work item names at most 65535 systems")
+        for basis, _ in systems:
+            if basis.l_max > FOCK_LMAX:
+                raise ValueError(f"shells up to l = {basis.l_max}: the F kernels are built through "
+                                 f"l = {FOCK_LMAX}")
+        self.systems = systems
+        self.basis = systems[0][0]
         self.tau = float(tau)
-        self.threads = int(threads)
+        if isinstance(threads, dict):
+            self.class_threads = {tuple(k): int(v) for k, v in threads.items()}
+            self.threads = 64
+        else:
+            self.class_threads = {}
+            self.threads = int(threads)
         self.mixed = bool(mixed)
         self.fp32_error = float(fp32_error)
-        self.lane_flops = float(lane_flops)
+        if isinstance(lane_flops, dict):
+            self.class_lane_flops = {tuple(k): float(v) for k, v in lane_flops.items()}
+            self.lane_flops = None
+        else:
+            self.class_lane_flops = {}
+            self.lane_flops = None if lane_flops is None else float(lane_flops)
         t0 = time.time()
-        kept, self.deleted, self.eps, self.g_max, self.n_prim_total, _ = P.kept_pairs(
-            basis, D, self.tau, 1)
-        big = [p for p in kept if p.degree > P.MAX_PRIM_PAIRS]
-        if big:
-            raise ValueError(f"{len(big)} pairs keep more than {P.MAX_PRIM_PAIRS} primitive "
-                             "pairs, the staging cap of the F kernels")
         by_cls: dict = {}
-        for p in kept:
-            by_cls.setdefault((p.la, p.lb), []).append(p)
+        self.deleted, self.n_prim_total = [], 0
+        self.Dsh_plans = []
+        for m, (basis, D) in enumerate(systems):
+            kept, deleted, self.eps, self.g_max, n_prim, _ = P.kept_pairs(basis, D, self.tau, 1)
+            self.deleted += deleted
+            self.n_prim_total += n_prim
+            big = [p for p in kept if p.degree > P.MAX_PRIM_PAIRS]
+            if big:
+                raise ValueError(f"{len(big)} pairs keep more than {P.MAX_PRIM_PAIRS} primitive "
+                                 "pairs, the staging cap of the F kernels")
+            for p in kept:
+                by_cls.setdefault((p.la, p.lb), []).append((m, p))
+            self.Dsh_plans.append(P.density_blocks(basis, D))
+        self.Dsh_plan = self.Dsh_plans[0]
         self.tables = {}
-        for c, ps in sorted(by_cls.items()):
-            ps = sorted(ps, key=lambda p: (-p.degree, -p.G))
-            self.tables[c] = SideTable(
-                c, ps, np.array([p.G for p in ps]), np.array([p.degree for p in ps]),
-                np.array([(p.a, p.b) for p in ps], dtype=np.int64))
-            self.tables[c].amp = np.array([fp32_amplification(basis, p) for p in ps])
-            self.tables[c].feat = side_arrays(basis, ps)
-        self.Dsh_plan = P.density_blocks(basis, D)
+        for c, mps in sorted(by_cls.items()):
+            if len(mps) >= KET_INDEX_LIMIT:
+                raise ValueError(f"{len(mps)} shell pairs of class {c}: a ket-list entry holds "
+                                 f"pair indices below {KET_INDEX_LIMIT}")
+            mps = sorted(mps, key=lambda mp: (mp[0], -mp[1].degree, -mp[1].G))
+            ps = [p for _, p in mps]
+            mol = np.array([m for m, _ in mps], dtype=np.int64)
+            t = SideTable(c, ps, np.array([p.G for p in ps]), np.array([p.degree for p in ps]),
+                          np.array([(p.a, p.b) for p in ps], dtype=np.int64))
+            t.mol = mol
+            t.mol_range = {}
+            for m in np.unique(mol):
+                idx = np.nonzero(mol == m)[0]
+                t.mol_range[int(m)] = (int(idx[0]), int(idx[-1]) + 1)
+            t.amp = np.array([fp32_amplification(systems[m][0], p) for m, p in mps])
+            feats = [side_arrays(systems[m][0], [p for mm, p in mps if mm == m])
+                     for m in sorted(t.mol_range)]
+            t.feat = {k: np.concatenate([f[k] for f in feats]) for k in feats[0]}
+            self.tables[c] = t
         # selector(features [n, N_FEATURES]) -> bool [n]: True moves an FP64
         # tile of the bound rule to the FP32 tier (mixed mode only)
         self.selector = selector
@@ -132,15 +205,27 @@ class FockBuilder:
     def _bounds(self, bra: SideTable, ket: SideTable, b: int) -> np.ndarray:
         return self._bounds_dmax(bra, ket, b)[0]
 
-    def _bounds_dmax(self, bra: SideTable, ket: SideTable, b: int):
-        """(G_ab G_cd dmax, dmax) of bra pair b against every ket pair."""
-        Dsh = self.Dsh_plan
+    def _bounds_range(self, bra: SideTable, ket: SideTable, b: int):
+        """(lo, G_ab G_cd dmax, dmax) of bra pair b against the ket pairs
+        lo .. lo + n of its own system."""
+        m = int(bra.mol[b])
+        lo, hi = ket.mol_range.get(m, (0, 0))
+        Dsh = self.Dsh_plans[m]
         sa, sb = bra.sh[b]
-        sc, sd = ket.sh[:, 0], ket.sh[:, 1]
+        sc, sd = ket.sh[lo:hi, 0], ket.sh[lo:hi, 1]
         dmax = np.maximum.reduce([
-            np.full(ket.n, 2.0 * Dsh[sa, sb]), 2.0 * Dsh[sc, sd], 0.5 * Dsh[sa, sc],
+            np.full(hi - lo, 2.0 * Dsh[sa, sb]), 2.0 * Dsh[sc, sd], 0.5 * Dsh[sa, sc],
             0.5 * Dsh[sa, sd], 0.5 * Dsh[sb, sc], 0.5 * Dsh[sb, sd]])
-        return bra.G[b] * (ket.G * dmax), dmax
+        return lo, bra.G[b] * (ket.G[lo:hi] * dmax), dmax
+
+    def _bounds_dmax(self, bra: SideTable, ket: SideTable, b: int):
+        """(G_ab G_cd dmax, dmax) of bra pair b against every ket pair (zero
+        for the kets of other systems)."""
+        lo, bound, dmax = self._bounds_range(bra, ket, b)
+        fb, fd = np.zeros(ket.n), np.zeros(ket.n)
+        fb[lo:lo + len(bound)] = bound
+        fd[lo:lo + len(dmax)] = dmax
+        return fb, fd
 
     def _tiles(self) -> list:
         """Per class pair: the live (bra, ket) quartets of each bra, split by
@@ -158,20 +243,22 @@ class FockBuilder:
                 diag = X == Y
                 tiles, feats = [], []
                 for b in range(bra.n):
-                    bound, dmax = self._bounds_dmax(bra, ket, b)
+                    lo, bound, dmax = self._bounds_range(bra, ket, b)
                     live = bound >= self.tau
                     if diag:
-                        live[:b] = False
-                    ks = np.nonzero(live)[0]
-                    if not len(ks):
+                        live[:max(0, b - lo)] = False
+                    kl = np.nonzero(live)[0]
+                    if not len(kl):
                         continue
+                    ks = kl + lo
+                    bound, dmax = bound[kl], dmax[kl]
                     if self.mixed:
-                        est = bound[ks] * bra.amp[b] * ket.amp[ks] * FP32_EST_SCALE
-                        lo = est < self.fp32_error
-                        parts = ((FP64, ks[~lo]), (FP32, ks[lo]))
+                        est = bound * bra.amp[b] * ket.amp[ks] * FP32_EST_SCALE
+                        small = est < self.fp32_error
+                        parts = ((FP64, ks[~small], dmax[~small]), (FP32, ks[small], dmax[small]))
                     else:
-                        parts = ((FP64, ks),)
-                    for tier, kt in parts:
+                        parts = ((FP64, ks, dmax),)
+                    for tier, kt, dm in parts:
                         if not len(kt):
                             continue
                         starts = np.arange(0, len(kt), T)
@@ -179,16 +266,33 @@ class FockBuilder:
                             tiles.append((b, kt[t0:t0 + T], tier))
                         if want_feats and tier == FP64:
                             feats.append(tile_features((*X, *Y), bra.feat, ket.feat, b, kt, starts,
-                                                       dmax[kt]))
+                                                       dm))
                 f = np.concatenate(feats) if feats else None
                 out.append((X, Y, tiles, f))
         return out
+
+    def _lane_flops(self, groups) -> float:
+        """The primitive-loop split: `lane_flops` if given; else, for a build
+        below SPLIT_MAX_FLOPS, its FLOPs spread over LANES_TARGET lanes (a
+        small system is split down to single primitive quartets per lane so
+        that its few quartets still fill the chip), and LANE_FLOPS_MAX for
+        larger builds."""
+        if self.lane_flops is not None:
+            return self.lane_flops
+        total = 0.0
+        for X, Y, tiles, _ in groups:
+            c = cost((*X, *Y))
+            bra, ket = self.tables[X], self.tables[Y]
+            for b, seg, _t in tiles:
+                total += c.per_prim * bra.deg[b] * ket.deg[seg].sum() + c.per_quartet * len(seg)
+        if total >= SPLIT_MAX_FLOPS:
+            return LANE_FLOPS_MAX
+        return float(min(LANE_FLOPS_MAX, max(1.0, total / LANES_TARGET)))
 
     def _plan(self) -> list:
         """Tiles -> tiers (bound rule, then the learned selector on the
         FP64 tiles) -> one work item per tile (or per primitive-loop chunk)
         per (class pair, tier) launch."""
-        T = self.threads
         groups = self._tiles()
         if self.mixed and self.selector is not None:
             allf = [f for *_, f in groups if f is not None]
@@ -206,11 +310,15 @@ class FockBuilder:
                             if sel[j]:
                                 tiles[t] = (b, seg, FP32)
                             j += 1
+        auto_lf = self._lane_flops(groups)
+        self.lane_flops_used = auto_lf
         out = []
         for X, Y, tiles, _ in groups:
             bra, ket = self.tables[X], self.tables[Y]
             cls = (*X, *Y)
             c = cost(cls)
+            lane_flops = self.class_lane_flops.get(cls, auto_lf)
+            T = self.class_threads.get(cls, self.threads)
             diag = int(X == Y)
             work = {FP64: [], FP32: []}
             cost_w = {FP64: [], FP32: []}
@@ -219,22 +327,37 @@ class FockBuilder:
             stats = {FP64: [0, 0, 0.0], FP32: [0, 0, 0.0]}
             for b, seg, tr in tiles:
                 kab = int(bra.deg[b])
-                # a lane's primitive loop is cut into chunks of at most
-                # lane_flops / per_prim primitive quartets, so no work item
-                # runs much longer than the others
-                chunk = max(1, min(255, max(8, int(self.lane_flops // c.per_prim)) // kab))
+                kcd = ket.deg[seg]
+                # split the primitive loop so that no lane's share exceeds
+                # lane_flops: bra primitive pairs per work item (cb), then
+                # ket primitive pairs per lane (ck)
+                cb = kab if kab * c.per_prim <= lane_flops else max(1, int(lane_flops // c.per_prim))
+                ck = max(1, int(lane_flops // (cb * c.per_prim)))
+                lanes = []
+                lane_prims = []
+                for k, n in zip(seg, kcd):
+                    if n <= ck:
+                        lanes.append(int(k))
+                        lane_prims.append(int(n))
+                    else:
+                        for lo in range(0, int(n), ck):
+                            cnt = 0 if lo + ck >= n else ck
+                            lanes.append(int(k) | (lo << 17) | (cnt << 24))
+                            lane_prims.append(min(ck, int(n) - lo))
+                lanes = np.array(lanes, dtype=np.int64)
+                lane_prims = np.array(lane_prims)
                 st = n_list[tr]
-                lists[tr].append(seg)
-                n_list[tr] += len(seg)
-                kmax = int(ket.deg[seg].max())
-                if kmax <= chunk:
-                    work[tr].append((b, st, 0, st + len(seg)))
-                    cost_w[tr].append(kab * kmax)
-                else:
-                    for kc in range(0, kmax, chunk):
-                        work[tr].append((b, st, (kc << 16) | (chunk << 24), st + len(seg)))
-                        cost_w[tr].append(kab * min(chunk, kmax - kc))
-                nprim = int(kab * ket.deg[seg].sum())
+                lists[tr].append(lanes)
+                n_list[tr] += len(lanes)
+                for t0 in range(0, len(lanes), T):
+                    t1 = min(len(lanes), t0 + T)
+                    pmax = int(lane_prims[t0:t1].max())
+                    m = int(bra.mol[b])
+                    for blo in range(0, kab, cb):
+                        bn = 0 if cb >= kab else min(cb, kab - blo)
+                        work[tr].append((b, st + t0, m | (blo << 16) | (bn << 24), st + t1))
+                        cost_w[tr].append(min(cb, kab - blo) * pmax)
+                nprim = int(kab * kcd.sum())
                 s_ = stats[tr]
                 s_[0] += len(seg)
                 s_[1] += nprim
@@ -244,6 +367,7 @@ class FockBuilder:
                     # costliest work items first: the dispatcher starts them early
                     order = np.argsort(-np.array(cost_w[tier]), kind="stable")
                     L = Launch(cls, bra, ket, diag, tier, np.array(work[tier], dtype=np.int32)[order])
+                    L.threads = T
                     L.ket_list = np.concatenate(lists[tier]).astype(np.int32)
                     L.n_quartets, L.n_prim, L.flops = stats[tier]
                     out.append(L)
@@ -255,15 +379,18 @@ class FockBuilder:
     def to_device(self) -> None:
         import torch
         dev = native.device()
-        b = self.basis
         for t in self.tables.values():
             recs = np.concatenate([p.rec for p in t.pairs], axis=0)
             off = np.concatenate([[0], np.cumsum(t.deg)]).astype(np.int32)
-            sep = np.array([b.shell_centre[p.a] - b.shell_centre[p.b] for p in t.pairs])
-            ao = np.array([(b.shell_ao[p.a], b.shell_ao[p.b]) for p in t.pairs], dtype=np.int32)
+            sep, ao = [], []
+            for m, p in zip(t.mol, t.pairs):
+                bs = self.systems[m][0]
+                sep.append(bs.shell_centre[p.a] - bs.shell_centre[p.b])
+                ao.append((bs.shell_ao[p.a], bs.shell_ao[p.b]))
             t.dev = {"prim": native.to_dev(recs), "off": native.to_dev(off),
-                     "sep": native.to_dev(sep), "G": native.to_dev(t.G),
-                     "sh": native.to_dev(t.sh.astype(np.int32)), "ao": native.to_dev(ao)}
+                     "sep": native.to_dev(np.array(sep)), "G": native.to_dev(t.G),
+                     "sh": native.to_dev(t.sh.astype(np.int32)),
+                     "ao": native.to_dev(np.array(ao, dtype=np.int32))}
             d = t.dev
             t.side = native.FjSide(d["prim"].data_ptr(), d["off"].data_ptr(), 0,
                                    d["sep"].data_ptr(), d["G"].data_ptr(), d["sh"].data_ptr(),
@@ -271,43 +398,99 @@ class FockBuilder:
         for L in self.launches:
             L.work_dev = native.to_dev(L.work)
             L.ket_list_dev = native.to_dev(L.ket_list)
-        self.shell_ao = native.to_dev(b.shell_ao.astype(np.int32))
-        self.shell_size = native.to_dev(b.shell_size.astype(np.int32))
-        self.Dsh = torch.zeros((b.n_sh, b.n_sh), dtype=torch.float64, device=dev)
-        self.F = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
+        # per system: shell offsets/sizes, Dsh and F (views of flat buffers)
+        self.shell_ao, self.shell_size = [], []
+        n_ao = [b.n_ao for b, _ in self.systems]
+        n_sh = [b.n_sh for b, _ in self.systems]
+        self._F_flat = torch.zeros(sum(n * n for n in n_ao), dtype=torch.float64, device=dev)
+        self._Dsh_flat = torch.zeros(sum(n * n for n in n_sh), dtype=torch.float64, device=dev)
+        self.Fs, self.Dshs = [], []
+        fo = so = 0
+        for (b, _), na, ns in zip(self.systems, n_ao, n_sh):
+            self.shell_ao.append(native.to_dev(b.shell_ao.astype(np.int32)))
+            self.shell_size.append(native.to_dev(b.shell_size.astype(np.int32)))
+            self.Fs.append(self._F_flat[fo:fo + na * na].view(na, na))
+            self.Dshs.append(self._Dsh_flat[so:so + ns * ns].view(ns, ns))
+            fo += na * na
+            so += ns * ns
+        self.F, self.Dsh = self.Fs[0], self.Dshs[0]
         self._sys_for = None
         self._dev = True
 
-    def _sys(self, D_dev):
-        """The one-entry system table for density D_dev (rebuilt when the
+    def _sys(self, D_devs):
+        """The system table for the device densities D_devs (rebuilt when a
         density buffer changes)."""
-        key = D_dev.data_ptr()
+        key = tuple(D.data_ptr() for D in D_devs)
         if self._sys_for != key:
-            self._D_ref = D_dev
-            self.sys = native.sys_table([(D_dev, self.Dsh, self.F)])
+            self._D_ref = list(D_devs)
+            self.sys = native.sys_table(list(zip(D_devs, self.Dshs, self.Fs)))
             self._sys_for = key
         return self.sys
 
-    def launch(self, L: "Launch", sys, stream=None) -> None:
-        native.call_fx(L.cls, self.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
-                       L.work.shape[0], sys, L.diag, self.tau, stream, L.ket_list_dev)
+    def _workspace(self, i: int):
+        """Launch workspace i (one per stream the build uses): the largest
+        any launch asks for, capped at WS_MAX_BYTES (a workspace class then
+        runs on fewer persistent workgroups)."""
+        import torch
+        ws = getattr(self, "_ws", None)
+        if ws is None:
+            ws = self._ws = []
+        need = min(WS_MAX_BYTES, max([native.workspace_bytes(L.cls, L.threads, L.work.shape[0])
+                                      for L in self.launches] + [0]))
+        while len(ws) <= i:
+            ws.append(torch.empty(need // 8, dtype=torch.float64, device=native.device())
+                      if need else None)
+        return ws[i]
 
-    def build(self, D_dev, stream=None, close: bool = True):
-        """G for the device density D_dev (n_ao x n_ao float64), on `stream`.
-        Returns the builder's F tensor (overwritten by the next build)."""
+    def launch(self, L: "Launch", sys, stream=None, ws=None) -> None:
+        native.call_fx(L.cls, L.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
+                       L.work.shape[0], sys, L.diag, self.tau, stream, L.ket_list_dev, ws)
+
+    def _side_streams(self, n: int) -> list:
+        import torch
+        if len(getattr(self, "_streams", [])) != n:
+            self._streams = [torch.cuda.Stream() for _ in range(n)]   # default priority
+        return self._streams
+
+    def build(self, D_dev, stream=None, close: bool = True, n_streams: int = 1):
+        """G for the device density D_dev (n_ao x n_ao float64; for a batch
+        builder, a list with one per system), on `stream`. With n_streams > 1
+        the class-pair launches are dealt round-robin (costliest first) to
+        that many side streams that fork from and join back into `stream`,
+        so launches too small to fill the chip overlap. Returns the F tensor
+        (the list of them for a batch; overwritten by the next build)."""
+        import torch
         if not self._dev:
             self.to_device()
-        sys = self._sys(D_dev)
-        self.F.zero_()
-        native.density_blocks(self.shell_ao, self.shell_size, D_dev, self.Dsh, stream)
-        for L in self.launches:
-            self.launch(L, sys, stream)
+        D_devs = list(D_dev) if isinstance(D_dev, (list, tuple)) else [D_dev]
+        if len(D_devs) != len(self.systems):
+            raise ValueError(f"{len(D_devs)} densities for {len(self.systems)} systems")
+        sys = self._sys(D_devs)
+        main = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(main):
+            self._F_flat.zero_()
+        for m, D in enumerate(D_devs):
+            native.density_blocks(self.shell_ao[m], self.shell_size[m], D, self.Dshs[m], main)
+        if n_streams <= 1:
+            ws = self._workspace(0)
+            for L in self.launches:
+                self.launch(L, sys, main, ws)
+        else:
+            side = self._side_streams(n_streams)
+            wss = [self._workspace(j) for j in range(n_streams)]
+            for s in side:
+                s.wait_stream(main)
+            for i, L in enumerate(self.launches):
+                self.launch(L, sys, side[i % n_streams], wss[i % n_streams])
+            for s in side:
+                main.wait_stream(s)
         if close:
-            native.close(self.F, stream)
-        return self.F
+            for F in self.Fs:
+                native.close(F, main)
+        return self.Fs if isinstance(D_dev, (list, tuple)) else self.F
 
     # -- graph capture --------------------------------------------------------
-    def capture(self, D_dev, stream=None):
+    def capture(self, D_dev, stream=None, n_streams: int = 1):
         """Capture one whole build (zero F, density block maxima, every
         launch, close) into a HIP graph on a side stream; `replay()` then
         reruns it with one launch. `D_dev` must keep its address: to build
@@ -316,7 +499,7 @@ class FockBuilder:
         import torch
         if not self._dev:
             self.to_device()
-        self.build(D_dev)           # warm: tables resident, code objects loaded
+        self.build(D_dev, n_streams=n_streams)   # warm: tables resident, code loaded
         torch.cuda.synchronize()
         self._graph_D = D_dev
         g = torch.cuda.CUDAGraph()
@@ -324,7 +507,7 @@ class FockBuilder:
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
-                self.build(D_dev)
+                self.build(D_dev, stream=s, n_streams=n_streams)
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         self.graph = g
@@ -332,7 +515,7 @@ class FockBuilder:
 
     def replay(self):
         self.graph.replay()
-        return self.F
+        return self.Fs if isinstance(self._graph_D, (list, tuple)) else self.F
 
     # -- counts -------------------------------------------------------------
     @property

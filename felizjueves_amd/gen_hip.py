@@ -105,7 +105,7 @@ def emit_quartet(cls) -> str:
     E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
     E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
     E("double s_j, double s_k, const double* __restrict__ wab,")
-    E("double* __restrict__ fab, double* __restrict__ F) {", 1)
+    E("double* __restrict__ fab, double* __restrict__ F, double* __restrict__ ws, size_t st) {", 1)
     for i in range(len(v.targets)):
         E(f"R c{i} = R(0);")
     E("for (int kc = 0; kc < k_cd; ++kc) {")
@@ -238,27 +238,59 @@ def emit_class(cls) -> str:
                      if lb else ("0.0",) * 3)
     cdx, cdy, cdz = (("a.ket.sep[3 * k]", "a.ket.sep[3 * k + 1]", "a.ket.sep[3 * k + 2]")
                      if ld else ("0.0",) * 3)
+    W = ws_doubles(cls)
+    if W:
+        kernel_body = (
+            "    // persistent: one workspace slot per lane of the grid, items strided\n"
+            "    const size_t st = (size_t)gridDim.x * blockDim.x;\n"
+            "    double* ws = a.ws + (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
+            "    for (int it = (int)blockIdx.x; it < a.n_items; it += (int)gridDim.x) {\n"
+            f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, ws, st);\n"
+            "        __syncthreads();\n"
+            "    }")
+        launch_ws = (
+            f"    // workspace: {W} doubles per lane; the grid is cut to the slots it holds\n"
+            f"    {{\n"
+            f"        const long long slots = a.ws ? a.ws_doubles / {W} : 0;\n"
+            f"        const long long blocks = slots / threads;\n"
+            f"        if (blocks < 1) return FJ_ERR_WORKSPACE;\n"
+            f"        grid = dim3((unsigned)(blocks < (long long)a.n_items ? blocks : a.n_items));\n"
+            f"    }}")
+        f_ws = (f"    {{\n"
+                f"        const long long need = (long long)8 * {W} * threads * FJ_LIB_WS_BLOCKS;\n"
+                f"        a.ws = fj_library_workspace(need);\n"
+                f"        if (!a.ws) return FJ_ERR_WORKSPACE;\n"
+                f"        a.ws_doubles = need / 8;\n"
+                f"    }}")
+    else:
+        kernel_body = "    fj_item_" + name + "<R>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0);"
+        launch_ws = ""
+        f_ws = ""
     out.append(f"""
+// One work item: workgroup-uniform early returns only (the persistent
+// kernel calls it in a loop with a barrier after each item).
 template <typename R>
-__global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
-    __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
+__device__ __forceinline__ void fj_item_{name}(const FjFArgs& a, int bx, int by, double* bsh,
+                                            double* __restrict__ ws, size_t st) {{
     // work item (bra pair, first ket, system, ket end) from the list, else
     // (x, y * blockDim, the single system, n_ket)
-    // The third field packs the system (bits 0-15) and a chunk of the ket
-    // primitive loop: first kc (bits 16-23) and count (bits 24-31, 0 = all).
-    // The HRR and the digestion are linear in the contracted boundary, so a
-    // quartet split over kc chunks sums exactly through F's atomics.
-    int bra, k0, k1, kc_lo = 0, kc_n = 0;
+    // The third field packs the system (bits 0-15) and a chunk of the bra
+    // primitive pairs: first (bits 16-23) and count (bits 24-31, 0 = all).
+    // A ket-list entry packs the ket pair (bits 0-16) and a chunk of its
+    // primitive pairs: first (bits 17-23) and count (bits 24-30, 0 = to the
+    // end). The HRR and the digestion are linear in the contracted boundary,
+    // so a quartet split over primitive chunks sums exactly through F's atomics.
+    int bra, k0, k1, kb_lo = 0, kb_n = 0;
     FjSys S;
     if (a.work) {{
-        const int4 w = a.work[blockIdx.x];
+        const int4 w = a.work[bx];
         bra = w.x; k0 = w.y; k1 = w.w;
         S = a.sys[w.z & 0xffff];
-        kc_lo = (w.z >> 16) & 0xff;
-        kc_n = (w.z >> 24) & 0xff;
+        kb_lo = (w.z >> 16) & 0xff;
+        kb_n = (w.z >> 24) & 0xff;
     }} else {{
-        bra = (int)blockIdx.x;
-        k0 = (int)(blockIdx.y * blockDim.x);
+        bra = bx;
+        k0 = by * (int)blockDim.x;
         S = a.one;
         k1 = a.ket.n;
     }}
@@ -267,6 +299,9 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     int kb0, kab;
     fj_side_range(a.bra, bra, kb0, kab);
     if (kab > FJ_MAX_PRIM_PAIRS) return;   // the host refuses such a pair; never overrun the staging
+    if (kb_lo >= kab) return;              // an empty bra chunk (the host emits none)
+    kb0 += kb_lo;
+    kab = kb_n ? min(kab - kb_lo, kb_n) : kab - kb_lo;
     for (int i = threadIdx.x; i < kab; i += blockDim.x) {{
         const double* __restrict__ r = a.bra.prim + 8 * (size_t)(kb0 + i);
         double* s = bsh + FJ_SREC * i;
@@ -282,7 +317,13 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     const int n_ao = S.n_ao;
     // with a ket list the work item names list[k0 .. k1), the live kets only
     const int kk = k0 + (int)threadIdx.x;
-    const int k = a.ket_list ? (kk < k1 ? a.ket_list[kk] : 0) : kk;
+    int k = kk, kc_lo = 0, kc_n = 0;
+    if (a.ket_list) {{
+        const int v = kk < k1 ? a.ket_list[kk] : 0;
+        k = v & 0x1ffff;
+        kc_lo = (v >> 17) & 0x7f;
+        kc_n = (v >> 24) & 0x7f;
+    }}
     double fab[{nab}];
     double wab[{nab}];
 #pragma unroll
@@ -321,7 +362,7 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
         kcd = kc_n ? min(kcd - kc_lo, kc_n) : kcd - kc_lo;
         const double CDx = {cdx}, CDy = {cdy}, CDz = {cdz};
         fj_q_{name}<R>(bsh, kab, a.ket.prim + 8 * (size_t)kc0, kcd, ABx, ABy, ABz, CDx, CDy, CDz,
-                        S.D, n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, S.F);
+                        S.D, n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, S.F, ws, st);
     }}
     // F_ab: one bra per workgroup, so every lane of a wavefront adds into the
     // same block; reduce across the wavefront, then one atomic per element.
@@ -333,15 +374,24 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
     }}
 }}
 
+template <typename R>
+__global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
+    __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
+{kernel_body}
+}}
+
 // the FP32-tier instantiation lives in its own translation unit (f_{name}_r32.hip)
 int fj_launch_{name}_r32(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a);
 //@@SPLIT@@
-static int fj_launch_{name}(hipStream_t s, int threads, int precision, const FjFArgs& a, int n_blocks) {{
+static int fj_launch_{name}(hipStream_t s, int threads, int precision, FjFArgs a, int n_blocks) {{
     if (threads <= 0) threads = FJ_DEFAULT_THREADS;
     if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
     if (precision != 0 && precision != 1) return FJ_ERR_ARG;
     if (n_blocks <= 0 || a.ket.n <= 0) return 0;
     dim3 grid((unsigned)n_blocks, a.work ? 1u : (unsigned)((a.ket.n + threads - 1) / threads));
+    a.n_bx = (int)grid.x;
+    a.n_items = (int)(grid.x * grid.y);
+{launch_ws}
     if (precision == 0) {{
         hipLaunchKernelGGL(fj_f_{name}_k<double>, grid, dim3(threads), 0, s, a);
         return (int)hipGetLastError();
@@ -364,13 +414,16 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
     a.sys = nullptr;
     a.one = FjSys{{D, Dsh, F, n_ao, n_sh}};
     a.diag = diag; a.tau = tau;
+    a.ws = nullptr; a.ws_doubles = 0;
+{f_ws}
     return fj_launch_{name}(s, threads, 0, a, n_bra);
 }}
 
 extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
                          const FjSide* bra, const FjSide* ket, const int* work, int n_work,
-                         const int* ket_list, const FjSys* sys, int diag, double tau) {{
-    if (!bra || !ket || !work || !sys) return FJ_ERR_ARG;
+                         const int* ket_list, const FjSys* sys, int diag, double tau,
+                         double* ws, long long ws_bytes) {{
+    if (!bra || !ket || !work || !sys || ws_bytes < 0) return FJ_ERR_ARG;
     FjFArgs a;
     a.bra = *bra; a.ket = *ket;
     a.work = reinterpret_cast<const int4*>(work);
@@ -378,6 +431,7 @@ extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
     a.sys = sys;
     a.one = FjSys{{nullptr, nullptr, nullptr, 0, 0}};
     a.diag = diag; a.tau = tau;
+    a.ws = ws; a.ws_doubles = ws ? ws_bytes / 8 : 0;
     return fj_launch_{name}(s, threads, precision, a, n_work);
 }}
 """)
@@ -453,7 +507,33 @@ def generate(classes, out: Path = GEN) -> list[Path]:
             if not p.exists() or p.read_text() != text:
                 p.write_text(text)
         files += [out / f"f_{n}.hip", out / f"f_{n}_r32.hip"]
+    wt = out / "fj_ws_table.hip"
+    text = emit_ws_table(classes)
+    if not wt.exists() or wt.read_text() != text:
+        wt.write_text(text)
+    files.append(wt)
     return files
+
+
+def emit_ws_table(classes) -> str:
+    """fj_fx_workspace_bytes: the workspace each built class asks for."""
+    lines = ["// generated by felizjueves_amd/gen_hip.py -- launch workspace per class",
+             '#include "../fj_device.h"', "",
+             'extern "C" long long fj_fx_workspace_bytes(int la, int lb, int lc, int ld, int threads,',
+             "                                           int n_work) {",
+             "    if (threads <= 0) threads = FJ_DEFAULT_THREADS;",
+             "    if (n_work <= 0) return 0;",
+             "    long long per_lane = 0;",
+             "    switch (((la * 4 + lb) * 4 + lc) * 4 + ld) {"]
+    for cls in classes:
+        w = ws_doubles(cls)
+        if w:
+            la, lb, lc, ld = cls
+            lines.append(f"        case {((la * 4 + lb) * 4 + lc) * 4 + ld}: per_lane = {w}; break;")
+    lines += ["        default: return 0;", "    }",
+              "    const long long blocks = n_work < FJ_WS_MAX_BLOCKS ? n_work : FJ_WS_MAX_BLOCKS;",
+              "    return 8 * per_lane * threads * blocks;", "}", ""]
+    return "\n".join(lines)
 
 
 def header_decls(classes) -> str:
@@ -473,7 +553,7 @@ def header_decls(classes) -> str:
             f"int fj_fx_{n}(fj_stream_t stream, int threads, int precision, const FjSide* bra,"
             f" const FjSide* ket,\n"
             f"    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,\n"
-            f"    double tau);")
+            f"    double tau, double* ws, long long ws_bytes);")
     return "\n".join(out) + "\n"
 
 
@@ -505,6 +585,28 @@ def write_header(classes) -> None:
 # blocks flushed per ab).
 
 FUSED_LIVE_MAX = 120
+
+# A staged class keeps its contracted boundary, half-transformed block and
+# F_cd accumulators in a launch workspace in HBM instead of private scratch:
+# the runtime sizes a queue's scratch for every wavefront the chip can hold,
+# and ~17 KB per lane on four queues at once (or ~100 KB on one) is more
+# than it will allocate. Its kernel is persistent over the work items, one
+# workspace slot per resident lane, the slots interleaved so that a
+# wavefront's accesses coalesce, and each of its (e, f) VRR groups is a
+# separate non-inlined function so registers are allocated per group.
+EF_LIVE_MAX = 120   # live values per (e, f) VRR group of a workspace class
+
+
+def ws_doubles(cls) -> int:
+    """Workspace doubles per lane of class cls (0: a fused class, all in
+    registers)."""
+    if not staged(cls):
+        return 0
+    la, lb, lc, ld = cls
+    ne = sum(n_cart(n) for n in range(la, la + lb + 1))
+    nf = sum(n_cart(n) for n in range(lc, lc + ld + 1))
+    w = ne * nf + n_cart(la) * n_cart(lb) * nf
+    return w + n_cart(lc) * n_cart(ld)
 
 
 def staged(cls) -> bool:
@@ -538,6 +640,32 @@ def _hrr_side(la: int, lb: int):
                                          tuple(-x for x in n[1])))
     leaves = [e for n in range(la, la + lb + 1) for e in cartesians(n)]
     return order, terms, targets, leaves
+
+
+def _emit_side_hrr_dfs(E: _Emit, la: int, lb: int, sep: str, src, dst, ind: int) -> None:
+    """One side's HRR emitted depth-first per target, each target handed to
+    dst(j, expr) right after its statement (targets feed no other node), so
+    only the memoised intermediates that later targets still need stay
+    live."""
+    order, terms, targets, leaves = _hrr_side(la, lb)
+    li = {e: i for i, e in enumerate(leaves)}
+    name: dict = {}
+
+    def need(n):
+        if n in name:
+            return name[n]
+        r = terms[n]
+        nm = f"t{_fidx(n[0])}_{_fidx(n[1])}"
+        if not r:
+            E(f"const double {nm} = {src(li[n[0]])};", ind)
+        else:
+            k, moved, lowered = r
+            x, y = need(moved), need(lowered)
+            E(f"const double {nm} = {x} + {sep}{AX[k]} * {y};", ind)
+        name[n] = nm
+        return nm
+    for j, t in enumerate(targets):
+        dst(j, need(t))
 
 
 def _emit_side_hrr(E: _Emit, la: int, lb: int, sep: str, src, dst, ind: int) -> None:
@@ -575,19 +703,24 @@ def emit_quartet_staged(cls) -> str:
     groups = e_groups(tuple(cls))
 
     E = _Emit()
-    E(f"// ({la}{lb}|{lc}{ld}), staged: {len(groups)} VRR groups over {NE} bra components, "
+    W = ws_doubles(cls)
+    if W:
+        # (e, f) groups: each its own non-inlined function, registers allocated
+        # per group; the boundary lives in the lane's workspace slice
+        from .hgp import ef_groups
+        tgroups = [list(g) for g in ef_groups(tuple(cls), EF_LIVE_MAX)]
+        bnd_ref = lambda i: f"ws[(size_t)({i}) * st]"  # noqa: E731
+        mid_ref = lambda i: f"ws[(size_t)({NF * NE} + {i}) * st]"  # noqa: E731
+    else:
+        tgroups = [[(e, f) for e, f in v.targets if e in g] for g in groups]
+        bnd_ref = lambda i: f"bnd[{i}]"  # noqa: E731
+        mid_ref = lambda i: f"mid[{i}]"  # noqa: E731
+    E(f"// ({la}{lb}|{lc}{ld}), staged: {len(tgroups)} VRR groups, "
       f"{len(v.targets)} boundary values", 0)
-    E("template <typename R>", 0)
-    E(f"__device__ __forceinline__ void fj_q_{name}(", 0)
-    E("const double* __restrict__ bp, int k_ab,")
-    E("const double* __restrict__ kp, int k_cd,")
-    E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
-    E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
-    E("double s_j, double s_k, const double* __restrict__ wab,")
-    E("double* __restrict__ fab, double* __restrict__ F) {", 1)
-    E(f"double bnd[{NF * NE}];   // contracted (e0|f0), [f][e]")
-    for gi, g in enumerate(groups):
-        tg = [(e, f) for e, f in v.targets if e in g]
+
+    def group_body(tg, ind):
+        """The primitive nest of one group: its accumulators, the closure of
+        its targets, the contracted values stored to the boundary."""
         nodes = closure(v, [(e, f, 0) for e, f in tg])
         used = set()
         for n in nodes:
@@ -597,50 +730,50 @@ def emit_quartet_staged(cls) -> str:
                 for t in ts:
                     used.add((t.coef, d if t.coef in ("PA", "WP", "QC", "WQ") else None))
         need = lambda c: any(u[0] == c for u in used)  # noqa: E731
-        E(f"{{   // group {gi}: e = {list(g)}")
         for i in range(len(tg)):
-            E(f"R c{i} = R(0);", 2)
-        E("for (int kc = 0; kc < k_cd; ++kc) {", 2)
-        E("const double* __restrict__ q_ = kp + 8 * kc;", 3)
-        E("const double Qx = q_[0], Qy = q_[1], Qz = q_[2], q = q_[6], Uq = q_[7];", 3)
+            E(f"R c{i} = R(0);", ind)
+        E("for (int kc = 0; kc < k_cd; ++kc) {", ind)
+        E("const double* __restrict__ q_ = kp + 8 * kc;", ind + 1)
+        E("const double Qx = q_[0], Qy = q_[1], Qz = q_[2], q = q_[6], Uq = q_[7];", ind + 1)
         for d in range(3):
             if ("QC", d) in used:
-                E(f"const R QC{AX[d]} = R(q_[{3 + d}]);", 3)
+                E(f"const R QC{AX[d]} = R(q_[{3 + d}]);", ind + 1)
         if need("i2q") or need("i2q_r"):
-            E("const double iq = 1.0 / q;", 3)
-        E("for (int kb = 0; kb < k_ab; ++kb) {", 3)
-        E("const double* __restrict__ p_ = bp + FJ_SREC * kb;", 4)
-        E("const double PQx = p_[0] - Qx, PQy = p_[1] - Qy, PQz = p_[2] - Qz;", 4)
-        E("const double p = p_[6];", 4)
-        E("const double ippq = 1.0 / (p + q);", 4)
-        E("const double rho = p * q * ippq;", 4)
-        E("const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);", 4)
-        E(f"double Fm[{L + 1}];", 4)
-        E(f"fj_boys<{L}>(T, Fm);", 4)
-        E("const double S = p_[7] * Uq * (FJ_TWO_OVER_SQRT_PI * sqrt(rho));", 4)
+            E("const double iq = 1.0 / q;", ind + 1)
+        E("for (int kb = 0; kb < k_ab; ++kb) {", ind + 1)
+        i2 = ind + 2
+        E("const double* __restrict__ p_ = bp + FJ_SREC * kb;", i2)
+        E("const double PQx = p_[0] - Qx, PQy = p_[1] - Qy, PQz = p_[2] - Qz;", i2)
+        E("const double p = p_[6];", i2)
+        E("const double ippq = 1.0 / (p + q);", i2)
+        E("const double rho = p * q * ippq;", i2)
+        E("const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);", i2)
+        E(f"double Fm[{L + 1}];", i2)
+        E(f"fj_boys<{L}>(T, Fm);", i2)
+        E("const double S = p_[7] * Uq * (FJ_TWO_OVER_SQRT_PI * sqrt(rho));", i2)
         for d in range(3):
             if ("PA", d) in used:
-                E(f"const R PA{AX[d]} = R(p_[{3 + d}]);", 4)
+                E(f"const R PA{AX[d]} = R(p_[{3 + d}]);", i2)
             if ("WP", d) in used:
-                E(f"const R WP{AX[d]} = R(-q * ippq * PQ{AX[d]});", 4)
+                E(f"const R WP{AX[d]} = R(-q * ippq * PQ{AX[d]});", i2)
             if ("WQ", d) in used:
-                E(f"const R WQ{AX[d]} = R(p * ippq * PQ{AX[d]});", 4)
+                E(f"const R WQ{AX[d]} = R(p * ippq * PQ{AX[d]});", i2)
         if need("i2p"):
-            E("const R i2p = R(0.5 * p_[8]);", 4)
+            E("const R i2p = R(0.5 * p_[8]);", i2)
         if need("i2p_r"):
-            E("const R i2p_r = R(0.5 * p_[8] * rho * p_[8]);", 4)
+            E("const R i2p_r = R(0.5 * p_[8] * rho * p_[8]);", i2)
         if need("i2q"):
-            E("const R i2q = R(0.5 * iq);", 4)
+            E("const R i2q = R(0.5 * iq);", i2)
         if need("i2q_r"):
-            E("const R i2q_r = R(0.5 * iq * rho * iq);", 4)
+            E("const R i2q_r = R(0.5 * iq * rho * iq);", i2)
         if need("i2pq"):
-            E("const R i2pq = R(0.5 * ippq);", 4)
+            E("const R i2pq = R(0.5 * ippq);", i2)
         for n in v.order:
             if n not in nodes:
                 continue
             r = v.terms[n]
             if not r:
-                E(f"const R {_vname(n)} = R(S * Fm[{n[2]}]);", 4)
+                E(f"const R {_vname(n)} = R(S * Fm[{n[2]}]);", i2)
                 continue
             d, ts = r
             parts = []
@@ -655,42 +788,69 @@ def emit_quartet_staged(cls) -> str:
                     f = c if abs(m) == 1 else f"(R({abs(m)}) * {c})"
                     parts.append(f"{sign} {f} * {ch}")
             expr = parts[0]
-            for p in parts[1:]:
-                expr += (" " + p) if p[0] in "+-" else (" + " + p)
-            E(f"const R {_vname(n)} = {expr};", 4)
+            for q_ in parts[1:]:
+                expr += (" " + q_) if q_[0] in "+-" else (" + " + q_)
+            E(f"const R {_vname(n)} = {expr};", i2)
         for i, (e, f) in enumerate(tg):
-            E(f"c{i} += {_vname((e, f, 0))};", 4)
-        E("}", 3)
-        E("}", 2)
+            E(f"c{i} += {_vname((e, f, 0))};", i2)
+        E("}", ind + 1)
+        E("}", ind)
         for i, (e, f) in enumerate(tg):
-            E(f"bnd[{fi[f] * NE + ei[e]}] = double(c{i});", 2)
-        E("}")
+            E(f"{bnd_ref(fi[f] * NE + ei[e])} = double(c{i});", ind)
+
+    if W:
+        for gi, tg in enumerate(tgroups):
+            E("template <typename R>", 0)
+            E(f"__device__ __noinline__ void fj_g{gi}_{name}(const double* __restrict__ bp, int k_ab,", 0)
+            E("const double* __restrict__ kp, int k_cd, double* __restrict__ ws, size_t st) {", 1)
+            group_body(tg, 1)
+            E("}", 0)
+    E("template <typename R>", 0)
+    E(f"__device__ __forceinline__ void fj_q_{name}(", 0)
+    E("const double* __restrict__ bp, int k_ab,")
+    E("const double* __restrict__ kp, int k_cd,")
+    E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
+    E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
+    E("double s_j, double s_k, const double* __restrict__ wab,")
+    E("double* __restrict__ fab, double* __restrict__ F, double* __restrict__ ws, size_t st) {", 1)
+    if W:
+        E(f"// (e0|f0) [f][e], (ab|f0) [ab][f] and F_cd in the workspace ({W} doubles per lane)")
+        for gi in range(len(tgroups)):
+            E(f"fj_g{gi}_{name}<R>(bp, k_ab, kp, k_cd, ws, st);")
+    else:
+        E(f"double bnd[{NF * NE}];   // contracted (e0|f0), [f][e]")
+        for gi, tg in enumerate(tgroups):
+            E(f"{{   // group {gi}: e = {sorted(set(e for e, _ in tg))}")
+            group_body(tg, 2)
+            E("}")
     # phase B: bra HRR per f
-    E(f"double mid[{NAB * NF}];   // (ab|f0), [ab][f]")
+    if not W:
+        E(f"double mid[{NAB * NF}];   // (ab|f0), [ab][f]")
     if lb == 0:
         E("#pragma unroll 1", 0)
-        E(f"for (int i = 0; i < {NAB * NF}; ++i) mid[(i % {NE}) * {NF} + i / {NE}] = bnd[i];")
+        E(f"for (int i = 0; i < {NAB * NF}; ++i) {mid_ref(f'(i % {NE}) * {NF} + i / {NE}')} = {bnd_ref('i')};")
     else:
         E("#pragma unroll 1", 0)
         E(f"for (int f = 0; f < {NF}; ++f) {{")
-        E(f"const double* __restrict__ x = bnd + f * {NE};", 2)
-        _emit_side_hrr(E, la, lb, "AB", lambda i: f"x[{i}]",
-                       lambda j, nm: E(f"mid[{j} * {NF} + f] = {nm};", 2), 2)
+        _emit_side_hrr_dfs(E, la, lb, "AB", lambda i: bnd_ref(f"f * {NE} + {i}"),
+                           lambda j, nm: E(f"{mid_ref(f'{j} * {NF} + f')} = {nm};", 2), 2)
         E("}")
-    # phase C: ket HRR per ab + digestion
-    for c in range(nc):
-        for d in range(nd):
-            E(f"const double wcd{c}_{d} = s_j * D[(oc + {c}) * n_ao + od + {d}];")
-            E(f"double fcd{c}_{d} = 0.0;")
+    # phase C: ket HRR per ab + digestion, each ERI digested as it is made.
+    # F_cd accumulates over ab: in registers, or (workspace classes) in the
+    # lane's workspace slice with its density weights read from D.
+    NCD = nc * nd
+    fcd_ref = lambda j: f"ws[(size_t)({NF * NE + NAB * NF} + {j}) * st]"  # noqa: E731
+    if W:
+        E("#pragma unroll 1", 0)
+        E(f"for (int j = 0; j < {NCD}; ++j) {fcd_ref('j')} = 0.0;")
+    else:
+        for c in range(nc):
+            for d in range(nd):
+                E(f"const double wcd{c}_{d} = s_j * D[(oc + {c}) * n_ao + od + {d}];")
+                E(f"double fcd{c}_{d} = 0.0;")
     E("#pragma unroll 1", 0)
     E(f"for (int ab = 0; ab < {NAB}; ++ab) {{")
     E(f"const int a = ab / {nb}, b = ab % {nb};", 2)
-    E(f"const double* __restrict__ y = mid + ab * {NF};", 2)
-    xs = {}
-
-    def keep(j, nm):
-        xs[j] = nm
-    _emit_side_hrr(E, lc, ld, "CD", lambda i: f"y[{i}]", keep, 2)
     E("const double w_ab = s_j * D[(oa + a) * n_ao + ob + b];", 2)
     E("const double* __restrict__ Db = D + (size_t)(ob + b) * n_ao;", 2)
     E("const double* __restrict__ Da = D + (size_t)(oa + a) * n_ao;", 2)
@@ -703,12 +863,17 @@ def emit_quartet_staged(cls) -> str:
         E(f"double fac{c} = 0.0, fbc{c} = 0.0;", 2)
     for d in range(nd):
         E(f"double fad{d} = 0.0, fbd{d} = 0.0;", 2)
-    for c in range(nc):
-        for d in range(nd):
-            x = xs[c * nd + d]
+
+    def digest(j, x):
+        c, d = divmod(j, nd)
+        if W:
+            E(f"sab += (s_j * D[(oc + {c}) * n_ao + od + {d}]) * {x}; "
+              f"{fcd_ref(j)} += w_ab * {x};", 2)
+        else:
             E(f"sab += wcd{c}_{d} * {x}; fcd{c}_{d} += w_ab * {x};", 2)
-            E(f"fac{c} += dbd{d} * {x}; fad{d} += dbc{c} * {x}; "
-              f"fbc{c} += dad{d} * {x}; fbd{d} += dac{c} * {x};", 2)
+        E(f"fac{c} += dbd{d} * {x}; fad{d} += dbc{c} * {x}; "
+          f"fbc{c} += dad{d} * {x}; fbd{d} += dac{c} * {x};", 2)
+    _emit_side_hrr_dfs(E, lc, ld, "CD", lambda i: mid_ref(f"ab * {NF} + {i}"), digest, 2)
     E("fab[ab] += sab;", 2)
     for c in range(nc):
         E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + oc + {c}, fac{c});", 2)
@@ -717,8 +882,13 @@ def emit_quartet_staged(cls) -> str:
         E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + od + {d}, fad{d});", 2)
         E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + od + {d}, fbd{d});", 2)
     E("}")
-    for c in range(nc):
-        for d in range(nd):
-            E(f"unsafeAtomicAdd(F + (size_t)(oc + {c}) * n_ao + od + {d}, fcd{c}_{d});")
+    if W:
+        E("#pragma unroll 1", 0)
+        E(f"for (int j = 0; j < {NCD}; ++j)")
+        E(f"unsafeAtomicAdd(F + (size_t)(oc + j / {nd}) * n_ao + od + j % {nd}, {fcd_ref('j')});", 2)
+    else:
+        for c in range(nc):
+            for d in range(nd):
+                E(f"unsafeAtomicAdd(F + (size_t)(oc + {c}) * n_ao + od + {d}, fcd{c}_{d});")
     E("}", 0)
     return E.text()

@@ -301,3 +301,39 @@ def e_groups(cls, live_max: int = 100):
             return tuple(groups)
     del nF
     return tuple((e,) for e in es)
+
+
+@lru_cache(maxsize=None)
+def ef_groups(cls, live_max: int = 120):
+    """Split the boundary targets (e, f), in build order, into consecutive
+    groups whose closures (plus their accumulators) keep at most `live_max`
+    values live: finer than e_groups, for classes where even one bra
+    component's targets need more. Cached on disk (the search takes seconds
+    for (ff|ff))."""
+    import json
+    from pathlib import Path
+    cache = Path(__file__).resolve().parent / "_build" / "ef_groups.json"
+    key = f"{''.join(map(str, cls))}:{live_max}"
+    try:
+        saved = json.loads(cache.read_text())
+    except (OSError, ValueError):
+        saved = {}
+    if key in saved:
+        return tuple(tuple((tuple(e), tuple(f)) for e, f in g) for g in saved[key])
+    dag = vrr_dag(tuple(cls))
+
+    def live(tg) -> int:
+        return peak_live(dag, closure(dag, [(e, f, 0) for e, f in tg])) + len(tg)
+
+    groups, cur = [], []
+    for t in dag.targets:
+        if cur and live(cur + [t]) > live_max:
+            groups.append(tuple(cur))
+            cur = [t]
+        else:
+            cur.append(t)
+    groups.append(tuple(cur))
+    saved[key] = [[[list(e), list(f)] for e, f in g] for g in groups]
+    cache.parent.mkdir(exist_ok=True)
+    cache.write_text(json.dumps(saved))
+    return tuple(groups)

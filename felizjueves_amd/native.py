@@ -128,16 +128,26 @@ def call_f(cls, threads: int, args: dict, F, stream=None) -> None:
 
 
 def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, work, n_work: int,
-            sys, diag: int, tau: float, stream=None, ket_list=None) -> None:
+            sys, diag: int, tau: float, stream=None, ket_list=None, ws=None) -> None:
     """One launch of fj_fx_<cls>: `work` a device int32 [n_work, 4] tensor of
     (bra, first ket, system, ket end), `sys` a device FjSys table; with
-    `ket_list` the (first, end) range indexes that device int32 list."""
+    `ket_list` the (first, end) range indexes that device int32 list; `ws`
+    a device tensor used as the launch workspace (classes that need one)."""
     fn = getattr(lib(), "fj_fx_" + "".join(map(str, cls)))
     fn.restype = C.c_int
+    ws_bytes = 0 if ws is None else ws.numel() * ws.element_size()
     rc = fn(C.c_void_p(stream_handle(stream)), C.c_int(threads), C.c_int(precision),
             C.byref(bra), C.byref(ket), C.c_void_p(ptr(work)), C.c_int(n_work),
-            C.c_void_p(ptr(ket_list)), C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau))
+            C.c_void_p(ptr(ket_list)), C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau),
+            C.c_void_p(ptr(ws)), C.c_longlong(ws_bytes))
     check(rc, fn.__name__)
+
+
+def workspace_bytes(cls, threads: int, n_work: int) -> int:
+    """Bytes of launch workspace fj_fx_<cls> uses for n_work items."""
+    fn = lib().fj_fx_workspace_bytes
+    fn.restype = C.c_longlong
+    return int(fn(*[C.c_int(x) for x in cls], C.c_int(threads), C.c_int(n_work)))
 
 
 def close(F, stream=None) -> None:

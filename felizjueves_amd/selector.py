@@ -20,7 +20,7 @@ def item_features(fb: FockBuilder, L, w) -> np.ndarray:
     """The tile features (features.tile_features) of work item
     w = (bra, first, sys|chunk, end) of launch L."""
     b = int(w[0])
-    ks = L.ket_list[w[1]:w[3]]
+    ks = np.unique(L.ket_list[w[1]:w[3]] & 0x1ffff)
     _, dmax = fb._bounds_dmax(L.bra, L.ket, b)
     return tile_features(L.cls, L.bra.feat, L.ket.feat, b, ks, np.array([0]), dmax[ks])[0]
 
@@ -41,7 +41,7 @@ def probe_errors(fb: FockBuilder, D: np.ndarray, max_items: int = 4096, seed: in
         sel = rng.choice(len(items), size=max_items, replace=False)
         items = [items[i] for i in sorted(sel)]
     feats, errs, des = [], [], []
-    native.density_blocks(fb.shell_ao, fb.shell_size, Dd, fb.Dsh)
+    native.density_blocks(fb.shell_ao[0], fb.shell_size[0], Dd, fb.Dsh)
     batch = 512
     for s0 in range(0, len(items), batch):
         chunk = items[s0:s0 + batch]
@@ -58,8 +58,8 @@ def probe_errors(fb: FockBuilder, D: np.ndarray, max_items: int = 4096, seed: in
                 w = L.work[[i for _, i in lst]].copy()
                 w[:, 2] = (w[:, 2] & ~0xffff) | np.array([j for j, _ in lst], dtype=np.int32)
                 wd = native.to_dev(w)
-                native.call_fx(L.cls, fb.threads, tier, L.bra.side, L.ket.side, wd, w.shape[0],
-                               sys, L.diag, fb.tau, None, L.ket_list_dev)
+                native.call_fx(L.cls, L.threads, tier, L.bra.side, L.ket.side, wd, w.shape[0],
+                               sys, L.diag, fb.tau, None, L.ket_list_dev, fb._workspace(0))
         torch.cuda.synchronize()
         d = (F[FP32] - F[FP64])
         d = d + d.transpose(1, 2)

@@ -38,6 +38,8 @@ typedef struct ihipStream_t* fj_stream_t; /* a hipStream_t; NULL = default strea
 #define FJ_ERR_SHAPE 9003    /* inconsistent sizes (e.g. diag with n_bra != n_ket) */
 #define FJ_ERR_CLASS 9004    /* no kernel for the requested class */
 #define FJ_ERR_ARG 9005      /* another invalid argument */
+#define FJ_ERR_WORKSPACE 9006 /* the class needs a launch workspace and none (or too
+                                small a one for a single workgroup) was given */
 
 /* One side of an F launch, as device pointers.  Pair i's primitive records
  * are prim[off[i] .. off[i+1]) (off == NULL: uniform degree k, records
@@ -93,13 +95,25 @@ int fj_boys_eval(fj_stream_t stream, int M, int n, const double* T, double* out)
  * pointers; `work` is a device int array [n_work][4] of (bra pair, first
  * ket, system index into the device array `sys`, ket end exclusive), one
  * workgroup each, `threads` consecutive kets from the first (the system
- * field packs the system index in bits 0-15 and, optionally, a chunk of each
- * ket's primitive pairs: first in bits 16-23, count in bits 24-31, 0 = all;
- * chunks of one quartet add up exactly in F); with a device
+ * field packs the system index in bits 0-15 and, optionally, a chunk of the
+ * bra pair's primitive pairs: first in bits 16-23, count in bits 24-31,
+ * 0 = all; chunks of one quartet add up exactly in F); with a device
  * `ket_list` (may be NULL) the (first, end) range indexes that list instead,
- * so a work item can name a compacted set of kets; `precision` 0
+ * one lane per entry, so a work item can name a compacted set of kets; an
+ * entry packs the ket pair in bits 0-16 and, optionally, a chunk of its
+ * primitive pairs: first in bits 17-23, count in bits 24-30 (0 = to the
+ * end), so one quartet can be spread over several lanes; `precision` 0
  * is the FP64 kernel, 1 the FP32 tier (VRR and boundary in FP32; Boys, the
- * seed, the HRR and every F update in FP64). */
+ * seed, the HRR and every F update in FP64).  `ws` (ws_bytes) is a device
+ * workspace: the classes whose contracted boundary is too large for private
+ * memory (fj_fx_workspace_bytes > 0: the f classes) keep it there, one slot
+ * per resident lane, and run persistent over the work items with as many
+ * workgroups as the workspace holds slots for; other classes ignore it (may
+ * be NULL).  Launches that share a workspace must not overlap in time. */
+/* Workspace bytes fj_fx_<la lb lc ld> uses for n_work items of `threads`
+ * lanes (0: the class needs none; never more than FJ_WS_MAX_BLOCKS
+ * workgroups' worth). */
+long long fj_fx_workspace_bytes(int la, int lb, int lc, int ld, int threads, int n_work);
 /* BEGIN F CLASSES (written by felizjueves_amd/gen_hip.py) */
 int fj_f_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
@@ -108,7 +122,7 @@ int fj_f_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0000(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -116,7 +130,7 @@ int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0001(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -124,7 +138,15 @@ int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0003(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0003(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -132,7 +154,7 @@ int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0011(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -140,7 +162,15 @@ int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0013(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0013(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -148,7 +178,23 @@ int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0023(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0023(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0033(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -156,7 +202,7 @@ int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0101(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -164,7 +210,15 @@ int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0103(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0103(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -172,7 +226,7 @@ int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -180,7 +234,15 @@ int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0113(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -188,7 +250,23 @@ int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -196,7 +274,15 @@ int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0203(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0203(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -204,7 +290,7 @@ int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0211(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -212,7 +298,15 @@ int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0213(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -220,7 +314,79 @@ int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0303(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0303(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0311(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0311(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0312(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0312(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0313(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0322(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_0333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -228,7 +394,7 @@ int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_1111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
 int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -236,7 +402,15 @@ int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1113(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -244,7 +418,23 @@ int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -252,7 +442,15 @@ int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1213(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -260,7 +458,55 @@ int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1313(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1322(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_1333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -268,7 +514,47 @@ int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
     const double* Dsh, const double* D, double* F);
 int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau);
+    double tau, double* ws, long long ws_bytes);
+int fj_f_2223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_2233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_2323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_2333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_f_3333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_3333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
 /* END F CLASSES */
 
 /* The MMD families, one entry per class, each with the upstream kernel's

@@ -1,0 +1,93 @@
+"""The builder's work lists (CPU): every live quartet's primitive loop is
+covered exactly once by the (work item bra chunk) x (ket-list lane chunk)
+products, for whole tiles, for the automatic split and for a forced fine
+split, in both tiers."""
+
+import numpy as np
+import pytest
+
+from felizjueves_amd.fock import FP32, FockBuilder
+from felizjueves_amd.systems import config1, config2
+
+
+def _decode_cover(fb):
+    """{(class, bra, ket): [kab x kcd] coverage counts} from the work lists,
+    decoding the fields as the kernel front end does."""
+    cover = {}
+    for L in fb.launches:
+        bra, ket = L.bra, L.ket
+        for b, k0, z, k1 in L.work:
+            assert z & 0xFFFF == bra.mol[b]
+            kab = int(bra.deg[b])
+            kb_lo, kb_n = (z >> 16) & 0xFF, (z >> 24) & 0xFF
+            assert kb_lo < kab
+            b1 = kab if kb_n == 0 else min(kab, kb_lo + kb_n)
+            assert k1 - k0 <= L.threads
+            for v in L.ket_list[k0:k1]:
+                k, lo, n = v & 0x1FFFF, (v >> 17) & 0x7F, (v >> 24) & 0x7F
+                assert ket.mol[k] == bra.mol[b]
+                kcd = int(ket.deg[k])
+                c1 = kcd if n == 0 else min(kcd, lo + n)
+                key = (L.cls, int(b), int(k))
+                m = cover.setdefault(key, np.zeros((kab, kcd), dtype=np.int32))
+                m[kb_lo:b1, lo:c1] += 1
+    return cover
+
+
+def _expected(fb):
+    out = set()
+    classes = sorted(fb.tables)
+    for i, X in enumerate(classes):
+        for Y in classes[i:]:
+            bra, ket = fb.tables[X], fb.tables[Y]
+            for b in range(bra.n):
+                live = fb._bounds(bra, ket, b) >= fb.tau
+                if X == Y:
+                    live[:b] = False
+                for k in np.nonzero(live)[0]:
+                    out.add(((*X, *Y), b, int(k)))
+    return out
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(lane_flops=3e4), dict(lane_flops=1e30),
+                                dict(mixed=True, selector=lambda X: np.arange(len(X)) % 2 == 0)])
+def test_work_lists_cover_each_primitive_quartet_once(kw):
+    s = config1()
+    fb = FockBuilder(s.basis, s.D, tau=1e-12, **kw)
+    cover = _decode_cover(fb)
+    assert set(cover) == _expected(fb)
+    for key, m in cover.items():
+        assert (m == 1).all(), key
+    if "selector" in kw:
+        assert any(L.precision == FP32 for L in fb.launches)
+
+
+def test_small_system_is_split_finer_than_a_large_one():
+    small = FockBuilder(config1().basis, config1().D, tau=1e-12)
+    s = config2(seed=11, n_atoms=10)
+    large = FockBuilder(s.basis, s.D, tau=1e-12)
+    assert small.lane_flops_used < large.lane_flops_used
+    n_lanes = sum(len(L.ket_list) for L in small.launches)
+    assert n_lanes > 5 * small.n_quartets
+
+
+def test_batch_work_lists_match_the_single_system_plans():
+    """A batch of three systems: each system's quartets covered once, kets
+    only from the bra's own system, and the same quartets as each system's
+    own plan."""
+    from felizjueves_amd.systems import config1, config2
+    systems = [config1(seed=1), config2(seed=5, n_atoms=4), config1(seed=3)]
+    fb = FockBuilder.batch([(s.basis, s.D) for s in systems], tau=1e-12)
+    cover = _decode_cover(fb)
+    for key, m in cover.items():
+        assert (m == 1).all(), key
+    assert set(cover) == _expected(fb)
+    for i, s in enumerate(systems):
+        one = FockBuilder(s.basis, s.D, tau=1e-12)
+        mine = {(c, fb.tables[c[:2]].pairs[b].a, fb.tables[c[:2]].pairs[b].b,
+                 fb.tables[c[2:]].pairs[k].a, fb.tables[c[2:]].pairs[k].b)
+                for c, b, k in cover if fb.tables[c[:2]].mol[b] == i}
+        theirs = {(c, one.tables[c[:2]].pairs[b].a, one.tables[c[:2]].pairs[b].b,
+                   one.tables[c[2:]].pairs[k].a, one.tables[c[2:]].pairs[k].b)
+                  for c, b, k in _decode_cover(one)}
+        assert mine == theirs, i
