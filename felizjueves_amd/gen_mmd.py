@@ -45,9 +45,13 @@ def _tname(t) -> str:
 # pieces
 
 
-def emit_tower(E: _Emit, L: int, ind: int, need=None) -> dict:
+def emit_tower(E: _Emit, L: int, ind: int, need=None, f32: bool = False) -> dict:
     """[t]^(0) for |t| <= L (or the set `need`) as locals r<t>; expects
-    doubles PQx PQy PQz, rho and T in scope. Returns {t: name}."""
+    doubles PQx PQy PQz, rho and T in scope. Returns {t: name}.
+
+    f32: the J mixed lever (upstream eri/j_coulomb/levers/mixed.py:82-191):
+    PQ, theta = sqrt(rho), T, the [0]^(m) and every [t]^(m) in float; F_m
+    evaluated in double on the widened float T and narrowed."""
     targets = list(need) if need is not None else hermite_order(L)
     nodes: dict = {}
     todo = [(t, 0) for t in targets]
@@ -66,15 +70,31 @@ def emit_tower(E: _Emit, L: int, ind: int, need=None) -> dict:
         nodes[(t, m)] = (d, ch)
         todo += ch
     mmax = max(m for _, m in nodes)
-    E(f"double Fm[{mmax + 1}];", ind)
-    E(f"fj_boys<{mmax}>(T, Fm);", ind)
-    E("const double w0 = FJ_TWO_OVER_SQRT_PI * sqrt(rho), m2r = -2.0 * rho;", ind)
-    E("double wm = w0;", ind)
-    for m in range(mmax + 1):
-        if ((0, 0, 0), m) in nodes:
-            E(f"const double R0_0_0m{m} = wm * Fm[{m}];", ind)
-        if m < mmax:
-            E("wm *= m2r;", ind)
+    real = "float" if f32 else "double"
+    if f32:
+        E("const float PQxf = (float)PQx, PQyf = (float)PQy, PQzf = (float)PQz;", ind)
+        E("const float thf = (float)sqrt(rho), t2f = thf * thf;", ind)
+        E("const float Tf = t2f * ((PQxf * PQxf + PQyf * PQyf) + PQzf * PQzf);", ind)
+        E(f"double Fm[{mmax + 1}];", ind)
+        E(f"fj_boys<{mmax}>((double)Tf, Fm);", ind)
+        E("const float m2r = -2.0f * t2f;", ind)
+        E("float wm = (float)FJ_TWO_OVER_SQRT_PI * thf;", ind)
+        for m in range(mmax + 1):
+            if ((0, 0, 0), m) in nodes:
+                E(f"const float R0_0_0m{m} = wm * (float)Fm[{m}];", ind)
+            if m < mmax:
+                E("wm *= m2r;", ind)
+    else:
+        E(f"double Fm[{mmax + 1}];", ind)
+        E(f"fj_boys<{mmax}>(T, Fm);", ind)
+        E("const double w0 = FJ_TWO_OVER_SQRT_PI * sqrt(rho), m2r = -2.0 * rho;", ind)
+        E("double wm = w0;", ind)
+        for m in range(mmax + 1):
+            if ((0, 0, 0), m) in nodes:
+                E(f"const double R0_0_0m{m} = wm * Fm[{m}];", ind)
+            if m < mmax:
+                E("wm *= m2r;", ind)
+    sfx = "f" if f32 else ""
     order = sorted(nodes, key=lambda n: (-n[1], sum(n[0])))
     for n in order:
         r = nodes[n]
@@ -83,12 +103,14 @@ def emit_tower(E: _Emit, L: int, ind: int, need=None) -> dict:
             continue
         d, ch = r
         nm = f"R{_tname(t)}m{m}"
-        e = f"PQ{AX[d]} * R{_tname(ch[0][0])}m{ch[0][1]}"
+        e = f"PQ{AX[d]}{sfx} * R{_tname(ch[0][0])}m{ch[0][1]}"
         if len(ch) > 1:
             k = t[d] - 1
             e += (f" + R{_tname(ch[1][0])}m{ch[1][1]}" if k == 1 else
-                  f" + {float(k)} * R{_tname(ch[1][0])}m{ch[1][1]}")
-        E(f"const double {nm} = {e};", ind)
+                  f" + {float(k)}{sfx} * R{_tname(ch[1][0])}m{ch[1][1]}")
+        E(f"const {real} {nm} = {e};", ind)
+    if f32:
+        return {t: f"(double)R{_tname(t)}m0" for t in targets}
     return {t: f"R{_tname(t)}m0" for t in targets}
 
 
@@ -282,15 +304,19 @@ def emit_dq(lc: int, ld: int) -> str:
 # at its first ket under tau; J_p = sum_k sum_q (-1)^|q| [p+q]^(0) D_q.
 
 
-def emit_pq(lp: int, lq: int) -> str:
+def emit_pq(lp: int, lq: int, f32: bool = False) -> str:
+    """fj_pq_<n> (FP64) or fj_pq32_<n> (the J mixed lever: the Hermite
+    recurrence in FP32 inside the FP64 boundary -- D_q, the J_p accumulator,
+    the wavefront reduction and the write stay FP64)."""
     L = lp + lq
     n = f"L{L}_p{lp}_q{lq}"
+    kn = ("pq32_" if f32 else "pq_") + n
     ps, qs = hermite_order(lp), hermite_order(lq)
     E = _Emit()
     E(f"// generated by felizjueves_amd/gen_mmd.py -- [p|q] class (lp={lp}, lq={lq})", 0)
     E('#include "../fj_device.h"', 0)
     E("", 0)
-    E(f"__global__ void __launch_bounds__(256) fj_pq_{n}_k(int n_bra, int n_ket, double tau,", 0)
+    E(f"__global__ void __launch_bounds__(256) fj_{kn}_k(int n_bra, int n_ket, double tau,", 0)
     E("const double* __restrict__ P, const double* __restrict__ zeta,")
     E("const double* __restrict__ Ip, const double* __restrict__ Q,")
     E("const double* __restrict__ eta, const double* __restrict__ Iq,")
@@ -309,7 +335,7 @@ def emit_pq(lp: int, lq: int) -> str:
     E("const double PQx = Px - Q[3 * k], PQy = Py - Q[3 * k + 1], PQz = Pz - Q[3 * k + 2];", 2)
     E("const double rho = z * e / (z + e);", 2)
     E("const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);", 2)
-    R = emit_tower(E, L, 2)
+    R = emit_tower(E, L, 2, f32=f32)
     E(f"const double* __restrict__ dq = D + (size_t)k * {len(qs)};", 2)
     for qi in range(len(qs)):
         E(f"const double d{qi} = dq[{qi}];", 2)
@@ -324,7 +350,7 @@ def emit_pq(lp: int, lq: int) -> str:
         E(f"{{ const double v = fj_wave_sum(j{i}); if (lane == 0) J[(size_t)bra * {len(ps)} + {i}] = v; }}")
     E("}", 0)
     E("", 0)
-    E(f'extern "C" int fj_pq_{n}(hipStream_t s, int threads, int n_bra, int n_ket, double tau,', 0)
+    E(f'extern "C" int fj_{kn}(hipStream_t s, int threads, int n_bra, int n_ket, double tau,', 0)
     E("const double* P, const double* zeta, const double* Ip, const double* Q,")
     E("const double* eta, const double* Iq, const double* Dmax, const double* D, double* J) {")
     E("if (threads <= 0) threads = 256;")
@@ -332,7 +358,7 @@ def emit_pq(lp: int, lq: int) -> str:
     E("if (n_bra < 0 || n_ket < 0) return FJ_ERR_SHAPE;")
     E("if (n_bra == 0) return 0;")
     E("const int waves = threads / 64;")
-    E(f"hipLaunchKernelGGL(fj_pq_{n}_k, dim3((n_bra + waves - 1) / waves), dim3(threads), 0, s,")
+    E(f"hipLaunchKernelGGL(fj_{kn}_k, dim3((n_bra + waves - 1) / waves), dim3(threads), 0, s,")
     E("n_bra, n_ket, tau, P, zeta, Ip, Q, eta, Iq, Dmax, D, J);", 2)
     E("return (int)hipGetLastError();")
     E("}", 0)
@@ -520,6 +546,7 @@ def units(lmax: int) -> dict[str, str]:
     for lp in range(2 * lmax + 1):
         for lq in range(2 * lmax + 1):
             out[f"pq_L{lp + lq}_p{lp}_q{lq}.hip"] = emit_pq(lp, lq)
+            out[f"pq32_L{lp + lq}_p{lp}_q{lq}.hip"] = emit_pq(lp, lq, f32=True)
     for c in k_classes(lmax):
         out[f"k_{''.join(map(str, c))}.hip"] = emit_k(*c)
     return out
@@ -554,12 +581,13 @@ def header_decls(lmax: int) -> str:
             f"    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);")
     for lp in range(2 * lmax + 1):
         for lq in range(2 * lmax + 1):
-            out.append(
-                f"int fj_pq_L{lp + lq}_p{lp}_q{lq}(fj_stream_t stream, int threads, int n_bra,"
-                f" int n_ket, double tau,\n"
-                f"    const double* P, const double* zeta, const double* Ip, const double* Q,\n"
-                f"    const double* eta, const double* Iq, const double* Dmax, const double* D,"
-                f" double* J);")
+            for kn in ("pq", "pq32"):
+                out.append(
+                    f"int fj_{kn}_L{lp + lq}_p{lp}_q{lq}(fj_stream_t stream, int threads, int n_bra,"
+                    f" int n_ket, double tau,\n"
+                    f"    const double* P, const double* zeta, const double* Ip, const double* Q,\n"
+                    f"    const double* eta, const double* Iq, const double* Dmax, const double* D,"
+                    f" double* J);")
     for c in k_classes(lmax):
         out.append(
             f"int fj_k_{''.join(map(str, c))}(fj_stream_t stream, int threads, int n_bra, int n_ket,"

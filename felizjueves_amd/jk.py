@@ -122,11 +122,16 @@ def schwarz_matrix(basis: Basis, threads: int = 64):
 
 
 class CoulombChain:
-    """The J chain for one basis; `run(D)` returns J on the device."""
+    """The J chain for one basis; `run(D)` returns J on the device.
 
-    def __init__(self, basis: Basis, tau: float = 0.0):
+    mixed: the J mixed lever (upstream eri/j_coulomb/levers/mixed.py:82-191)
+    -- every [p|q] launch evaluates the Hermite recurrence in FP32
+    (fj_pq32_*), with Boys, D_q, the J_p accumulation and JAB in FP64."""
+
+    def __init__(self, basis: Basis, tau: float = 0.0, mixed: bool = False):
         self.basis = basis
         self.tau = float(tau)
+        self.mixed = bool(mixed)
         self.classes = _pair_classes(basis)
 
     def run(self, D: np.ndarray):
@@ -183,7 +188,8 @@ class CoulombChain:
             for (lc, ld), k in kets.items():
                 lq = lc + ld
                 Jp = torch.zeros((len(z), npp), dtype=torch.float64, device=dev)
-                _call(f"fj_pq_L{lp + lq}_p{lp}_q{lq}", C.c_int(256), C.c_int(len(z)), C.c_int(k["n"]),
+                kn = "pq32" if self.mixed else "pq"
+                _call(f"fj_{kn}_L{lp + lq}_p{lp}_q{lq}", C.c_int(256), C.c_int(len(z)), C.c_int(k["n"]),
                       C.c_double(self.tau), _c(dP), _c(dz), _c(dIp), _c(k["Q"]), _c(k["eta"]),
                       _c(k["Iq"]), _c(k["Dmax"]), _c(k["D"]), _c(Jp))
                 _call(f"fj_jab_{la}{lb}", C.c_int(64), C.c_int(len(z)), C.c_int(b.n_ao),

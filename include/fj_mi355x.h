@@ -659,76 +659,151 @@ int fj_jab_22(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
 int fj_pq_L0_p0_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L0_p0_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L1_p0_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L1_p0_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L2_p0_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L2_p0_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L3_p0_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L3_p0_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L4_p0_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L4_p0_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L1_p1_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L1_p1_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L2_p1_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L2_p1_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L3_p1_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L3_p1_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L4_p1_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L4_p1_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L5_p1_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L5_p1_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L2_p2_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L2_p2_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L3_p2_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L3_p2_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L4_p2_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L4_p2_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L5_p2_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L5_p2_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L6_p2_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L6_p2_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L3_p3_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L3_p3_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L4_p3_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L4_p3_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L5_p3_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L5_p3_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L6_p3_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L6_p3_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L7_p3_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L7_p3_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L4_p4_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L4_p4_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L5_p4_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L5_p4_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L6_p4_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L6_p4_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L7_p4_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L7_p4_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L8_p4_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L8_p4_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_k_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,

@@ -75,3 +75,20 @@ def test_mmd_jk_vs_fused_hgp_config1():
     d = np.abs(G.cpu().numpy() - Gf.cpu().numpy()).max()
     print("config1 MMD G vs fused HGP G", d, flush=True)
     assert d < 1e-10
+
+
+@pytest.mark.parametrize("name", FIX)
+def test_j_chain_mixed_lever_vs_libcint(name):
+    """The J mixed lever (FP32 Hermite recurrence, FP64 boundary) against
+    libcint's vj: within the upstream FP32 tier bar, F32_REL = 1e-12 * 2^29
+    relative to the fixture's scale (upstream test/oracle/tolerance.py:40),
+    and no closer to FP64 than FP32 round-off allows (it is the FP32 path)."""
+    fx = load_fixture(name)
+    J = _host(jk.CoulombChain(fx.basis, mixed=True).run(fx.density))
+    J64 = _host(jk.CoulombChain(fx.basis).run(fx.density))
+    ref = fx.arrays["vj"]
+    scale = max(1.0, np.abs(ref).max())
+    err = np.abs(J - ref).max()
+    print(name, "mixed max |J - vj|", err, "vs fp64 path", np.abs(J - J64).max(), flush=True)
+    assert err < 1e-12 * 2.0 ** 29 * scale
+    assert np.abs(J - J64).max() > 0.0
