@@ -21,7 +21,7 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libfj_mi355x.so"
 ARCH = "gfx950"
-LMAX = 3
+LMAX = 2
 MMD_LMAX = 2
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -55,7 +55,7 @@ def fp32_amplification(basis, p) -> float:
     d = float(np.linalg.norm(basis.shell_centre[p.a] - basis.shell_centre[p.b]))
     z = p.rec[:, 6]
     return (1.0 + d) ** (p.la + p.lb) * math.sqrt(z.max() / z.min())
-FOCK_LMAX = 3  # the top class built: (ff|ff)
+FOCK_LMAX = 2  # the top class built: (dd|dd)
 
 # The primitive-loop split (see FockBuilder._lane_flops): a lane's share of
 # a quartet's primitive loop is at most LANE_FLOPS_MAX algorithmic FLOPs, and

@@ -139,14 +139,6 @@ int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_0003(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0003(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -163,36 +155,12 @@ int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_0013(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0013(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0023(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0023(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0033(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -211,14 +179,6 @@ int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_0103(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0103(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -235,14 +195,6 @@ int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_0113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0113(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -251,36 +203,12 @@ int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_0123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0203(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0203(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -299,92 +227,12 @@ int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_0213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0213(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0303(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0303(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0311(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0311(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0312(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0312(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0313(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0322(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_0333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_0333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -403,36 +251,12 @@ int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_1113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1113(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_1123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_1133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -443,14 +267,6 @@ int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_1213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1213(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -459,100 +275,12 @@ int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
-int fj_f_1223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_1233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_1313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1313(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_1322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1322(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_1323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_1333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_1333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
     const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
     const double* Dsh, const double* D, double* F);
 int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_2223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_2223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_2233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_2233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_2323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_2323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_2333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_2333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
-    double tau, double* ws, long long ws_bytes);
-int fj_f_3333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-    int n_sh, int n_ao, int diag, double tau,
-    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
-    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
-    const double* Dsh, const double* D, double* F);
-int fj_fx_3333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 /* END F CLASSES */
