@@ -1,16 +1,27 @@
 """Benchmark: Fock builds/s and quartets/s on one MI355X, mixed vs FP64.
 
-Workload (default): the 40-atom random cluster (s,s,p,d per atom, n_ao 440,
-tau 1e-12, seed 2), one step = one closed two-electron Fock build
-G = J - K/2 from device-resident pair tables and density.  Timed: the plain
-FP64 build and the mixed build (FP32 tier for the kets whose estimated FP32
-error -- the six-block bound times the HRR amplification, fock.py -- is
-below 1e-11 per tile), K builds each after W warm-up builds.
+Headline workload: the 40-atom random cluster (s,s,p,d per atom, n_ao 440,
+tau 1e-12, seed 2); one step = one closed two-electron Fock build
+G = J - K/2 from device-resident pair tables and density. `value` is the
+production build: the autotuned plan (per-class lanes and primitive-loop
+split, launches dealt to several streams), captured in a HIP graph and
+replayed, mixed precision (FP32 tier for the tiles the amplified bound rule
+and, when a trained selector is present, the learned selector send there).
+Its accuracy against the plain FP64 build rides along, with the plain FP64
+build's own rate (eager, one stream, default plan).
 
-Prints ONE JSON line.  `value` is the mixed build's Fock builds/s; the FP64
-build's figure, the quartet rates, |dE| and max|dG| of mixed vs FP64, the
-roofline of the dominant kernel (HIP events on the launch stream) and the
-CPU baseline (the oracle on the host cores, a bounded sample) ride along.
+Also reported:
+* "c3": the mixed configuration on the three-atom s,s,p,d system -- plain
+  FP64 build vs the production build (tuned, streams, graph, mixed,
+  learned selector), |dE|, max|dG|, FP32 share, the J mixed lever's max|dJ|,
+  and the literal bound rule (FP32 iff Q_ab Q_cd max|D| < 1e-6) for
+  comparison;
+* "c5": learned selector vs the bound rule on held-out systems;
+* "roofline" of the dominant kernel (HIP events around its eager launches
+  on the stream it runs on) and "cpu_baseline" (the oracle on the host
+  cores, a bounded sample).
+
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 
 from __future__ import annotations
@@ -29,58 +40,66 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = ("quartets/s & Fock builds/s, mixed vs FP64, |ΔE|≤1e-6 Eh, 1xMI355X")
 FP64_PEAK_TFLOPS = 78.6
+TUNED = ROOT / "felizjueves_amd" / "tuned"
 
 
-def timed_builds(fb, Dd, steps, warmup, probe=None):
-    """Run warm-up then `steps` timed builds; optionally record HIP events
-    around launch index `probe` of each timed build."""
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup):
+    """Seconds for exactly `steps` calls of fn after `warmup` untimed ones,
+    bracketed by device synchronisations."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t
+
+
+def launch_times(fb, Dd, reps=1):
+    """Per-launch durations (ms) of eager one-stream builds, HIP events on
+    the stream the launches run on (torch's current stream)."""
     import torch
     from felizjueves_amd import native
-    for _ in range(warmup):
-        fb.build(Dd)
-    torch.cuda.synchronize()
-    evs = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        if probe is None:
-            fb.build(Dd)
-            continue
-        # the same launches as FockBuilder.build, with events around `probe`
-        sys = fb._sys([Dd])
+    sys_ = fb._sys([Dd])
+    ws = fb._workspace(0)
+    out = np.zeros((reps, len(fb.launches)))
+    for r in range(reps):
         fb.F.zero_()
         native.density_blocks(fb.shell_ao[0], fb.shell_size[0], Dd, fb.Dsh)
-        for i, L in enumerate(fb.launches):
-            if i == probe:
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-            fb.launch(L, sys, None, fb._workspace(0))
-            if i == probe:
-                e1.record()
-                evs.append((e0, e1))
+        evs = []
+        for L in fb.launches:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fb.launch(L, sys_, None, ws)
+            e1.record()
+            evs.append((e0, e1))
         native.close(fb.F)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms = [a.elapsed_time(b) for a, b in evs]
-    return dt, ms
+        torch.cuda.synchronize()
+        out[r] = [a.elapsed_time(b) for a, b in evs]
+    return out
 
 
-def launch_times(fb, Dd):
-    """Per-launch durations (ms) of one build, HIP events on the stream."""
-    import torch
-    from felizjueves_amd import native
-    sys = fb._sys([Dd])
-    fb.F.zero_()
-    native.density_blocks(fb.shell_ao[0], fb.shell_size[0], Dd, fb.Dsh)
-    evs = []
-    for L in fb.launches:
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        fb.launch(L, sys, None, fb._workspace(0))
-        e1.record()
-        evs.append((e0, e1))
-    native.close(fb.F)
-    torch.cuda.synchronize()
-    return [a.elapsed_time(b) for a, b in evs]
+def load_tuning(name):
+    from felizjueves_amd import autotune
+    p = TUNED / name
+    if not p.exists():
+        return {}, 1
+    cfg = autotune.load(p)
+    return autotune.builder_kwargs(cfg), int(cfg.get("n_streams", 1))
+
+
+def load_selector(path):
+    from felizjueves_amd.selector import LearnedSelector
+    p = Path(path)
+    if not p.with_suffix(".net.npz").exists():
+        return None
+    return LearnedSelector.load(p)
 
 
 def cpu_baseline(fb, D, budget_s=15.0):
@@ -141,14 +160,94 @@ def cpu_baseline(fb, D, budget_s=15.0):
     }
 
 
+def accuracy(D, G, Gref):
+    from felizjueves_amd.fock import energy
+    d = G - Gref
+    return {"max_dG": float(np.abs(d).max()), "dE": abs(energy(D, d))}
+
+
+def production(s, Dd, tuning, n_streams, selector, steps, warmup):
+    """(builder, seconds for `steps` graph replays) of the production build."""
+    from felizjueves_amd.fock import FockBuilder
+    fb = FockBuilder(s.basis, s.D, tau=1e-12, mixed=True, selector=selector, **tuning)
+    fb.capture(Dd, n_streams=n_streams)
+    t = timed(fb.replay, steps, warmup)
+    return fb, t
+
+
+def c3_block(steps, warmup, selector):
+    """The mixed configuration on the three-atom system."""
+    import torch
+    from felizjueves_amd import jk, native
+    from felizjueves_amd.fock import FockBuilder
+    from felizjueves_amd.systems import config1
+    s = config1()
+    Dd = native.to_dev(s.D)
+    plain = FockBuilder(s.basis, s.D, tau=1e-12, lane_flops=1e6)
+    t_plain = timed(lambda: plain.build(Dd), steps, warmup)
+    G64 = plain.F.cpu().numpy()
+    tuning, ns = load_tuning("c1_3atom.json")
+    fb, t_prod = production(s, Dd, tuning, ns, selector, steps, warmup)
+    Gp = fb.replay().cpu().numpy()
+    torch.cuda.synchronize()
+    lit = FockBuilder(s.basis, s.D, tau=1e-12, mixed=True, fp32_bound=1e-6)
+    Gl = lit.build(Dd).cpu().numpy()
+    J64 = jk.CoulombChain(s.basis, tau=1e-12).run(s.D)
+    J32 = jk.CoulombChain(s.basis, tau=1e-12, mixed=True).run(s.D)
+    torch.cuda.synchronize()
+    out = {
+        "workload": s.name, "n_ao": s.basis.n_ao, "quartets": plain.n_quartets,
+        "plain_fp64_builds_per_s": steps / t_plain,
+        "production_builds_per_s": steps / t_prod,
+        "speedup_vs_plain_fp64": t_plain / t_prod,
+        "production": {"n_streams": ns, "graph": True, "tuned": bool(tuning),
+                       "selector": selector is not None, "fp32_share": fb.fp32_share(),
+                       **accuracy(s.D, Gp, G64)},
+        "literal_bound_rule_1e-6": {"fp32_share": lit.fp32_share(), **accuracy(s.D, Gl, G64)},
+        "j_mixed_lever_max_dJ": float((J32 - J64).abs().max().item()),
+        "k_mixed": "none: K runs in FP64 (no FP32 K kernel)",
+    }
+    log("c3", json.dumps({k: v for k, v in out.items() if k not in ("production", "literal_bound_rule_1e-6")}))
+    return out
+
+
+def c5_block(steps, warmup, selector):
+    """Learned selector vs the bound rule on systems outside its training
+    set (the three-atom system, seed 1, and a 10-atom cluster, seed 11)."""
+    import torch
+    from felizjueves_amd import native
+    from felizjueves_amd.fock import FockBuilder
+    from felizjueves_amd.systems import config1, config2
+    if selector is None:
+        return None
+    out = {}
+    for s in (config1(), config2(seed=11, n_atoms=10)):
+        Dd = native.to_dev(s.D)
+        ref = FockBuilder(s.basis, s.D, tau=1e-12)
+        G64 = ref.build(Dd).cpu().numpy()
+        row = {}
+        for name, sel in (("bound_rule", None), ("learned", selector)):
+            fb = FockBuilder(s.basis, s.D, tau=1e-12, mixed=True, selector=sel)
+            fb.capture(Dd, n_streams=4)
+            t = timed(fb.replay, steps, warmup)
+            G = fb.replay().cpu().numpy()
+            torch.cuda.synchronize()
+            row[name] = {"fp32_share": fb.fp32_share(), "builds_per_s": steps / t,
+                         **accuracy(s.D, G, G64)}
+        out[s.name] = row
+    log("c5", json.dumps(out))
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--atoms", type=int, default=40)
-    ap.add_argument("--threads", type=int, default=64)
+    ap.add_argument("--selector", default=str(TUNED / "selector"))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the c3 / c5 blocks")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the Fock build is a "
@@ -157,36 +256,42 @@ def main() -> None:
 
     import torch
     from felizjueves_amd import native
-    from felizjueves_amd.fock import FockBuilder, energy, FP64
+    from felizjueves_amd.fock import FockBuilder
     from felizjueves_amd.systems import config2
 
+    selector = load_selector(a.selector)
     s = config2(n_atoms=a.atoms)
     D = s.D
-    fb64 = FockBuilder(s.basis, D, tau=1e-12, threads=a.threads)
-    fbmx = FockBuilder(s.basis, D, tau=1e-12, threads=a.threads, mixed=True)
     Dd = native.to_dev(D)
-    print(f"[bench] {s.name}: n_ao {s.basis.n_ao}, quartets {fb64.n_quartets:.3e}, "
-          f"primitive quartets {fb64.n_prim_quartets:.3e}, {fb64.flops / 1e9:.1f} GFLOP/build, "
-          f"plan {fb64.plan_seconds:.1f}s", file=sys.stderr, flush=True)
+    plain = FockBuilder(s.basis, D, tau=1e-12)
+    log(f"{s.name}: n_ao {s.basis.n_ao}, quartets {plain.n_quartets:.3e}, primitive quartets "
+        f"{plain.n_prim_quartets:.3e}, {plain.flops / 1e9:.1f} GFLOP/build, plan {plain.plan_seconds:.1f}s")
 
-    # dominant kernel of the FP64 build
-    fb64.build(Dd)
+    # roofline of the dominant kernel: eager one-stream launches, events
+    plain.build(Dd)
     torch.cuda.synchronize()
-    lt = launch_times(fb64, Dd)
-    probe = int(np.argmax(lt))
-    Lp = fb64.launches[probe]
+    lt = launch_times(plain, Dd, reps=3)
+    probe = int(np.argmax(lt.mean(0)))
+    Lp = plain.launches[probe]
+    avg_ms = float(lt[:, probe].mean())
 
-    t64, ms_probe = timed_builds(fb64, Dd, a.steps, a.warmup, probe=probe)
-    G64 = fb64.F.cpu().numpy()
-    print(f"[bench] fp64: {a.steps / t64:.3f} builds/s", file=sys.stderr, flush=True)
-    tmx, _ = timed_builds(fbmx, Dd, a.steps, a.warmup)
-    Gmx = fbmx.F.cpu().numpy()
-    print(f"[bench] mixed: {a.steps / tmx:.3f} builds/s (fp32 share {fbmx.fp32_share():.3f})",
-          file=sys.stderr, flush=True)
+    t_plain = timed(lambda: plain.build(Dd), a.steps, a.warmup)
+    G64 = plain.F.cpu().numpy()
+    log(f"plain fp64: {a.steps / t_plain:.3f} builds/s")
 
-    dE = abs(energy(D, Gmx) - energy(D, G64))
-    dG = float(np.abs(Gmx - G64).max())
-    avg_ms = float(np.mean(ms_probe))
+    tuning, ns = load_tuning("c2_40atom.json") if a.atoms == 40 else ({}, 4)
+    fb64 = FockBuilder(s.basis, D, tau=1e-12, **tuning)
+    fb64.capture(Dd, n_streams=ns)
+    t_t64 = timed(fb64.replay, a.steps, a.warmup)
+    log(f"tuned fp64 graph: {a.steps / t_t64:.3f} builds/s")
+
+    fbp, t_prod = production(s, Dd, tuning, ns, selector, a.steps, a.warmup)
+    Gp = fbp.replay().cpu().numpy()
+    torch.cuda.synchronize()
+    acc = accuracy(D, Gp, G64)
+    log(f"production (mixed, tuned, {ns} streams, graph): {a.steps / t_prod:.3f} builds/s, "
+        f"fp32 share {fbp.fp32_share():.4f}, {acc}")
+
     achieved = Lp.flops / (avg_ms * 1e-3) / 1e12
     roofline = {
         "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -194,28 +299,34 @@ def main() -> None:
         "kernel": f"fj_f_{''.join(map(str, Lp.cls))}_k<double>",
         "note": ("compute-bound on FP64 VALU (no MFMA in this kernel); 78.6 TFLOP/s is MI355X's FP64 "
                  "vector peak (equal to its FP64 matrix peak); achieved = algorithmic FLOPs of the "
-                 "launch (upstream hgp/cost.py convention, no FMA credit) / mean launch time"),
+                 "launch (hgp.Cost: VRR + boundary per primitive quartet, HRR + digestion per "
+                 "quartet, no FMA credit) / mean duration of its eager launches (default plan, one "
+                 "stream, HIP events on that stream)"),
         "launch_ms": avg_ms, "launch_gflop": Lp.flops / 1e9,
     }
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline(fb64, D)
+    extra = {}
+    if not a.no_extra:
+        extra["c3"] = c3_block(max(10, a.steps), a.warmup, selector)
+        extra["c5"] = c5_block(max(5, a.steps // 2), a.warmup, selector)
+    cpu = None if a.no_cpu else cpu_baseline(plain, D)
     out = {
-        "metric": METRIC, "value": a.steps / tmx, "unit": "Fock builds/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * tmx / a.steps,
+        "metric": METRIC, "value": a.steps / t_prod, "unit": "Fock builds/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * t_prod / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+f32-tier", "data": "synthetic",
+        "dtype": "f64 (f32 VRR tier on selected tiles)", "data": "synthetic",
         "config": {"workload": s.name, "n_atoms": a.atoms, "n_ao": s.basis.n_ao, "tau": 1e-12,
-                   "fp32_error_per_tile": fbmx.fp32_error, "threads_per_workgroup": a.threads},
-        "fp64_builds_per_s": a.steps / t64, "fp64_ms_per_build": 1e3 * t64 / a.steps,
-        "quartets_per_s_mixed": fbmx.n_quartets * a.steps / tmx,
-        "quartets_per_s_fp64": fb64.n_quartets * a.steps / t64,
-        "prim_quartets_per_s_fp64": fb64.n_prim_quartets * a.steps / t64,
-        "quartets_per_build": fb64.n_quartets, "gflop_per_build": fb64.flops / 1e9,
-        "fp64_tflops_algorithmic": fb64.flops * a.steps / t64 / 1e12,
-        "mixed_speedup_vs_fp64": t64 / tmx, "fp32_share_quartets": fbmx.fp32_share(),
-        "dE_mixed_vs_fp64": dE, "max_dG_mixed_vs_fp64": dG,
-        "roofline": roofline, "cpu_baseline": cpu,
+                   "fp32_error_per_tile": fbp.fp32_error, "n_streams": ns, "graph": True,
+                   "tuned": bool(tuning), "selector": selector is not None},
+        "fp64_plain_builds_per_s": a.steps / t_plain, "fp64_plain_ms_per_build": 1e3 * t_plain / a.steps,
+        "fp64_tuned_graph_builds_per_s": a.steps / t_t64,
+        "quartets_per_s": plain.n_quartets * a.steps / t_prod,
+        "quartets_per_s_fp64_plain": plain.n_quartets * a.steps / t_plain,
+        "prim_quartets_per_s": plain.n_prim_quartets * a.steps / t_prod,
+        "quartets_per_build": plain.n_quartets, "gflop_per_build": plain.flops / 1e9,
+        "tflops_algorithmic": plain.flops * a.steps / t_prod / 1e12,
+        "speedup_vs_fp64_plain": t_plain / t_prod, "fp32_share_quartets": fbp.fp32_share(),
+        "dE_vs_fp64": acc["dE"], "max_dG_vs_fp64": acc["max_dG"],
+        **extra, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

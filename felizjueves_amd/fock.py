@@ -115,9 +115,9 @@ class FockBuilder:
 
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads=64,
                  mixed: bool = False, fp32_error: float = 1e-11, selector=None,
-                 lane_flops=None):
+                 lane_flops=None, fp32_bound: float | None = None):
         self._setup([(basis, D)], tau=tau, threads=threads, mixed=mixed, fp32_error=fp32_error,
-                     selector=selector, lane_flops=lane_flops)
+                     selector=selector, lane_flops=lane_flops, fp32_bound=fp32_bound)
 
     @classmethod
     def batch(cls, systems, **kw) -> "FockBuilder":
@@ -129,10 +129,16 @@ class FockBuilder:
         return self
 
     def _setup(self, systems, *, tau: float = 1e-12, threads=64, mixed: bool = False,
-               fp32_error: float = 1e-11, selector=None, lane_flops=None):
+               fp32_error: float = 1e-11, selector=None, lane_flops=None,
+               fp32_bound: float | None = None):
         """threads, lane_flops: one value for every class pair, or a dict
         {(la, lb, lc, ld): value} (the autotuner's choice per class pair;
-        missing ones take the default 64 and the automatic split)."""
+        missing ones take the default 64 and the automatic split).
+        fp32_bound: instead of the amplified estimate, the plain rule "FP32
+        iff G_ab G_cd dmax < fp32_bound" (the rule the working-size mixed
+        configuration is specified with; not accurate enough for the FP64
+        G bars, kept for comparison)."""
+        self.fp32_bound = fp32_bound
         if not systems:
             raise ValueError("no systems")
         if len(systems) > 0xFFFF:
@@ -253,8 +259,11 @@ class FockBuilder:
                     ks = kl + lo
                     bound, dmax = bound[kl], dmax[kl]
                     if self.mixed:
-                        est = bound * bra.amp[b] * ket.amp[ks] * FP32_EST_SCALE
-                        small = est < self.fp32_error
+                        if self.fp32_bound is not None:
+                            small = bound < self.fp32_bound
+                        else:
+                            est = bound * bra.amp[b] * ket.amp[ks] * FP32_EST_SCALE
+                            small = est < self.fp32_error
                         parts = ((FP64, ks[~small], dmax[~small]), (FP32, ks[small], dmax[small]))
                     else:
                         parts = ((FP64, ks, dmax),)

@@ -162,10 +162,11 @@ def collect(systems, max_items: int = 20000, seed: int = 0, log=None):
     return np.concatenate(Xs), np.concatenate(ys)
 
 
-def train_selector(X: np.ndarray, y: np.ndarray, budget: float = 1e-11, quantile: float = 0.9999,
-                   epochs: int = 40, seed: int = 0, log=None) -> LearnedSelector:
+def train_selector(X: np.ndarray, y: np.ndarray, budget: float = 1e-11, quantile: float = 0.999,
+                   epochs: int = 60, seed: int = 0, log=None) -> LearnedSelector:
     """Train the network on the device (90/10 split), set the margin from
-    the held-out residuals, return the selector."""
+    the held-out residuals of the tiles predicted within three decades of
+    the budget (the ones the decision depends on), return the selector."""
     from .mlp import MLP
     rng = np.random.default_rng(seed)
     idx = rng.permutation(len(X))
@@ -173,9 +174,40 @@ def train_selector(X: np.ndarray, y: np.ndarray, budget: float = 1e-11, quantile
     va, tr = idx[:n_val], idx[n_val:]
     net = MLP(N_FEATURES, seed=seed)
     net.fit(X[tr], y[tr], epochs=epochs, batch=4096, lr=2e-3, seed=seed, log=log)
-    r = y[va] - net.predict(X[va])
-    margin = float(np.quantile(r, quantile))
+    pred = net.predict(X[va])
+    r = y[va] - pred
+    near = pred < np.log10(budget) + 3.0
+    margin = float(np.quantile(r[near] if near.sum() >= 1000 else r, quantile))
     classes = {tuple(int(v) for v in row) for row in np.unique(X[:, :4], axis=0)}
     sel = LearnedSelector(net, budget, margin, X.min(0), X.max(0), classes)
     sel.val_rmse = float(np.sqrt(np.mean(r * r)))
     return sel
+
+
+def main() -> None:
+    """Generate training data on the device (FP32 vs FP64 per tile of seeded
+    molecules), train the network with the library's kernels, save."""
+    import argparse
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--systems", type=int, default=200)
+    ap.add_argument("--max-items", type=int, default=20000)
+    ap.add_argument("--budget", type=float, default=1e-11)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--data", default=None, help="also save the (features, log error) pairs here")
+    a = ap.parse_args()
+    t = time.time()
+    X, y = collect(training_systems(a.systems), max_items=a.max_items,
+                   log=lambda i, s, n: print(f"[selector] probe {i} {s.name} tiles {n} "
+                                             f"{time.time() - t:.0f}s", flush=True))
+    if a.data:
+        np.savez(a.data, X=X, y=y)
+    print(f"[selector] {len(X)} pairs in {time.time() - t:.0f}s", flush=True)
+    sel = train_selector(X, y, budget=a.budget,
+                         log=lambda e, l: print(f"[selector] epoch {e} loss {l:.4f}", flush=True))
+    print(f"[selector] val rmse {sel.val_rmse:.3f} margin {sel.margin:.3f}", flush=True)
+    sel.save(a.out)
+
+
+if __name__ == "__main__":
+    main()
