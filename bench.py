@@ -248,6 +248,9 @@ def main() -> None:
     ap.add_argument("--selector", default=str(TUNED / "selector"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the c3 / c5 blocks")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for HBM counter runs: one elementwise copy of 512 MiB (the counters' "
+                         "scale) and one eager build of the default plan, then exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the Fock build is a "
@@ -258,6 +261,22 @@ def main() -> None:
     from felizjueves_amd import native
     from felizjueves_amd.fock import FockBuilder
     from felizjueves_amd.systems import config2
+
+    if a.traffic_probe:
+        s = config2(n_atoms=a.atoms)
+        Dd = native.to_dev(s.D)
+        n = 64 << 20
+        src = torch.ones(n, dtype=torch.float64, device=Dd.device)
+        dst = torch.empty_like(src)
+        torch.add(src, 0.0, out=dst)          # reads and writes 512 MiB
+        plain = FockBuilder(s.basis, s.D, tau=1e-12)
+        plain.build(Dd)
+        torch.cuda.synchronize()
+        lt = launch_times(plain, Dd, reps=1)
+        Lp = plain.launches[int(np.argmax(lt[0]))]
+        print(json.dumps({"copy_bytes_each_way": n * 8, "dominant": f"fj_f_{''.join(map(str, Lp.cls))}_k",
+                          "dominant_gflop": Lp.flops / 1e9}), flush=True)
+        return
 
     selector = load_selector(a.selector)
     s = config2(n_atoms=a.atoms)

@@ -42,6 +42,11 @@ struct FjFArgs {
 // (allocated on first use, grown when asked for more; NULL on failure).
 double* fj_library_workspace(long long bytes);
 
+// An empty asm statement that the compiler must assume redefines the
+// per-lane pointer p (a VGPR pair) and the uniform stride s (an SGPR pair):
+// values derived from them are recomputed after it, not hoisted above it.
+#define FJ_LAUNDER(p, s) __asm__ volatile("" : "+v"(p), "+s"(s))
+
 // First primitive record and degree of pair i of one side.
 __device__ __forceinline__ void fj_side_range(const FjSide& s, int i, int& k0, int& k) {
     if (s.off) {

@@ -245,7 +245,14 @@ def emit_class(cls) -> str:
             "    const size_t st = (size_t)gridDim.x * blockDim.x;\n"
             "    double* ws = a.ws + (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
             "    for (int it = (int)blockIdx.x; it < a.n_items; it += (int)gridDim.x) {\n"
-            f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, ws, st);\n"
+            "        // an empty asm that 'redefines' the slot pointer and stride each\n"
+            "        // item, so the compiler recomputes the workspace addresses where\n"
+            "        // they are used instead of hoisting thousands of them out of the\n"
+            "        // item loop (which spills)\n"
+            "        double* wsi = ws;\n"
+            "        size_t sti = st;\n"
+            "        FJ_LAUNDER(wsi, sti);\n"
+            f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti);\n"
             "        __syncthreads();\n"
             "    }")
         launch_ws = (
@@ -704,11 +711,14 @@ def emit_quartet_staged(cls) -> str:
 
     E = _Emit()
     W = ws_doubles(cls)
+    # f classes: (e, f) groups, each its own non-inlined function (registers
+    # allocated per group); d classes: groups of bra components inline, with
+    # the primitive records read straight from LDS
+    fns = max(cls) >= 3
     if W:
-        # (e, f) groups: each its own non-inlined function, registers allocated
-        # per group; the boundary lives in the lane's workspace slice
         from .hgp import ef_groups
-        tgroups = [list(g) for g in ef_groups(tuple(cls), EF_LIVE_MAX)]
+        tgroups = ([list(g) for g in ef_groups(tuple(cls), EF_LIVE_MAX)] if fns else
+                   [[(e, f) for e, f in v.targets if e in g] for g in groups])
         bnd_ref = lambda i: f"ws[(size_t)({i}) * st]"  # noqa: E731
         mid_ref = lambda i: f"ws[(size_t)({NF * NE} + {i}) * st]"  # noqa: E731
     else:
@@ -798,7 +808,7 @@ def emit_quartet_staged(cls) -> str:
         for i, (e, f) in enumerate(tg):
             E(f"{bnd_ref(fi[f] * NE + ei[e])} = double(c{i});", ind)
 
-    if W:
+    if W and fns:
         for gi, tg in enumerate(tgroups):
             E("template <typename R>", 0)
             E(f"__device__ __noinline__ void fj_g{gi}_{name}(const double* __restrict__ bp, int k_ab,", 0)
@@ -815,10 +825,12 @@ def emit_quartet_staged(cls) -> str:
     E("double* __restrict__ fab, double* __restrict__ F, double* __restrict__ ws, size_t st) {", 1)
     if W:
         E(f"// (e0|f0) [f][e], (ab|f0) [ab][f] and F_cd in the workspace ({W} doubles per lane)")
+    if W and fns:
         for gi in range(len(tgroups)):
             E(f"fj_g{gi}_{name}<R>(bp, k_ab, kp, k_cd, ws, st);")
     else:
-        E(f"double bnd[{NF * NE}];   // contracted (e0|f0), [f][e]")
+        if not W:
+            E(f"double bnd[{NF * NE}];   // contracted (e0|f0), [f][e]")
         for gi, tg in enumerate(tgroups):
             E(f"{{   // group {gi}: e = {sorted(set(e for e, _ in tg))}")
             group_body(tg, 2)
@@ -832,15 +844,17 @@ def emit_quartet_staged(cls) -> str:
     else:
         E("#pragma unroll 1", 0)
         E(f"for (int f = 0; f < {NF}; ++f) {{")
-        _emit_side_hrr_dfs(E, la, lb, "AB", lambda i: bnd_ref(f"f * {NE} + {i}"),
-                           lambda j, nm: E(f"{mid_ref(f'{j} * {NF} + f')} = {nm};", 2), 2)
+        hrr = _emit_side_hrr_dfs if fns else _emit_side_hrr
+        hrr(E, la, lb, "AB", lambda i: bnd_ref(f"f * {NE} + {i}"),
+            lambda j, nm: E(f"{mid_ref(f'{j} * {NF} + f')} = {nm};", 2), 2)
         E("}")
     # phase C: ket HRR per ab + digestion, each ERI digested as it is made.
     # F_cd accumulates over ab: in registers, or (workspace classes) in the
     # lane's workspace slice with its density weights read from D.
     NCD = nc * nd
+    fcd_ws = bool(W) and fns   # f classes: F_cd too large for registers
     fcd_ref = lambda j: f"ws[(size_t)({NF * NE + NAB * NF} + {j}) * st]"  # noqa: E731
-    if W:
+    if fcd_ws:
         E("#pragma unroll 1", 0)
         E(f"for (int j = 0; j < {NCD}; ++j) {fcd_ref('j')} = 0.0;")
     else:
@@ -866,14 +880,15 @@ def emit_quartet_staged(cls) -> str:
 
     def digest(j, x):
         c, d = divmod(j, nd)
-        if W:
+        if fcd_ws:
             E(f"sab += (s_j * D[(oc + {c}) * n_ao + od + {d}]) * {x}; "
               f"{fcd_ref(j)} += w_ab * {x};", 2)
         else:
             E(f"sab += wcd{c}_{d} * {x}; fcd{c}_{d} += w_ab * {x};", 2)
         E(f"fac{c} += dbd{d} * {x}; fad{d} += dbc{c} * {x}; "
           f"fbc{c} += dad{d} * {x}; fbd{d} += dac{c} * {x};", 2)
-    _emit_side_hrr_dfs(E, lc, ld, "CD", lambda i: mid_ref(f"ab * {NF} + {i}"), digest, 2)
+    (_emit_side_hrr_dfs if fns else _emit_side_hrr)(
+        E, lc, ld, "CD", lambda i: mid_ref(f"ab * {NF} + {i}"), digest, 2)
     E("fab[ab] += sab;", 2)
     for c in range(nc):
         E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + oc + {c}, fac{c});", 2)
@@ -882,7 +897,7 @@ def emit_quartet_staged(cls) -> str:
         E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + od + {d}, fad{d});", 2)
         E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + od + {d}, fbd{d});", 2)
     E("}")
-    if W:
+    if fcd_ws:
         E("#pragma unroll 1", 0)
         E(f"for (int j = 0; j < {NCD}; ++j)")
         E(f"unsafeAtomicAdd(F + (size_t)(oc + j / {nd}) * n_ao + od + j % {nd}, {fcd_ref('j')});", 2)
