@@ -323,6 +323,13 @@ def main() -> None:
                  "stream, HIP events on that stream)"),
         "launch_ms": avg_ms, "launch_gflop": Lp.flops / 1e9,
     }
+    tr = TUNED / f"traffic_c2_{a.atoms}atom.json"
+    if tr.exists():
+        t = json.loads(tr.read_text())
+        if t.get("kernel") == roofline["kernel"]:
+            roofline["traffic"] = t["traffic_bytes"]
+            roofline["traffic_note"] = (t["source"] + f", scaled by a 512 MiB copy (fetch x{t['fetch_scale']:.3f}, "
+                                        f"write x{t['write_scale']:.3f}); mostly the launch workspace")
     extra = {}
     if not a.no_extra:
         extra["c3"] = c3_block(max(10, a.steps), a.warmup, selector)
