@@ -42,7 +42,7 @@ def sources(lmax: int = LMAX) -> list[Path]:
     return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", *files, *mmd]
 
 
-def _compile(src: Path, extra: list[str]) -> tuple[Path, float, str]:
+def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, float, str]:
     OBJ.mkdir(exist_ok=True)
     obj = OBJ / (src.stem + ".o")
     deps = [src, CSRC / "fj_device.h", HERE.parent / "include" / "fj_mi355x.h",
@@ -50,7 +50,8 @@ def _compile(src: Path, extra: list[str]) -> tuple[Path, float, str]:
     hdr = src.parent / (src.stem.replace("_r32", "") + ".h")
     if hdr.exists():
         deps.append(hdr)
-    if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps if d.exists()):
+    if not force and obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime
+                                          for d in deps if d.exists()):
         return obj, 0.0, ""
     t = time.time()
     r = subprocess.run([HIPCC, *FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage", "-c",
@@ -62,20 +63,22 @@ def _compile(src: Path, extra: list[str]) -> tuple[Path, float, str]:
 
 
 def build(lmax: int = LMAX, jobs: int | None = None, extra: list[str] | None = None,
-          verbose: bool = True) -> Path:
+          verbose: bool = True, force: bool = False) -> Path:
+    """Generate, compile (every unit when `force`, else the out-of-date ones)
+    and link the library; refuse it if a kernel needs too much scratch."""
     srcs = sources(lmax)
     jobs = jobs or min(16, os.cpu_count() or 4)
     t0 = time.time()
     # biggest units first so the long compiles start early
     srcs.sort(key=lambda p: -p.stat().st_size)
     with ThreadPoolExecutor(jobs) as ex:
-        res = list(ex.map(lambda s: _compile(s, extra or []), srcs))
+        res = list(ex.map(lambda s: _compile(s, extra or [], force), srcs))
     if verbose:
         slow = sorted(res, key=lambda r: -r[1])[:3]
         print(f"[fj build] {len(res)} units in {time.time() - t0:.1f}s; slowest: " +
               ", ".join(f"{p.name} {dt:.1f}s" for p, dt, _ in slow), flush=True)
     objs = [str(p) for p, _, _ in res]
-    check_resources()
+    check_resources(objs)
     tmp = LIB.with_suffix(".so.tmp")
     r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs],
                        capture_output=True, text=True)
@@ -90,12 +93,16 @@ def build(lmax: int = LMAX, jobs: int | None = None, extra: list[str] | None = N
 SCRATCH_MAX = 6 * 1024
 
 
-def resources() -> dict:
-    """{kernel: (VGPRs, scratch bytes per lane, occupancy)} of the last
-    build, from hipcc's kernel resource remarks."""
+def resources(objs=None) -> dict:
+    """{kernel: (VGPRs, scratch bytes per lane, occupancy)} from hipcc's
+    kernel resource remarks of the given objects (default: every one in
+    the build folder)."""
     import re
     out = {}
-    for f in sorted(OBJ.glob("*.res")):
+    files = [Path(o).with_suffix(".res") for o in objs] if objs is not None else sorted(OBJ.glob("*.res"))
+    for f in files:
+        if not f.exists():
+            continue
         name = None
         cur: dict = {}
         for line in f.read_text().splitlines():
@@ -113,10 +120,10 @@ def resources() -> dict:
     return out
 
 
-def check_resources() -> None:
+def check_resources(objs=None) -> None:
     """Refuse a build in which any kernel asks for more private scratch per
     lane than SCRATCH_MAX (the runtime fails such launches)."""
-    res = resources()
+    res = resources(objs)
     big = {k: v["ScratchSize"] for k, v in res.items() if v.get("ScratchSize", 0) > SCRATCH_MAX}
     if big:
         raise RuntimeError(f"kernels over {SCRATCH_MAX} B of scratch per lane: {big}")
