@@ -334,34 +334,40 @@ class FockBuilder:
             lists = {FP64: [], FP32: []}
             n_list = {FP64: 0, FP32: 0}
             stats = {FP64: [0, 0, 0.0], FP32: [0, 0, 0.0]}
+            # every tile of one bra pair and tier is expanded into lanes
+            # together: (ket, primitive chunk), chunks of one ket equal in
+            # size, lanes sorted by primitive count so that each work item's
+            # lanes loop the same number of times
+            by_bra: dict = {}
             for b, seg, tr in tiles:
+                by_bra.setdefault((b, tr), []).append(seg)
+            for (b, tr), segs in by_bra.items():
+                seg = np.concatenate(segs)
                 kab = int(bra.deg[b])
-                kcd = ket.deg[seg]
+                kcd = ket.deg[seg].astype(np.int64)
                 # split the primitive loop so that no lane's share exceeds
                 # lane_flops: bra primitive pairs per work item (cb), then
                 # ket primitive pairs per lane (ck)
                 cb = kab if kab * c.per_prim <= lane_flops else max(1, int(lane_flops // c.per_prim))
-                ck = max(1, int(lane_flops // (cb * c.per_prim)))
-                lanes = []
-                lane_prims = []
-                for k, n in zip(seg, kcd):
-                    if n <= ck:
-                        lanes.append(int(k))
-                        lane_prims.append(int(n))
-                    else:
-                        for lo in range(0, int(n), ck):
-                            cnt = 0 if lo + ck >= n else ck
-                            lanes.append(int(k) | (lo << 17) | (cnt << 24))
-                            lane_prims.append(min(ck, int(n) - lo))
-                lanes = np.array(lanes, dtype=np.int64)
-                lane_prims = np.array(lane_prims)
+                ck = max(1, min(P.MAX_PRIM_PAIRS, int(lane_flops // (cb * c.per_prim))))
+                n_ch = -(-kcd // ck)                       # chunks per ket
+                size = -(-kcd // n_ch)                     # equal chunk size
+                rep = np.repeat(np.arange(len(seg)), n_ch)
+                j = np.arange(len(rep)) - np.repeat(np.cumsum(n_ch) - n_ch, n_ch)
+                lo = j * size[rep]
+                cnt = np.minimum(size[rep], kcd[rep] - lo)
+                whole = n_ch[rep] == 1
+                enc = np.where(whole, 0, np.where(lo + cnt >= kcd[rep], 0, cnt))
+                lanes = seg[rep].astype(np.int64) | (np.where(whole, 0, lo) << 17) | (enc << 24)
+                order = np.argsort(-cnt, kind="stable")
+                lanes, lane_prims = lanes[order], cnt[order]
                 st = n_list[tr]
                 lists[tr].append(lanes)
                 n_list[tr] += len(lanes)
+                m = int(bra.mol[b])
                 for t0 in range(0, len(lanes), T):
                     t1 = min(len(lanes), t0 + T)
-                    pmax = int(lane_prims[t0:t1].max())
-                    m = int(bra.mol[b])
+                    pmax = int(lane_prims[t0])
                     for blo in range(0, kab, cb):
                         bn = 0 if cb >= kab else min(cb, kab - blo)
                         work[tr].append((b, st + t0, m | (blo << 16) | (bn << 24), st + t1))
