@@ -601,7 +601,12 @@ FUSED_LIVE_MAX = 120
 # workspace slot per resident lane, the slots interleaved so that a
 # wavefront's accesses coalesce, and each of its (e, f) VRR groups is a
 # separate non-inlined function so registers are allocated per group.
-EF_LIVE_MAX = 120   # live values per (e, f) VRR group of a workspace class
+EF_LIVE_MAX = 120   # live values per (e, f) VRR group of an f class
+# live values per bra-component group of a d class: measured on MI355X
+# (40-atom build, eager): 100 -> 98.2 ms, 220 -> 92.0 ms, 300 -> 121 ms
+# (pairs of components halve the Boys evaluations and cut the recompute
+# from 1.75x to 1.62x at (dd|dd); larger groups spill)
+E_LIVE_MAX = 220
 
 
 def ws_doubles(cls) -> int:
@@ -707,7 +712,7 @@ def emit_quartet_staged(cls) -> str:
     NAB, NCD = na * nb, nc * nd
     ei = {e: i for i, e in enumerate(eR)}
     fi = {f: i for i, f in enumerate(fR)}
-    groups = e_groups(tuple(cls))
+    groups = e_groups(tuple(cls), E_LIVE_MAX)
 
     E = _Emit()
     W = ws_doubles(cls)
