@@ -37,10 +37,6 @@ struct FjFArgs {
 // sized for FJ_LIB_WS_BLOCKS.
 #define FJ_WS_MAX_BLOCKS 1024
 #define FJ_LIB_WS_BLOCKS 256
-// A workspace starts with FJ_WS_HEAD doubles of bookkeeping: two int
-// counters (the next work item, the workgroups done) that must be zero when
-// the workspace is first used; every launch leaves them zero again.
-#define FJ_WS_HEAD 32
 
 // The library-owned workspace of at least `bytes` on the current device
 // (allocated on first use, grown when asked for more; NULL on failure).

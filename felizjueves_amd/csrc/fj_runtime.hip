@@ -27,7 +27,6 @@ double* fj_library_workspace(long long bytes) {
         }
         void* p = nullptr;
         if (hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;
-        if (hipMemset(p, 0, 8 * FJ_WS_HEAD) != hipSuccess) return nullptr;   // the item counters
         buf[dev] = static_cast<double*>(p);
         have[dev] = bytes;
     }

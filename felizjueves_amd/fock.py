@@ -453,8 +453,7 @@ class FockBuilder:
         need = min(WS_MAX_BYTES, max([native.workspace_bytes(L.cls, L.threads, L.work.shape[0])
                                       for L in self.launches] + [0]))
         while len(ws) <= i:
-            # zeroed once: the head holds the persistent kernels' work counters
-            ws.append(torch.zeros(need // 8, dtype=torch.float64, device=native.device())
+            ws.append(torch.empty(need // 8, dtype=torch.float64, device=native.device())
                       if need else None)
         return ws[i]
 

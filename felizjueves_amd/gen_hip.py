@@ -241,19 +241,10 @@ def emit_class(cls) -> str:
     W = ws_doubles(cls)
     if W:
         kernel_body = (
-            "    // persistent: one workspace slot per lane of the grid; work items\n"
-            "    // are taken from a counter at the head of the workspace (the\n"
-            "    // costliest first, as listed), and the last workgroup to finish\n"
-            "    // resets the counters for the next launch on this workspace\n"
-            "    int* ctr = reinterpret_cast<int*>(a.ws);\n"
-            "    __shared__ int s_item;\n"
+            "    // persistent: one workspace slot per lane of the grid, items strided\n"
             "    const size_t st = (size_t)gridDim.x * blockDim.x;\n"
-            "    double* ws = a.ws + FJ_WS_HEAD + (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
-            "    for (;;) {\n"
-            "        if (threadIdx.x == 0) s_item = atomicAdd(ctr, 1);\n"
-            "        __syncthreads();\n"
-            "        const int it = s_item;\n"
-            "        if (it < 0 || it >= a.n_items) break;   // uniform: every thread read the same s_item\n"
+            "    double* ws = a.ws + (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
+            "    for (int it = (int)blockIdx.x; it < a.n_items; it += (int)gridDim.x) {\n"
             "        // an empty asm that 'redefines' the slot pointer and stride each\n"
             "        // item, so the compiler recomputes the workspace addresses where\n"
             "        // they are used instead of hoisting thousands of them out of the\n"
@@ -262,26 +253,18 @@ def emit_class(cls) -> str:
             "        size_t sti = st;\n"
             "        FJ_LAUNDER(wsi, sti);\n"
             f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti);\n"
-            "        __syncthreads();   // s_item is rewritten only after every thread is done with it\n"
-            "    }\n"
-            "    if (threadIdx.x == 0) {\n"
-            "        __threadfence();\n"
-            "        if (atomicAdd(ctr + 1, 1) == (int)gridDim.x - 1) {\n"
-            "            atomicExch(ctr, 0);\n"
-            "            atomicExch(ctr + 1, 0);\n"
-            "        }\n"
+            "        __syncthreads();\n"
             "    }")
         launch_ws = (
-            f"    // workspace: a head of FJ_WS_HEAD doubles (the item counters), then\n"
-            f"    // {W} doubles per lane; the grid is cut to the slots it holds\n"
+            f"    // workspace: {W} doubles per lane; the grid is cut to the slots it holds\n"
             f"    {{\n"
-            f"        const long long slots = a.ws ? (a.ws_doubles - FJ_WS_HEAD) / {W} : 0;\n"
+            f"        const long long slots = a.ws ? a.ws_doubles / {W} : 0;\n"
             f"        const long long blocks = slots / threads;\n"
             f"        if (blocks < 1) return FJ_ERR_WORKSPACE;\n"
             f"        grid = dim3((unsigned)(blocks < (long long)a.n_items ? blocks : a.n_items));\n"
             f"    }}")
         f_ws = (f"    {{\n"
-                f"        const long long need = (long long)8 * ({W} * threads * FJ_LIB_WS_BLOCKS + FJ_WS_HEAD);\n"
+                f"        const long long need = (long long)8 * {W} * threads * FJ_LIB_WS_BLOCKS;\n"
                 f"        a.ws = fj_library_workspace(need);\n"
                 f"        if (!a.ws) return FJ_ERR_WORKSPACE;\n"
                 f"        a.ws_doubles = need / 8;\n"
@@ -556,7 +539,7 @@ def emit_ws_table(classes) -> str:
             lines.append(f"        case {((la * 4 + lb) * 4 + lc) * 4 + ld}: per_lane = {w}; break;")
     lines += ["        default: return 0;", "    }",
               "    const long long blocks = n_work < FJ_WS_MAX_BLOCKS ? n_work : FJ_WS_MAX_BLOCKS;",
-              "    return 8 * (per_lane * threads * blocks + FJ_WS_HEAD);", "}", ""]
+              "    return 8 * per_lane * threads * blocks;", "}", ""]
     return "\n".join(lines)
 
 

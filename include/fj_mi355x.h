@@ -105,13 +105,11 @@ int fj_boys_eval(fj_stream_t stream, int M, int n, const double* T, double* out)
  * end), so one quartet can be spread over several lanes; `precision` 0
  * is the FP64 kernel, 1 the FP32 tier (VRR and boundary in FP32; Boys, the
  * seed, the HRR and every F update in FP64).  `ws` (ws_bytes) is a device
- * workspace: the staged classes (fj_fx_workspace_bytes > 0) keep their
- * contracted boundary there, one slot per resident lane, and run persistent
- * over the work items with as many workgroups as the workspace holds slots
- * for; other classes ignore it (may be NULL).  Its first 256 bytes are two
- * work counters: zero them once when the workspace is allocated (every
- * launch leaves them zero).  Launches that share a workspace must not
- * overlap in time. */
+ * workspace: the classes whose contracted boundary is too large for private
+ * memory (fj_fx_workspace_bytes > 0: the f classes) keep it there, one slot
+ * per resident lane, and run persistent over the work items with as many
+ * workgroups as the workspace holds slots for; other classes ignore it (may
+ * be NULL).  Launches that share a workspace must not overlap in time. */
 /* Workspace bytes fj_fx_<la lb lc ld> uses for n_work items of `threads`
  * lanes (0: the class needs none; never more than FJ_WS_MAX_BLOCKS
  * workgroups' worth). */

@@ -41,9 +41,9 @@ int main(int argc, char** argv) {
     if (W > 0) {
         // the launcher's rule: the library workspace holds FJ_LIB_WS_BLOCKS
         // workgroups' slots; the persistent grid is cut to min(that, items)
-        const long long need = 8 * (W * threads * 256 + FJ_WS_HEAD);
-        a.ws = (double*)calloc(need / 8, 8); a.ws_doubles = need / 8;
-        long long blocks = (a.ws_doubles - FJ_WS_HEAD) / W / threads;
+        const long long need = 8 * W * threads * 256;
+        a.ws = (double*)malloc(need); a.ws_doubles = need / 8;
+        long long blocks = a.ws_doubles / W / threads;
         unsigned grid = (unsigned)std::min<long long>(blocks, a.n_items);
         gridDim = {grid, 1, 1};
         for (unsigned b = 0; b < grid; ++b) blocks_xy.push_back({b, 0});

@@ -53,10 +53,6 @@ inline double __shfl_xor(double v, int o, int w) {
     return r;
 }
 inline void unsafeAtomicAdd(double* p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
-inline int atomicAdd(int* p, int v) { return std::atomic_ref<int>(*p).fetch_add(v); }
-inline int atomicExch(int* p, int v) { return std::atomic_ref<int>(*p).exchange(v); }
-inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
-#define FJ_WS_HEAD 32
 inline void fj_side_range(const FjSide& s, int i, int& k0, int& k) {
     if (s.off) { k0 = s.off[i]; k = s.off[i + 1] - k0; } else { k0 = i * s.k; k = s.k; }
 }
