@@ -39,7 +39,7 @@ def sources(lmax: int = LMAX) -> list[Path]:
     gen_hip.write_header(classes)
     mmd = gen_mmd.generate(MMD_LMAX)
     gen_mmd.write_header(MMD_LMAX)
-    return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", *files, *mmd]
+    return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", *files, *mmd]
 
 
 def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, float, str]:

@@ -115,9 +115,10 @@ class FockBuilder:
 
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads=64,
                  mixed: bool = False, fp32_error: float = 1e-11, selector=None,
-                 lane_flops=None, fp32_bound: float | None = None):
+                 lane_flops=None, fp32_bound: float | None = None, planner: str = "native"):
         self._setup([(basis, D)], tau=tau, threads=threads, mixed=mixed, fp32_error=fp32_error,
-                     selector=selector, lane_flops=lane_flops, fp32_bound=fp32_bound)
+                     selector=selector, lane_flops=lane_flops, fp32_bound=fp32_bound,
+                     planner=planner)
 
     @classmethod
     def batch(cls, systems, **kw) -> "FockBuilder":
@@ -130,7 +131,7 @@ class FockBuilder:
 
     def _setup(self, systems, *, tau: float = 1e-12, threads=64, mixed: bool = False,
                fp32_error: float = 1e-11, selector=None, lane_flops=None,
-               fp32_bound: float | None = None):
+               fp32_bound: float | None = None, planner: str = "native"):
         """threads, lane_flops: one value for every class pair, or a dict
         {(la, lb, lc, ld): value} (the autotuner's choice per class pair;
         missing ones take the default 64 and the automatic split).
@@ -203,7 +204,10 @@ class FockBuilder:
         # selector(features [n, N_FEATURES]) -> bool [n]: True moves an FP64
         # tile of the bound rule to the FP32 tier (mixed mode only)
         self.selector = selector
-        self.launches = self._plan()
+        # the native planner unless the learned selector needs the tiles'
+        # features (the Python planner, which it reproduces exactly)
+        self.planner = "python" if (planner == "python" or (mixed and selector is not None)) else "native"
+        self.launches = self._plan() if self.planner == "python" else self._plan_native()
         self.plan_seconds = time.time() - t0
         self._dev = False
 
@@ -278,6 +282,51 @@ class FockBuilder:
                                                        dm))
                 f = np.concatenate(feats) if feats else None
                 out.append((X, Y, tiles, f))
+        return out
+
+    def _plan_native(self) -> list:
+        """The same plan as _plan, from the native planner (planner.py):
+        a counting pass for the automatic split, then one call per class
+        pair."""
+        from . import planner as NP
+        sides = {c: NP.Side(t) for c, t in self.tables.items()}
+        systems = NP.Systems(self.Dsh_plans)
+        classes = sorted(self.tables)
+        pairs = [(X, Y) for i, X in enumerate(classes) for Y in classes[i:]]
+        common = dict(tau=self.tau, mixed=self.mixed, fp32_error=self.fp32_error,
+                      fp32_bound=self.fp32_bound, est_scale=FP32_EST_SCALE,
+                      max_prim=P.MAX_PRIM_PAIRS)
+        if self.lane_flops is None:
+            total = 0.0
+            for X, Y in pairs:
+                c = cost((*X, *Y))
+                r = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems, diag=int(X == Y),
+                                  threads=64, lane_flops=0.0, per_prim=c.per_prim,
+                                  per_quartet=c.per_quartet, **common)
+                total += r[0][2][2] + r[1][2][2]
+            auto_lf = LANE_FLOPS_MAX if total >= SPLIT_MAX_FLOPS else \
+                float(min(LANE_FLOPS_MAX, max(1.0, total / LANES_TARGET)))
+        else:
+            auto_lf = self.lane_flops
+        self.lane_flops_used = auto_lf
+        out = []
+        for X, Y in pairs:
+            cls = (*X, *Y)
+            c = cost(cls)
+            T = self.class_threads.get(cls, self.threads)
+            lf = self.class_lane_flops.get(cls, auto_lf)
+            r = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems, diag=int(X == Y),
+                              threads=T, lane_flops=lf, per_prim=c.per_prim,
+                              per_quartet=c.per_quartet, **common)
+            for tier, (w, k, stats) in zip((FP64, FP32), r):
+                if w is None:
+                    continue
+                L = Launch(cls, self.tables[X], self.tables[Y], int(X == Y), tier, w)
+                L.threads = T
+                L.ket_list = k
+                L.n_quartets, L.n_prim, L.flops = stats
+                out.append(L)
+        out.sort(key=lambda L: -L.flops)
         return out
 
     def _lane_flops(self, groups) -> float:

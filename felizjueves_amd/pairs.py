@@ -288,17 +288,21 @@ def kept_pairs(basis: Basis, D: np.ndarray, tau: float, n: int, tables=None):
     batch stream and the native builder). Returns (pairs, deleted, eps,
     g_max, n_prim_total, mags)."""
     tables = tables or pair_tables(basis)
-    sums = {}
+    sums, bounds_at = {}, {}
     for cls, (ps, pp) in tables.items():
-        sums[cls] = np.array([math.fsum(pp.bound[pp.pair == k]) for k in range(len(ps))])
+        # primitive rows are grouped by pair, pairs ascending
+        at = np.searchsorted(pp.pair, np.arange(len(ps) + 1))
+        bounds_at[cls] = at
+        sums[cls] = np.array([math.fsum(pp.bound[at[k]:at[k + 1]]) for k in range(len(ps))])
     g_max = max((s.max() for s in sums.values() if len(s)), default=0.0)
     eps = tau / (2.0 * g_max) if tau > 0.0 and g_max > 0.0 else 0.0
     mags = magnitudes(tau, n) if tau > 0 else tuple([1.0] + [0.0] * (n - 1))
     pairs, deleted, n_prim_total = [], [], 0
     for cls, (ps, pp) in tables.items():
         kb_of = basis.shell_nprim
+        at = bounds_at[cls]
         for k, (a, b) in enumerate(ps):
-            rows = np.nonzero(pp.pair == k)[0]
+            rows = np.arange(at[k], at[k + 1])
             n_prim_total += len(rows)
             keep_local = prune(pp.bound[rows], eps)
             if len(keep_local) == 0:

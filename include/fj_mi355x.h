@@ -114,6 +114,40 @@ int fj_boys_eval(fj_stream_t stream, int M, int n, const double* T, double* out)
  * lanes (0: the class needs none; never more than FJ_WS_MAX_BLOCKS
  * workgroups' worth). */
 long long fj_fx_workspace_bytes(int la, int lb, int lc, int ld, int threads, int n_work);
+/* The native planner of one class pair (host code; no upstream counterpart:
+ * the upstream plans in Python, eri/fock/launch_f.py:390-470, per
+ * (degree, magnitude) bin).  Per bra pair of `bra`, the kets of the same
+ * system (ket pairs mol_lo[m] .. mol_hi[m]) that pass the six-block screen
+ * G_ab G_cd dmax >= tau (dmax from Dsh[m], n_sh[m] x n_sh[m]; with `diag`,
+ * kets k >= b only), split by the FP32 rule when `mixed` (fp32_bound >= 0:
+ * bound < fp32_bound; else bound amp_b amp_k est_scale < fp32_error), are
+ * expanded into lanes (ket pair, chunk of <= lane_flops / (cb per_prim)
+ * primitive pairs, equal chunks per ket) sorted by primitive count, cut
+ * into work items of `threads` lanes and bra chunks of cb primitive pairs,
+ * costliest first.  lane_flops <= 0 only counts (n_quartets, n_prim,
+ * flops).  Output arrays are allocated here; release them with
+ * fj_plan_free. */
+typedef struct FjPlanSide {
+    const double* G;      /* pair bounds [n] */
+    const int* deg;       /* primitive pairs per pair [n] */
+    const int* sh;        /* shell indices within the pair's system [n][2] */
+    const int* mol;       /* system of each pair [n] (pairs grouped by system) */
+    const double* amp;    /* FP32 error amplification per pair [n] */
+    int n;
+} FjPlanSide;
+typedef struct FjPlanResult {
+    int* work[2];         /* per tier (0 FP64, 1 FP32): int4 work items */
+    long long n_work[2];
+    int* kets[2];         /* per tier: ket-list entries */
+    long long n_kets[2];
+    double n_quartets[2], n_prim[2], flops[2];
+} FjPlanResult;
+int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const int* mol_lo,
+    const int* mol_hi, const double* const* Dsh, const int* n_sh, int diag, double tau, int mixed,
+    double fp32_error, double fp32_bound, double est_scale, int threads, double lane_flops,
+    double per_prim, double per_quartet, int max_prim, FjPlanResult* out);
+void fj_plan_free(FjPlanResult* out);
+
 /* BEGIN F CLASSES (written by felizjueves_amd/gen_hip.py) */
 int fj_f_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,

@@ -91,3 +91,28 @@ def test_batch_work_lists_match_the_single_system_plans():
                    one.tables[c[2:]].pairs[k].a, one.tables[c[2:]].pairs[k].b)
                   for c, b, k in _decode_cover(one)}
         assert mine == theirs, i
+
+
+def _same_plan(a, b):
+    assert len(a.launches) == len(b.launches)
+    for La, Lb in zip(a.launches, b.launches):
+        assert (La.cls, La.precision, La.threads) == (Lb.cls, Lb.precision, Lb.threads)
+        assert np.array_equal(La.work, Lb.work), La.cls
+        assert np.array_equal(La.ket_list, Lb.ket_list), La.cls
+        assert (La.n_quartets, La.n_prim) == (Lb.n_quartets, Lb.n_prim)
+        assert abs(La.flops - Lb.flops) <= 1e-9 * La.flops
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(lane_flops=3e4), dict(mixed=True),
+                                dict(mixed=True, fp32_bound=1e-6), dict(threads=128, lane_flops=1e5)])
+def test_native_planner_equals_python_planner(kw):
+    s = config1()
+    _same_plan(FockBuilder(s.basis, s.D, tau=1e-12, planner="python", **kw),
+               FockBuilder(s.basis, s.D, tau=1e-12, planner="native", **kw))
+
+
+def test_native_planner_equals_python_planner_batch():
+    systems = [config1(seed=1), config2(seed=5, n_atoms=4), config2(seed=11, n_atoms=10)]
+    sy = [(s.basis, s.D) for s in systems]
+    _same_plan(FockBuilder.batch(sy, tau=1e-12, planner="python"),
+               FockBuilder.batch(sy, tau=1e-12, planner="native"))
