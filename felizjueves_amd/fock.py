@@ -510,6 +510,17 @@ class FockBuilder:
         native.call_fx(L.cls, L.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
                        L.work.shape[0], sys, L.diag, self.tau, stream, L.ket_list_dev, ws)
 
+    def _stream_assignment(self, n: int) -> list:
+        """Stream of each launch: the launches, costliest first, each to the
+        stream with the least work so far (by their FLOP counts)."""
+        load = [0.0] * n
+        out = []
+        for L in self.launches:
+            j = min(range(n), key=lambda i: load[i])
+            load[j] += L.flops
+            out.append(j)
+        return out
+
     def _side_streams(self, n: int) -> list:
         import torch
         if len(getattr(self, "_streams", [])) != n:
@@ -519,8 +530,9 @@ class FockBuilder:
     def build(self, D_dev, stream=None, close: bool = True, n_streams: int = 1):
         """G for the device density D_dev (n_ao x n_ao float64; for a batch
         builder, a list with one per system), on `stream`. With n_streams > 1
-        the class-pair launches are dealt round-robin (costliest first) to
-        that many side streams that fork from and join back into `stream`,
+        the class-pair launches are dealt (costliest first, each to the least
+        loaded) to that many side streams that fork from and join back into
+        `stream`,
         so launches too small to fill the chip overlap. Returns the F tensor
         (the list of them for a batch; overwritten by the next build)."""
         import torch
@@ -544,8 +556,8 @@ class FockBuilder:
             wss = [self._workspace(j) for j in range(n_streams)]
             for s in side:
                 s.wait_stream(main)
-            for i, L in enumerate(self.launches):
-                self.launch(L, sys, side[i % n_streams], wss[i % n_streams])
+            for L, j in zip(self.launches, self._stream_assignment(n_streams)):
+                self.launch(L, sys, side[j], wss[j])
             for s in side:
                 main.wait_stream(s)
         if close:
