@@ -591,7 +591,10 @@ def write_header(classes) -> None:
 # with that row's digestion (F_ab and F_cd in registers, the four exchange
 # blocks flushed per ab).
 
-FUSED_LIVE_MAX = 120
+# classes whose whole VRR keeps at most this many values live are emitted
+# fused (all in registers); measured on MI355X (40-atom build): 120 -> 91.5
+# ms, 200 -> 88.5 ms (e.g. (sd|sd) 6.2 -> 4.1 ms), 400 -> 116 ms (spills)
+FUSED_LIVE_MAX = 200
 
 # A staged class keeps its contracted boundary, half-transformed block and
 # F_cd accumulators in a launch workspace in HBM instead of private scratch:

@@ -24,8 +24,8 @@ from oracle import oracle as O
 ROOT = Path(__file__).resolve().parent.parent
 EMU = ROOT / "tests" / "emu"
 
-# (class, launch index): an f workspace class with lb = 0, one with both
-# HRRs, a fused f class and a d workspace class
+# (class, launch index): fused f classes, an f workspace class with both
+# HRRs (its VRR groups as separate functions), a d workspace class
 CASES = [((0, 0, 3, 3), 0), ((0, 1, 3, 3), 2), ((0, 0, 2, 3), 1), ((1, 2, 2, 2), 0)]
 
 
