@@ -86,9 +86,12 @@ __global__ void __launch_bounds__(256) fj_mlp_forward_k(int n, int F, const floa
     Y[i] = fj_mlp_eval<false>(F, x, sW, h1, nullptr);
 }
 
+// quantile < 0: mean squared error; 0 < quantile < 1: the pinball loss of
+// that quantile, mean over samples of q (t - y) if t >= y else (q - 1)(t - y)
 __global__ void __launch_bounds__(256) fj_mlp_backward_k(int n, int F, const float* __restrict__ X,
                                                          const float* __restrict__ T, FjMlpW W,
-                                                         float* __restrict__ G, float* __restrict__ loss) {
+                                                         float* __restrict__ G, float* __restrict__ loss,
+                                                         float quantile) {
     extern __shared__ float smem[];
     const int np = fj_mlp_nparam(F);
     float* sW = smem;
@@ -102,8 +105,17 @@ __global__ void __launch_bounds__(256) fj_mlp_backward_k(int n, int F, const flo
         for (int k = 0; k < F; ++k) x[k] = X[(size_t)i * F + k];
         const float y = fj_mlp_eval<true>(F, x, sW, h1, h2);
         const float r = y - T[i];
-        l = r * r / (float)n;
-        const float dy = 2.0f * r / (float)n;
+        float dy;
+        if (quantile < 0.0f) {
+            l = r * r / (float)n;
+            dy = 2.0f * r / (float)n;
+        } else if (r <= 0.0f) {   // t >= y
+            l = -quantile * r / (float)n;
+            dy = -quantile / (float)n;
+        } else {
+            l = (1.0f - quantile) * r / (float)n;
+            dy = (1.0f - quantile) / (float)n;
+        }
         const float* W2 = sW + FJ_MLP_H * F + FJ_MLP_H;
         const float* w3 = W2 + FJ_MLP_H * FJ_MLP_H + FJ_MLP_H;
         const int o_b1 = FJ_MLP_H * F, o_W2 = o_b1 + FJ_MLP_H, o_b2 = o_W2 + FJ_MLP_H * FJ_MLP_H;
@@ -177,7 +189,19 @@ extern "C" int fj_mlp_backward(hipStream_t s, int n, int F, const float* X, cons
     FjMlpW W{Wb, P};
     const size_t lds = 2 * sizeof(float) * fj_mlp_nparam(F);
     hipLaunchKernelGGL(fj_mlp_backward_k, dim3((n + 255) / 256), dim3(256), lds, s, n, F, X, T, W, G,
-                       loss);
+                       loss, -1.0f);
+    return (int)hipGetLastError();
+}
+
+extern "C" int fj_mlp_backward_quantile(hipStream_t s, int n, int F, const float* X, const float* T,
+                                        const uint16_t* Wb, const float* P, float* G, float* loss,
+                                        float quantile) {
+    if (n < 0 || F < 1 || F > FJ_MLP_FMAX || !(quantile > 0.0f && quantile < 1.0f)) return FJ_ERR_ARG;
+    if (n == 0) return 0;
+    FjMlpW W{Wb, P};
+    const size_t lds = 2 * sizeof(float) * fj_mlp_nparam(F);
+    hipLaunchKernelGGL(fj_mlp_backward_k, dim3((n + 255) / 256), dim3(256), lds, s, n, F, X, T, W, G,
+                       loss, quantile);
     return (int)hipGetLastError();
 }
 

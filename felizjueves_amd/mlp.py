@@ -82,12 +82,17 @@ class MLP:
         torch.cuda.synchronize()
         return Y.cpu().numpy().astype(np.float64)
 
-    def grad(self, Xd, Td):
-        """Zero G and loss, then accumulate dL/dP and L on the device."""
+    def grad(self, Xd, Td, quantile: float | None = None):
+        """Zero G and loss, then accumulate dL/dP and L on the device (mean
+        squared error, or the pinball loss of `quantile`)."""
         self.G.zero_()
         self.loss.zero_()
-        _call("fj_mlp_backward", C.c_int(Xd.shape[0]), C.c_int(self.F), Xd, Td, self.Wb, self.P,
-              self.G, self.loss)
+        if quantile is None:
+            _call("fj_mlp_backward", C.c_int(Xd.shape[0]), C.c_int(self.F), Xd, Td, self.Wb, self.P,
+                  self.G, self.loss)
+        else:
+            _call("fj_mlp_backward_quantile", C.c_int(Xd.shape[0]), C.c_int(self.F), Xd, Td, self.Wb,
+                  self.P, self.G, self.loss, C.c_float(quantile))
 
     def step(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8) -> None:
         self.t += 1
@@ -95,8 +100,9 @@ class MLP:
               C.c_float(lr), C.c_float(b1), C.c_float(b2), C.c_float(eps), C.c_int(self.t))
 
     def fit(self, X: np.ndarray, y: np.ndarray, epochs: int = 60, batch: int = 4096,
-            lr: float = 2e-3, seed: int = 0, log=None) -> list[float]:
-        """Minibatch Adam on (standardised X, y) with the device kernels."""
+            lr: float = 2e-3, seed: int = 0, log=None, quantile: float | None = None) -> list[float]:
+        """Minibatch Adam on (standardised X, y) with the device kernels
+        (mean squared error, or the pinball loss of `quantile`)."""
         import torch
         self.mean = X.mean(axis=0)
         self.std = X.std(axis=0) + 1e-6
@@ -111,7 +117,7 @@ class MLP:
             for s in range(0, n, batch):
                 idx = perm[s:s + batch]
                 xb, tb = Xd[idx].contiguous(), Td[idx].contiguous()
-                self.grad(xb, tb)
+                self.grad(xb, tb, quantile)
                 self.step(lr * (0.5 ** (ep // max(1, epochs // 3))))
                 tot += float(self.loss.item()) * len(idx)
             hist.append(tot / n)

@@ -163,17 +163,26 @@ def collect(systems, max_items: int = 20000, seed: int = 0, log=None):
 
 
 def train_selector(X: np.ndarray, y: np.ndarray, budget: float = 1e-11, quantile: float = 0.999,
-                   epochs: int = 60, seed: int = 0, log=None) -> LearnedSelector:
+                   epochs: int = 60, seed: int = 0, log=None,
+                   fit_quantile: float | None = 0.99) -> LearnedSelector:
     """Train the network on the device (90/10 split), set the margin from
     the held-out residuals of the tiles predicted within three decades of
-    the budget (the ones the decision depends on), return the selector."""
+    the budget (the ones the decision depends on), return the selector.
+
+    With `fit_quantile` q the network is fitted with the pinball loss of q,
+    so it predicts the q-quantile of a tile's log error rather than its
+    mean; the margin then only covers the gap from q to `quantile` (a few
+    tenths of a decade instead of several decades for the mean fit), and
+    far more tiles fall under the budget at the same held-out risk.
+    `fit_quantile=None` fits the mean (squared error)."""
     from .mlp import MLP
     rng = np.random.default_rng(seed)
     idx = rng.permutation(len(X))
     n_val = max(1, len(X) // 10)
     va, tr = idx[:n_val], idx[n_val:]
     net = MLP(N_FEATURES, seed=seed)
-    net.fit(X[tr], y[tr], epochs=epochs, batch=4096, lr=2e-3, seed=seed, log=log)
+    net.fit(X[tr], y[tr], epochs=epochs, batch=4096, lr=2e-3, seed=seed, log=log,
+            quantile=fit_quantile)
     pred = net.predict(X[va])
     r = y[va] - pred
     near = pred < np.log10(budget) + 3.0
@@ -195,15 +204,23 @@ def main() -> None:
     ap.add_argument("--budget", type=float, default=1e-11)
     ap.add_argument("--out", required=True)
     ap.add_argument("--data", default=None, help="also save the (features, log error) pairs here")
+    ap.add_argument("--from-data", default=None, help="train on pairs saved by --data instead of probing")
+    ap.add_argument("--fit-quantile", type=float, default=0.99,
+                    help="pinball-loss quantile of the fit (<= 0: squared error)")
     a = ap.parse_args()
     t = time.time()
-    X, y = collect(training_systems(a.systems), max_items=a.max_items,
-                   log=lambda i, s, n: print(f"[selector] probe {i} {s.name} tiles {n} "
-                                             f"{time.time() - t:.0f}s", flush=True))
+    if a.from_data:
+        with np.load(a.from_data, allow_pickle=False) as z:
+            X, y = z["X"], z["y"]
+    else:
+        X, y = collect(training_systems(a.systems), max_items=a.max_items,
+                       log=lambda i, s, n: print(f"[selector] probe {i} {s.name} tiles {n} "
+                                                 f"{time.time() - t:.0f}s", flush=True))
     if a.data:
         np.savez(a.data, X=X, y=y)
     print(f"[selector] {len(X)} pairs in {time.time() - t:.0f}s", flush=True)
     sel = train_selector(X, y, budget=a.budget,
+                         fit_quantile=a.fit_quantile if a.fit_quantile > 0 else None,
                          log=lambda e, l: print(f"[selector] epoch {e} loss {l:.4f}", flush=True))
     print(f"[selector] val rmse {sel.val_rmse:.3f} margin {sel.margin:.3f}", flush=True)
     sel.save(a.out)

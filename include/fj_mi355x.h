@@ -351,6 +351,10 @@ int fj_mlp_forward(fj_stream_t stream, int n, int F, const float* X, const unsig
  * the caller). */
 int fj_mlp_backward(fj_stream_t stream, int n, int F, const float* X, const float* T,
                     const unsigned short* Wb, const float* P, float* G, float* loss);
+/* The same gradients for the pinball loss of `quantile` (0 < q < 1): mean
+ * over samples of q (t - y) when t >= y, else (q - 1)(t - y). */
+int fj_mlp_backward_quantile(fj_stream_t stream, int n, int F, const float* X, const float* T,
+    const unsigned short* Wb, const float* P, float* G, float* loss, float quantile);
 /* One Adam step (step t >= 1) on P from G, moments M, V; refreshes Wb. */
 int fj_mlp_adam(fj_stream_t stream, int np, float* P, const float* G, float* M, float* V,
                 unsigned short* Wb, float lr, float beta1, float beta2, float eps, int t);

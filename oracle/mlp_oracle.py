@@ -44,13 +44,20 @@ def forward(P, X, F):
     return y, (X, a1, h1, a2, h2, W2, w3)
 
 
-def grad(P, X, T, F):
-    """(loss, dL/dP) for L = mean (y - t)^2, the straight-through gradient
-    of the FP32 master weights (d/dW of the BF16 copy, as the kernel)."""
+def grad(P, X, T, F, quantile=None):
+    """(loss, dL/dP) for L = mean (y - t)^2 (or, with `quantile` q, the
+    pinball loss mean of q (t - y) if t >= y else (q - 1)(t - y)), the
+    straight-through gradient of the FP32 master weights (d/dW of the BF16
+    copy, as the kernel)."""
     y, (X, a1, h1, a2, h2, W2, w3) = forward(P, X, F)
     n = len(y)
     r = y - np.asarray(T, dtype=np.float64)
-    dy = 2.0 * r / n
+    if quantile is None:
+        dy = 2.0 * r / n
+        loss = float(np.mean(r * r))
+    else:
+        dy = np.where(r <= 0, -quantile, 1.0 - quantile) / n
+        loss = float(np.mean(np.where(r <= 0, -quantile * r, (1.0 - quantile) * r)))
     gw3 = h2.T @ dy
     gb3 = dy.sum()
     d2 = (dy[:, None] * w3[None, :]) * (a2 > 0)
@@ -60,4 +67,4 @@ def grad(P, X, T, F):
     gW1 = d1.T @ X
     gb1 = d1.sum(0)
     g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])
-    return float(np.mean(r * r)), g
+    return loss, g

@@ -54,6 +54,22 @@ def test_backward_vs_oracle():
     assert err <= 1e-4 * np.abs(g).max()
 
 
+def test_quantile_backward_vs_oracle():
+    torch, native, net, X, T = _setup(n=3000, seed=4)
+    # shift the targets to straddle the predictions (residuals of both signs)
+    y0, _ = O.forward(net.params(), X, F)
+    T = (y0 + np.random.default_rng(5).normal(0, 1, len(y0))).astype(np.float32)
+    for q in (0.99, 0.3):
+        net.grad(native.to_dev(X), native.to_dev(T), quantile=q)
+        torch.cuda.synchronize()
+        loss, g = O.grad(net.params(), X, T, F, quantile=q)
+        gd = net.G.cpu().numpy()
+        err = np.abs(gd - g).max()
+        print("q", q, "loss", net.loss.item(), loss, "grad max|dg|", err, flush=True)
+        assert abs(net.loss.item() - loss) <= 1e-5 * loss
+        assert err <= 1e-4 * np.abs(g).max()
+
+
 def test_adam_step_vs_numpy():
     torch, native, net, X, T = _setup(n=800, seed=2)
     P0 = net.params().astype(np.float64)

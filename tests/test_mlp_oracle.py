@@ -1,6 +1,6 @@
 """Pins of the selector network's CPU oracle (CPU only): BF16 rounding
-against torch's bfloat16 conversion, and the MSE gradients against torch
-autograd in float64 on the same BF16 effective weights."""
+against torch's bfloat16 conversion, and the MSE and pinball-loss gradients
+against torch autograd in float64 on the same BF16 effective weights."""
 
 import numpy as np
 import torch
@@ -25,6 +25,49 @@ def test_bf16_matches_torch_round_to_nearest_even():
     x = np.concatenate([x, ties.view(np.float32)])
     ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
     assert np.array_equal(M.bf16(x).view(np.uint32), ref.view(np.uint32))
+
+
+def _autograd(P, X, T, quantile):
+    W1, b1, W2, b2, w3, b3 = M.unpack(P, F)
+    ts = [torch.tensor(M.bf16(W1), dtype=torch.float64, requires_grad=True),
+          torch.tensor(b1, dtype=torch.float64, requires_grad=True),
+          torch.tensor(M.bf16(W2), dtype=torch.float64, requires_grad=True),
+          torch.tensor(b2, dtype=torch.float64, requires_grad=True),
+          torch.tensor(M.bf16(w3), dtype=torch.float64, requires_grad=True),
+          torch.tensor(float(b3), dtype=torch.float64, requires_grad=True)]
+    x = torch.tensor(X)
+    h1 = torch.relu(x @ ts[0].T + ts[1])
+    h2 = torch.relu(h1 @ ts[2].T + ts[3])
+    y = h2 @ ts[4] + ts[5]
+    # pinball loss written as max(q u, (q - 1) u), u = t - y: a form of its
+    # own, not the oracle's branch on the sign of the residual
+    u = torch.tensor(T) - y
+    L = torch.maximum(quantile * u, (quantile - 1.0) * u).mean()
+    L.backward()
+    return L.item(), np.concatenate([t.grad.numpy().ravel() for t in ts])
+
+
+def test_quantile_grad_matches_autograd():
+    P = _params(6)
+    rng = np.random.default_rng(7)
+    X = rng.normal(0, 1, (400, F))
+    T = rng.normal(0, 1, 400)
+    for q in (0.1, 0.5, 0.99):
+        loss, g = M.grad(P, X, T, F, quantile=q)
+        L, ref = _autograd(P, X, T, q)
+        assert abs(loss - L) < 1e-12 * max(1.0, L)
+        assert np.abs(g - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_quantile_half_is_half_absolute_error():
+    """q = 1/2: the pinball loss is half the mean absolute error."""
+    P = _params(8)
+    rng = np.random.default_rng(9)
+    X = rng.normal(0, 1, (100, F))
+    T = rng.normal(0, 1, 100)
+    y, _ = M.forward(P, X, F)
+    loss, _ = M.grad(P, X, T, F, quantile=0.5)
+    assert abs(loss - 0.5 * np.mean(np.abs(y - T))) < 1e-13
 
 
 def test_grad_matches_autograd():
