@@ -73,13 +73,16 @@ __device__ __forceinline__ double fj_wave_sum(double v) {
 //  T >= 40: F_0 = (1/2) sqrt(pi/T) (erfc(sqrt 40) < 2e-19) and the upward
 //          recursion F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable for T > m.
 // The table holds correctly rounded values of the Kummer series.
-template <int M>
-__device__ __forceinline__ void fj_boys(double T, double (&F)[M + 1]) {
+// fj_boys_from reads row i of a table with ROW doubles per row: e^-T_i at
+// column 0 and F_M(T_i) .. F_{M+8}(T_i) from column FO on.
+template <int M, int ROW, int FO>
+__device__ __forceinline__ void fj_boys_from(double T, double (&F)[M + 1],
+                                             const double* __restrict__ tab) {
     static_assert(M + 8 <= 20, "Boys table holds F_0..F_20");
     if (T < FJ_BOYS_TMAX) {
         const int i = (int)(T * 8.0 + 0.5);
         const double x = (double)i * 0.125 - T;
-        const double* __restrict__ r = fj_boys_tab + FJ_BOYS_ROW * i + 1 + M;
+        const double* __restrict__ r = tab + ROW * i + FO;
         double f = r[8];
         f = fma(f, x * (1.0 / 8.0), r[7]);
         f = fma(f, x * (1.0 / 7.0), r[6]);
@@ -101,7 +104,7 @@ __device__ __forceinline__ void fj_boys(double T, double (&F)[M + 1]) {
             ex = fma(ex, x, 0.5);
             ex = fma(ex, x, 1.0);
             ex = fma(ex, x, 1.0);
-            const double e = fj_boys_tab[FJ_BOYS_ROW * i] * ex;
+            const double e = tab[ROW * i] * ex;
             const double t2 = 2.0 * T;
             F[M] = f;
 #pragma unroll
@@ -123,4 +126,29 @@ __device__ __forceinline__ void fj_boys(double T, double (&F)[M + 1]) {
             }
         }
     }
+}
+
+template <int M>
+__device__ __forceinline__ void fj_boys(double T, double (&F)[M + 1]) {
+    fj_boys_from<M, FJ_BOYS_ROW, 1 + M>(T, F, fj_boys_tab);
+}
+
+// The columns of the Boys table one order M needs (e^-T_i, F_M .. F_{M+8}),
+// FJ_BOYS_LROW doubles per row, staged in LDS by a workgroup that evaluates
+// many Boys functions (a persistent kernel stages it once): the Taylor step
+// then reads LDS instead of scattered global loads.
+#define FJ_BOYS_LROW 10
+
+template <int M>
+__device__ __forceinline__ void fj_boys_stage(double* __restrict__ tb) {
+    static_assert(M + 8 <= 20, "Boys table holds F_0..F_20");
+    for (int j = (int)threadIdx.x; j < FJ_BOYS_NROW * FJ_BOYS_LROW; j += (int)blockDim.x) {
+        const int i = j / FJ_BOYS_LROW, c = j - i * FJ_BOYS_LROW;
+        tb[j] = fj_boys_tab[FJ_BOYS_ROW * i + (c ? M + c : 0)];
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void fj_boys_lds(double T, double (&F)[M + 1], const double* __restrict__ tb) {
+    fj_boys_from<M, FJ_BOYS_LROW, 1>(T, F, tb);
 }

@@ -241,6 +241,10 @@ def emit_class(cls) -> str:
     W = ws_doubles(cls)
     if W:
         kernel_body = (
+            "    // the Boys table columns of this class, staged once per workgroup\n"
+            "    __shared__ double tb[FJ_BOYS_NROW * FJ_BOYS_LROW];\n"
+            f"    fj_boys_stage<{sum(cls)}>(tb);\n"
+            "    __syncthreads();\n"
             "    // persistent: one workspace slot per lane of the grid, items strided\n"
             "    const size_t st = (size_t)gridDim.x * blockDim.x;\n"
             "    double* ws = a.ws + (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
@@ -252,7 +256,7 @@ def emit_class(cls) -> str:
             "        double* wsi = ws;\n"
             "        size_t sti = st;\n"
             "        FJ_LAUNDER(wsi, sti);\n"
-            f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti);\n"
+            f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n"
             "        __syncthreads();\n"
             "    }")
         launch_ws = (
@@ -270,7 +274,7 @@ def emit_class(cls) -> str:
                 f"        a.ws_doubles = need / 8;\n"
                 f"    }}")
     else:
-        kernel_body = "    fj_item_" + name + "<R>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0);"
+        kernel_body = "    fj_item_" + name + "<R>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);"
         launch_ws = ""
         f_ws = ""
     out.append(f"""
@@ -278,7 +282,8 @@ def emit_class(cls) -> str:
 // kernel calls it in a loop with a barrier after each item).
 template <typename R>
 __device__ __forceinline__ void fj_item_{name}(const FjFArgs& a, int bx, int by, double* bsh,
-                                            double* __restrict__ ws, size_t st) {{
+                                            double* __restrict__ ws, size_t st,
+                                            const double* __restrict__ tb) {{
     // work item (bra pair, first ket, system, ket end) from the list, else
     // (x, y * blockDim, the single system, n_ket)
     // The third field packs the system (bits 0-15) and a chunk of the bra
@@ -369,7 +374,7 @@ __device__ __forceinline__ void fj_item_{name}(const FjFArgs& a, int bx, int by,
         kcd = kc_n ? min(kcd - kc_lo, kc_n) : kcd - kc_lo;
         const double CDx = {cdx}, CDy = {cdy}, CDz = {cdz};
         fj_q_{name}<R>(bsh, kab, a.ket.prim + 8 * (size_t)kc0, kcd, ABx, ABy, ABz, CDx, CDy, CDz,
-                        S.D, n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, S.F, ws, st);
+                        S.D, n_ao, oc, od, oa, ob, s_j, s_k, wab, fab, S.F, ws, st{", tb" if W else ""});
     }}
     // F_ab: one bra per workgroup, so every lane of a wavefront adds into the
     // same block; reduce across the wavefront, then one atomic per element.
@@ -767,7 +772,7 @@ def emit_quartet_staged(cls) -> str:
         E("const double rho = p * q * ippq;", i2)
         E("const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);", i2)
         E(f"double Fm[{L + 1}];", i2)
-        E(f"fj_boys<{L}>(T, Fm);", i2)
+        E(f"fj_boys_lds<{L}>(T, Fm, tb);" if W else f"fj_boys<{L}>(T, Fm);", i2)
         E("const double S = p_[7] * Uq * (FJ_TWO_OVER_SQRT_PI * sqrt(rho));", i2)
         for d in range(3):
             if ("PA", d) in used:
@@ -820,7 +825,8 @@ def emit_quartet_staged(cls) -> str:
         for gi, tg in enumerate(tgroups):
             E("template <typename R>", 0)
             E(f"__device__ __noinline__ void fj_g{gi}_{name}(const double* __restrict__ bp, int k_ab,", 0)
-            E("const double* __restrict__ kp, int k_cd, double* __restrict__ ws, size_t st) {", 1)
+            E("const double* __restrict__ kp, int k_cd, double* __restrict__ ws, size_t st,", 1)
+            E("const double* __restrict__ tb) {", 1)
             group_body(tg, 1)
             E("}", 0)
     E("template <typename R>", 0)
@@ -830,12 +836,13 @@ def emit_quartet_staged(cls) -> str:
     E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
     E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
     E("double s_j, double s_k, const double* __restrict__ wab,")
-    E("double* __restrict__ fab, double* __restrict__ F, double* __restrict__ ws, size_t st) {", 1)
+    E("double* __restrict__ fab, double* __restrict__ F, double* __restrict__ ws, size_t st,", 1)
+    E("const double* __restrict__ tb) {", 1)
     if W:
         E(f"// (e0|f0) [f][e], (ab|f0) [ab][f] and F_cd in the workspace ({W} doubles per lane)")
     if W and fns:
         for gi in range(len(tgroups)):
-            E(f"fj_g{gi}_{name}<R>(bp, k_ab, kp, k_cd, ws, st);")
+            E(f"fj_g{gi}_{name}<R>(bp, k_ab, kp, k_cd, ws, st, tb);")
     else:
         if not W:
             E(f"double bnd[{NF * NE}];   // contracted (e0|f0), [f][e]")

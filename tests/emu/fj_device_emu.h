@@ -80,3 +80,29 @@ inline void fj_boys(double T, double (&F)[M + 1]) {
             for (int m = 1; m <= M; ++m) { f = ((double)(2 * m - 1) * f - e) * h; F[m] = f; } }
     }
 }
+#define FJ_BOYS_LROW 10
+template <int M>
+inline void fj_boys_stage(double* tb) {
+    static_assert(M + 8 <= 20, "Boys table holds F_0..F_20");
+    for (int j = (int)threadIdx.x; j < FJ_BOYS_NROW * FJ_BOYS_LROW; j += (int)blockDim.x) {
+        const int i = j / FJ_BOYS_LROW, c = j - i * FJ_BOYS_LROW;
+        const int g = FJ_BOYS_ROW * i + (c ? M + c : 0);
+        if (g < 0 || g >= FJ_BOYS_NROW * FJ_BOYS_ROW || (c && M + c >= FJ_BOYS_ROW)) {
+            fprintf(stderr, "boys stage index %d\n", g); abort();
+        }
+        tb[j] = fj_boys_tab[g];
+    }
+}
+// the LDS copy must hold exactly what the global table holds for order M
+template <int M>
+inline void fj_boys_lds(double T, double (&F)[M + 1], const double* tb) {
+    if (T < FJ_BOYS_TMAX) {
+        const int i = (int)(T * 8.0 + 0.5);
+        if (i < 0 || i >= FJ_BOYS_NROW) { fprintf(stderr, "boys lds row %d T %g\n", i, T); abort(); }
+        for (int c = 0; c < FJ_BOYS_LROW; ++c) {
+            const double want = fj_boys_tab[FJ_BOYS_ROW * i + (c ? M + c : 0)];
+            if (tb[FJ_BOYS_LROW * i + c] != want) { fprintf(stderr, "boys lds row %d col %d differs\n", i, c); abort(); }
+        }
+    }
+    fj_boys<M>(T, F);
+}
