@@ -113,6 +113,29 @@ def test_builder_config1_mixed():
     assert dG <= 1e-8 and dE <= 1e-6
 
 
+@pytest.mark.parametrize("which", ["c1", "c2-10"])
+def test_builder_all_fp32_tier(which):
+    """Every tile in the FP32 tier (fp32_bound above every bound), so every
+    class's FP32 kernel runs, the staged ones with their own larger VRR
+    groups: against the FP64 build. FP32 VRR values carry ~1e-7 relative
+    error, amplified by the HRR, so the bar is 1e-3 max|G| -- far above
+    that, far below the O(max|G|) error a wrong group, target or
+    workspace index gives. (The mixed-precision bars are held by the
+    mixed tests, where the rule keeps such tiles in FP64.)"""
+    import torch
+    s = config1() if which == "c1" else config2(seed=11, n_atoms=10)
+    Dd = native.to_dev(s.D)
+    G64 = FockBuilder(s.basis, s.D, tau=1e-12).build(Dd).clone()
+    fb = FockBuilder(s.basis, s.D, tau=1e-12, mixed=True, fp32_bound=1e30)
+    G32 = fb.build(Dd)
+    torch.cuda.synchronize()
+    assert fb.fp32_share() > 1.0 - 1e-12
+    scale = G64.abs().max().item()
+    err = (G32 - G64).abs().max().item()
+    _print(which, "all-FP32 max|dG|", err, "max|G|", scale)
+    assert err <= 1e-3 * scale
+
+
 def test_builder_repeatable():
     """Two builds of the same density agree to the atomics' reordering."""
     import torch
