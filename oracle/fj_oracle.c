@@ -17,6 +17,7 @@
  *                                                     hgp/hrr.py:101-120,
  *                                                     hgp/recurrence.py:308-335
  *   oracle_f_launch      one fused-F launch (same args as the device entry)
+ *   oracle_fx_launch_mmd the same launch with MMD integrals (no HRR)
  *                                                     eri/fock/kernel_f.py:1-165,
  *                                                     levers/seams.py:400-770
  *   oracle_fock_ordered  J, K by EVERY ordered quartet, weight 1
@@ -548,6 +549,78 @@ void oracle_fx_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
         }
         free(V);
         free(C);
+        free(x);
+    }
+    for (int t = 0; t < nthr; ++t)
+        for (size_t i = 0; i < (size_t)n_ao * n_ao; ++i) F[i] += Fp[(size_t)t * n_ao * n_ao + i];
+    free(Fp);
+}
+
+/* The same launch (screen, canonical half, weights, six-block digestion)
+ * with every quartet's integrals from the MMD path over the shells' full
+ * primitive lists instead of the HGP VRR/HRR over the launch's records:
+ * MMD has no HRR, so it keeps full FP64 accuracy where the HGP transfer
+ * cancels (a very tight primitive paired with a distant diffuse shell). */
+void oracle_fx_launch_mmd(int la, int lb, int lc, int ld, int n_bra, int n_ket, int n_sh,
+                          int n_ao, int diag, double tau, const double *G_ab, const int *bra_sh,
+                          const int *bra_ao, const double *G_cd, const int *ket_sh,
+                          const int *ket_ao, const double *Dsh, const double *D,
+                          const int *shell_l, const double *centre, const int *nprim,
+                          const int *pstart, const double *exps, const double *coefs,
+                          double *F) {
+    int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    double *Fp = calloc((size_t)nthr * n_ao * n_ao, sizeof(double));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *Ft = Fp + (size_t)tid * n_ao * n_ao;
+        double *x = malloc(sizeof(double) * na * nb * nc * nd);
+#pragma omp for schedule(dynamic, 1)
+        for (int bra = 0; bra < n_bra; ++bra) {
+            int sa = bra_sh[2 * bra], sb = bra_sh[2 * bra + 1];
+            int oa = bra_ao[2 * bra], ob = bra_ao[2 * bra + 1];
+            shell_t A = mkshell(sa, shell_l, centre, nprim, pstart, exps, coefs);
+            shell_t B = mkshell(sb, shell_l, centre, nprim, pstart, exps, coefs);
+            for (int k = diag * bra; k < n_ket; ++k) {
+                int sc = ket_sh[2 * k], sd = ket_sh[2 * k + 1];
+                int oc = ket_ao[2 * k], od = ket_ao[2 * k + 1];
+                double dm = 2.0 * Dsh[sa * n_sh + sb], t;
+                if ((t = 2.0 * Dsh[sc * n_sh + sd]) > dm) dm = t;
+                if ((t = 0.5 * Dsh[sa * n_sh + sc]) > dm) dm = t;
+                if ((t = 0.5 * Dsh[sa * n_sh + sd]) > dm) dm = t;
+                if ((t = 0.5 * Dsh[sb * n_sh + sc]) > dm) dm = t;
+                if ((t = 0.5 * Dsh[sb * n_sh + sd]) > dm) dm = t;
+                if (G_ab[bra] * (G_cd[k] * dm) < tau) continue;
+                shell_t Cs = mkshell(sc, shell_l, centre, nprim, pstart, exps, coefs);
+                shell_t Ds = mkshell(sd, shell_l, centre, nprim, pstart, exps, coefs);
+                eri_mmd_sh(&A, &B, &Cs, &Ds, x);
+                double deg = 1.0;
+                if (sa == sb) deg *= 0.5;
+                if (sc == sd) deg *= 0.5;
+                if (diag && k == bra) deg *= 0.5;
+                double sj = 2.0 * deg, sk = -0.5 * deg;
+                for (int i = 0; i < na; ++i)
+                    for (int j = 0; j < nb; ++j)
+                        for (int c = 0; c < nc; ++c)
+                            for (int d = 0; d < nd; ++d) {
+                                double v = x[((i * nb + j) * nc + c) * nd + d];
+                                int Ai = oa + i, Bj = ob + j, Cc = oc + c, Dd = od + d;
+                                Ft[Ai * n_ao + Bj] += sj * D[Cc * n_ao + Dd] * v;
+                                Ft[Cc * n_ao + Dd] += sj * D[Ai * n_ao + Bj] * v;
+                                Ft[Ai * n_ao + Cc] += sk * D[Bj * n_ao + Dd] * v;
+                                Ft[Ai * n_ao + Dd] += sk * D[Bj * n_ao + Cc] * v;
+                                Ft[Bj * n_ao + Cc] += sk * D[Ai * n_ao + Dd] * v;
+                                Ft[Bj * n_ao + Dd] += sk * D[Ai * n_ao + Cc] * v;
+                            }
+            }
+        }
         free(x);
     }
     for (int t = 0; t < nthr; ++t)

@@ -46,6 +46,7 @@ def lib():
                                       [vp] * 13)
         L.oracle_fx_launch.argtypes = ([C.c_int] * 8 + [vp] * 2 + [C.c_int] * 3 +
                                        [C.c_double] + [vp] * 13)
+        L.oracle_fx_launch_mmd.argtypes = [C.c_int] * 9 + [C.c_double] + [vp] * 15
         _lib = L
     return _lib
 
@@ -113,3 +114,19 @@ def fx_launch(cls, bra: dict, ket: dict, diag: int, n_sh: int, n_ao: int, tau: f
         n_sh, n_ao, diag, float(tau), _ptr(b["prim"]), _ptr(b["sep"]), _ptr(b["G"]),
         _ptr(b["sh"]), _ptr(b["ao"]), _ptr(k["prim"]), _ptr(k["sep"]), _ptr(k["G"]),
         _ptr(k["sh"]), _ptr(k["ao"]), _ptr(c(Dsh)), _ptr(c(D)), _ptr(F))
+
+
+def fx_launch_mmd(cls, bra: dict, ket: dict, diag: int, basis, tau: float, Dsh, D, F) -> None:
+    """fx_launch's screen, weights and digestion with every quartet's
+    integrals by MMD over the shells' full primitive lists (no HRR: full
+    FP64 accuracy for tight-diffuse pairs, where the HGP transfer cancels).
+    Each side needs G, sh, ao."""
+    la, lb, lc, ld = cls
+    c = lambda a: np.ascontiguousarray(a)  # noqa: E731
+    b, k = {x: c(v) for x, v in bra.items()}, {x: c(v) for x, v in ket.items()}
+    Dsh, D = c(np.asarray(Dsh, dtype=np.float64)), c(np.asarray(D, dtype=np.float64))
+    arrs = [c(a) for a in _b(basis)]
+    lib().oracle_fx_launch_mmd(
+        la, lb, lc, ld, len(b["G"]), len(k["G"]), basis.n_sh, basis.n_ao, diag, tau,
+        _ptr(b["G"]), _ptr(b["sh"]), _ptr(b["ao"]), _ptr(k["G"]), _ptr(k["sh"]), _ptr(k["ao"]),
+        _ptr(Dsh), _ptr(D), *[_ptr(a) for a in arrs], _ptr(F))
