@@ -30,6 +30,7 @@ struct FjFArgs {
     long long ws_doubles;  // its size
     int n_bx;              // work items: n_items = n_bx * (grid y of a walk launch)
     int n_items;
+    int n_fp64;            // a mixed launch: items [0, n_fp64) FP64, the rest FP32
 };
 
 // Workgroups of a workspace class: at most FJ_WS_MAX_BLOCKS in one launch;

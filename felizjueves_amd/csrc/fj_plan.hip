@@ -22,6 +22,9 @@
 
 namespace {
 
+constexpr size_t WAVE = 64;              // lanes of a wavefront
+constexpr long long FILL_MIN_PRIMS = 4;  // halve a lane to fill idle lanes from this many pairs
+
 struct Tier {
     std::vector<int> work;      // int4 per item
     std::vector<long long> cost;
@@ -52,8 +55,24 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
         const int b1 = (int)((long long)bra->n * (ci + 1) / n_chunk);
         Tier* tier = &part[2 * (size_t)ci];
         std::vector<int> kl[2];
-        std::vector<long long> cnt, lanes;
-        std::vector<int> order;
+        std::vector<long long> cnt;
+        std::vector<int> order, lk, llo;
+        // stable sort of the lanes by primitive count, descending
+        auto sort_lanes = [&]() {
+            order.resize(lk.size());
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+            std::vector<int> k2(lk.size()), l2(lk.size());
+            std::vector<long long> c2(lk.size());
+            for (size_t i = 0; i < order.size(); ++i) {
+                k2[i] = lk[order[i]];
+                l2[i] = llo[order[i]];
+                c2[i] = cnt[order[i]];
+            }
+            lk.swap(k2);
+            llo.swap(l2);
+            cnt.swap(c2);
+        };
         for (int b = b0; b < b1; ++b) {
             const int m = bra->mol[b];
             const int lo = mol_lo[m], hi = mol_hi[m];
@@ -99,7 +118,9 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
                                    ? kab : std::max(1, (int)std::floor(lane_flops / per_prim));
                 const double ckd = std::floor(lane_flops / (cb * per_prim));
                 const int ck = std::max(1, (int)std::min((double)max_prim, ckd));
-                lanes.clear();
+                // lanes (ket, first primitive pair, count), equal chunks per ket
+                lk.clear();
+                llo.clear();
                 cnt.clear();
                 for (int k : kl[t]) {
                     const int kcd = ket->deg[k];
@@ -107,24 +128,45 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
                     const int size = (kcd + n_ch - 1) / n_ch;
                     for (int j = 0; j < n_ch; ++j) {
                         const int lo_p = j * size;
-                        const int c = std::min(size, kcd - lo_p);
-                        long long v = k;
-                        if (n_ch > 1) {
-                            const int enc = (lo_p + c >= kcd) ? 0 : c;
-                            v |= ((long long)lo_p << 17) | ((long long)enc << 24);
-                        }
-                        lanes.push_back(v);
-                        cnt.push_back(c);
+                        lk.push_back(k);
+                        llo.push_back(lo_p);
+                        cnt.push_back(std::min(size, kcd - lo_p));
                     }
                 }
-                order.resize(lanes.size());
-                std::iota(order.begin(), order.end(), 0);
-                std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+                sort_lanes();
+                // fill the last wavefront's idle lanes: halve the (first)
+                // lane with the most primitive pairs while it has at least
+                // FILL_MIN_PRIMS, ceil half in place, the rest appended
+                const size_t target = (lk.size() + WAVE - 1) / WAVE * WAVE;
+                if (target != lk.size()) {
+                    while (lk.size() < target) {
+                        size_t i = 0;
+                        for (size_t x = 1; x < cnt.size(); ++x)
+                            if (cnt[x] > cnt[i]) i = x;
+                        const long long c = cnt[i];
+                        if (c < FILL_MIN_PRIMS) break;
+                        const long long h = (c + 1) / 2;
+                        cnt[i] = h;
+                        lk.push_back(lk[i]);
+                        llo.push_back(llo[i] + (int)h);
+                        cnt.push_back(c - h);
+                    }
+                    sort_lanes();
+                }
                 const long long st = (long long)T.kets.size();
-                for (int i : order) T.kets.push_back((int)lanes[i]);
-                for (size_t t0 = 0; t0 < order.size(); t0 += threads) {
-                    const size_t t1 = std::min(order.size(), t0 + (size_t)threads);
-                    const long long pmax = cnt[order[t0]];
+                for (size_t i = 0; i < lk.size(); ++i) {
+                    const int k = lk[i], lo_p = llo[i];
+                    const int kcd = ket->deg[k];
+                    long long v = k;
+                    if (!(lo_p == 0 && cnt[i] == kcd)) {
+                        const long long enc = (lo_p + cnt[i] >= kcd) ? 0 : cnt[i];
+                        v |= ((long long)lo_p << 17) | (enc << 24);
+                    }
+                    T.kets.push_back((int)v);
+                }
+                for (size_t t0 = 0; t0 < lk.size(); t0 += threads) {
+                    const size_t t1 = std::min(lk.size(), t0 + (size_t)threads);
+                    const long long pmax = cnt[t0];
                     for (int blo = 0; blo < kab; blo += cb) {
                         const int bn = (cb >= kab) ? 0 : std::min(cb, kab - blo);
                         T.work.push_back(b);

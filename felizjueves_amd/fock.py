@@ -38,6 +38,7 @@ from .features import side_arrays, tile_features
 from .hgp import cost
 
 FP64, FP32 = 0, 1
+MIXED = 2   # one launch for both tiers: items [0, n_fp64) FP64, the rest FP32
 
 # The FP32 tier's error rule (the fallback of the learned selector): the
 # FP32 VRR's error is not bounded by the final integrals' Schwarz bound
@@ -106,6 +107,70 @@ class Launch:
     n_quartets: int = 0       # computed (post-screen) contracted quartets
     n_prim: int = 0           # primitive quartets of those
     flops: float = 0.0        # algorithmic FLOPs of those
+    n_fp64: int = 0           # MIXED: work items [0, n_fp64) run FP64, the rest FP32
+    n_quartets32: int = 0     # MIXED: the FP32 items' quartets
+
+
+WAVE = 64             # lanes of a wavefront
+FILL_MIN_PRIMS = 4    # a lane is halved to fill idle lanes only from this many primitive pairs
+
+
+def fill_wave(k: np.ndarray, lo: np.ndarray, cnt: np.ndarray):
+    """Lanes (ket, first primitive pair, count), sorted by count descending,
+    of one bra pair and tier: the last wavefront's idle lanes are filled by
+    halving the lane with the most primitive pairs (the first such, ceil
+    half kept in place, the rest appended) while that count is at least
+    FILL_MIN_PRIMS, then the lanes are stably re-sorted by count. A
+    wavefront runs as long as its longest lane; the halves run in lanes
+    that would idle, so the same wavefronts finish sooner."""
+    n = len(cnt)
+    target = -(-n // WAVE) * WAVE
+    if n == target or n == 0:
+        return k, lo, cnt
+    k, lo, cnt = list(k), list(lo), list(cnt)
+    while len(cnt) < target:
+        i = max(range(len(cnt)), key=lambda x: (cnt[x], -x))
+        c = cnt[i]
+        if c < FILL_MIN_PRIMS:
+            break
+        h = (c + 1) // 2
+        cnt[i] = h
+        k.append(k[i])
+        lo.append(lo[i] + h)
+        cnt.append(c - h)
+    k, lo, cnt = np.array(k), np.array(lo, dtype=np.int64), np.array(cnt, dtype=np.int64)
+    order = np.argsort(-cnt, kind="stable")
+    return k[order], lo[order], cnt[order]
+
+
+def merge_tier_launches(launches: list) -> list:
+    """Each class pair's FP64 and FP32 launches as one MIXED launch: the
+    FP64 work items (in their order) then the FP32 ones, the FP32 items'
+    ket-list ranges moved past the FP64 list; biggest launches first."""
+    by = {}
+    for L in launches:
+        by.setdefault(L.cls, {})[L.precision] = L
+    out = []
+    for cls, d in by.items():
+        if FP64 in d and FP32 in d:
+            a, b = d[FP64], d[FP32]
+            wb = b.work.copy()
+            off = len(a.ket_list)
+            wb[:, 1] += off
+            wb[:, 3] += off
+            M = Launch(cls, a.bra, a.ket, a.diag, MIXED, np.concatenate([a.work, wb]).astype(np.int32))
+            M.threads = a.threads
+            M.ket_list = np.concatenate([a.ket_list, b.ket_list]).astype(np.int32)
+            M.n_fp64 = a.work.shape[0]
+            M.n_quartets = a.n_quartets + b.n_quartets
+            M.n_quartets32 = b.n_quartets
+            M.n_prim = a.n_prim + b.n_prim
+            M.flops = a.flops + b.flops
+            out.append(M)
+        else:
+            out.extend(d.values())
+    out.sort(key=lambda L: -L.flops)
+    return out
 
 
 class FockBuilder:
@@ -115,10 +180,11 @@ class FockBuilder:
 
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads=64,
                  mixed: bool = False, fp32_error: float = 1e-11, selector=None,
-                 lane_flops=None, fp32_bound: float | None = None, planner: str = "native"):
+                 lane_flops=None, fp32_bound: float | None = None, planner: str = "native",
+                 merge_tiers: bool = True):
         self._setup([(basis, D)], tau=tau, threads=threads, mixed=mixed, fp32_error=fp32_error,
                      selector=selector, lane_flops=lane_flops, fp32_bound=fp32_bound,
-                     planner=planner)
+                     planner=planner, merge_tiers=merge_tiers)
 
     @classmethod
     def batch(cls, systems, **kw) -> "FockBuilder":
@@ -131,14 +197,19 @@ class FockBuilder:
 
     def _setup(self, systems, *, tau: float = 1e-12, threads=64, mixed: bool = False,
                fp32_error: float = 1e-11, selector=None, lane_flops=None,
-               fp32_bound: float | None = None, planner: str = "native"):
+               fp32_bound: float | None = None, planner: str = "native",
+               merge_tiers: bool = True):
         """threads, lane_flops: one value for every class pair, or a dict
         {(la, lb, lc, ld): value} (the autotuner's choice per class pair;
         missing ones take the default 64 and the automatic split).
         fp32_bound: instead of the amplified estimate, the plain rule "FP32
         iff G_ab G_cd dmax < fp32_bound" (the rule the working-size mixed
         configuration is specified with; not accurate enough for the FP64
-        G bars, kept for comparison)."""
+        G bars, kept for comparison).
+        merge_tiers: a class pair with tiles in both tiers runs as ONE
+        launch (fj_fxm_*: each work item in its tier) instead of one per
+        tier, so the FP32 tiles add no launches and no launch tails."""
+        self.merge_tiers = bool(merge_tiers)
         self.fp32_bound = fp32_bound
         if not systems:
             raise ValueError("no systems")
@@ -208,6 +279,8 @@ class FockBuilder:
         # features (the Python planner, which it reproduces exactly)
         self.planner = "python" if (planner == "python" or (mixed and selector is not None)) else "native"
         self.launches = self._plan() if self.planner == "python" else self._plan_native()
+        if self.merge_tiers:
+            self.launches = merge_tier_launches(self.launches)
         self.plan_seconds = time.time() - t0
         self._dev = False
 
@@ -405,11 +478,12 @@ class FockBuilder:
                 j = np.arange(len(rep)) - np.repeat(np.cumsum(n_ch) - n_ch, n_ch)
                 lo = j * size[rep]
                 cnt = np.minimum(size[rep], kcd[rep] - lo)
-                whole = n_ch[rep] == 1
+                order = np.argsort(-cnt, kind="stable")
+                rep, lo, cnt = fill_wave(rep[order], lo[order], cnt[order])
+                whole = (lo == 0) & (cnt == kcd[rep])
                 enc = np.where(whole, 0, np.where(lo + cnt >= kcd[rep], 0, cnt))
                 lanes = seg[rep].astype(np.int64) | (np.where(whole, 0, lo) << 17) | (enc << 24)
-                order = np.argsort(-cnt, kind="stable")
-                lanes, lane_prims = lanes[order], cnt[order]
+                lane_prims = cnt
                 st = n_list[tr]
                 lists[tr].append(lanes)
                 n_list[tr] += len(lanes)
@@ -507,6 +581,10 @@ class FockBuilder:
         return ws[i]
 
     def launch(self, L: "Launch", sys, stream=None, ws=None) -> None:
+        if L.precision == MIXED:
+            native.call_fxm(L.cls, L.threads, L.bra.side, L.ket.side, L.work_dev, L.work.shape[0],
+                            L.n_fp64, sys, L.diag, self.tau, stream, L.ket_list_dev, ws)
+            return
         native.call_fx(L.cls, L.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
                        L.work.shape[0], sys, L.diag, self.tau, stream, L.ket_list_dev, ws)
 
@@ -608,7 +686,8 @@ class FockBuilder:
 
     def fp32_share(self) -> float:
         q = self.n_quartets
-        return sum(L.n_quartets for L in self.launches if L.precision == FP32) / q if q else 0.0
+        return sum(L.n_quartets if L.precision == FP32 else L.n_quartets32 if L.precision == MIXED
+                   else 0 for L in self.launches) / q if q else 0.0
 
 
 def energy(D: np.ndarray, G: np.ndarray) -> float:

@@ -277,6 +277,20 @@ def emit_class(cls) -> str:
         kernel_body = "    fj_item_" + name + "<R>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);"
         launch_ws = ""
         f_ws = ""
+    # the mixed kernel: the same body, each work item in its tier (items
+    # [0, n_fp64) FP64, the rest FP32; uniform over the workgroup)
+    mixed_body = (kernel_body
+                  .replace(f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n",
+                           f"        if (it % a.n_bx < a.n_fp64)\n"
+                           f"            fj_item_{name}<double>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n"
+                           f"        else\n"
+                           f"            fj_item_{name}<float>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n")
+                  .replace("    fj_item_" + name + "<R>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);",
+                           "    if ((int)blockIdx.x < a.n_fp64)\n"
+                           "        fj_item_" + name + "<double>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);\n"
+                           "    else\n"
+                           "        fj_item_" + name + "<float>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);"))
+    assert mixed_body != kernel_body
     out.append(f"""
 // One work item: workgroup-uniform early returns only (the persistent
 // kernel calls it in a loop with a barrier after each item).
@@ -392,13 +406,20 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
 {kernel_body}
 }}
 
-// the FP32-tier instantiation lives in its own translation unit (f_{name}_r32.hip)
+// the FP32-tier instantiation and the mixed kernel live in their own
+// translation units (f_{name}_r32.hip, f_{name}_mx.hip)
 int fj_launch_{name}_r32(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a);
+int fj_launch_{name}_mx(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a);
+//@@MIXED@@
+__global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_km(FjFArgs a) {{
+    __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
+{mixed_body}
+}}
 //@@SPLIT@@
 static int fj_launch_{name}(hipStream_t s, int threads, int precision, FjFArgs a, int n_blocks) {{
     if (threads <= 0) threads = FJ_DEFAULT_THREADS;
     if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
-    if (precision != 0 && precision != 1) return FJ_ERR_ARG;
+    if (precision < 0 || precision > 2) return FJ_ERR_ARG;
     if (n_blocks <= 0 || a.ket.n <= 0) return 0;
     dim3 grid((unsigned)n_blocks, a.work ? 1u : (unsigned)((a.ket.n + threads - 1) / threads));
     a.n_bx = (int)grid.x;
@@ -408,6 +429,7 @@ static int fj_launch_{name}(hipStream_t s, int threads, int precision, FjFArgs a
         hipLaunchKernelGGL(fj_f_{name}_k<double>, grid, dim3(threads), 0, s, a);
         return (int)hipGetLastError();
     }}
+    if (precision == 2) return fj_launch_{name}_mx(s, grid, dim3(threads), a);
     return fj_launch_{name}_r32(s, grid, dim3(threads), a);
 }}
 
@@ -427,6 +449,7 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
     a.one = FjSys{{D, Dsh, F, n_ao, n_sh}};
     a.diag = diag; a.tau = tau;
     a.ws = nullptr; a.ws_doubles = 0;
+    a.n_fp64 = 0;
 {f_ws}
     return fj_launch_{name}(s, threads, 0, a, n_bra);
 }}
@@ -444,7 +467,26 @@ extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
     a.one = FjSys{{nullptr, nullptr, nullptr, 0, 0}};
     a.diag = diag; a.tau = tau;
     a.ws = ws; a.ws_doubles = ws ? ws_bytes / 8 : 0;
+    a.n_fp64 = 0;
+    if (precision != 0 && precision != 1) return FJ_ERR_ARG;
     return fj_launch_{name}(s, threads, precision, a, n_work);
+}}
+
+extern "C" int fj_fxm_{name}(hipStream_t s, int threads, const FjSide* bra, const FjSide* ket,
+                          const int* work, int n_work, int n_fp64, const int* ket_list,
+                          const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes) {{
+    if (!bra || !ket || !work || !sys || ws_bytes < 0) return FJ_ERR_ARG;
+    if (n_fp64 < 0 || n_fp64 > n_work) return FJ_ERR_ARG;
+    FjFArgs a;
+    a.bra = *bra; a.ket = *ket;
+    a.work = reinterpret_cast<const int4*>(work);
+    a.ket_list = ket_list;
+    a.sys = sys;
+    a.one = FjSys{{nullptr, nullptr, nullptr, 0, 0}};
+    a.diag = diag; a.tau = tau;
+    a.ws = ws; a.ws_doubles = ws ? ws_bytes / 8 : 0;
+    a.n_fp64 = n_fp64;
+    return fj_launch_{name}(s, threads, 2, a, n_work);
 }}
 """)
     return "\n".join(out)
@@ -506,6 +548,12 @@ def generate(classes, out: Path = GEN) -> list[Path]:
     for cls in classes:
         n = cls_name(cls)
         head, main = emit_class(cls).split("//@@SPLIT@@\n")
+        head, mixed = head.split("//@@MIXED@@\n")
+        mx = (f"// generated by felizjueves_amd/gen_hip.py -- mixed-tier kernel of ({n})\n"
+              f'#include "f_{n}.h"\n\n' + mixed +
+              f"\nint fj_launch_{n}_mx(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a) {{\n"
+              f"    hipLaunchKernelGGL(fj_f_{n}_km, grid, block, 0, s, a);\n"
+              f"    return (int)hipGetLastError();\n}}\n")
         r32 = (f"// generated by felizjueves_amd/gen_hip.py -- FP32-tier kernel of ({n})\n"
                f'#include "f_{n}.h"\n\n'
                f"int fj_launch_{n}_r32(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a) {{\n"
@@ -513,12 +561,13 @@ def generate(classes, out: Path = GEN) -> list[Path]:
                f"    return (int)hipGetLastError();\n}}\n")
         texts = {f"f_{n}.h": head,
                  f"f_{n}.hip": f'// generated by felizjueves_amd/gen_hip.py\n#include "f_{n}.h"\n' + main,
-                 f"f_{n}_r32.hip": r32}
+                 f"f_{n}_r32.hip": r32,
+                 f"f_{n}_mx.hip": mx}
         for fname, text in texts.items():
             p = out / fname
             if not p.exists() or p.read_text() != text:
                 p.write_text(text)
-        files += [out / f"f_{n}.hip", out / f"f_{n}_r32.hip"]
+        files += [out / f"f_{n}.hip", out / f"f_{n}_r32.hip", out / f"f_{n}_mx.hip"]
     wt = out / "fj_ws_table.hip"
     text = emit_ws_table(classes)
     if not wt.exists() or wt.read_text() != text:
@@ -565,7 +614,10 @@ def header_decls(classes) -> str:
             f"int fj_fx_{n}(fj_stream_t stream, int threads, int precision, const FjSide* bra,"
             f" const FjSide* ket,\n"
             f"    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,\n"
-            f"    double tau, double* ws, long long ws_bytes);")
+            f"    double tau, double* ws, long long ws_bytes);\n"
+            f"int fj_fxm_{n}(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,\n"
+            f"    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,\n"
+            f"    int diag, double tau, double* ws, long long ws_bytes);")
     return "\n".join(out) + "\n"
 
 

@@ -143,6 +143,20 @@ def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, work, n
     check(rc, fn.__name__)
 
 
+def call_fxm(cls, threads: int, bra: FjSide, ket: FjSide, work, n_work: int, n_fp64: int,
+             sys, diag: int, tau: float, stream=None, ket_list=None, ws=None) -> None:
+    """One mixed-tier launch of fj_fxm_<cls>: work items [0, n_fp64) run in
+    FP64, the rest in FP32 (otherwise as call_fx)."""
+    fn = getattr(lib(), "fj_fxm_" + "".join(map(str, cls)))
+    fn.restype = C.c_int
+    ws_bytes = 0 if ws is None else ws.numel() * ws.element_size()
+    rc = fn(C.c_void_p(stream_handle(stream)), C.c_int(threads), C.byref(bra), C.byref(ket),
+            C.c_void_p(ptr(work)), C.c_int(n_work), C.c_int(n_fp64), C.c_void_p(ptr(ket_list)),
+            C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau), C.c_void_p(ptr(ws)),
+            C.c_longlong(ws_bytes))
+    check(rc, fn.__name__)
+
+
 def workspace_bytes(cls, threads: int, n_work: int) -> int:
     """Bytes of launch workspace fj_fx_<cls> uses for n_work items."""
     fn = lib().fj_fx_workspace_bytes

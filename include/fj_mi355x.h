@@ -157,6 +157,9 @@ int fj_f_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0000(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0000(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -165,6 +168,9 @@ int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0001(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0001(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -173,6 +179,9 @@ int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0002(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -181,6 +190,9 @@ int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0011(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0011(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -189,6 +201,9 @@ int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0012(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -197,6 +212,9 @@ int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -205,6 +223,9 @@ int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0101(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0101(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -213,6 +234,9 @@ int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0102(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -221,6 +245,9 @@ int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0111(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -229,6 +256,9 @@ int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -237,6 +267,9 @@ int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -245,6 +278,9 @@ int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0202(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -253,6 +289,9 @@ int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0211(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0211(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -261,6 +300,9 @@ int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -269,6 +311,9 @@ int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_0222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -277,6 +322,9 @@ int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1111(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_1111(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -285,6 +333,9 @@ int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_1112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -293,6 +344,9 @@ int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_1122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -301,6 +355,9 @@ int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_1212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -309,6 +366,9 @@ int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_1222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -317,6 +377,9 @@ int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, i
 int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
+int fj_fxm_2222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 /* END F CLASSES */
 
 /* The MMD families, one entry per class, each with the upstream kernel's

@@ -38,7 +38,7 @@ typedef struct ihipStream_t* hipStream_t;
 #define FJ_BOYS_TMAX 40.0
 struct FjFArgs {
     FjSide bra, ket; const int4* work; const int* ket_list; const FjSys* sys; FjSys one;
-    int diag; double tau; double* ws; long long ws_doubles; int n_bx; int n_items;
+    int diag; double tau; double* ws; long long ws_doubles; int n_bx; int n_items; int n_fp64;
 };
 extern std::barrier<>* g_bar;
 extern double g_shfl[256];

@@ -6,7 +6,7 @@ split, in both tiers."""
 import numpy as np
 import pytest
 
-from felizjueves_amd.fock import FP32, FockBuilder
+from felizjueves_amd.fock import FP32, MIXED, FockBuilder
 from felizjueves_amd.systems import config1, config2
 
 
@@ -59,7 +59,12 @@ def test_work_lists_cover_each_primitive_quartet_once(kw):
     for key, m in cover.items():
         assert (m == 1).all(), key
     if "selector" in kw:
-        assert any(L.precision == FP32 for L in fb.launches)
+        # both tiers of a class pair run as one launch, each item in its tier
+        assert any(L.precision == MIXED for L in fb.launches)
+        split = FockBuilder(s.basis, s.D, tau=1e-12, merge_tiers=False, **kw)
+        assert any(L.precision == FP32 for L in split.launches)
+        assert fb.fp32_share() == split.fp32_share() > 0.0
+        assert _decode_cover(split).keys() == cover.keys()
 
 
 def test_small_system_is_split_finer_than_a_large_one():
