@@ -99,7 +99,7 @@ class Launch:
     ket: SideTable
     diag: int
     precision: int
-    work: np.ndarray          # [n, 2] int32
+    work: np.ndarray          # [n, 4] int32 items (bra, first ket entry, system | bra chunk, end entry)
     work_dev: object = None
     ket_list: np.ndarray = None
     ket_list_dev: object = None
