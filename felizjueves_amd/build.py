@@ -14,7 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 from . import gen_hip, gen_mmd
-from .hgp import fock_classes
+from .hgp import builder_classes
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
@@ -34,7 +34,7 @@ FLAGS = [
 
 
 def sources(lmax: int = LMAX) -> list[Path]:
-    classes = fock_classes(lmax)
+    classes = builder_classes(lmax)
     files = gen_hip.generate(classes)
     gen_hip.write_header(classes)
     mmd = gen_mmd.generate(MMD_LMAX)

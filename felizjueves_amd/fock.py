@@ -143,6 +143,73 @@ def fill_wave(k: np.ndarray, lo: np.ndarray, cnt: np.ndarray):
     return k[order], lo[order], cnt[order]
 
 
+# A pair's primitive pairs are kept together (built on one centre) while the
+# estimated FP64 amplification of the ones on the far side stays below this
+# (relative error ~ 1e-16 x this); otherwise the pair is split by centre.
+ORIENT_AMP_MAX = 10.0
+
+
+def orient_pairs(basis: Basis, pairs: list) -> list:
+    """Build every pair on the centre its primitive pairs sit near. The
+    HGP boundary is built on a pair's first centre and the HRR moves lb
+    units of angular momentum across |AB|: a primitive pair whose product
+    centre P lies near B has moments about A larger than its result by
+    about (1 + |PA| sqrt(2 zeta))^lb, and those cancel in the HRR, so its
+    rounding error is amplified by that much (against an MMD reference,
+    a 10-atom cluster's G was 1.3e-10 off built on A only, 1.7e-12
+    oriented). Per pair, with size_j = |U_j| (2 zeta_j)^(-(la+lb)/2) the
+    scale of primitive pair j's contribution and amp_j its amplification
+    in the other orientation: keep the pair (a, b) if the size-weighted
+    amplification of its near-B primitive pairs is below ORIENT_AMP_MAX,
+    else take it whole as (b, a) (class (lb, la), records (P, P - B, zeta,
+    U)) if that of its near-A ones is, else split it into the two parts.
+    The parts sum to the pair (the integral is linear in its primitive
+    pairs), and a bra part (b, a) reaches F[b][a] instead of F[a][b], which
+    the closing F + F^T makes the same."""
+    import copy
+    out = []
+    for p in pairs:
+        if p.a == p.b:
+            out.append(p)
+            continue
+        A = basis.shell_centre[p.a]
+        B = basis.shell_centre[p.b]
+        if not np.any(A != B):
+            out.append(p)
+            continue
+        P_ = p.rec[:, 0:3]
+        z = p.rec[:, 6]
+        dA = np.sqrt(((P_ - A) ** 2).sum(1))
+        dB = np.sqrt(((P_ - B) ** 2).sum(1))
+        near_b = dB < dA
+        size = np.abs(p.rec[:, 7]) * (2.0 * z) ** (-0.5 * (p.la + p.lb))
+        tot = size.sum()
+        amp_on_a = (1.0 + dA * np.sqrt(2.0 * z)) ** p.lb     # built on A
+        amp_on_b = (1.0 + dB * np.sqrt(2.0 * z)) ** p.la     # built on B
+        err_a = (size * amp_on_a)[near_b].sum() / tot
+        err_b = (size * amp_on_b)[~near_b].sum() / tot
+
+        def on_b(q, sel):
+            r = copy.copy(q)
+            r.a, r.b, r.la, r.lb = q.b, q.a, q.lb, q.la
+            r.prims = q.prims[sel]
+            rec = q.rec[sel].copy()
+            rec[:, 3:6] = rec[:, 0:3] - B
+            r.rec = np.ascontiguousarray(rec)
+            return r
+        if err_a <= ORIENT_AMP_MAX:
+            out.append(p)
+        elif err_b <= ORIENT_AMP_MAX:
+            out.append(on_b(p, np.ones(len(z), dtype=bool)))
+        else:
+            q = copy.copy(p)
+            q.prims = p.prims[~near_b]
+            q.rec = np.ascontiguousarray(p.rec[~near_b])
+            out.append(q)
+            out.append(on_b(p, near_b))
+    return out
+
+
 def merge_tier_launches(launches: list) -> list:
     """Each class pair's FP64 and FP32 launches as one MIXED launch: the
     FP64 work items (in their order) then the FP32 ones, the FP32 items'
@@ -181,10 +248,10 @@ class FockBuilder:
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads=64,
                  mixed: bool = False, fp32_error: float = 1e-11, selector=None,
                  lane_flops=None, fp32_bound: float | None = None, planner: str = "native",
-                 merge_tiers: bool = True):
+                 merge_tiers: bool = True, orient: bool = True):
         self._setup([(basis, D)], tau=tau, threads=threads, mixed=mixed, fp32_error=fp32_error,
                      selector=selector, lane_flops=lane_flops, fp32_bound=fp32_bound,
-                     planner=planner, merge_tiers=merge_tiers)
+                     planner=planner, merge_tiers=merge_tiers, orient=orient)
 
     @classmethod
     def batch(cls, systems, **kw) -> "FockBuilder":
@@ -198,7 +265,7 @@ class FockBuilder:
     def _setup(self, systems, *, tau: float = 1e-12, threads=64, mixed: bool = False,
                fp32_error: float = 1e-11, selector=None, lane_flops=None,
                fp32_bound: float | None = None, planner: str = "native",
-               merge_tiers: bool = True):
+               merge_tiers: bool = True, orient: bool = True):
         """threads, lane_flops: one value for every class pair, or a dict
         {(la, lb, lc, ld): value} (the autotuner's choice per class pair;
         missing ones take the default 64 and the automatic split).
@@ -208,8 +275,14 @@ class FockBuilder:
         G bars, kept for comparison).
         merge_tiers: a class pair with tiles in both tiers runs as ONE
         launch (fj_fxm_*: each work item in its tier) instead of one per
-        tier, so the FP32 tiles add no launches and no launch tails."""
+        tier, so the FP32 tiles add no launches and no launch tails.
+        orient: every shell pair's primitive pairs are built on the centre
+        nearer their product centre (`orient_pairs`): a pair may run as
+        (b, a), in class (lb, la), and a mixed pair as two partial pairs.
+        n_quartets then counts the partial quartets; shell_quartets() the
+        contracted shell quartets."""
         self.merge_tiers = bool(merge_tiers)
+        self.orient = bool(orient)
         self.fp32_bound = fp32_bound
         if not systems:
             raise ValueError("no systems")
@@ -248,6 +321,8 @@ class FockBuilder:
             if big:
                 raise ValueError(f"{len(big)} pairs keep more than {P.MAX_PRIM_PAIRS} primitive "
                                  "pairs, the staging cap of the F kernels")
+            if self.orient:
+                kept = orient_pairs(basis, kept)
             for p in kept:
                 by_cls.setdefault((p.la, p.lb), []).append((m, p))
             self.Dsh_plans.append(P.density_blocks(basis, D))
@@ -683,6 +758,17 @@ class FockBuilder:
     @property
     def flops(self) -> float:
         return sum(L.flops for L in self.launches)
+
+    def shell_quartets(self) -> int:
+        """Contracted shell quartets this build computes (the screened
+        canonical quartets, however the pairs were oriented or split)."""
+        if not self.orient:
+            return int(self.n_quartets)
+        if not hasattr(self, "_shell_quartets"):
+            one = FockBuilder.__new__(FockBuilder)
+            one._setup(self.systems, tau=self.tau, orient=False, merge_tiers=False)
+            self._shell_quartets = int(one.n_quartets)
+        return self._shell_quartets
 
     def fp32_share(self) -> float:
         q = self.n_quartets

@@ -28,7 +28,7 @@ from __future__ import annotations
 from pathlib import Path
 
 from .basis import cartesians, n_cart
-from .hgp import ZERO, hrr_dag, vrr_dag
+from .hgp import ZERO, canonical, hrr_dag, vrr_dag
 
 HERE = Path(__file__).resolve().parent
 GEN = HERE / "csrc" / "gen"
@@ -433,6 +433,7 @@ static int fj_launch_{name}(hipStream_t s, int threads, int precision, FjFArgs a
     return fj_launch_{name}_r32(s, grid, dim3(threads), a);
 }}
 
+//@@UPSTREAM@@
 extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
                         int n_sh, int n_ao, int diag, double tau,
                         const double* bra_prim,{" const double* AB," if lb else ""} const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -453,6 +454,7 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
 {f_ws}
     return fj_launch_{name}(s, threads, 0, a, n_bra);
 }}
+//@@END_UPSTREAM@@
 
 extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
                          const FjSide* bra, const FjSide* ket, const int* work, int n_work,
@@ -489,7 +491,12 @@ extern "C" int fj_fxm_{name}(hipStream_t s, int threads, const FjSide* bra, cons
     return fj_launch_{name}(s, threads, 2, a, n_work);
 }}
 """)
-    return "\n".join(out)
+    text = "\n".join(out)
+    i, j = text.index("//@@UPSTREAM@@\n"), text.index("//@@END_UPSTREAM@@\n")
+    if canonical(cls):
+        # the upstream-ABI entry (walker launches use canonical classes only)
+        return text[:i] + text[i + len("//@@UPSTREAM@@\n"):j] + text[j + len("//@@END_UPSTREAM@@\n"):]
+    return text[:i] + text[j + len("//@@END_UPSTREAM@@\n"):]
 
 
 # ---------------------------------------------------------------------------
@@ -603,14 +610,16 @@ def header_decls(classes) -> str:
     for cls in classes:
         la, lb, lc, ld = cls
         n = cls_name(cls)
-        out.append(
+        up = (
             f"int fj_f_{n}(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,\n"
             f"    int n_sh, int n_ao, int diag, double tau,\n"
             f"    const double* bra_prim,{' const double* AB,' if lb else ''} const double* G_ab,"
             f" const int* bra_sh, const int* bra_ao,\n"
             f"    const double* ket_prim,{' const double* CD,' if ld else ''} const double* G_cd,"
             f" const int* ket_sh, const int* ket_ao,\n"
-            f"    const double* Dsh, const double* D, double* F);\n"
+            f"    const double* Dsh, const double* D, double* F);\n") if canonical(cls) else ""
+        out.append(
+            up +
             f"int fj_fx_{n}(fj_stream_t stream, int threads, int precision, const FjSide* bra,"
             f" const FjSide* ket,\n"
             f"    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,\n"

@@ -223,6 +223,20 @@ def cost(cls) -> Cost:
                 vf, 2 * interior, sum(cls))
 
 
+def builder_classes(lmax: int) -> tuple[tuple[int, int, int, int], ...]:
+    """The classes the native builder launches: pair classes (la, lb) in
+    both orders (a shell pair's primitive pairs are built on the centre
+    nearer their product centre, so a (p, d) pair may run as (d, p)), bra
+    pair class <= ket pair class: 45 for lmax = 2. The canonical ones
+    (fock_classes) also carry the upstream-ABI entry."""
+    pairs = [(a, b) for a in range(lmax + 1) for b in range(lmax + 1)]
+    return tuple(p + q for i, p in enumerate(pairs) for q in pairs[i:])
+
+
+def canonical(cls) -> bool:
+    return cls[0] <= cls[1] and cls[2] <= cls[3]
+
+
 def fock_classes(lmax: int) -> tuple[tuple[int, int, int, int], ...]:
     """Every AO-order class (la lb|lc ld), la <= lb, lc <= ld, bra pair <=
     ket pair (upstream `hgp/vrr.py:89-91` for lmax = 2: 21 classes)."""

@@ -182,6 +182,12 @@ int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0002(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0010(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0010(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -202,6 +208,18 @@ int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0012(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0020(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0021(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0021(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -237,6 +255,12 @@ int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0102(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0110(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0110(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -257,6 +281,18 @@ int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0120(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0120(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0121(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0121(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -281,6 +317,12 @@ int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0202(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0210(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0210(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -303,6 +345,18 @@ int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0220(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0220(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0221(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0221(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -312,6 +366,42 @@ int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1010(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1010(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1011(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1011(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1012(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1020(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1021(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1021(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -336,6 +426,18 @@ int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_1112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1120(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1120(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1121(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1121(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -358,6 +460,18 @@ int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_1212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1220(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1220(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1221(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1221(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -367,6 +481,36 @@ int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2020(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2021(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2021(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2121(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2121(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,

@@ -9,7 +9,7 @@ from pathlib import Path
 import pytest
 
 from felizjueves_amd import gen_hip, native
-from felizjueves_amd.hgp import fock_classes
+from felizjueves_amd.hgp import builder_classes, fock_classes
 
 ROOT = Path(__file__).resolve().parent.parent
 HEADER = ROOT / "include" / "fj_mi355x.h"
@@ -27,12 +27,18 @@ def test_header_declares_entries():
     for cls in fock_classes(2):
         n = "".join(map(str, cls))
         assert f"fj_f_{n}" in names and f"fj_fx_{n}" in names
+    # the builder's classes (both pair orders) have the builder entries;
+    # the upstream-ABI entry exists for the canonical classes only
+    for cls in builder_classes(2):
+        n = "".join(map(str, cls))
+        assert f"fj_fx_{n}" in names and f"fj_fxm_{n}" in names
+        assert (f"fj_f_{n}" in names) == (cls in fock_classes(2))
 
 
 def test_header_matches_generator():
     cur = HEADER.read_text()
-    classes = tuple(c for c in fock_classes(3)
-                    if f"fj_f_{''.join(map(str, c))}(" in cur)
+    classes = tuple(c for c in builder_classes(3)
+                    if f"fj_fx_{''.join(map(str, c))}(" in cur)
     assert gen_hip.header_text(classes, cur) == cur
 
 

@@ -43,7 +43,9 @@ def oracle_build_mmd(fb: FockBuilder, D: np.ndarray) -> np.ndarray:
     """oracle_build with every quartet's integrals by MMD over the shells'
     full primitive lists (no HRR): the accurate FP64 reference where HGP's
     transfer cancels (very tight primitives paired with distant diffuse
-    shells)."""
+    shells). The integrals are per shell quartet, so it takes the
+    unoriented tables (one entry per shell pair, whatever fb's orientation)."""
+    fb = FockBuilder(fb.basis, D, tau=fb.tau, orient=False, merge_tiers=False)
     b = fb.basis
     host = builder_tables_host(fb)
     Dsh = P.density_blocks(b, D)
@@ -91,4 +93,4 @@ def test_builder_plan_vs_walk_config1():
     assert np.abs(G - w.F).max() < 1e-11
     # the plan's counts: every computed quartet is a unique quartet
     n_pairs = s.basis.n_sh * (s.basis.n_sh + 1) // 2
-    assert 0 < fb.n_quartets <= n_pairs * (n_pairs + 1) // 2
+    assert 0 < fb.shell_quartets() <= n_pairs * (n_pairs + 1) // 2
