@@ -196,7 +196,7 @@ def c3_block(steps, warmup, selector):
     J32 = jk.CoulombChain(s.basis, tau=1e-12, mixed=True).run(s.D)
     torch.cuda.synchronize()
     out = {
-        "workload": s.name, "n_ao": s.basis.n_ao, "quartets": plain.n_quartets,
+        "workload": s.name, "n_ao": s.basis.n_ao, "quartets": plain.shell_quartets(),
         "plain_fp64_builds_per_s": steps / t_plain,
         "production_builds_per_s": steps / t_prod,
         "speedup_vs_plain_fp64": t_plain / t_prod,
@@ -252,7 +252,7 @@ def c4_block(a) -> None:
     tuning, ns = load_tuning("c4_64mol_d.json")
     fb = FockBuilder.batch([(s.basis, s.D) for s in systems], tau=1e-12, **tuning)
     plan_s = time.time() - t0
-    log(f"c4d: 64 molecules, n_ao {sum(s.basis.n_ao for s in systems)}, quartets {fb.n_quartets:.3e}, "
+    log(f"c4d: 64 molecules, n_ao {sum(s.basis.n_ao for s in systems)}, quartets {fb.shell_quartets():.3e}, "
         f"{fb.flops / 1e9:.0f} GFLOP/build, plan {plan_s:.0f}s")
     Ds = [native.to_dev(s.D) for s in systems]
     ns = ns if tuning else 4
@@ -273,7 +273,7 @@ def c4_block(a) -> None:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "c4-64mol-8to80atoms-seed4+i-spd", "shells": "s,s,p,d (f not built)",
                    "tau": 1e-12, "n_streams": ns, "graph": True, "tuned": bool(tuning)},
-        "quartets_per_build": fb.n_quartets, "quartets_per_s": fb.n_quartets * a.steps / t,
+        "quartets_per_build": fb.shell_quartets(), "quartets_per_s": fb.shell_quartets() * a.steps / t,
         "gflop_per_build": fb.flops / 1e9, "tflops_algorithmic": fb.flops * a.steps / t / 1e12,
         "plan_seconds": plan_s, "max_dG_vs_molecule_alone": worst,
         "molecules_checked": checked,
@@ -332,7 +332,7 @@ def main() -> None:
     D = s.D
     Dd = native.to_dev(D)
     plain = FockBuilder(s.basis, D, tau=1e-12)
-    log(f"{s.name}: n_ao {s.basis.n_ao}, quartets {plain.n_quartets:.3e}, primitive quartets "
+    log(f"{s.name}: n_ao {s.basis.n_ao}, quartets {plain.shell_quartets():.3e}, primitive quartets "
         f"{plain.n_prim_quartets:.3e}, {plain.flops / 1e9:.1f} GFLOP/build, plan {plain.plan_seconds:.1f}s")
 
     # roofline of the dominant kernel: eager one-stream launches, events
@@ -394,10 +394,10 @@ def main() -> None:
                    "tuned": bool(tuning), "selector": selector is not None},
         "fp64_plain_builds_per_s": a.steps / t_plain, "fp64_plain_ms_per_build": 1e3 * t_plain / a.steps,
         "fp64_tuned_graph_builds_per_s": a.steps / t_t64,
-        "quartets_per_s": plain.n_quartets * a.steps / t_prod,
-        "quartets_per_s_fp64_plain": plain.n_quartets * a.steps / t_plain,
+        "quartets_per_s": plain.shell_quartets() * a.steps / t_prod,
+        "quartets_per_s_fp64_plain": plain.shell_quartets() * a.steps / t_plain,
         "prim_quartets_per_s": plain.n_prim_quartets * a.steps / t_prod,
-        "quartets_per_build": plain.n_quartets, "gflop_per_build": plain.flops / 1e9,
+        "quartets_per_build": plain.shell_quartets(), "gflop_per_build": plain.flops / 1e9,
         "tflops_algorithmic": plain.flops * a.steps / t_prod / 1e12,
         "speedup_vs_fp64_plain": t_plain / t_prod, "fp32_share_quartets": fbp.fp32_share(),
         "dE_vs_fp64": acc["dE"], "max_dG_vs_fp64": acc["max_dG"],
