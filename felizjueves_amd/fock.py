@@ -147,6 +147,13 @@ def fill_wave(k: np.ndarray, lo: np.ndarray, cnt: np.ndarray):
 # estimated FP64 amplification of the ones on the far side stays below this
 # (relative error ~ 1e-16 x this); otherwise the pair is split by centre.
 ORIENT_AMP_MAX = 10.0
+# pairs whose shells are all below this angular momentum stay as they are.
+# Orienting s/p pairs too changes little in FP64 (10-atom cluster: the same
+# G to 1e-14) but matters for the FP32 tier, where their one-unit HRR still
+# amplifies FP32 rounding: the 40-atom mixed build was 1.9e-8 from FP64
+# with them left as they are, 2.1e-11 oriented; leaving them saved launches
+# (45 -> 36 class pairs) but no time in the graph build.
+ORIENT_L_MIN = 0
 
 
 def orient_pairs(basis: Basis, pairs: list) -> list:
@@ -169,7 +176,7 @@ def orient_pairs(basis: Basis, pairs: list) -> list:
     import copy
     out = []
     for p in pairs:
-        if p.a == p.b:
+        if p.a == p.b or max(p.la, p.lb) < ORIENT_L_MIN:
             out.append(p)
             continue
         A = basis.shell_centre[p.a]

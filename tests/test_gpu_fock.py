@@ -194,6 +194,31 @@ def test_builder_small_cluster_vs_oracle():
     assert dG <= 1e-10 and dE <= 1e-9
 
 
+def test_builder_c2_fp64_vs_reference():
+    """The 40-atom working-size cluster (n_ao 440), FP64 bars at full size
+    against the committed reference G (tests/golden/make_c2_reference.py:
+    the oracle's HGP launches over fully oriented pairs)."""
+    import torch
+    from pathlib import Path
+    z = np.load(Path(__file__).resolve().parent / "golden" / "c2_40atom_fp64_reference.npz",
+                allow_pickle=False)
+    s = config2()
+    n = int(z["n_ao"])
+    assert n == s.basis.n_ao and str(z["system"]) == s.name
+    ref = np.zeros((n, n))
+    ref[np.triu_indices(n)] = z["G_upper"]
+    ref = ref + np.triu(ref, 1).T
+    fb = FockBuilder(s.basis, s.D, tau=float(z["tau"]))
+    G = fb.build(native.to_dev(s.D))
+    torch.cuda.synchronize()
+    G = G.cpu().numpy()
+    dG = np.abs(G - ref).max()
+    dE = abs(energy(s.D, G) - energy(s.D, ref))
+    _print("c2-40 fp64 vs reference max|dG|", dG, "|dE|", dE, "shell quartets", fb.shell_quartets())
+    assert dG <= 1e-10 and dE <= 1e-9
+    assert np.abs(G - G.T).max() == 0.0
+
+
 def _two_atom_spdf():
     """Two atoms 1.9 bohr apart, shells s, s, p, d, f each (seeded): every
     one of the 55 classes through (ff|ff) occurs."""
