@@ -239,6 +239,36 @@ def c5_block(steps, warmup, selector):
     return out
 
 
+def jk_block(steps, warmup):
+    """The MMD families (Schwarz-screened J chain: DQ, [p|q], JAB; the K
+    kernel and its scatter): J and K of the three-atom system and the
+    10-atom cluster, timed per call (tables built once), and G = J - K/2
+    against the fused F build."""
+    import torch
+    from felizjueves_amd import jk, native
+    from felizjueves_amd.fock import FockBuilder
+    from felizjueves_amd.systems import config1, config2
+    out = {}
+    for s in (config1(), config2(seed=11, n_atoms=10)):
+        Dd = native.to_dev(s.D)
+        cj, ck = jk.CoulombChain(s.basis, tau=1e-12), jk.ExchangeBuild(s.basis, tau=1e-12)
+        state = {}
+
+        def run_j():
+            state["J"] = cj.run(Dd)
+
+        def run_k():
+            state["K"] = ck.run(Dd)
+        tj = timed(run_j, steps, warmup) / steps
+        tk = timed(run_k, steps, warmup) / steps
+        G = FockBuilder(s.basis, s.D, tau=1e-12).build(Dd)
+        torch.cuda.synchronize()
+        dG = float((state["J"] - 0.5 * state["K"] - G).abs().max().item())
+        out[s.name] = {"j_ms": 1e3 * tj, "k_ms": 1e3 * tk, "max_dG_mmd_vs_fused": dG}
+    log("jk", json.dumps(out))
+    return out
+
+
 def c4_block(a) -> None:
     """The batched workload: 64 molecules in one builder (one launch per
     class pair and tier for all of them, dealt to streams, captured in a
@@ -383,6 +413,7 @@ def main() -> None:
     if not a.no_extra:
         extra["c3"] = c3_block(max(10, a.steps), a.warmup, selector)
         extra["c5"] = c5_block(max(5, a.steps // 2), a.warmup, selector)
+        extra["jk"] = jk_block(3, 1)
     cpu = None if a.no_cpu else cpu_baseline(plain, D)
     out = {
         "metric": METRIC, "value": a.steps / t_prod, "unit": "Fock builds/s", "n_gpus": 1,

@@ -121,8 +121,28 @@ def schwarz_matrix(basis: Basis, threads: int = 64):
 # ---------------------------------------------------------------------------
 
 
+def _block_index(basis: Basis, prs, l1: int, l2: int) -> np.ndarray:
+    """Flat indices into D (row-major n_ao x n_ao) of the (l1, l2) blocks of
+    the shell pairs prs, one row of n_cart(l1) * n_cart(l2) per pair."""
+    n = basis.n_ao
+    r = np.arange(n_cart(l1))[:, None] * n + np.arange(n_cart(l2))[None, :]
+    return np.array([basis.shell_ao[c] * n + basis.shell_ao[d] + r.ravel() for c, d in prs],
+                    dtype=np.int64).reshape(len(prs), -1)
+
+
+def _dev_density(D):
+    """D as a contiguous float64 device tensor (a host array is copied)."""
+    import torch
+    if isinstance(D, torch.Tensor):
+        return D.to(device=native.device(), dtype=torch.float64).contiguous()
+    return native.to_dev(np.ascontiguousarray(D, dtype=np.float64))
+
+
 class CoulombChain:
-    """The J chain for one basis; `run(D)` returns J on the device.
+    """The J chain for one basis; `run(D)` returns J on the device. The
+    pair records and their device tables are built once here; `run` only
+    gathers D's blocks on the device and launches (D a host array or a
+    device tensor).
 
     mixed: the J mixed lever (upstream eri/j_coulomb/levers/mixed.py:82-191)
     -- every [p|q] launch evaluates the Hermite recurrence in FP32
@@ -133,77 +153,67 @@ class CoulombChain:
         self.tau = float(tau)
         self.mixed = bool(mixed)
         self.classes = _pair_classes(basis)
-
-    def run(self, D: np.ndarray):
-        import torch
-        b = self.basis
-        dev = native.device()
-        J = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
-        # the ket side: DQ per class, then Dmax read back and the sort
-        kets = {}
-        for (lc, ld), prs in self.classes.items():
+        b = basis
+        self.side = {}
+        for (l1, l2), prs in self.classes.items():
             recs = [_records(b, c, d) for c, d in prs]
+            cat = lambda k: np.concatenate([r[k] for r in recs])  # noqa: E731
+            P, z, s, X1, X2 = (cat(k) for k in range(5))
             pair = np.concatenate([np.full(len(r[1]), k) for k, r in enumerate(recs)]).astype(np.int32)
             w = np.concatenate([np.full(len(r[1]), 1.0 if c == d else 2.0)
                                 for (c, d), r in zip(prs, recs)])
-            ncd = n_cart(lc) * n_cart(ld)
-            Dcd = np.array([D[b.shell_ao[c]:b.shell_ao[c] + n_cart(lc),
-                              b.shell_ao[d]:b.shell_ao[d] + n_cart(ld)].ravel() for c, d in prs])
-            P = np.concatenate([r[0] for r in recs])
-            z = np.concatenate([r[1] for r in recs])
-            s = np.concatenate([r[2] for r in recs])
-            XA = np.concatenate([r[3] for r in recs])
-            XB = np.concatenate([r[4] for r in recs])
+            oa = np.concatenate([np.full(len(r[1]), b.shell_ao[c]) for (c, _), r in zip(prs, recs)])
+            ob = np.concatenate([np.full(len(r[1]), b.shell_ao[d]) for (_, d), r in zip(prs, recs)])
+            dev = lambda x: native.to_dev(np.ascontiguousarray(x))  # noqa: E731
+            self.side[(l1, l2)] = dict(
+                n=len(z), n_pairs=len(prs), P=dev(P), z=dev(z), s=dev(s), X1=dev(X1), X2=dev(X2),
+                pair=dev(pair), w=dev(w), Iq=dev(record_bound(z, l1 + l2)),
+                Ip=dev(record_bound(z, l1 + l2)), oa=dev(oa.astype(np.int32)),
+                ob=dev(ob.astype(np.int32)), rows=dev(np.arange(len(z), dtype=np.int32)),
+                didx=dev(_block_index(b, prs, l1, l2)))
+
+    def run(self, D):
+        import torch
+        b = self.basis
+        dev = native.device()
+        Dd = _dev_density(D).view(-1)
+        J = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
+        # the ket side: DQ per class, then the kets sorted by bound x Dmax
+        # (descending, stable) on the device
+        kets = {}
+        for (lc, ld), t in self.side.items():
             nq = len(hermite_order(lc + ld))
-            Dq = torch.zeros((len(z), nq), dtype=torch.float64, device=dev)
-            Dm = torch.zeros(len(z), dtype=torch.float64, device=dev)
-            _call(f"fj_dq_{lc}_{ld}", C.c_int(64), C.c_int(len(z)), C.c_int(len(prs)),
-                  native.to_dev(z), native.to_dev(XA), native.to_dev(XB), native.to_dev(s),
-                  native.to_dev(w), native.to_dev(pair), native.to_dev(Dcd.reshape(len(prs), ncd)),
-                  Dq, Dm)
-            torch.cuda.synchronize()
-            Iq = record_bound(z, lc + ld)
-            key = Iq * Dm.cpu().numpy()
-            order = np.argsort(-key, kind="stable")
-            kets[(lc, ld)] = dict(Q=native.to_dev(P[order]), eta=native.to_dev(z[order]),
-                                  Iq=native.to_dev(Iq[order]), Dmax=Dm[order].contiguous(),
-                                  D=Dq[order].contiguous(), n=len(z))
+            Dq = torch.zeros((t["n"], nq), dtype=torch.float64, device=dev)
+            Dm = torch.zeros(t["n"], dtype=torch.float64, device=dev)
+            Dcd = Dd[t["didx"]].contiguous()
+            _call(f"fj_dq_{lc}_{ld}", C.c_int(64), C.c_int(t["n"]), C.c_int(t["n_pairs"]),
+                  t["z"], t["X1"], t["X2"], t["s"], t["w"], t["pair"], Dcd, Dq, Dm)
+            order = torch.argsort(-(t["Iq"] * Dm), stable=True)
+            kets[(lc, ld)] = dict(Q=t["P"][order].contiguous(), eta=t["z"][order].contiguous(),
+                                  Iq=t["Iq"][order].contiguous(), Dmax=Dm[order].contiguous(),
+                                  D=Dq[order].contiguous(), n=t["n"])
         # the bra side: [p|q] per ket class, then JAB per buffer
-        for (la, lb), prs in self.classes.items():
-            recs = [_records(b, a, c) for a, c in prs]
-            P = np.concatenate([r[0] for r in recs])
-            z = np.concatenate([r[1] for r in recs])
-            s = np.concatenate([r[2] for r in recs])
-            XA = np.concatenate([r[3] for r in recs])
-            XB = np.concatenate([r[4] for r in recs])
-            oa = np.concatenate([np.full(len(r[1]), b.shell_ao[a]) for (a, _), r in zip(prs, recs)])
-            ob = np.concatenate([np.full(len(r[1]), b.shell_ao[c]) for (_, c), r in zip(prs, recs)])
-            Ip = record_bound(z, la + lb)
-            dP, dz, dIp = native.to_dev(P), native.to_dev(z), native.to_dev(Ip)
-            rows = native.to_dev(np.arange(len(z), dtype=np.int32))
-            da, db = native.to_dev(oa.astype(np.int32)), native.to_dev(ob.astype(np.int32))
-            ds, dXA, dXB = native.to_dev(s), native.to_dev(XA), native.to_dev(XB)
+        for (la, lb), t in self.side.items():
             lp = la + lb
             npp = len(hermite_order(lp))
             for (lc, ld), k in kets.items():
                 lq = lc + ld
-                Jp = torch.zeros((len(z), npp), dtype=torch.float64, device=dev)
+                Jp = torch.zeros((t["n"], npp), dtype=torch.float64, device=dev)
                 kn = "pq32" if self.mixed else "pq"
-                _call(f"fj_{kn}_L{lp + lq}_p{lp}_q{lq}", C.c_int(256), C.c_int(len(z)), C.c_int(k["n"]),
-                      C.c_double(self.tau), _c(dP), _c(dz), _c(dIp), _c(k["Q"]), _c(k["eta"]),
-                      _c(k["Iq"]), _c(k["Dmax"]), _c(k["D"]), _c(Jp))
-                _call(f"fj_jab_{la}{lb}", C.c_int(64), C.c_int(len(z)), C.c_int(b.n_ao),
-                      C.c_int(len(z)), _c(dz), _c(dXA), _c(dXB), _c(ds), _c(da), _c(db),
-                      _c(rows), _c(Jp), _c(J))
+                _call(f"fj_{kn}_L{lp + lq}_p{lp}_q{lq}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
+                      C.c_double(self.tau), _c(t["P"]), _c(t["z"]), _c(t["Ip"]), _c(k["Q"]),
+                      _c(k["eta"]), _c(k["Iq"]), _c(k["Dmax"]), _c(k["D"]), _c(Jp))
+                _call(f"fj_jab_{la}{lb}", C.c_int(64), C.c_int(t["n"]), C.c_int(b.n_ao),
+                      C.c_int(t["n"]), _c(t["z"]), _c(t["X1"]), _c(t["X2"]), _c(t["s"]),
+                      _c(t["oa"]), _c(t["ob"]), _c(t["rows"]), _c(Jp), _c(J))
         return J
-
-
-# ---------------------------------------------------------------------------
 
 
 class ExchangeBuild:
     """K_ab = sum_cd D_cd (ac|bd) through the K kernel; `run(D)` returns K
-    on the device."""
+    on the device. The bra tables (per (la, lc)), the ket tables (per b
+    shell and ld, sorted by bound) and the D-block indices are built once
+    here; `run` gathers D's blocks on the device and launches."""
 
     def __init__(self, basis: Basis, tau: float = 0.0):
         self.basis = basis
@@ -213,6 +223,38 @@ class ExchangeBuild:
         for s in range(b.n_sh):
             self.by_l.setdefault(int(b.shell_l[s]), []).append(s)
         self.lmax = max(self.by_l)
+        L = sorted(self.by_l)
+        dev = lambda x: native.to_dev(np.ascontiguousarray(x))  # noqa: E731
+        self.bras, self.kets, self.dblk = {}, {}, {}
+        for la in L:
+            for lc in L:
+                bra = self._side(la, lc)
+                if bra is None:
+                    continue
+                self.bras[(la, lc)] = dict(
+                    n=len(bra["z"]), P=dev(bra["P"]), z=dev(bra["z"]), s=dev(bra["s"]),
+                    XA=dev(bra["X1"]), XC=dev(bra["X2"]), Ip=dev(record_bound(bra["z"], la + lc)),
+                    oa=dev(b.shell_ao[bra["s1"]].astype(np.int32)), s2=bra["s2"])
+        for lc in L:
+            for ld in L:
+                prs = [(c, d) for c in self.by_l[lc] for d in self.by_l[ld]]
+                self.dblk[(lc, ld)] = dict(n=len(prs), idx=dev(_block_index(b, prs, lc, ld)))
+        for (la, lc), t in self.bras.items():
+            for ld in L:
+                t[("dc", ld)] = dev((np.searchsorted(self.by_l[lc], t["s2"]) *
+                                     len(self.by_l[ld])).astype(np.int32))
+        for lb in L:
+            for ld in L:
+                for bs in self.by_l[lb]:
+                    ket = self._side(lb, ld, fixed=bs)
+                    if ket is None:
+                        continue
+                    Iq = record_bound(ket["z"], lb + ld)
+                    order = np.argsort(-Iq, kind="stable")
+                    self.kets[(lb, ld, bs)] = dict(
+                        n=len(ket["z"]), Q=dev(ket["P"][order]), eta=dev(ket["z"][order]),
+                        s=dev(ket["s"][order]), XB=dev(ket["X1"][order]), XD=dev(ket["X2"][order]),
+                        Iq=dev(Iq[order]), pos=dev(ket["pos2"][order].astype(np.int32)))
 
     def _side(self, l1: int, l2: int, fixed: int | None = None):
         """Records of every ordered pair (s1, s2), l(s1) = l1, l(s2) = l2
@@ -235,54 +277,41 @@ class ExchangeBuild:
         return dict(P=cat(0), z=cat(1), s=cat(2), X1=cat(3), X2=cat(4),
                     s1=np.concatenate(s1), s2=np.concatenate(s2), pos2=np.concatenate(pos2))
 
-    def run(self, D: np.ndarray):
+    def run(self, D):
         import torch
         b = self.basis
         dev = native.device()
+        Dd = _dev_density(D)
         K = torch.zeros((b.n_ao, b.n_ao), dtype=torch.float64, device=dev)
-        maxD = float(np.abs(D).max())
+        maxD = float(Dd.abs().max().item())
+        Dflat = Dd.view(-1)
+        Dcd = {k: Dflat[t["idx"]].contiguous() for k, t in self.dblk.items()}
         L = sorted(self.by_l)
-        for la in L:
-            for lc in L:
-                bra = self._side(la, lc)
-                if bra is None:
-                    continue
-                Ip = record_bound(bra["z"], la + lc)
-                dbra = {k: native.to_dev(v) for k, v in
-                        dict(P=bra["P"], z=bra["z"], s=bra["s"], XA=bra["X1"], XC=bra["X2"],
-                             Ip=Ip).items()}
-                oa = native.to_dev(b.shell_ao[bra["s1"]].astype(np.int32))
-                for ld in L:
-                    nc_, nd_ = n_cart(lc), n_cart(ld)
-                    cs, ds = self.by_l[lc], self.by_l[ld]
-                    # Dcd rows: (c position, d position) blocks, (c, d) components
-                    Dcd = np.array([D[b.shell_ao[c]:b.shell_ao[c] + nc_,
-                                      b.shell_ao[d]:b.shell_ao[d] + nd_].ravel()
-                                    for c in cs for d in ds])
-                    dc = native.to_dev((np.searchsorted(cs, bra["s2"]) * len(ds)).astype(np.int32))
-                    dDcd = native.to_dev(Dcd)
-                    for lb in L:
-                        for bs in self.by_l[lb]:
-                            ket = self._side(lb, ld, fixed=bs)
-                            if ket is None:
-                                continue
-                            Iq = record_bound(ket["z"], lb + ld)
-                            order = np.argsort(-Iq, kind="stable")
-                            dk = {k: native.to_dev(v[order]) for k, v in
-                                  dict(Q=ket["P"], eta=ket["z"], s=ket["s"], XB=ket["X1"],
-                                       XD=ket["X2"], Iq=Iq).items()}
-                            dd = native.to_dev(ket["pos2"][order].astype(np.int32))
-                            nab = n_cart(la) * n_cart(lb)
-                            Kr = torch.zeros((len(bra["z"]), nab), dtype=torch.float64, device=dev)
-                            _call(f"fj_k_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(len(bra["z"])),
-                                  C.c_int(len(ket["z"])), C.c_int(Dcd.shape[0]), C.c_double(self.tau),
-                                  C.c_double(maxD), _c(dbra["P"]), _c(dbra["z"]), _c(dbra["XA"]),
-                                  _c(dbra["XC"]), _c(dbra["s"]), _c(dbra["Ip"]), _c(dc),
-                                  _c(dk["Q"]), _c(dk["eta"]), _c(dk["XB"]), _c(dk["XD"]),
-                                  _c(dk["s"]), _c(dk["Iq"]), _c(dd), _c(dDcd), _c(Kr))
-                            _call("fj_k_scatter", C.c_int(len(bra["z"])), C.c_int(n_cart(la)),
-                                  C.c_int(n_cart(lb)), C.c_int(b.n_ao), _c(oa),
-                                  C.c_int(int(b.shell_ao[bs])), _c(Kr), _c(K))
+        # one scratch for every launch's rows: the K kernel writes each bra
+        # row whole, and the scatter reads it before the next launch on the
+        # same stream writes it again
+        n_rows = max(t["n"] * n_cart(la) * n_cart(lb)
+                     for (la, _), t in self.bras.items() for lb in L)
+        scratch = torch.empty(n_rows, dtype=torch.float64, device=dev)
+        for (la, lc), t in self.bras.items():
+            for ld in L:
+                dDcd = Dcd[(lc, ld)]
+                for lb in L:
+                    nab = n_cart(la) * n_cart(lb)
+                    for bs in self.by_l[lb]:
+                        k = self.kets.get((lb, ld, bs))
+                        if k is None:
+                            continue
+                        Kr = scratch[:t["n"] * nab]
+                        _call(f"fj_k_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
+                              C.c_int(self.dblk[(lc, ld)]["n"]), C.c_double(self.tau),
+                              C.c_double(maxD), _c(t["P"]), _c(t["z"]), _c(t["XA"]), _c(t["XC"]),
+                              _c(t["s"]), _c(t["Ip"]), _c(t[("dc", ld)]), _c(k["Q"]), _c(k["eta"]),
+                              _c(k["XB"]), _c(k["XD"]), _c(k["s"]), _c(k["Iq"]), _c(k["pos"]),
+                              _c(dDcd), _c(Kr))
+                        _call("fj_k_scatter", C.c_int(t["n"]), C.c_int(n_cart(la)),
+                              C.c_int(n_cart(lb)), C.c_int(b.n_ao), _c(t["oa"]),
+                              C.c_int(int(b.shell_ao[bs])), _c(Kr), _c(K))
         return K
 
 
