@@ -121,3 +121,45 @@ def test_native_planner_equals_python_planner_batch():
     sy = [(s.basis, s.D) for s in systems]
     _same_plan(FockBuilder.batch(sy, tau=1e-12, planner="python"),
                FockBuilder.batch(sy, tau=1e-12, planner="native"))
+
+
+def test_orient_pairs_partition_each_pair():
+    """Oriented tables: each shell pair's primitive pairs appear exactly
+    once over its parts, every part's records are (P, P - first centre,
+    zeta, U) of its own orientation, and only pairs of separate centres are
+    swapped or split."""
+    from felizjueves_amd import pairs as PP
+    from felizjueves_amd.fock import ORIENT_AMP_MAX, orient_pairs
+    s = config2(seed=11, n_atoms=10)
+    kept = PP.kept_pairs(s.basis, s.D, 1e-12, 1)[0]
+    parts = orient_pairs(s.basis, kept)
+    seen = {}
+    for p in parts:
+        key = (min(p.a, p.b), max(p.a, p.b)) if p.a != p.b else (p.a, p.b)
+        first = s.basis.shell_centre[p.a]
+        assert np.allclose(p.rec[:, 3:6], p.rec[:, 0:3] - first, rtol=0, atol=1e-12)
+        assert (p.la, p.lb) == (int(s.basis.shell_l[p.a]), int(s.basis.shell_l[p.b]))
+        seen.setdefault(key, []).append(p)
+    swapped = split = 0
+    for p in kept:
+        key = (min(p.a, p.b), max(p.a, p.b)) if p.a != p.b else (p.a, p.b)
+        got = seen[key]
+        zs = np.sort(np.concatenate([q.rec[:, 6] for q in got]))
+        us = np.sort(np.concatenate([q.rec[:, 7] for q in got]))
+        assert np.array_equal(zs, np.sort(p.rec[:, 6])) and np.array_equal(us, np.sort(p.rec[:, 7]))
+        same_centre = not np.any(s.basis.shell_centre[p.a] != s.basis.shell_centre[p.b])
+        if p.a == p.b or same_centre:
+            assert len(got) == 1 and (got[0].a, got[0].b) == (p.a, p.b)
+        swapped += any((q.a, q.b) == (p.b, p.a) for q in got) and len(got) == 1
+        split += len(got) == 2
+    assert swapped > 0 and ORIENT_AMP_MAX > 0
+
+
+def test_orientation_keeps_the_fock_matrix():
+    """The oriented plan computes the same G as the unoriented one (C1, the
+    CPU oracle's HGP launches over each plan's tables)."""
+    from test_builder_oracle import oracle_build
+    s = config1()
+    a = oracle_build(FockBuilder(s.basis, s.D, tau=1e-12, orient=False), s.D)
+    b = oracle_build(FockBuilder(s.basis, s.D, tau=1e-12), s.D)
+    assert np.abs(a - b).max() < 1e-11
