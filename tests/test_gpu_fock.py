@@ -159,6 +159,37 @@ def test_mixed_one_launch_per_class_matches_split():
     assert err < 1e-12
 
 
+def test_builder_everything_screened_gives_zero():
+    """tau above every bound: no launch at all; eager and graph builds give
+    an exact zero G (F is still cleared and closed)."""
+    import torch
+    s = config1()
+    fb = FockBuilder(s.basis, s.D, tau=1e30)
+    Dd = native.to_dev(s.D)
+    assert fb.launches == []
+    G = fb.build(Dd).clone()
+    fb.capture(Dd)
+    Gr = fb.replay().clone()
+    torch.cuda.synchronize()
+    assert G.abs().max().item() == 0.0 and Gr.abs().max().item() == 0.0
+
+
+def test_builder_single_s_function_vs_oracle():
+    """One s shell (one AO, two primitives): the (ss|ss) diagonal launch of
+    one pair, FP64 bars against the oracle."""
+    import torch
+    from felizjueves_amd.basis import Basis, Shell
+    b = Basis([Shell(0, (0.1, -0.2, 0.3), (1.3, 0.4), (0.6, 0.5))])
+    D = np.array([[0.7]])
+    fb = FockBuilder(b, D, tau=1e-12)
+    G = fb.build(native.to_dev(D))
+    torch.cuda.synchronize()
+    G = G.cpu().numpy()
+    Go = oracle_build(fb, D)
+    _print("single s G", G[0, 0], "oracle", Go[0, 0])
+    assert abs(G[0, 0] - Go[0, 0]) <= 1e-12 * max(1.0, abs(Go[0, 0]))
+
+
 def test_builder_repeatable():
     """Two builds of the same density agree to the atomics' reordering."""
     import torch
