@@ -226,6 +226,14 @@ def _blkname(p0: int, p1: int) -> str:
     return "abcd"[p0] + "abcd"[p1]
 
 
+# Waves per SIMD to ask of the mixed-tier kernel (both instantiations in one
+# kernel take the larger register count), for the classes whose FP64 kernel
+# alone reaches more than the mixed one did unasked (hipcc's resource
+# remarks for gfx950: e.g. (ss|dd) 2 vs 1, (sp|sp) 3 vs 2).
+MIXED_MIN_WAVES = {(0, 0, 2, 2): 2, (0, 0, 1, 2): 3, (0, 1, 0, 1): 3, (0, 0, 1, 1): 4,
+                   (0, 0, 0, 2): 4, (1, 0, 2, 0): 3, (0, 0, 0, 0): 8}
+
+
 def emit_class(cls) -> str:
     la, lb, lc, ld = cls
     name = cls_name(cls)
@@ -291,6 +299,8 @@ def emit_class(cls) -> str:
                            "    else\n"
                            "        fj_item_" + name + "<float>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);"))
     assert mixed_body != kernel_body
+    w = MIXED_MIN_WAVES.get(tuple(cls))
+    mixed_waves = f", {w}" if w else ""
     out.append(f"""
 // One work item: workgroup-uniform early returns only (the persistent
 // kernel calls it in a loop with a barrier after each item).
@@ -411,7 +421,7 @@ __global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_k(FjFArgs a) {{
 int fj_launch_{name}_r32(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a);
 int fj_launch_{name}_mx(hipStream_t s, dim3 grid, dim3 block, const FjFArgs& a);
 //@@MIXED@@
-__global__ void __launch_bounds__(FJ_MAX_THREADS) fj_f_{name}_km(FjFArgs a) {{
+__global__ void __launch_bounds__(FJ_MAX_THREADS{mixed_waves}) fj_f_{name}_km(FjFArgs a) {{
     __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
 {mixed_body}
 }}
