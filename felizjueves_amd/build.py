@@ -22,7 +22,8 @@ OBJ = HERE / "_build"
 LIB = HERE / "libfj_mi355x.so"
 ARCH = "gfx950"
 LMAX = 2
-MMD_LMAX = 2
+MMD_LMAX = 3     # Schwarz, DQ, [p|q] and JAB through f
+MMD_K_LMAX = 2   # the K kernels through d
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -37,8 +38,8 @@ def sources(lmax: int = LMAX) -> list[Path]:
     classes = builder_classes(lmax)
     files = gen_hip.generate(classes)
     gen_hip.write_header(classes)
-    mmd = gen_mmd.generate(MMD_LMAX)
-    gen_mmd.write_header(MMD_LMAX)
+    mmd = gen_mmd.generate(MMD_LMAX, k_lmax=MMD_K_LMAX)
+    gen_mmd.write_header(MMD_LMAX, MMD_K_LMAX)
     return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", *files, *mmd]
 
 

@@ -537,25 +537,37 @@ def k_classes(lmax: int):
     return [(a, c, b, d) for a in r for c in r for b in r for d in r]
 
 
-def units(lmax: int) -> dict[str, str]:
+# Schwarz units only for pairs with la + lb <= this: the (ab|ab) tower of a
+# (df) or (ff) pair keeps too much live (8.5 / 19 KB of scratch per lane,
+# above the library's 6 KB cap); the J chain screens with host bounds and
+# does not need them.
+SCHWARZ_LSUM_MAX = 4
+
+
+def units(lmax: int, k_lmax: int | None = None) -> dict[str, str]:
+    """Every unit through angular momentum lmax; the K classes only through
+    k_lmax (their straight-line Hermite towers grow as L^6: (ff|ff) alone
+    would be 18 MB of source)."""
+    k_lmax = lmax if k_lmax is None else k_lmax
     out = {}
     for la, lb in pair_classes(lmax):
-        out[f"schwarz_{la}{lb}.hip"] = emit_schwarz(la, lb)
+        if la + lb <= SCHWARZ_LSUM_MAX:
+            out[f"schwarz_{la}{lb}.hip"] = emit_schwarz(la, lb)
         out[f"dq_{la}_{lb}.hip"] = emit_dq(la, lb)
         out[f"jab_{la}{lb}.hip"] = emit_jab(la, lb)
     for lp in range(2 * lmax + 1):
         for lq in range(2 * lmax + 1):
             out[f"pq_L{lp + lq}_p{lp}_q{lq}.hip"] = emit_pq(lp, lq)
             out[f"pq32_L{lp + lq}_p{lp}_q{lq}.hip"] = emit_pq(lp, lq, f32=True)
-    for c in k_classes(lmax):
+    for c in k_classes(k_lmax):
         out[f"k_{''.join(map(str, c))}.hip"] = emit_k(*c)
     return out
 
 
-def generate(lmax: int, out: Path = GEN) -> list[Path]:
+def generate(lmax: int, out: Path = GEN, k_lmax: int | None = None) -> list[Path]:
     out.mkdir(parents=True, exist_ok=True)
     files = []
-    for name, text in units(lmax).items():
+    for name, text in units(lmax, k_lmax).items():
         p = out / name
         if not p.exists() or p.read_text() != text:
             p.write_text(text)
@@ -563,10 +575,12 @@ def generate(lmax: int, out: Path = GEN) -> list[Path]:
     return files
 
 
-def header_decls(lmax: int) -> str:
+def header_decls(lmax: int, k_lmax: int | None = None) -> str:
+    k_lmax = lmax if k_lmax is None else k_lmax
     out = []
     for la, lb in pair_classes(lmax):
-        out.append(
+        if la + lb <= SCHWARZ_LSUM_MAX:
+            out.append(
             f"int fj_schwarz_{la}{lb}(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,\n"
             f"    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,\n"
             f"    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,\n"
@@ -588,7 +602,7 @@ def header_decls(lmax: int) -> str:
                     f"    const double* P, const double* zeta, const double* Ip, const double* Q,\n"
                     f"    const double* eta, const double* Iq, const double* Dmax, const double* D,"
                     f" double* J);")
-    for c in k_classes(lmax):
+    for c in k_classes(k_lmax):
         out.append(
             f"int fj_k_{''.join(map(str, c))}(fj_stream_t stream, int threads, int n_bra, int n_ket,"
             f" int n_dc,\n"
@@ -604,13 +618,13 @@ BEGIN, END = ("/* BEGIN MMD CLASSES (written by felizjueves_amd/gen_mmd.py) */",
               "/* END MMD CLASSES */")
 
 
-def header_text(lmax: int, current: str) -> str:
+def header_text(lmax: int, current: str, k_lmax: int | None = None) -> str:
     i, j = current.index(BEGIN), current.index(END)
-    return current[:i + len(BEGIN)] + "\n" + header_decls(lmax) + current[j:]
+    return current[:i + len(BEGIN)] + "\n" + header_decls(lmax, k_lmax) + current[j:]
 
 
-def write_header(lmax: int) -> None:
+def write_header(lmax: int, k_lmax: int | None = None) -> None:
     cur = HEADER.read_text()
-    new = header_text(lmax, cur)
+    new = header_text(lmax, cur, k_lmax)
     if new != cur:
         HEADER.write_text(new)

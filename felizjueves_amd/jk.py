@@ -37,6 +37,9 @@ from .basis import Basis, n_cart
 from .pairs import hermite_bound_sq, hermite_order
 
 
+K_LMAX = 2   # the K kernels are built through d (Schwarz and the J chain through f)
+
+
 def _records(basis: Basis, a: int, b: int):
     """(P [n,3], zeta [n], s [n], XPA [n,3], XPB [n,3]) of ordered pair (a, b)."""
     A, B = basis.shell_centre[a], basis.shell_centre[b]
@@ -101,9 +104,15 @@ def _pair_classes(basis: Basis):
 def schwarz_matrix(basis: Basis, threads: int = 64):
     """Q[m, n] = sqrt(max((mn|mn), 0)) on the device (torch float64)."""
     import torch
+    from .gen_mmd import SCHWARZ_LSUM_MAX
+    classes = _pair_classes(basis)
+    big = [c for c in classes if sum(c) > SCHWARZ_LSUM_MAX]
+    if big:
+        raise ValueError(f"pair classes {big}: the Schwarz kernels are built for la + lb <= "
+                         f"{SCHWARZ_LSUM_MAX}")
     dev = native.device()
     Q = torch.zeros((basis.n_ao, basis.n_ao), dtype=torch.float64, device=dev)
-    for (la, lb), prs in _pair_classes(basis).items():
+    for (la, lb), prs in classes.items():
         recs = [_records(basis, a, b) for a, b in prs]
         n_rec = np.cumsum([0] + [len(r[1]) for r in recs])
         cat = lambda k: native.to_dev(np.concatenate([r[k] for r in recs]))  # noqa: E731
@@ -223,6 +232,9 @@ class ExchangeBuild:
         for s in range(b.n_sh):
             self.by_l.setdefault(int(b.shell_l[s]), []).append(s)
         self.lmax = max(self.by_l)
+        if self.lmax > K_LMAX:
+            raise ValueError(f"shells up to l = {self.lmax}: the K kernels are built through "
+                             f"l = {K_LMAX}")
         L = sorted(self.by_l)
         dev = lambda x: native.to_dev(np.ascontiguousarray(x))  # noqa: E731
         self.bras, self.kets, self.dblk = {}, {}, {}

@@ -599,6 +599,16 @@ int fj_dq_0_2(fj_stream_t stream, int threads, int n_rec, int n_pair,
 int fj_jab_02(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
     const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
     const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_03(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_0_3(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_03(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
 int fj_schwarz_11(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
     const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
     const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
@@ -619,6 +629,16 @@ int fj_dq_1_2(fj_stream_t stream, int threads, int n_rec, int n_pair,
 int fj_jab_12(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
     const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
     const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_13(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
+int fj_dq_1_3(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_13(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
 int fj_schwarz_22(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
     const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
     const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
@@ -627,6 +647,18 @@ int fj_dq_2_2(fj_stream_t stream, int threads, int n_rec, int n_pair,
     const double* eta, const double* XPA, const double* XPB, const double* s,
     const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
 int fj_jab_22(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_dq_2_3(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_23(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
+    const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
+    const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_dq_3_3(fj_stream_t stream, int threads, int n_rec, int n_pair,
+    const double* eta, const double* XPA, const double* XPB, const double* s,
+    const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
+int fj_jab_33(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
     const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
     const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
 int fj_pq_L0_p0_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
@@ -659,6 +691,18 @@ int fj_pq_L4_p0_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double
 int fj_pq32_L4_p0_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L5_p0_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L5_p0_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L6_p0_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L6_p0_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L1_p1_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
@@ -687,6 +731,18 @@ int fj_pq_L5_p1_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq32_L5_p1_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L6_p1_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L6_p1_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L7_p1_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L7_p1_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L2_p2_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
@@ -719,6 +775,18 @@ int fj_pq_L6_p2_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double
 int fj_pq32_L6_p2_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L7_p2_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L7_p2_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L8_p2_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L8_p2_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L3_p3_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
@@ -749,6 +817,18 @@ int fj_pq_L7_p3_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double
 int fj_pq32_L7_p3_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L8_p3_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L8_p3_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L9_p3_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L9_p3_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq_L4_p4_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
@@ -777,6 +857,102 @@ int fj_pq_L8_p4_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_pq32_L8_p4_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L9_p4_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L9_p4_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L10_p4_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L10_p4_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L5_p5_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L5_p5_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L6_p5_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L6_p5_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L7_p5_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L7_p5_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L8_p5_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L8_p5_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L9_p5_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L9_p5_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L10_p5_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L10_p5_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L11_p5_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L11_p5_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L6_p6_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L6_p6_q0(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L7_p6_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L7_p6_q1(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L8_p6_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L8_p6_q2(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L9_p6_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L9_p6_q3(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L10_p6_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L10_p6_q4(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L11_p6_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L11_p6_q5(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq_L12_p6_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
+    const double* P, const double* zeta, const double* Ip, const double* Q,
+    const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
+int fj_pq32_L12_p6_q6(fj_stream_t stream, int threads, int n_bra, int n_ket, double tau,
     const double* P, const double* zeta, const double* Ip, const double* Q,
     const double* eta, const double* Iq, const double* Dmax, const double* D, double* J);
 int fj_k_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,

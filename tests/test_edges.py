@@ -106,3 +106,17 @@ def test_entries_refuse_bad_arguments_before_any_launch():
     rc = h(None, C.c_int(96), C.c_int(0), C.byref(side), C.byref(side), work, C.c_int(1), None,
            sysd, C.c_int(0), C.c_double(0.0), None, C.c_longlong(0))
     assert rc == ERR_THREADS
+
+
+def test_k_build_refuses_shells_above_d():
+    from felizjueves_amd import jk
+    from felizjueves_amd.basis import load_fixture
+    with pytest.raises(ValueError, match="K kernels"):
+        jk.ExchangeBuild(load_fixture("co_ccpvtz").basis)
+
+
+def test_schwarz_refuses_pairs_above_its_built_classes():
+    from felizjueves_amd import jk
+    from felizjueves_amd.basis import load_fixture
+    with pytest.raises(ValueError, match="Schwarz"):
+        jk.schwarz_matrix(load_fixture("co_ccpvtz").basis)
