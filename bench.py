@@ -329,7 +329,8 @@ def main() -> None:
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the Fock build is a "
-              "single-GPU path here; see DESIGN.md for how it would shard.", file=sys.stderr)
+              "single-GPU path here (across GPUs it would split the cost-ordered work items "
+              "and sum the private F matrices with one all-reduce).", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -5,7 +5,7 @@ A work item is one tile of a FockBuilder launch: one bra pair and up to
 `threads` ket pairs (and optionally a chunk of the ket primitive loop). Its
 FP32 error is measured, not modelled: the probe runs the same work items in
 both tiers, each item into its own F buffer (one FjSys entry per item, all
-sharing D and Dsh), and compares. The FP64 result is the judge.
+sharing D and Dsh), and compares. The FP64 result is the reference.
 """
 
 from __future__ import annotations
