@@ -253,10 +253,17 @@ def emit_class(cls) -> str:
             "    __shared__ double tb[FJ_BOYS_NROW * FJ_BOYS_LROW];\n"
             f"    fj_boys_stage<{sum(cls)}>(tb);\n"
             "    __syncthreads();\n"
-            "    // persistent: one workspace slot per lane of the grid, items strided\n"
+            "    // persistent: one workspace slot per lane of the grid. The items are\n"
+            "    // listed costliest first and dealt in snake order (round r takes\n"
+            "    // item r G + b, or r G + G - 1 - b in odd rounds), so a workgroup\n"
+            "    // that drew a costly item draws a cheap one next (greedy balance\n"
+            "    // without atomics); the choice is uniform over the workgroup\n"
             "    const size_t st = (size_t)gridDim.x * blockDim.x;\n"
             "    double* ws = a.ws + (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
-            "    for (int it = (int)blockIdx.x; it < a.n_items; it += (int)gridDim.x) {\n"
+            "    const int G = (int)gridDim.x, b = (int)blockIdx.x;\n"
+            "    for (int r = 0; r * G < a.n_items; ++r) {\n"
+            "        const int it = r * G + ((r & 1) ? G - 1 - b : b);\n"
+            "        if (it >= a.n_items) continue;   // uniform: a partial last round\n"
             "        // an empty asm that 'redefines' the slot pointer and stride each\n"
             "        // item, so the compiler recomputes the workspace addresses where\n"
             "        // they are used instead of hoisting thousands of them out of the\n"
