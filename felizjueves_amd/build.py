@@ -23,7 +23,7 @@ LIB = HERE / "libfj_mi355x.so"
 ARCH = "gfx950"
 LMAX = 2
 MMD_LMAX = 3     # Schwarz, DQ, [p|q] and JAB through f
-MMD_K_LMAX = 2   # the K kernels through d
+MMD_K_LMAX = 2   # straight-line K kernels through d; the f classes in loop form (fj_kloop.h)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -40,7 +40,8 @@ def sources(lmax: int = LMAX) -> list[Path]:
     gen_hip.write_header(classes)
     mmd = gen_mmd.generate(MMD_LMAX, k_lmax=MMD_K_LMAX)
     gen_mmd.write_header(MMD_LMAX, MMD_K_LMAX)
-    return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", *files, *mmd]
+    return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", CSRC / "fj_kloop.hip",
+            *files, *mmd]
 
 
 def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, float, str]:

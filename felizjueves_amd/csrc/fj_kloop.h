@@ -1,0 +1,242 @@
+// fj_kloop.h -- the K kernel in loop form, every class through f:
+// K_ab += sum_cd D_cd (ac|bd) per bra record (upstream
+// eri/k_exchange/kernel_k.py:314, host k_quartet k.py:395), with the class
+// (la lc | lb ld) as run-time arguments instead of straight-line code.
+//
+// The straight-line K kernels (gen_mmd.emit_k) grow as L^6 in source; this
+// one is a single kernel whose loops read small index tables. One wavefront
+// per workgroup and per bra record; the kets of a bra are taken one at a
+// time by the whole wavefront, cooperating through LDS:
+//
+//   per-axis E tables   E^{ij}_t of the bra (once) and of the ket, one lane
+//                       per axis (upstream mmd/hermite.py:45-79)
+//   Hermite tower       [t]^(m), m = L .. 0, one level per step, lanes over t
+//                       (upstream reference/pq.py:54-133), two LDS rows
+//   X[c][q]             s_bd (-1)^|q| sum_d D_cd E^{bd}_q   (per b)
+//   Y[c][p]             sum_q [p + q] X[c][q]
+//   kac[a][c][b]       += sum_p E^{ac}_p Y[c][p]
+//
+// and K[bra][a][b] = s_ac sum_c kac[a][c][b] at the end. Every loop bound is
+// uniform across the wavefront, so each barrier is reached by all lanes.
+//
+// The includer provides fj_device.h (the device one, or the CPU emulation).
+#pragma once
+
+#define FJ_KL_LMAX 3                       // through f
+#define FJ_KL_MMAX (4 * FJ_KL_LMAX)        // the tower's top order, L <= 12
+#define FJ_KL_NH 455                       // Hermite functions with |t| <= 12
+#define FJ_KL_NP 84                        // |p| <= 6 (one side)
+#define FJ_KL_NC 10                        // Cartesians of an f shell
+#define FJ_KL_AX 112                       // one axis's E table: 4 x 4 x 7
+
+// Hermite functions with |t| < n.
+__device__ __forceinline__ int fj_herm_base(int n) { return n * (n + 1) * (n + 2) / 6; }
+
+// Index of t = (x, y, z) in the Hermite order: |t| ascending, then x
+// descending, then y descending (the order of the Cartesians of one shell).
+__device__ __forceinline__ int fj_herm_index(int x, int y, int z) {
+    const int u = y + z;
+    return fj_herm_base(x + u) + u * (u + 1) / 2 + z;
+}
+
+// The components of Hermite index j, packed x | y << 4 | z << 8.
+__device__ __forceinline__ int fj_herm_decode(int j) {
+    int n = 0;
+    while (fj_herm_base(n + 1) <= j) ++n;
+    const int pos = j - fj_herm_base(n);
+    int u = 0;
+    while ((u + 1) * (u + 2) / 2 <= pos) ++u;
+    const int z = pos - u * (u + 1) / 2;
+    return (n - u) | ((u - z) << 4) | (z << 8);
+}
+
+// One axis's E^{ij}_t, i <= l1, j <= l2, at e[(i (l2 + 1) + j) (l1 + l2 + 1) + t],
+// zero where t > i + j:  E^{00}_0 = exp(zeta X_PA X_PB),
+// E^{i+1,j}_t = E^{ij}_{t-1} / (2 zeta) + X_PA E^{ij}_t + (t + 1) E^{ij}_{t+1},
+// the j raise with X_PB.
+__device__ __forceinline__ void fj_kl_eaxis(double* e, int l1, int l2, double zeta, double pa,
+                                            double pb) {
+    const int nj = l2 + 1, nt = l1 + l2 + 1;
+    for (int k = 0; k < (l1 + 1) * nj * nt; ++k) e[k] = 0.0;
+    const double h = 0.5 / zeta;
+    e[0] = exp(zeta * pa * pb);
+    for (int i = 0; i <= l1; ++i) {
+        for (int j = (i == 0 ? 1 : 0); j <= l2; ++j) {
+            // (i, 0) from (i - 1, 0); (i, j > 0) from (i, j - 1)
+            const double* s = j == 0 ? e + (i - 1) * nj * nt : e + (i * nj + j - 1) * nt;
+            const double x = j == 0 ? pa : pb;
+            const int top = i + j;
+            double* d = e + (i * nj + j) * nt;
+            for (int t = 0; t <= top; ++t) {
+                double v = 0.0;
+                if (t >= 1) v += h * s[t - 1];
+                if (t <= top - 1) v += x * s[t];
+                if (t + 1 <= top - 1) v += (double)(t + 1) * s[t + 1];
+                d[t] = v;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) fj_kloop_k(
+    int la, int lc, int lb, int ld, int n_bra, int n_ket, int n_dc, double tau, double maxD,
+    const double* __restrict__ P, const double* __restrict__ zeta_ac,
+    const double* __restrict__ XPA, const double* __restrict__ XPC,
+    const double* __restrict__ s_ac, const double* __restrict__ Ip,
+    const int* __restrict__ dc, const double* __restrict__ Q,
+    const double* __restrict__ eta, const double* __restrict__ XQB,
+    const double* __restrict__ XQD, const double* __restrict__ s_bd,
+    const double* __restrict__ Iq, const int* __restrict__ dd,
+    const double* __restrict__ Dcd, double* __restrict__ K) {
+    __shared__ double R[2][FJ_KL_NH];
+    __shared__ double ea[3 * FJ_KL_AX], eb[3 * FJ_KL_AX];
+    __shared__ double X[FJ_KL_NC * FJ_KL_NP], Y[FJ_KL_NC * FJ_KL_NP];
+    __shared__ double kac[FJ_KL_NC * FJ_KL_NC * FJ_KL_NC];
+    __shared__ double dsh[FJ_KL_NC * FJ_KL_NC];
+    __shared__ double wf[FJ_KL_MMAX + 1];
+    __shared__ int hc[FJ_KL_NH];
+    __shared__ int tw[FJ_KL_NH];
+    __shared__ unsigned short pqi[FJ_KL_NP * FJ_KL_NP];
+
+    const int bra = (int)blockIdx.x, lane = (int)threadIdx.x;
+    if (bra >= n_bra) return;   // uniform per workgroup
+    const int lp = la + lc, lq = lb + ld, L = lp + lq;
+    const int na = (la + 1) * (la + 2) / 2, nc = (lc + 1) * (lc + 2) / 2;
+    const int nb = (lb + 1) * (lb + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
+    const int np = fj_herm_base(lp + 1), nq = fj_herm_base(lq + 1), nh = fj_herm_base(L + 1);
+    const int oa = fj_herm_base(la), ob = fj_herm_base(lb), oc = fj_herm_base(lc),
+              od = fj_herm_base(ld);
+    const int ta = lp + 1, tb = lq + 1;   // t extents of the two E tables
+
+    // index tables: components of every Hermite index, the tower's step of
+    // each t (first nonzero axis d: [t] = PQ_d [t - 1_d] + (t_d - 1) [t - 2_d]),
+    // and the index of p + q
+    for (int j = lane; j < nh; j += 64) {
+        const int c = fj_herm_decode(j);
+        const int x = c & 15, y = (c >> 4) & 15, z = c >> 8;
+        hc[j] = c;
+        int w = 0;
+        if (j > 0) {
+            const int d = x ? 0 : (y ? 1 : 2);
+            const int td = d == 0 ? x : (d == 1 ? y : z);
+            const int i1 = fj_herm_index(x - (d == 0), y - (d == 1), z - (d == 2));
+            const int i2 = td >= 2 ? fj_herm_index(x - 2 * (d == 0), y - 2 * (d == 1), z - 2 * (d == 2))
+                                   : 0;
+            w = i1 | (i2 << 10) | (d << 20) | ((td - 1) << 22);
+        }
+        tw[j] = w;
+    }
+    for (int j = lane; j < np * nq; j += 64) {
+        const int p = j / nq, q = j - p * nq;
+        const int cp = fj_herm_decode(p), cq = fj_herm_decode(q);
+        pqi[j] = (unsigned short)fj_herm_index((cp & 15) + (cq & 15),
+                                               ((cp >> 4) & 15) + ((cq >> 4) & 15),
+                                               (cp >> 8) + (cq >> 8));
+    }
+    for (int j = lane; j < na * nc * nb; j += 64) kac[j] = 0.0;
+    const double z = zeta_ac[bra];
+    const double Px = P[3 * bra], Py = P[3 * bra + 1], Pz = P[3 * bra + 2];
+    if (lane < 3) fj_kl_eaxis(ea + lane * FJ_KL_AX, la, lc, z, XPA[3 * bra + lane], XPC[3 * bra + lane]);
+    const double ip = Ip[bra];
+    const int dcb = dc[bra];
+    __syncthreads();
+
+    for (int k = 0; k < n_ket; ++k) {
+        if (ip * Iq[k] * maxD < tau) break;   // kets sorted by Iq: the rest are smaller
+        const int blk = dcb + dd[k];
+        if (blk < 0 || blk >= n_dc) continue;   // the host never names such a block
+        const double e = eta[k], sb = s_bd[k];
+        const double PQx = Px - Q[3 * k], PQy = Py - Q[3 * k + 1], PQz = Pz - Q[3 * k + 2];
+        const double rho = z * e / (z + e);
+        if (lane < 3) fj_kl_eaxis(eb + lane * FJ_KL_AX, lb, ld, e, XQB[3 * k + lane], XQD[3 * k + lane]);
+        const double* __restrict__ dm = Dcd + (size_t)blk * (nc * nd);
+        for (int j = lane; j < nc * nd; j += 64) dsh[j] = dm[j];
+        if (lane == 0) {
+            // [0]^(m) = (2 / sqrt(pi)) sqrt(rho) (-2 rho)^m F_m(T)
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            double Fm[FJ_KL_MMAX + 1];
+            fj_boys<FJ_KL_MMAX>(T, Fm);
+            const double m2r = -2.0 * rho;
+            double wm = FJ_TWO_OVER_SQRT_PI * sqrt(rho);
+#pragma unroll
+            for (int m = 0; m <= FJ_KL_MMAX; ++m) {
+                wf[m] = wm * Fm[m];
+                wm *= m2r;
+            }
+            R[L & 1][0] = wf[L];
+        }
+        __syncthreads();
+        // level m holds [t]^(m) for |t| <= L - m; level 0 ends in R[0]
+        for (int m = L - 1; m >= 0; --m) {
+            double* __restrict__ cur = R[m & 1];
+            const double* __restrict__ prv = R[(m + 1) & 1];
+            const int n_m = fj_herm_base(L - m + 1);
+            for (int j = lane; j < n_m; j += 64) {
+                double v;
+                if (j == 0) {
+                    v = wf[m];
+                } else {
+                    const int w = tw[j];
+                    const int d = (w >> 20) & 3, c1 = w >> 22;
+                    const double pq = d == 0 ? PQx : (d == 1 ? PQy : PQz);
+                    v = pq * prv[w & 1023];
+                    if (c1 > 0) v += (double)c1 * prv[(w >> 10) & 1023];
+                }
+                cur[j] = v;
+            }
+            __syncthreads();
+        }
+        const double* __restrict__ R0 = R[0];
+        for (int ib = 0; ib < nb; ++ib) {
+            const int cb = hc[ob + ib];
+            const int bx = cb & 15, by = (cb >> 4) & 15, bz = cb >> 8;
+            for (int j = lane; j < nc * nq; j += 64) {
+                const int c = j / nq, q = j - c * nq;
+                const int cq = hc[q];
+                const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
+                double s = 0.0;
+                for (int d = 0; d < nd; ++d) {
+                    const int cd = hc[od + d];
+                    const double ex = eb[(bx * (ld + 1) + (cd & 15)) * tb + qx];
+                    const double ey = eb[FJ_KL_AX + (by * (ld + 1) + ((cd >> 4) & 15)) * tb + qy];
+                    const double ez = eb[2 * FJ_KL_AX + (bz * (ld + 1) + (cd >> 8)) * tb + qz];
+                    s += dsh[c * nd + d] * (ex * ey * ez);
+                }
+                X[j] = ((qx + qy + qz) & 1 ? -sb : sb) * s;
+            }
+            __syncthreads();
+            for (int j = lane; j < nc * np; j += 64) {
+                const int c = j / np, p = j - c * np;
+                const double* __restrict__ xr = X + c * nq;
+                const unsigned short* __restrict__ ir = pqi + p * nq;
+                double s = 0.0;
+                for (int q = 0; q < nq; ++q) s += R0[ir[q]] * xr[q];
+                Y[j] = s;
+            }
+            __syncthreads();
+            // X of the next b is written before the next barrier, Y after it
+            for (int j = lane; j < na * nc; j += 64) {
+                const int a = j / nc, c = j - a * nc;
+                const int ca = hc[oa + a], cc = hc[oc + c];
+                const double* __restrict__ ex = ea + ((ca & 15) * (lc + 1) + (cc & 15)) * ta;
+                const double* __restrict__ ey = ea + FJ_KL_AX + (((ca >> 4) & 15) * (lc + 1) + ((cc >> 4) & 15)) * ta;
+                const double* __restrict__ ez = ea + 2 * FJ_KL_AX + ((ca >> 8) * (lc + 1) + (cc >> 8)) * ta;
+                const double* __restrict__ yr = Y + c * np;
+                double s = 0.0;
+                for (int p = 0; p < np; ++p) {
+                    const int cp = hc[p];
+                    s += (ex[cp & 15] * ey[(cp >> 4) & 15] * ez[cp >> 8]) * yr[p];
+                }
+                kac[j * nb + ib] += s;
+            }
+        }
+    }
+    __syncthreads();
+    const double sa = s_ac[bra];
+    for (int j = lane; j < na * nb; j += 64) {
+        const int a = j / nb, b = j - a * nb;
+        double s = 0.0;
+        for (int c = 0; c < nc; ++c) s += kac[(a * nc + c) * nb + b];
+        K[(size_t)bra * (na * nb) + j] = sa * s;
+    }
+}

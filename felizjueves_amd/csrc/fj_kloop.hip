@@ -1,0 +1,37 @@
+// fj_kloop.hip -- launcher of the loop-form K kernel (fj_kloop.h): the
+// entries fj_k_<la><lc><lb><ld> of the classes with an f shell and the
+// generic fj_k_loop.
+#include "fj_device.h"
+#include "fj_kloop.h"
+
+int fj_kloop_launch(hipStream_t s, int la, int lc, int lb, int ld, int threads, int n_bra, int n_ket,
+                    int n_dc, double tau, double maxD, const double* P, const double* zeta_ac,
+                    const double* XPA, const double* XPC, const double* s_ac, const double* Ip,
+                    const int* dc, const double* Q, const double* eta, const double* XQB,
+                    const double* XQD, const double* s_bd, const double* Iq, const int* dd,
+                    const double* Dcd, double* K) {
+    if (la < 0 || la > FJ_KL_LMAX || lc < 0 || lc > FJ_KL_LMAX || lb < 0 || lb > FJ_KL_LMAX ||
+        ld < 0 || ld > FJ_KL_LMAX)
+        return FJ_ERR_ARG;
+    // threads is checked as for the straight-line kernels; this kernel runs
+    // one wavefront per workgroup whatever it asks
+    if (threads <= 0) threads = 256;
+    if (threads % 64 || threads > 256) return FJ_ERR_THREADS;
+    if (n_bra < 0 || n_ket < 0 || n_dc < 0) return FJ_ERR_SHAPE;
+    if (n_bra == 0) return 0;
+    hipLaunchKernelGGL(fj_kloop_k, dim3((unsigned)n_bra), dim3(64), 0, s, la, lc, lb, ld, n_bra, n_ket,
+                       n_dc, tau, maxD, P, zeta_ac, XPA, XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, s_bd, Iq,
+                       dd, Dcd, K);
+    return (int)hipGetLastError();
+}
+
+extern "C" int fj_k_loop(hipStream_t s, int la, int lc, int lb, int ld, int threads, int n_bra,
+                         int n_ket, int n_dc, double tau, double maxD, const double* P,
+                         const double* zeta_ac, const double* XPA, const double* XPC,
+                         const double* s_ac, const double* Ip, const int* dc, const double* Q,
+                         const double* eta, const double* XQB, const double* XQD,
+                         const double* s_bd, const double* Iq, const int* dd, const double* Dcd,
+                         double* K) {
+    return fj_kloop_launch(s, la, lc, lb, ld, threads, n_bra, n_ket, n_dc, tau, maxD, P, zeta_ac, XPA,
+                           XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, s_bd, Iq, dd, Dcd, K);
+}

@@ -545,9 +545,10 @@ SCHWARZ_LSUM_MAX = 4
 
 
 def units(lmax: int, k_lmax: int | None = None) -> dict[str, str]:
-    """Every unit through angular momentum lmax; the K classes only through
-    k_lmax (their straight-line Hermite towers grow as L^6: (ff|ff) alone
-    would be 18 MB of source)."""
+    """Every unit through angular momentum lmax. The K classes are
+    straight-line kernels only through k_lmax (their Hermite towers grow as
+    L^6 in source: (ff|ff) alone would be 18 MB); the classes past it get
+    entries onto the loop-form kernel (csrc/fj_kloop.h)."""
     k_lmax = lmax if k_lmax is None else k_lmax
     out = {}
     for la, lb in pair_classes(lmax):
@@ -561,7 +562,38 @@ def units(lmax: int, k_lmax: int | None = None) -> dict[str, str]:
             out[f"pq32_L{lp + lq}_p{lp}_q{lq}.hip"] = emit_pq(lp, lq, f32=True)
     for c in k_classes(k_lmax):
         out[f"k_{''.join(map(str, c))}.hip"] = emit_k(*c)
+    loop = [c for c in k_classes(lmax) if max(c) > k_lmax]
+    if loop:
+        out["k_loop_entries.hip"] = emit_k_loop_entries(loop)
     return out
+
+
+K_ARGS = ("int n_bra, int n_ket, int n_dc, double tau, double maxD, const double* P,\n"
+          "    const double* zeta_ac, const double* XPA, const double* XPC, const double* s_ac,\n"
+          "    const double* Ip, const int* dc, const double* Q, const double* eta,\n"
+          "    const double* XQB, const double* XQD, const double* s_bd, const double* Iq,\n"
+          "    const int* dd, const double* Dcd, double* K")
+K_NAMES = ("n_bra, n_ket, n_dc, tau, maxD, P, zeta_ac, XPA, XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, "
+           "s_bd, Iq, dd, Dcd, K")
+
+
+def emit_k_loop_entries(classes) -> str:
+    """The fj_k_<cls> entries of the classes past the straight-line K
+    kernels: the same parameter list, launched through the loop-form kernel
+    (csrc/fj_kloop.h) with the class as arguments."""
+    E = _Emit()
+    E("// generated by felizjueves_amd/gen_mmd.py -- K classes in loop form", 0)
+    E('#include "../fj_device.h"', 0)
+    E("", 0)
+    E("int fj_kloop_launch(hipStream_t s, int la, int lc, int lb, int ld, int threads,", 0)
+    E(K_ARGS.replace("\n    ", " ") + ");")
+    for c in classes:
+        n = "".join(map(str, c))
+        E("", 0)
+        E(f'extern "C" int fj_k_{n}(hipStream_t s, int threads, ' + K_ARGS.replace("\n    ", " ") + ") {", 0)
+        E(f"return fj_kloop_launch(s, {c[0]}, {c[1]}, {c[2]}, {c[3]}, threads, {K_NAMES});")
+        E("}", 0)
+    return E.text()
 
 
 def generate(lmax: int, out: Path = GEN, k_lmax: int | None = None) -> list[Path]:
@@ -602,7 +634,7 @@ def header_decls(lmax: int, k_lmax: int | None = None) -> str:
                     f"    const double* P, const double* zeta, const double* Ip, const double* Q,\n"
                     f"    const double* eta, const double* Iq, const double* Dmax, const double* D,"
                     f" double* J);")
-    for c in k_classes(k_lmax):
+    for c in k_classes(lmax):
         out.append(
             f"int fj_k_{''.join(map(str, c))}(fj_stream_t stream, int threads, int n_bra, int n_ket,"
             f" int n_dc,\n"

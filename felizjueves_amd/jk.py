@@ -37,7 +37,7 @@ from .basis import Basis, n_cart
 from .pairs import hermite_bound_sq, hermite_order
 
 
-K_LMAX = 2   # the K kernels are built through d (Schwarz and the J chain through f)
+K_LMAX = 3   # the K kernels through f (straight-line through d, the f classes in loop form)
 
 
 def _records(basis: Basis, a: int, b: int):
@@ -218,76 +218,103 @@ class CoulombChain:
         return J
 
 
+def _exchange_side(basis: Basis, by_l: dict, l1: int, l2: int, fixed: int | None = None):
+    """Records of every ordered pair (s1, s2), l(s1) = l1, l(s2) = l2 (s1 ==
+    fixed if given), with each record's s1 index, s2 index and s2's position
+    among the shells of momentum l2."""
+    firsts = [fixed] if fixed is not None else by_l.get(l1, [])
+    recs, s1, s2, pos2 = [], [], [], []
+    for u in firsts:
+        for j, v in enumerate(by_l.get(l2, [])):
+            r = _records(basis, u, v)
+            recs.append(r)
+            n = len(r[1])
+            s1.append(np.full(n, u))
+            s2.append(np.full(n, v))
+            pos2.append(np.full(n, j))
+    if not recs:
+        return None
+    cat = lambda k: np.concatenate([r[k] for r in recs])  # noqa: E731
+    return dict(P=cat(0), z=cat(1), s=cat(2), X1=cat(3), X2=cat(4),
+                s1=np.concatenate(s1), s2=np.concatenate(s2), pos2=np.concatenate(pos2))
+
+
+def exchange_tables(basis: Basis) -> dict:
+    """The K kernel's host tables (numpy): bra records per (la, lc) with
+    their D-block row per ld, ket records per (lb, ld, b shell) sorted by
+    bound, and the D-block gather indices per (lc, ld) (upstream
+    eri/k_exchange/launch_k.py:308-401 builds the same records per launch)."""
+    b = basis
+    by_l = {}
+    for s in range(b.n_sh):
+        by_l.setdefault(int(b.shell_l[s]), []).append(s)
+    lmax = max(by_l)
+    if lmax > K_LMAX:
+        raise ValueError(f"shells up to l = {lmax}: the K kernels are built through l = {K_LMAX}")
+    L = sorted(by_l)
+    bras, kets, dblk = {}, {}, {}
+    for la in L:
+        for lc in L:
+            bra = _exchange_side(b, by_l, la, lc)
+            if bra is None:
+                continue
+            t = dict(n=len(bra["z"]), P=bra["P"], z=bra["z"], s=bra["s"], XA=bra["X1"],
+                     XC=bra["X2"], Ip=record_bound(bra["z"], la + lc),
+                     oa=b.shell_ao[bra["s1"]].astype(np.int32))
+            for ld in L:
+                t[("dc", ld)] = (np.searchsorted(by_l[lc], bra["s2"]) * len(by_l[ld])).astype(np.int32)
+            bras[(la, lc)] = t
+    for lc in L:
+        for ld in L:
+            prs = [(c, d) for c in by_l[lc] for d in by_l[ld]]
+            dblk[(lc, ld)] = dict(n=len(prs), idx=_block_index(b, prs, lc, ld))
+    for lb in L:
+        for ld in L:
+            for bs in by_l[lb]:
+                ket = _exchange_side(b, by_l, lb, ld, fixed=bs)
+                if ket is None:
+                    continue
+                Iq = record_bound(ket["z"], lb + ld)
+                order = np.argsort(-Iq, kind="stable")
+                kets[(lb, ld, bs)] = dict(
+                    n=len(ket["z"]), Q=ket["P"][order], eta=ket["z"][order], s=ket["s"][order],
+                    XB=ket["X1"][order], XD=ket["X2"][order], Iq=Iq[order],
+                    pos=ket["pos2"][order].astype(np.int32))
+    return dict(by_l=by_l, bras=bras, kets=kets, dblk=dblk)
+
+
+def exchange_launches(tables: dict):
+    """(la, lc, lb, ld, b shell) of every K launch, in launch order: each
+    writes the K rows of the (la, lc) bra records against the kets of one b
+    shell, scattered into K's (a, b) block."""
+    by_l = tables["by_l"]
+    L = sorted(by_l)
+    for (la, lc) in tables["bras"]:
+        for ld in L:
+            for lb in L:
+                for bs in by_l[lb]:
+                    if (lb, ld, bs) in tables["kets"]:
+                        yield la, lc, lb, ld, bs
+
+
 class ExchangeBuild:
     """K_ab = sum_cd D_cd (ac|bd) through the K kernel; `run(D)` returns K
-    on the device. The bra tables (per (la, lc)), the ket tables (per b
-    shell and ld, sorted by bound) and the D-block indices are built once
-    here; `run` gathers D's blocks on the device and launches."""
+    on the device. The host tables (`exchange_tables`) are built and copied
+    to the device once here; `run` gathers D's blocks on the device and
+    launches."""
 
     def __init__(self, basis: Basis, tau: float = 0.0):
         self.basis = basis
         self.tau = float(tau)
-        b = basis
-        self.by_l = {}
-        for s in range(b.n_sh):
-            self.by_l.setdefault(int(b.shell_l[s]), []).append(s)
+        host = exchange_tables(basis)
+        self.by_l = host["by_l"]
         self.lmax = max(self.by_l)
-        if self.lmax > K_LMAX:
-            raise ValueError(f"shells up to l = {self.lmax}: the K kernels are built through "
-                             f"l = {K_LMAX}")
-        L = sorted(self.by_l)
+        self.order = list(exchange_launches(host))
         dev = lambda x: native.to_dev(np.ascontiguousarray(x))  # noqa: E731
-        self.bras, self.kets, self.dblk = {}, {}, {}
-        for la in L:
-            for lc in L:
-                bra = self._side(la, lc)
-                if bra is None:
-                    continue
-                self.bras[(la, lc)] = dict(
-                    n=len(bra["z"]), P=dev(bra["P"]), z=dev(bra["z"]), s=dev(bra["s"]),
-                    XA=dev(bra["X1"]), XC=dev(bra["X2"]), Ip=dev(record_bound(bra["z"], la + lc)),
-                    oa=dev(b.shell_ao[bra["s1"]].astype(np.int32)), s2=bra["s2"])
-        for lc in L:
-            for ld in L:
-                prs = [(c, d) for c in self.by_l[lc] for d in self.by_l[ld]]
-                self.dblk[(lc, ld)] = dict(n=len(prs), idx=dev(_block_index(b, prs, lc, ld)))
-        for (la, lc), t in self.bras.items():
-            for ld in L:
-                t[("dc", ld)] = dev((np.searchsorted(self.by_l[lc], t["s2"]) *
-                                     len(self.by_l[ld])).astype(np.int32))
-        for lb in L:
-            for ld in L:
-                for bs in self.by_l[lb]:
-                    ket = self._side(lb, ld, fixed=bs)
-                    if ket is None:
-                        continue
-                    Iq = record_bound(ket["z"], lb + ld)
-                    order = np.argsort(-Iq, kind="stable")
-                    self.kets[(lb, ld, bs)] = dict(
-                        n=len(ket["z"]), Q=dev(ket["P"][order]), eta=dev(ket["z"][order]),
-                        s=dev(ket["s"][order]), XB=dev(ket["X1"][order]), XD=dev(ket["X2"][order]),
-                        Iq=dev(Iq[order]), pos=dev(ket["pos2"][order].astype(np.int32)))
-
-    def _side(self, l1: int, l2: int, fixed: int | None = None):
-        """Records of every ordered pair (s1, s2), l(s1) = l1, l(s2) = l2
-        (s1 == fixed if given), with each record's s1 index, s2 index and
-        s2's position among the shells of momentum l2."""
-        b = self.basis
-        firsts = [fixed] if fixed is not None else self.by_l.get(l1, [])
-        recs, s1, s2, pos2 = [], [], [], []
-        for u in firsts:
-            for j, v in enumerate(self.by_l.get(l2, [])):
-                r = _records(b, u, v)
-                recs.append(r)
-                n = len(r[1])
-                s1.append(np.full(n, u))
-                s2.append(np.full(n, v))
-                pos2.append(np.full(n, j))
-        if not recs:
-            return None
-        cat = lambda k: np.concatenate([r[k] for r in recs])  # noqa: E731
-        return dict(P=cat(0), z=cat(1), s=cat(2), X1=cat(3), X2=cat(4),
-                    s1=np.concatenate(s1), s2=np.concatenate(s2), pos2=np.concatenate(pos2))
+        up = lambda t: {k: (dev(v) if isinstance(v, np.ndarray) else v) for k, v in t.items()}  # noqa: E731
+        self.bras = {k: up(t) for k, t in host["bras"].items()}
+        self.kets = {k: up(t) for k, t in host["kets"].items()}
+        self.dblk = {k: up(t) for k, t in host["dblk"].items()}
 
     def run(self, D):
         import torch
@@ -298,32 +325,24 @@ class ExchangeBuild:
         maxD = float(Dd.abs().max().item())
         Dflat = Dd.view(-1)
         Dcd = {k: Dflat[t["idx"]].contiguous() for k, t in self.dblk.items()}
-        L = sorted(self.by_l)
         # one scratch for every launch's rows: the K kernel writes each bra
         # row whole, and the scatter reads it before the next launch on the
         # same stream writes it again
-        n_rows = max(t["n"] * n_cart(la) * n_cart(lb)
-                     for (la, _), t in self.bras.items() for lb in L)
+        n_rows = max(self.bras[(la, lc)]["n"] * n_cart(la) * n_cart(lb)
+                     for la, lc, lb, _, _ in self.order)
         scratch = torch.empty(n_rows, dtype=torch.float64, device=dev)
-        for (la, lc), t in self.bras.items():
-            for ld in L:
-                dDcd = Dcd[(lc, ld)]
-                for lb in L:
-                    nab = n_cart(la) * n_cart(lb)
-                    for bs in self.by_l[lb]:
-                        k = self.kets.get((lb, ld, bs))
-                        if k is None:
-                            continue
-                        Kr = scratch[:t["n"] * nab]
-                        _call(f"fj_k_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
-                              C.c_int(self.dblk[(lc, ld)]["n"]), C.c_double(self.tau),
-                              C.c_double(maxD), _c(t["P"]), _c(t["z"]), _c(t["XA"]), _c(t["XC"]),
-                              _c(t["s"]), _c(t["Ip"]), _c(t[("dc", ld)]), _c(k["Q"]), _c(k["eta"]),
-                              _c(k["XB"]), _c(k["XD"]), _c(k["s"]), _c(k["Iq"]), _c(k["pos"]),
-                              _c(dDcd), _c(Kr))
-                        _call("fj_k_scatter", C.c_int(t["n"]), C.c_int(n_cart(la)),
-                              C.c_int(n_cart(lb)), C.c_int(b.n_ao), _c(t["oa"]),
-                              C.c_int(int(b.shell_ao[bs])), _c(Kr), _c(K))
+        for la, lc, lb, ld, bs in self.order:
+            t, k = self.bras[(la, lc)], self.kets[(lb, ld, bs)]
+            Kr = scratch[:t["n"] * n_cart(la) * n_cart(lb)]
+            _call(f"fj_k_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
+                  C.c_int(self.dblk[(lc, ld)]["n"]), C.c_double(self.tau),
+                  C.c_double(maxD), _c(t["P"]), _c(t["z"]), _c(t["XA"]), _c(t["XC"]),
+                  _c(t["s"]), _c(t["Ip"]), _c(t[("dc", ld)]), _c(k["Q"]), _c(k["eta"]),
+                  _c(k["XB"]), _c(k["XD"]), _c(k["s"]), _c(k["Iq"]), _c(k["pos"]),
+                  _c(Dcd[(lc, ld)]), _c(Kr))
+            _call("fj_k_scatter", C.c_int(t["n"]), C.c_int(n_cart(la)),
+                  C.c_int(n_cart(lb)), C.c_int(b.n_ao), _c(t["oa"]),
+                  C.c_int(int(b.shell_ao[bs])), _c(Kr), _c(K))
         return K
 
 

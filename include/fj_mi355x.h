@@ -538,7 +538,21 @@ int fj_fxm_2222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide
  *   fj_jab_<la><lb>      J_ab = sum_p (s E_p) J_p scattered into J, atomics
  *                        (upstream eri/j_coulomb/kernel_jab.py:10-60)
  *   fj_k_<la><lc><lb><ld>  K[bra, a, b] = sum_ket D_cd (ac|bd), kets sorted
- *                        by Iq (upstream eri/k_exchange/kernel_k.py:1-110) */
+ *                        by Iq (upstream eri/k_exchange/kernel_k.py:1-110);
+ *                        straight-line kernels through d, the classes with
+ *                        an f shell on the loop-form kernel of fj_k_loop */
+/* The K kernel in loop form for any class (la lc | lb ld) through f, the
+ * class as arguments, otherwise the fj_k_<la><lc><lb><ld> parameter list
+ * (upstream eri/k_exchange/kernel_k.py:314, host k_quartet k.py:395). One
+ * wavefront per bra record whatever `threads` asks (checked as for the
+ * straight-line kernels); the f classes' entries call it, and it also runs
+ * the classes through d for checking one form against the other. */
+int fj_k_loop(fj_stream_t stream, int la, int lc, int lb, int ld, int threads, int n_bra, int n_ket,
+              int n_dc, double tau, double maxD, const double* P, const double* zeta_ac,
+              const double* XPA, const double* XPC, const double* s_ac, const double* Ip,
+              const int* dc, const double* Q, const double* eta, const double* XQB,
+              const double* XQD, const double* s_bd, const double* Iq, const int* dd,
+              const double* Dcd, double* K);
 /* K[oa[r] + i][ob + j] += Kr[r][i * nb + j] for n_rec records: the K
  * kernel's per-record blocks summed into K (the host-side sum and scatter
  * of upstream eri/k_exchange/launch_k.py:401-479, on the device). */
@@ -970,6 +984,11 @@ int fj_k_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0003(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -981,6 +1000,11 @@ int fj_k_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0013(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1000,6 +1024,31 @@ int fj_k_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0023(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0030(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0031(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0032(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0033(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1011,6 +1060,11 @@ int fj_k_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0103(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1030,6 +1084,11 @@ int fj_k_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0113(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1041,6 +1100,31 @@ int fj_k_0121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0123(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0130(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0131(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0132(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0133(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1060,6 +1144,11 @@ int fj_k_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0203(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1071,6 +1160,11 @@ int fj_k_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0213(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1090,6 +1184,111 @@ int fj_k_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0223(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0230(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0231(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0232(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0233(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0300(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0301(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0302(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0303(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0310(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0311(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0312(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0313(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0320(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0321(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0322(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0323(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0330(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0331(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0332(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_0333(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1101,6 +1300,11 @@ int fj_k_1001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1003(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1120,6 +1324,11 @@ int fj_k_1012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1013(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1131,6 +1340,31 @@ int fj_k_1021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1023(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1030(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1031(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1032(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1033(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1150,6 +1384,11 @@ int fj_k_1102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1103(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1161,6 +1400,11 @@ int fj_k_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1113(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1180,6 +1424,31 @@ int fj_k_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1123(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1130(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1131(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1132(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1133(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1191,6 +1460,11 @@ int fj_k_1201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1203(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1210,6 +1484,11 @@ int fj_k_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1213(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1221,6 +1500,111 @@ int fj_k_1221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1223(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1230(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1231(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1232(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1233(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1300(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1301(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1302(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1303(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1310(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1311(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1312(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1313(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1320(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1321(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1322(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1323(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1330(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1331(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1332(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_1333(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1240,6 +1624,11 @@ int fj_k_2002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2003(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1251,6 +1640,11 @@ int fj_k_2011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2013(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1270,6 +1664,31 @@ int fj_k_2022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2023(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2030(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2031(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2032(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2033(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1281,6 +1700,11 @@ int fj_k_2101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2103(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1300,6 +1724,11 @@ int fj_k_2112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2113(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1311,6 +1740,31 @@ int fj_k_2121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2123(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2130(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2131(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2132(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2133(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
@@ -1330,6 +1784,11 @@ int fj_k_2202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2203(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1345,6 +1804,11 @@ int fj_k_2212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2213(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
@@ -1356,6 +1820,431 @@ int fj_k_2221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 int fj_k_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2223(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2230(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2231(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2232(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2233(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2300(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2301(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2302(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2303(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2310(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2311(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2312(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2313(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2320(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2321(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2322(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2323(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2330(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2331(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2332(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_2333(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3003(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3013(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3023(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3030(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3031(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3032(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3033(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3103(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3113(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3123(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3130(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3131(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3132(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3133(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3203(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3213(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3223(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3230(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3231(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3232(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3233(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3300(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3301(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3302(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3303(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3310(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3311(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3312(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3313(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3320(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3321(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3322(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3323(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3330(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3331(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3332(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k_3333(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,

@@ -106,13 +106,31 @@ def test_entries_refuse_bad_arguments_before_any_launch():
     rc = h(None, C.c_int(96), C.c_int(0), C.byref(side), C.byref(side), work, C.c_int(1), None,
            sysd, C.c_int(0), C.c_double(0.0), None, C.c_longlong(0))
     assert rc == ERR_THREADS
+    # the loop-form K kernel: a class past f, then threads not a multiple of 64
+    k = lib.fj_k_loop
+    k.restype = C.c_int
+    nulls = [None] * 16
+    rc = k(None, C.c_int(4), C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(64), C.c_int(1),
+           C.c_int(1), C.c_int(1), C.c_double(0.0), C.c_double(1.0), *nulls)
+    assert rc == ERR_ARG
+    rc = k(None, C.c_int(3), C.c_int(3), C.c_int(3), C.c_int(3), C.c_int(100), C.c_int(1),
+           C.c_int(1), C.c_int(1), C.c_double(0.0), C.c_double(1.0), *nulls)
+    assert rc == ERR_THREADS
 
 
-def test_k_build_refuses_shells_above_d():
+def test_k_tables_cover_f_and_follow_the_launch_order():
+    """The K host tables reach f (CO/cc-pVTZ) and every launch names a bra
+    table, a ket table and a D-block table that exist."""
     from felizjueves_amd import jk
     from felizjueves_amd.basis import load_fixture
-    with pytest.raises(ValueError, match="K kernels"):
-        jk.ExchangeBuild(load_fixture("co_ccpvtz").basis)
+    t = jk.exchange_tables(load_fixture("co_ccpvtz").basis)
+    L = list(jk.exchange_launches(t))
+    assert max(la for la, *_ in L) == 3 and max(ld for _, _, _, ld, _ in L) == 3
+    for la, lc, lb, ld, bs in L:
+        assert (la, lc) in t["bras"] and (lb, ld, bs) in t["kets"] and (lc, ld) in t["dblk"]
+        # every D block a ket can name lies inside the gathered table
+        dc = t["bras"][(la, lc)][("dc", ld)]
+        assert dc.max() + t["kets"][(lb, ld, bs)]["pos"].max() < t["dblk"][(lc, ld)]["n"]
 
 
 def test_schwarz_refuses_pairs_above_its_built_classes():

@@ -1,0 +1,110 @@
+"""The loop-form K kernel (csrc/fj_kloop.h) runs unchanged on the CPU (CPU
+only): compiled with g++ against the emulation of the device header (one
+thread per lane, barriers for __syncthreads) under AddressSanitizer, it takes
+every K launch of a two-atom s,p,d,f basis -- the same host tables and launch
+order as `jk.ExchangeBuild` -- and, after the scatter, its K is compared with
+the oracle's K (every ordered shell quartet by McMurchie-Davidson in C).
+
+Every class through (ff|ff) runs through the loop form here, the d classes
+included, so the one kernel is checked over all 256 classes the library
+launches it for or could."""
+
+import os
+import shutil
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from felizjueves_amd import jk
+from felizjueves_amd.basis import Basis, Shell, n_cart, primitive_norm
+from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parent.parent
+EMU = ROOT / "tests" / "emu"
+CSRC = ROOT / "felizjueves_amd" / "csrc"
+
+
+def _shell(l, centre, exps, raw):  # noqa: E741
+    c = [w * primitive_norm(l, a) for w, a in zip(raw, exps)]
+    return Shell(l, tuple(centre), tuple(exps), tuple(c))
+
+
+def _system():
+    """Two atoms 1.9 bohr apart with s, p, d, f each (the first atom's s
+    contracted over two primitives); D a random symmetric matrix."""
+    A, B = (0.0, 0.0, 0.0), (0.7, -1.1, 1.3)
+    shells = [_shell(0, A, (3.1, 0.45), (0.6, 0.5))]
+    for X, sc in ((A, 1.0), (B, 0.8)):
+        if X is B:
+            shells.append(_shell(0, X, (0.5,), (1.0,)))
+        shells += [_shell(1, X, (1.3 * sc,), (1.0,)),
+                   _shell(2, X, (0.9 * sc,), (1.0,)),
+                   _shell(3, X, (0.7 * sc,), (1.0,))]
+    basis = Basis(shells)
+    rng = np.random.default_rng(5)
+    M = rng.normal(size=(basis.n_ao, basis.n_ao))
+    return basis, 0.1 * (M + M.T)
+
+
+def _dump(path: Path, basis, D, tables, launches, tau, maxD) -> None:
+    def w(f, x, dt):
+        x = np.ascontiguousarray(x, dtype=dt).ravel()
+        np.array([x.size], dtype=np.int64).tofile(f)
+        x.tofile(f)
+    with open(path, "wb") as f:
+        np.array([len(launches)], dtype=np.int32).tofile(f)
+        for la, lc, lb, ld, bs in launches:
+            t, k = tables["bras"][(la, lc)], tables["kets"][(lb, ld, bs)]
+            blk = tables["dblk"][(lc, ld)]
+            np.array([la, lc, lb, ld, t["n"], k["n"], blk["n"]], dtype=np.int32).tofile(f)
+            np.array([tau, maxD], dtype=np.float64).tofile(f)
+            for x in (t["P"], t["z"], t["XA"], t["XC"], t["s"], t["Ip"]):
+                w(f, x, np.float64)
+            w(f, t[("dc", ld)], np.int32)
+            for x in (k["Q"], k["eta"], k["XB"], k["XD"], k["s"], k["Iq"]):
+                w(f, x, np.float64)
+            w(f, k["pos"], np.int32)
+            w(f, D.ravel()[blk["idx"]], np.float64)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
+def test_loop_k_kernel_on_cpu_vs_oracle():
+    basis, D = _system()
+    tables = jk.exchange_tables(basis)
+    launches = list(jk.exchange_launches(tables))
+    maxD = float(np.abs(D).max())
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "gen").mkdir()
+        shutil.copy(EMU / "fj_device_emu.h", td / "fj_device.h")
+        shutil.copy(CSRC / "gen" / "fj_boys_table.h", td / "gen" / "fj_boys_table.h")
+        exe = td / "kloop_emu"
+        cmd = ["g++", "-std=c++20", "-O1", "-g0", "-fsanitize=address", f"-I{ROOT / 'include'}",
+               f"-I{td}", f'-DKLOOP_HEADER="{CSRC / "fj_kloop.h"}"', str(EMU / "kloop_emu.cpp"),
+               "-o", str(exe), "-lpthread"]
+        r = subprocess.run(cmd, cwd=td, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-3000:]
+        _dump(td / "case.bin", basis, D, tables, launches, 0.0, maxD)
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+        r = subprocess.run([str(exe), str(td / "case.bin"), str(td / "K.bin")], cwd=td, env=env,
+                           capture_output=True, text=True, timeout=900)
+        assert r.returncode == 0, r.stderr[-3000:]
+        rows = np.fromfile(td / "K.bin")
+    K = np.zeros((basis.n_ao, basis.n_ao))
+    o = 0
+    for la, lc, lb, ld, bs in launches:
+        t = tables["bras"][(la, lc)]
+        na, nb = n_cart(la), n_cart(lb)
+        Kr = rows[o:o + t["n"] * na * nb].reshape(t["n"], na, nb)
+        o += Kr.size
+        ob = int(basis.shell_ao[bs])
+        for r_, oa in enumerate(t["oa"]):
+            K[oa:oa + na, ob:ob + nb] += Kr[r_]
+    assert o == rows.size
+    _, Ko = O.fock_ordered(basis, D)
+    err = np.abs(K - Ko).max()
+    print("loop-form K vs oracle", err, "scale", np.abs(Ko).max(), flush=True)
+    assert err <= 1e-12 * max(1.0, np.abs(Ko).max())
