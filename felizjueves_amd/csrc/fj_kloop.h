@@ -1,4 +1,7 @@
-// fj_kloop.h -- the K kernel in loop form, every class through f:
+// fj_kloop.h -- the MMD kernels in loop form, every class through f: K
+// (below) and the Schwarz diagonal (fj_schwarz_loop_k, at the end).
+//
+// K:
 // K_ab += sum_cd D_cd (ac|bd) per bra record (upstream
 // eri/k_exchange/kernel_k.py:314, host k_quartet k.py:395), with the class
 // (la lc | lb ld) as run-time arguments instead of straight-line code.
@@ -78,6 +81,68 @@ __device__ __forceinline__ void fj_kl_eaxis(double* e, int l1, int l2, double ze
     }
 }
 
+// The tower's step table for every Hermite index j < nh: for t != 0 with
+// first nonzero axis d, [t] = PQ_d [t - 1_d] + (t_d - 1) [t - 2_d], packed as
+// index(t - 1_d) | index(t - 2_d) << 10 | d << 20 | (t_d - 1) << 22; with hc,
+// also the packed components of j.
+__device__ __forceinline__ void fj_kl_steps(int* tw, int* hc, int nh, int lane) {
+    for (int j = lane; j < nh; j += 64) {
+        const int c = fj_herm_decode(j);
+        const int x = c & 15, y = (c >> 4) & 15, z = c >> 8;
+        if (hc) hc[j] = c;
+        int w = 0;
+        if (j > 0) {
+            const int d = x ? 0 : (y ? 1 : 2);
+            const int td = d == 0 ? x : (d == 1 ? y : z);
+            const int i1 = fj_herm_index(x - (d == 0), y - (d == 1), z - (d == 2));
+            const int i2 = td >= 2 ? fj_herm_index(x - 2 * (d == 0), y - 2 * (d == 1), z - 2 * (d == 2))
+                                   : 0;
+            w = i1 | (i2 << 10) | (d << 20) | ((td - 1) << 22);
+        }
+        tw[j] = w;
+    }
+}
+
+// wf[m] = [0]^(m) = (2 / sqrt(pi)) sqrt(rho) (-2 rho)^m F_m(T), m <= 12.
+__device__ __forceinline__ void fj_kl_seed(double* wf, double rho, double T) {
+    double Fm[FJ_KL_MMAX + 1];
+    fj_boys<FJ_KL_MMAX>(T, Fm);
+    const double m2r = -2.0 * rho;
+    double wm = FJ_TWO_OVER_SQRT_PI * sqrt(rho);
+#pragma unroll
+    for (int m = 0; m <= FJ_KL_MMAX; ++m) {
+        wf[m] = wm * Fm[m];
+        wm *= m2r;
+    }
+}
+
+// Levels m = L - 1 .. 0 of the tower, lanes over t: level m holds [t]^(m)
+// for |t| <= L - m in R[m & 1], so [t]^(0) ends in R[0]. The caller has
+// written wf and R[L & 1][0] = wf[L] and passed a barrier; every lane of the
+// workgroup calls this (a barrier after each level).
+__device__ __forceinline__ void fj_kl_tower(double (*R)[FJ_KL_NH], const double* wf, const int* tw,
+                                            int L, double PQx, double PQy, double PQz, int lane) {
+    for (int m = L - 1; m >= 0; --m) {
+        double* __restrict__ cur = R[m & 1];
+        const double* __restrict__ prv = R[(m + 1) & 1];
+        const int n_m = fj_herm_base(L - m + 1);
+        for (int j = lane; j < n_m; j += 64) {
+            double v;
+            if (j == 0) {
+                v = wf[m];
+            } else {
+                const int w = tw[j];
+                const int d = (w >> 20) & 3, c1 = w >> 22;
+                const double pq = d == 0 ? PQx : (d == 1 ? PQy : PQz);
+                v = pq * prv[w & 1023];
+                if (c1 > 0) v += (double)c1 * prv[(w >> 10) & 1023];
+            }
+            cur[j] = v;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(64) fj_kloop_k(
     int la, int lc, int lb, int ld, int n_bra, int n_ket, int n_dc, double tau, double maxD,
     const double* __restrict__ P, const double* __restrict__ zeta_ac,
@@ -111,21 +176,7 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
     // index tables: components of every Hermite index, the tower's step of
     // each t (first nonzero axis d: [t] = PQ_d [t - 1_d] + (t_d - 1) [t - 2_d]),
     // and the index of p + q
-    for (int j = lane; j < nh; j += 64) {
-        const int c = fj_herm_decode(j);
-        const int x = c & 15, y = (c >> 4) & 15, z = c >> 8;
-        hc[j] = c;
-        int w = 0;
-        if (j > 0) {
-            const int d = x ? 0 : (y ? 1 : 2);
-            const int td = d == 0 ? x : (d == 1 ? y : z);
-            const int i1 = fj_herm_index(x - (d == 0), y - (d == 1), z - (d == 2));
-            const int i2 = td >= 2 ? fj_herm_index(x - 2 * (d == 0), y - 2 * (d == 1), z - 2 * (d == 2))
-                                   : 0;
-            w = i1 | (i2 << 10) | (d << 20) | ((td - 1) << 22);
-        }
-        tw[j] = w;
-    }
+    fj_kl_steps(tw, hc, nh, lane);
     for (int j = lane; j < np * nq; j += 64) {
         const int p = j / nq, q = j - p * nq;
         const int cp = fj_herm_decode(p), cq = fj_herm_decode(q);
@@ -152,40 +203,11 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
         const double* __restrict__ dm = Dcd + (size_t)blk * (nc * nd);
         for (int j = lane; j < nc * nd; j += 64) dsh[j] = dm[j];
         if (lane == 0) {
-            // [0]^(m) = (2 / sqrt(pi)) sqrt(rho) (-2 rho)^m F_m(T)
-            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            double Fm[FJ_KL_MMAX + 1];
-            fj_boys<FJ_KL_MMAX>(T, Fm);
-            const double m2r = -2.0 * rho;
-            double wm = FJ_TWO_OVER_SQRT_PI * sqrt(rho);
-#pragma unroll
-            for (int m = 0; m <= FJ_KL_MMAX; ++m) {
-                wf[m] = wm * Fm[m];
-                wm *= m2r;
-            }
+            fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz));
             R[L & 1][0] = wf[L];
         }
         __syncthreads();
-        // level m holds [t]^(m) for |t| <= L - m; level 0 ends in R[0]
-        for (int m = L - 1; m >= 0; --m) {
-            double* __restrict__ cur = R[m & 1];
-            const double* __restrict__ prv = R[(m + 1) & 1];
-            const int n_m = fj_herm_base(L - m + 1);
-            for (int j = lane; j < n_m; j += 64) {
-                double v;
-                if (j == 0) {
-                    v = wf[m];
-                } else {
-                    const int w = tw[j];
-                    const int d = (w >> 20) & 3, c1 = w >> 22;
-                    const double pq = d == 0 ? PQx : (d == 1 ? PQy : PQz);
-                    v = pq * prv[w & 1023];
-                    if (c1 > 0) v += (double)c1 * prv[(w >> 10) & 1023];
-                }
-                cur[j] = v;
-            }
-            __syncthreads();
-        }
+        fj_kl_tower(R, wf, tw, L, PQx, PQy, PQz, lane);
         const double* __restrict__ R0 = R[0];
         for (int ib = 0; ib < nb; ++ib) {
             const int cb = hc[ob + ib];
@@ -238,5 +260,94 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
         double s = 0.0;
         for (int c = 0; c < nc; ++c) s += kac[(a * nc + c) * nb + b];
         K[(size_t)bra * (na * nb) + j] = sa * s;
+    }
+}
+
+// Schwarz: Q_mn = sqrt(max((mn|mn), 0)) for every AO pair of one shell pair
+// (upstream eri/kernel_schwarz.py:249-412): (ab|ab) = sum_{i,k records}
+// s_i s_k sum_p E^i_p sum_q (-1)^|q| [p + q] E^k_q, p and q in the box
+// t_d <= a_d + b_d. One wavefront per shell pair; per record pair the E
+// tables and the tower in LDS, then lanes over the component rows (a, b).
+__global__ void __launch_bounds__(64) fj_schwarz_loop_k(
+    int la, int lb, int n_pair, int n_ao, const double* __restrict__ zeta_ab,
+    const double* __restrict__ P, const double* __restrict__ XPA, const double* __restrict__ XPB,
+    const double* __restrict__ s_ab, const int* __restrict__ rec0, const int* __restrict__ rec1,
+    const int* __restrict__ off_a, const int* __restrict__ off_b, double* __restrict__ Q) {
+    __shared__ double R[2][FJ_KL_NH];
+    __shared__ double ei[3 * FJ_KL_AX], ek[3 * FJ_KL_AX];
+    __shared__ double wf[FJ_KL_MMAX + 1];
+    __shared__ int tw[FJ_KL_NH];
+
+    const int pair = (int)blockIdx.x, lane = (int)threadIdx.x;
+    if (pair >= n_pair) return;   // uniform per workgroup
+    const int L = 2 * (la + lb), nh = fj_herm_base(L + 1);
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2, nr = na * nb;
+    const int nj = lb + 1, tt = la + lb + 1;   // E table strides
+    fj_kl_steps(tw, nullptr, nh, lane);
+    double acc[2] = {0.0, 0.0};   // rows lane and lane + 64 (nr <= 100)
+    const int r0 = rec0[pair], r1 = rec1[pair];
+    __syncthreads();
+    for (int i = r0; i < r1; ++i) {
+        for (int k = r0; k < r1; ++k) {
+            const double zi = zeta_ab[i], zk = zeta_ab[k];
+            if (lane < 3)
+                fj_kl_eaxis(ei + lane * FJ_KL_AX, la, lb, zi, XPA[3 * i + lane], XPB[3 * i + lane]);
+            else if (lane < 6)
+                fj_kl_eaxis(ek + (lane - 3) * FJ_KL_AX, la, lb, zk, XPA[3 * k + lane - 3],
+                            XPB[3 * k + lane - 3]);
+            const double PQx = P[3 * i] - P[3 * k], PQy = P[3 * i + 1] - P[3 * k + 1],
+                         PQz = P[3 * i + 2] - P[3 * k + 2];
+            const double rho = zi * zk / (zi + zk);
+            if (lane == 0) {
+                fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz));
+                R[L & 1][0] = wf[L];
+            }
+            __syncthreads();
+            fj_kl_tower(R, wf, tw, L, PQx, PQy, PQz, lane);
+            const double sik = s_ab[i] * s_ab[k];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int r = lane + 64 * h;
+                if (r >= nr) continue;
+                const int a = r / nb, b = r - a * nb;
+                const int ca = fj_herm_decode(fj_herm_base(la) + a), cb = fj_herm_decode(fj_herm_base(lb) + b);
+                const int ax = ca & 15, ay = (ca >> 4) & 15, az = ca >> 8;
+                const int bx = cb & 15, by = (cb >> 4) & 15, bz = cb >> 8;
+                const double* __restrict__ ix = ei + (ax * nj + bx) * tt;
+                const double* __restrict__ iy = ei + FJ_KL_AX + (ay * nj + by) * tt;
+                const double* __restrict__ iz = ei + 2 * FJ_KL_AX + (az * nj + bz) * tt;
+                const double* __restrict__ kx = ek + (ax * nj + bx) * tt;
+                const double* __restrict__ ky = ek + FJ_KL_AX + (ay * nj + by) * tt;
+                const double* __restrict__ kz = ek + 2 * FJ_KL_AX + (az * nj + bz) * tt;
+                double v = 0.0;
+                for (int px = 0; px <= ax + bx; ++px)
+                    for (int py = 0; py <= ay + by; ++py)
+                        for (int pz = 0; pz <= az + bz; ++pz) {
+                            double w = 0.0;
+                            for (int qx = 0; qx <= ax + bx; ++qx)
+                                for (int qy = 0; qy <= ay + by; ++qy)
+                                    for (int qz = 0; qz <= az + bz; ++qz) {
+                                        const double e = kx[qx] * ky[qy] * kz[qz];
+                                        const double rv = R[0][fj_herm_index(px + qx, py + qy, pz + qz)];
+                                        w += ((qx + qy + qz) & 1 ? -e : e) * rv;
+                                    }
+                            v += (ix[px] * iy[py] * iz[pz]) * w;
+                        }
+                acc[h] += sik * v;
+            }
+            __syncthreads();   // before the next record pair rewrites ei, ek, wf and R
+        }
+    }
+    const int oa = off_a[pair], ob = off_b[pair];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h;
+        if (r >= nr) continue;
+        const int a = r / nb, b = r - a * nb;
+        const double q = sqrt(fmax(acc[h], 0.0));
+        if (la != lb || oa != ob || a >= b) {
+            Q[(size_t)(oa + a) * n_ao + ob + b] = q;
+            Q[(size_t)(ob + b) * n_ao + oa + a] = q;
+        }
     }
 }

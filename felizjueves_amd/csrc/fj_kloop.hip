@@ -1,6 +1,7 @@
-// fj_kloop.hip -- launcher of the loop-form K kernel (fj_kloop.h): the
-// entries fj_k_<la><lc><lb><ld> of the classes with an f shell and the
-// generic fj_k_loop.
+// fj_kloop.hip -- launchers of the loop-form MMD kernels (fj_kloop.h): K
+// (the entries fj_k_<la><lc><lb><ld> of the classes with an f shell and the
+// generic fj_k_loop) and Schwarz (fj_schwarz_<la><lb> past la + lb = 4 and
+// the generic fj_schwarz_loop).
 #include "fj_device.h"
 #include "fj_kloop.h"
 
@@ -34,4 +35,29 @@ extern "C" int fj_k_loop(hipStream_t s, int la, int lc, int lb, int ld, int thre
                          double* K) {
     return fj_kloop_launch(s, la, lc, lb, ld, threads, n_bra, n_ket, n_dc, tau, maxD, P, zeta_ac, XPA,
                            XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, s_bd, Iq, dd, Dcd, K);
+}
+
+int fj_schwarz_loop_launch(hipStream_t s, int la, int lb, int threads, int n_pair, int n_rec, int n_ao,
+                           const double* zeta_ab, const double* P, const double* XPA,
+                           const double* XPB, const double* s_ab, const int* rec0, const int* rec1,
+                           const int* off_a, const int* off_b, double* schwarz_bound) {
+    if (la < 0 || la > FJ_KL_LMAX || lb < 0 || lb > FJ_KL_LMAX) return FJ_ERR_ARG;
+    // threads is checked as for the straight-line kernels; this kernel runs
+    // one wavefront per shell pair whatever it asks
+    if (threads <= 0) threads = 64;
+    if (threads % 64 || threads > 256) return FJ_ERR_THREADS;
+    if (n_pair < 0 || n_rec < 0 || n_ao <= 0) return FJ_ERR_SHAPE;
+    if (n_pair == 0) return 0;
+    hipLaunchKernelGGL(fj_schwarz_loop_k, dim3((unsigned)n_pair), dim3(64), 0, s, la, lb, n_pair, n_ao,
+                       zeta_ab, P, XPA, XPB, s_ab, rec0, rec1, off_a, off_b, schwarz_bound);
+    return (int)hipGetLastError();
+}
+
+extern "C" int fj_schwarz_loop(hipStream_t s, int la, int lb, int threads, int n_pair, int n_rec,
+                               int n_ao, const double* zeta_ab, const double* P, const double* XPA,
+                               const double* XPB, const double* s_ab, const int* rec0,
+                               const int* rec1, const int* off_a, const int* off_b,
+                               double* schwarz_bound) {
+    return fj_schwarz_loop_launch(s, la, lb, threads, n_pair, n_rec, n_ao, zeta_ab, P, XPA, XPB, s_ab,
+                                  rec0, rec1, off_a, off_b, schwarz_bound);
 }

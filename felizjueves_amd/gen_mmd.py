@@ -537,10 +537,10 @@ def k_classes(lmax: int):
     return [(a, c, b, d) for a in r for c in r for b in r for d in r]
 
 
-# Schwarz units only for pairs with la + lb <= this: the (ab|ab) tower of a
-# (df) or (ff) pair keeps too much live (8.5 / 19 KB of scratch per lane,
-# above the library's 6 KB cap); the J chain screens with host bounds and
-# does not need them.
+# Straight-line Schwarz units for pairs with la + lb <= this: the (ab|ab)
+# tower of a (df) or (ff) pair keeps too much live (8.5 / 19 KB of scratch
+# per lane, above the library's 6 KB cap); those two classes run on the
+# loop-form kernel (csrc/fj_kloop.h).
 SCHWARZ_LSUM_MAX = 4
 
 
@@ -556,6 +556,9 @@ def units(lmax: int, k_lmax: int | None = None) -> dict[str, str]:
             out[f"schwarz_{la}{lb}.hip"] = emit_schwarz(la, lb)
         out[f"dq_{la}_{lb}.hip"] = emit_dq(la, lb)
         out[f"jab_{la}{lb}.hip"] = emit_jab(la, lb)
+    big = [c for c in pair_classes(lmax) if sum(c) > SCHWARZ_LSUM_MAX]
+    if big:
+        out["schwarz_loop_entries.hip"] = emit_schwarz_loop_entries(big)
     for lp in range(2 * lmax + 1):
         for lq in range(2 * lmax + 1):
             out[f"pq_L{lp + lq}_p{lp}_q{lq}.hip"] = emit_pq(lp, lq)
@@ -575,6 +578,31 @@ K_ARGS = ("int n_bra, int n_ket, int n_dc, double tau, double maxD, const double
           "    const int* dd, const double* Dcd, double* K")
 K_NAMES = ("n_bra, n_ket, n_dc, tau, maxD, P, zeta_ac, XPA, XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, "
            "s_bd, Iq, dd, Dcd, K")
+
+
+S_ARGS = ("int n_pair, int n_rec, int n_ao, const double* zeta_ab, const double* P,\n"
+          "    const double* XPA, const double* XPB, const double* s_ab, const int* rec0,\n"
+          "    const int* rec1, const int* off_a, const int* off_b, double* schwarz_bound")
+S_NAMES = "n_pair, n_rec, n_ao, zeta_ab, P, XPA, XPB, s_ab, rec0, rec1, off_a, off_b, schwarz_bound"
+
+
+def emit_schwarz_loop_entries(classes) -> str:
+    """The fj_schwarz_<la><lb> entries of the pair classes past the
+    straight-line Schwarz kernels: the same parameter list, launched
+    through the loop-form kernel (csrc/fj_kloop.h)."""
+    E = _Emit()
+    E("// generated by felizjueves_amd/gen_mmd.py -- Schwarz classes in loop form", 0)
+    E('#include "../fj_device.h"', 0)
+    E("", 0)
+    E("int fj_schwarz_loop_launch(hipStream_t s, int la, int lb, int threads,", 0)
+    E(S_ARGS.replace("\n    ", " ") + ");")
+    for la, lb in classes:
+        E("", 0)
+        E(f'extern "C" int fj_schwarz_{la}{lb}(hipStream_t s, int threads, '
+          + S_ARGS.replace("\n    ", " ") + ") {", 0)
+        E(f"return fj_schwarz_loop_launch(s, {la}, {lb}, threads, {S_NAMES});")
+        E("}", 0)
+    return E.text()
 
 
 def emit_k_loop_entries(classes) -> str:
@@ -611,8 +639,7 @@ def header_decls(lmax: int, k_lmax: int | None = None) -> str:
     k_lmax = lmax if k_lmax is None else k_lmax
     out = []
     for la, lb in pair_classes(lmax):
-        if la + lb <= SCHWARZ_LSUM_MAX:
-            out.append(
+        out.append(
             f"int fj_schwarz_{la}{lb}(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,\n"
             f"    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,\n"
             f"    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,\n"

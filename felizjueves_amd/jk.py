@@ -101,29 +101,34 @@ def _pair_classes(basis: Basis):
 # ---------------------------------------------------------------------------
 
 
-def schwarz_matrix(basis: Basis, threads: int = 64):
-    """Q[m, n] = sqrt(max((mn|mn), 0)) on the device (torch float64)."""
-    import torch
-    from .gen_mmd import SCHWARZ_LSUM_MAX
-    classes = _pair_classes(basis)
-    big = [c for c in classes if sum(c) > SCHWARZ_LSUM_MAX]
-    if big:
-        raise ValueError(f"pair classes {big}: the Schwarz kernels are built for la + lb <= "
-                         f"{SCHWARZ_LSUM_MAX}")
-    dev = native.device()
-    Q = torch.zeros((basis.n_ao, basis.n_ao), dtype=torch.float64, device=dev)
-    for (la, lb), prs in classes.items():
+def schwarz_tables(basis: Basis) -> dict:
+    """Per pair class (la, lb): the records of its shell pairs (numpy), the
+    record range of each pair and the AO offsets of its two shells (upstream
+    eri/schwarz.py:152-176 builds the same records)."""
+    out = {}
+    for (la, lb), prs in _pair_classes(basis).items():
         recs = [_records(basis, a, b) for a, b in prs]
         n_rec = np.cumsum([0] + [len(r[1]) for r in recs])
-        cat = lambda k: native.to_dev(np.concatenate([r[k] for r in recs]))  # noqa: E731
-        P, z, s, XA, XB = cat(0), cat(1), cat(2), cat(3), cat(4)
-        rec0 = native.to_dev(n_rec[:-1].astype(np.int32))
-        rec1 = native.to_dev(n_rec[1:].astype(np.int32))
-        oa = native.to_dev(np.array([basis.shell_ao[a] for a, _ in prs], dtype=np.int32))
-        ob = native.to_dev(np.array([basis.shell_ao[b] for _, b in prs], dtype=np.int32))
-        _call(f"fj_schwarz_{la}{lb}", C.c_int(threads), C.c_int(len(prs)), C.c_int(int(n_rec[-1])),
-              C.c_int(basis.n_ao), _c(z), _c(P), _c(XA), _c(XB), _c(s), _c(rec0), _c(rec1),
-              _c(oa), _c(ob), _c(Q))
+        cat = lambda k: np.concatenate([r[k] for r in recs])  # noqa: E731
+        out[(la, lb)] = dict(
+            n_pair=len(prs), n_rec=int(n_rec[-1]), P=cat(0), z=cat(1), s=cat(2), XA=cat(3),
+            XB=cat(4), rec0=n_rec[:-1].astype(np.int32), rec1=n_rec[1:].astype(np.int32),
+            oa=np.array([basis.shell_ao[a] for a, _ in prs], dtype=np.int32),
+            ob=np.array([basis.shell_ao[b] for _, b in prs], dtype=np.int32))
+    return out
+
+
+def schwarz_matrix(basis: Basis, threads: int = 64):
+    """Q[m, n] = sqrt(max((mn|mn), 0)) on the device (torch float64); pair
+    classes through (ff), the two past la + lb = 4 on the loop-form kernel."""
+    import torch
+    dev = native.device()
+    Q = torch.zeros((basis.n_ao, basis.n_ao), dtype=torch.float64, device=dev)
+    for (la, lb), t in schwarz_tables(basis).items():
+        d = {k: native.to_dev(v) for k, v in t.items() if isinstance(v, np.ndarray)}
+        _call(f"fj_schwarz_{la}{lb}", C.c_int(threads), C.c_int(t["n_pair"]), C.c_int(t["n_rec"]),
+              C.c_int(basis.n_ao), d["z"], d["P"], d["XA"], d["XB"], d["s"], d["rec0"], d["rec1"],
+              d["oa"], d["ob"], Q)
     return Q
 
 

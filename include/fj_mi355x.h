@@ -529,7 +529,8 @@ int fj_fxm_2222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide
 /* The MMD families, one entry per class, each with the upstream kernel's
  * parameter list after (stream, threads):
  *   fj_schwarz_<la><lb>  Q_mn = sqrt(max((mn|mn), 0)), one thread per shell
- *                        pair (upstream eri/kernel_schwarz.py:76-101)
+ *                        pair (upstream eri/kernel_schwarz.py:76-101); (df)
+ *                        and (ff) on the loop-form kernel of fj_schwarz_loop
  *   fj_dq_<lc>_<ld>      D_q and Dmax per ket record
  *                        (upstream eri/j_coulomb/kernel_dq.py:10-60)
  *   fj_pq_L<L>_p<lp>_q<lq>  J_p = sum_k sum_q [p|q] D_q, kets sorted by
@@ -541,6 +542,14 @@ int fj_fxm_2222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide
  *                        by Iq (upstream eri/k_exchange/kernel_k.py:1-110);
  *                        straight-line kernels through d, the classes with
  *                        an f shell on the loop-form kernel of fj_k_loop */
+/* The Schwarz kernel in loop form for any pair class (la lb) through (ff),
+ * the class as arguments, otherwise the fj_schwarz_<la><lb> parameter list
+ * (upstream eri/kernel_schwarz.py:249-412). One wavefront per shell pair
+ * whatever `threads` asks; the entries of (df) and (ff) call it. */
+int fj_schwarz_loop(fj_stream_t stream, int la, int lb, int threads, int n_pair, int n_rec, int n_ao,
+                    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+                    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+                    const int* off_b, double* schwarz_bound);
 /* The K kernel in loop form for any class (la lc | lb ld) through f, the
  * class as arguments, otherwise the fj_k_<la><lc><lb><ld> parameter list
  * (upstream eri/k_exchange/kernel_k.py:314, host k_quartet k.py:395). One
@@ -663,12 +672,20 @@ int fj_dq_2_2(fj_stream_t stream, int threads, int n_rec, int n_pair,
 int fj_jab_22(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
     const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
     const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_23(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
 int fj_dq_2_3(fj_stream_t stream, int threads, int n_rec, int n_pair,
     const double* eta, const double* XPA, const double* XPB, const double* s,
     const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);
 int fj_jab_23(fj_stream_t stream, int threads, int n_rec, int n_ao, int n_rows,
     const double* zeta_ab, const double* XPA, const double* XPB, const double* s,
     const int* off_a, const int* off_b, const int* row, const double* Jp, double* J);
+int fj_schwarz_33(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
+    const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
+    const double* s_ab, const int* rec0, const int* rec1, const int* off_a,
+    const int* off_b, double* schwarz_bound);
 int fj_dq_3_3(fj_stream_t stream, int threads, int n_rec, int n_pair,
     const double* eta, const double* XPA, const double* XPB, const double* s,
     const double* w, const int* pair, const double* Dcd, double* D, double* Dmax);

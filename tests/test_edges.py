@@ -116,6 +116,14 @@ def test_entries_refuse_bad_arguments_before_any_launch():
     rc = k(None, C.c_int(3), C.c_int(3), C.c_int(3), C.c_int(3), C.c_int(100), C.c_int(1),
            C.c_int(1), C.c_int(1), C.c_double(0.0), C.c_double(1.0), *nulls)
     assert rc == ERR_THREADS
+    # the loop-form Schwarz kernel: a class past f, then a bad thread count
+    q = lib.fj_schwarz_loop
+    q.restype = C.c_int
+    nulls = [None] * 10
+    rc = q(None, C.c_int(4), C.c_int(0), C.c_int(64), C.c_int(1), C.c_int(1), C.c_int(10), *nulls)
+    assert rc == ERR_ARG
+    rc = q(None, C.c_int(3), C.c_int(3), C.c_int(320), C.c_int(1), C.c_int(1), C.c_int(10), *nulls)
+    assert rc == ERR_THREADS
 
 
 def test_k_tables_cover_f_and_follow_the_launch_order():
@@ -133,8 +141,13 @@ def test_k_tables_cover_f_and_follow_the_launch_order():
         assert dc.max() + t["kets"][(lb, ld, bs)]["pos"].max() < t["dblk"][(lc, ld)]["n"]
 
 
-def test_schwarz_refuses_pairs_above_its_built_classes():
+def test_schwarz_tables_cover_f_pairs():
+    """The Schwarz host tables reach (ff) on CO/cc-pVTZ; each pair's record
+    range lies inside its class's records."""
     from felizjueves_amd import jk
     from felizjueves_amd.basis import load_fixture
-    with pytest.raises(ValueError, match="Schwarz"):
-        jk.schwarz_matrix(load_fixture("co_ccpvtz").basis)
+    t = jk.schwarz_tables(load_fixture("co_ccpvtz").basis)
+    assert (2, 3) in t and (3, 3) in t
+    for c in t.values():
+        assert c["rec0"][0] == 0 and c["rec1"][-1] == c["n_rec"] == len(c["z"])
+        assert (c["rec1"] > c["rec0"]).all() and len(c["oa"]) == c["n_pair"]

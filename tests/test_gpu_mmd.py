@@ -45,6 +45,38 @@ def test_schwarz_vs_oracle_config1():
     assert rel < 1e-12
 
 
+def test_schwarz_through_ff_vs_oracle():
+    """CO/cc-pVTZ: every pair class through (ff) -- (df) and (ff) on the
+    loop-form kernel -- against the oracle's Q."""
+    fx = load_fixture("co_ccpvtz")
+    Q = _host(jk.schwarz_matrix(fx.basis))
+    Qo = O.schwarz(fx.basis)
+    rel = np.abs(Q - Qo).max() / Qo.max()
+    print("co_ccpvtz schwarz max rel", rel, flush=True)
+    assert rel < 1e-12
+    assert np.array_equal(Q, Q.T)
+
+
+def test_schwarz_loop_form_matches_straight_line_kernels():
+    """H2O/cc-pVDZ: fj_schwarz_loop on every pair class gives the straight-line
+    kernels' Q to FP64 round-off."""
+    import ctypes as C
+    import torch
+    fx = load_fixture("h2o_ccpvdz")
+    b = fx.basis
+    Q1 = jk.schwarz_matrix(b)
+    Q2 = torch.zeros_like(Q1)
+    for (la, lb), t in jk.schwarz_tables(b).items():
+        d = {k: native.to_dev(v) for k, v in t.items() if isinstance(v, np.ndarray)}
+        jk._call("fj_schwarz_loop", C.c_int(la), C.c_int(lb), C.c_int(64), C.c_int(t["n_pair"]),
+                 C.c_int(t["n_rec"]), C.c_int(b.n_ao), d["z"], d["P"], d["XA"], d["XB"], d["s"],
+                 d["rec0"], d["rec1"], d["oa"], d["ob"], Q2)
+    a, c = _host(Q1), _host(Q2)
+    rel = np.abs(a - c).max() / a.max()
+    print("schwarz loop vs straight-line max rel", rel, flush=True)
+    assert rel < 1e-13
+
+
 @pytest.mark.parametrize("name", FIX_J)
 def test_j_chain_vs_libcint(name):
     fx = load_fixture(name)

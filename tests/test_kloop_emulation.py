@@ -1,13 +1,16 @@
-"""The loop-form K kernel (csrc/fj_kloop.h) runs unchanged on the CPU (CPU
-only): compiled with g++ against the emulation of the device header (one
-thread per lane, barriers for __syncthreads) under AddressSanitizer, it takes
-every K launch of a two-atom s,p,d,f basis -- the same host tables and launch
-order as `jk.ExchangeBuild` -- and, after the scatter, its K is compared with
-the oracle's K (every ordered shell quartet by McMurchie-Davidson in C).
+"""The loop-form MMD kernels (csrc/fj_kloop.h) run unchanged on the CPU
+(CPU only): compiled with g++ against the emulation of the device header
+(one thread per lane, barriers for __syncthreads) under AddressSanitizer.
 
-Every class through (ff|ff) runs through the loop form here, the d classes
-included, so the one kernel is checked over all 256 classes the library
-launches it for or could."""
+K takes every K launch of a two-atom s,p,d,f basis -- the same host tables
+and launch order as `jk.ExchangeBuild` -- and, after the scatter, its K is
+compared with the oracle's K (every ordered shell quartet by
+McMurchie-Davidson in C). Schwarz takes every pair class of the same basis
+(`jk.schwarz_tables`) and its Q is compared with the oracle's.
+
+Every class through (ff|ff) and (ff) runs through the loop forms here, the
+lower classes included, so each kernel is checked over all the classes the
+library launches it for or could."""
 
 import os
 import shutil
@@ -70,6 +73,53 @@ def _dump(path: Path, basis, D, tables, launches, tau, maxD) -> None:
             w(f, D.ravel()[blk["idx"]], np.float64)
 
 
+def _build(td: Path, driver: str) -> Path:
+    (td / "gen").mkdir(exist_ok=True)
+    shutil.copy(EMU / "fj_device_emu.h", td / "fj_device.h")
+    shutil.copy(CSRC / "gen" / "fj_boys_table.h", td / "gen" / "fj_boys_table.h")
+    exe = td / Path(driver).stem
+    cmd = ["g++", "-std=c++20", "-O1", "-g0", "-fsanitize=address", f"-I{ROOT / 'include'}",
+           f"-I{td}", f'-DKLOOP_HEADER="{CSRC / "fj_kloop.h"}"', str(EMU / driver),
+           "-o", str(exe), "-lpthread"]
+    r = subprocess.run(cmd, cwd=td, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def _run(exe: Path, td: Path, out: str) -> np.ndarray:
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([str(exe), str(td / "case.bin"), str(td / out)], cwd=td, env=env,
+                       capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return np.fromfile(td / out)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
+def test_loop_schwarz_kernel_on_cpu_vs_oracle():
+    basis, _ = _system()
+    tables = jk.schwarz_tables(basis)
+    assert (3, 3) in tables and (2, 3) in tables
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        exe = _build(td, "schwarz_loop_emu.cpp")
+        with open(td / "case.bin", "wb") as f:
+            np.array([len(tables), basis.n_ao], dtype=np.int32).tofile(f)
+            for (la, lb), t in tables.items():
+                np.array([la, lb, t["n_pair"], t["n_rec"]], dtype=np.int32).tofile(f)
+                for k, dt in (("z", np.float64), ("P", np.float64), ("XA", np.float64),
+                              ("XB", np.float64), ("s", np.float64), ("rec0", np.int32),
+                              ("rec1", np.int32), ("oa", np.int32), ("ob", np.int32)):
+                    x = np.ascontiguousarray(t[k], dtype=dt).ravel()
+                    np.array([x.size], dtype=np.int64).tofile(f)
+                    x.tofile(f)
+        Q = _run(exe, td, "Q.bin").reshape(basis.n_ao, basis.n_ao)
+    Qo = O.schwarz(basis)
+    err = np.abs(Q - Qo).max() / Qo.max()
+    print("loop-form Schwarz vs oracle, max rel", err, flush=True)
+    assert err < 1e-12
+    assert np.array_equal(Q, Q.T)
+
+
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
 def test_loop_k_kernel_on_cpu_vs_oracle():
     basis, D = _system()
@@ -78,21 +128,9 @@ def test_loop_k_kernel_on_cpu_vs_oracle():
     maxD = float(np.abs(D).max())
     with tempfile.TemporaryDirectory() as td:
         td = Path(td)
-        (td / "gen").mkdir()
-        shutil.copy(EMU / "fj_device_emu.h", td / "fj_device.h")
-        shutil.copy(CSRC / "gen" / "fj_boys_table.h", td / "gen" / "fj_boys_table.h")
-        exe = td / "kloop_emu"
-        cmd = ["g++", "-std=c++20", "-O1", "-g0", "-fsanitize=address", f"-I{ROOT / 'include'}",
-               f"-I{td}", f'-DKLOOP_HEADER="{CSRC / "fj_kloop.h"}"', str(EMU / "kloop_emu.cpp"),
-               "-o", str(exe), "-lpthread"]
-        r = subprocess.run(cmd, cwd=td, capture_output=True, text=True)
-        assert r.returncode == 0, r.stderr[-3000:]
+        exe = _build(td, "kloop_emu.cpp")
         _dump(td / "case.bin", basis, D, tables, launches, 0.0, maxD)
-        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-        r = subprocess.run([str(exe), str(td / "case.bin"), str(td / "K.bin")], cwd=td, env=env,
-                           capture_output=True, text=True, timeout=900)
-        assert r.returncode == 0, r.stderr[-3000:]
-        rows = np.fromfile(td / "K.bin")
+        rows = _run(exe, td, "K.bin")
     K = np.zeros((basis.n_ao, basis.n_ao))
     o = 0
     for la, lc, lb, ld, bs in launches:
