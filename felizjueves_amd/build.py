@@ -23,6 +23,7 @@ LIB = HERE / "libfj_mi355x.so"
 ARCH = "gfx950"
 LMAX = 2
 MMD_LMAX = 3     # Schwarz, DQ, [p|q] and JAB through f
+LOOP_LMAX = 3    # fused F classes past LMAX run on the loop-form kernel (fj_floop.h)
 MMD_K_LMAX = 2   # straight-line K kernels through d; the f classes in loop form (fj_kloop.h)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -37,7 +38,16 @@ FLAGS = [
 def sources(lmax: int = LMAX) -> list[Path]:
     classes = builder_classes(lmax)
     files = gen_hip.generate(classes)
-    gen_hip.write_header(classes)
+    # the classes with an f shell: entries onto the loop-form fused kernel
+    loop = [c for c in builder_classes(LOOP_LMAX) if max(c) > lmax]
+    if loop:
+        p = gen_hip.GEN / "f_loop_entries.hip"
+        text = gen_hip.emit_loop_entries(loop)
+        if not p.exists() or p.read_text() != text:
+            p.write_text(text)
+        files.append(p)
+    built = set(classes) | set(loop)
+    gen_hip.write_header(tuple(c for c in builder_classes(LOOP_LMAX) if c in built))
     mmd = gen_mmd.generate(MMD_LMAX, k_lmax=MMD_K_LMAX)
     gen_mmd.write_header(MMD_LMAX, MMD_K_LMAX)
     return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", CSRC / "fj_kloop.hip",

@@ -1,0 +1,233 @@
+// fj_floop.h -- the fused F kernel in loop form for the classes with an f
+// shell: the same work items, screen, degeneracy and six-block digestion as
+// the generated HGP kernels (upstream eri/fock/kernel_f.py:439-487,529-628),
+// with each primitive quartet's integrals by McMurchie-Davidson instead of
+// the HGP VRR/HRR:
+//
+//   (ab|cd) = U_ab U_cd sum_p E^{ab}_p sum_q (-1)^|q| E^{cd}_q [p + q]
+//
+// (upstream mmd/kernel.py:232-487 for the Hermite form; the records' U
+// carries exp(-mu |AB|^2), so the E tables start from 1).
+//
+// One wavefront per work item. The item's kets are taken one at a time; per
+// live ket, the weighted density blocks and the six F accumulators sit in
+// LDS, and per primitive quartet the lanes build the E tables and the
+// Hermite tower, then, for the bra rows ab in chunks of FJ_FL_CH,
+//   W[ab][q] = sum_p E^{ab}_p [p + q]           (lanes over (ab, q))
+//   x(ab,cd) = U U sum_q W[ab][q] (-1)^|q| E^{cd}_q   (lanes over (ab, cd))
+// and digest x at once (LDS atomics into the six blocks). After the ket's
+// primitives the blocks go to F with one global atomic per element.
+// Every barrier sits in control flow that is uniform over the wavefront.
+//
+// The includer provides fj_device.h and fj_kloop.h.
+#pragma once
+
+#define FJ_FL_CH 16   // bra rows per W chunk
+
+__global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
+    __shared__ double R[2][FJ_KL_NH];
+    __shared__ double eab[3 * FJ_KL_AX], ecd[3 * FJ_KL_AX];
+    __shared__ double W[FJ_FL_CH * FJ_KL_NP];
+    __shared__ double wab[FJ_KL_NC * FJ_KL_NC], wcd[FJ_KL_NC * FJ_KL_NC], wac[FJ_KL_NC * FJ_KL_NC],
+        wad[FJ_KL_NC * FJ_KL_NC], wbc[FJ_KL_NC * FJ_KL_NC], wbd[FJ_KL_NC * FJ_KL_NC];
+    __shared__ double fab[FJ_KL_NC * FJ_KL_NC], fcd[FJ_KL_NC * FJ_KL_NC], fac[FJ_KL_NC * FJ_KL_NC],
+        fad[FJ_KL_NC * FJ_KL_NC], fbc[FJ_KL_NC * FJ_KL_NC], fbd[FJ_KL_NC * FJ_KL_NC];
+    __shared__ double wf[FJ_KL_MMAX + 1];
+    __shared__ int hc[FJ_KL_NH];
+    __shared__ int tw[FJ_KL_NH];
+    __shared__ unsigned short pqi[FJ_KL_NP * FJ_KL_NP];
+
+    const int lane = (int)threadIdx.x;
+    // the work item: (bra pair, first ket, system | bra chunk, ket end), or
+    // (x, y * span, the single system, n_ket) without a work list
+    int bra, k0, k1, kb_lo = 0, kb_n = 0;
+    FjSys S;
+    if (a.work) {
+        const int4 w = a.work[blockIdx.x];
+        bra = w.x; k0 = w.y; k1 = w.w;
+        S = a.sys[w.z & 0xffff];
+        kb_lo = (w.z >> 16) & 0xff;
+        kb_n = (w.z >> 24) & 0xff;
+    } else {
+        bra = (int)blockIdx.x;
+        k0 = (int)blockIdx.y * span;
+        S = a.one;
+        k1 = a.ket.n;
+    }
+    k1 = min(k1, k0 + span);   // an item covers at most `span` kets
+    const int kmin = a.diag * bra;
+    if (k0 >= k1 || (!a.ket_list && k0 + span <= kmin)) return;   // uniform
+    int kb0, kab;
+    fj_side_range(a.bra, bra, kb0, kab);
+    if (kab > FJ_MAX_PRIM_PAIRS || kb_lo >= kab) return;
+    kb0 += kb_lo;
+    kab = kb_n ? min(kab - kb_lo, kb_n) : kab - kb_lo;
+
+    const int lp = la + lb, lq = lc + ld, L = lp + lq;
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+    const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
+    const int nab = na * nb, ncd = nc * nd;
+    const int np = fj_herm_base(lp + 1), nq = fj_herm_base(lq + 1), nh = fj_herm_base(L + 1);
+    const int ha = fj_herm_base(la), hb = fj_herm_base(lb), hcc = fj_herm_base(lc),
+              hd = fj_herm_base(ld);
+    const int ta = lp + 1, tc = lq + 1;   // t extents of the two E tables
+
+    fj_kl_steps(tw, hc, nh, lane);
+    for (int j = lane; j < np * nq; j += 64) {
+        const int p = j / nq, q = j - p * nq;
+        const int cp = fj_herm_decode(p), cq = fj_herm_decode(q);
+        pqi[j] = (unsigned short)fj_herm_index((cp & 15) + (cq & 15),
+                                               ((cp >> 4) & 15) + ((cq >> 4) & 15),
+                                               (cp >> 8) + (cq >> 8));
+    }
+    const int sa = a.bra.sh[2 * bra], sb = a.bra.sh[2 * bra + 1];
+    const int oa = a.bra.ao[2 * bra], ob = a.bra.ao[2 * bra + 1];
+    const double Gb = a.bra.G[bra];
+    const double ABx = lb ? a.bra.sep[3 * bra] : 0.0, ABy = lb ? a.bra.sep[3 * bra + 1] : 0.0,
+                 ABz = lb ? a.bra.sep[3 * bra + 2] : 0.0;
+    const int n_ao = S.n_ao, ns = S.n_sh;
+    const double* __restrict__ D = S.D;
+    __syncthreads();
+
+    for (int kk = k0; kk < k1; ++kk) {
+        int k = kk, kc_lo = 0, kc_n = 0;
+        if (a.ket_list) {
+            const int v = a.ket_list[kk];
+            k = v & 0x1ffff;
+            kc_lo = (v >> 17) & 0x7f;
+            kc_n = (v >> 24) & 0x7f;
+        }
+        if (k < kmin) continue;   // uniform: every lane reads the same ket
+        const int sc = a.ket.sh[2 * k], sd = a.ket.sh[2 * k + 1];
+        const double* __restrict__ Dsh = S.Dsh;
+        double dmax = 2.0 * Dsh[sa * ns + sb];
+        dmax = fmax(dmax, 2.0 * Dsh[sc * ns + sd]);
+        dmax = fmax(dmax, 0.5 * Dsh[sa * ns + sc]);
+        dmax = fmax(dmax, 0.5 * Dsh[sa * ns + sd]);
+        dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sc]);
+        dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sd]);
+        if (Gb * (a.ket.G[k] * dmax) < a.tau) continue;
+        int kc0, kcd;
+        fj_side_range(a.ket, k, kc0, kcd);
+        if (kcd > FJ_MAX_PRIM_PAIRS || kcd <= kc_lo) continue;
+        kc0 += kc_lo;
+        kcd = kc_n ? min(kcd - kc_lo, kc_n) : kcd - kc_lo;
+        double deg = 1.0;
+        if (sa == sb) deg *= 0.5;
+        if (sc == sd) deg *= 0.5;
+        if (a.diag && k == bra) deg *= 0.5;
+        const double s_j = 2.0 * deg, s_k = -0.5 * deg;
+        const int oc = a.ket.ao[2 * k], od = a.ket.ao[2 * k + 1];
+        const double CDx = ld ? a.ket.sep[3 * k] : 0.0, CDy = ld ? a.ket.sep[3 * k + 1] : 0.0,
+                     CDz = ld ? a.ket.sep[3 * k + 2] : 0.0;
+        for (int j = lane; j < nab; j += 64) {
+            wab[j] = s_j * D[(size_t)(oa + j / nb) * n_ao + ob + j % nb];
+            fab[j] = 0.0;
+        }
+        for (int j = lane; j < ncd; j += 64) {
+            wcd[j] = s_j * D[(size_t)(oc + j / nd) * n_ao + od + j % nd];
+            fcd[j] = 0.0;
+        }
+        for (int j = lane; j < na * nc; j += 64) {
+            wac[j] = s_k * D[(size_t)(oa + j / nc) * n_ao + oc + j % nc];
+            fac[j] = 0.0;
+        }
+        for (int j = lane; j < na * nd; j += 64) {
+            wad[j] = s_k * D[(size_t)(oa + j / nd) * n_ao + od + j % nd];
+            fad[j] = 0.0;
+        }
+        for (int j = lane; j < nb * nc; j += 64) {
+            wbc[j] = s_k * D[(size_t)(ob + j / nc) * n_ao + oc + j % nc];
+            fbc[j] = 0.0;
+        }
+        for (int j = lane; j < nb * nd; j += 64) {
+            wbd[j] = s_k * D[(size_t)(ob + j / nd) * n_ao + od + j % nd];
+            fbd[j] = 0.0;
+        }
+        for (int ib = kb0; ib < kb0 + kab; ++ib) {
+            const double* __restrict__ rb = a.bra.prim + 8 * (size_t)ib;
+            for (int ik = kc0; ik < kc0 + kcd; ++ik) {
+                const double* __restrict__ rk = a.ket.prim + 8 * (size_t)ik;
+                const double p = rb[6], q = rk[6];
+                if (lane < 3) {
+                    const double pa = rb[3 + lane], ab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+                    fj_kl_eaxis_from(eab + lane * FJ_KL_AX, la, lb, p, pa, pa + ab, 1.0);
+                } else if (lane < 6) {
+                    const int x = lane - 3;
+                    const double qc = rk[3 + x], cd = x == 0 ? CDx : (x == 1 ? CDy : CDz);
+                    fj_kl_eaxis_from(ecd + x * FJ_KL_AX, lc, ld, q, qc, qc + cd, 1.0);
+                }
+                const double PQx = rb[0] - rk[0], PQy = rb[1] - rk[1], PQz = rb[2] - rk[2];
+                const double rho = p * q / (p + q);
+                if (lane == 0) {
+                    fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz));
+                    R[L & 1][0] = wf[L];
+                }
+                __syncthreads();
+                fj_kl_tower(R, wf, tw, L, PQx, PQy, PQz, lane);
+                const double* __restrict__ R0 = R[0];
+                const double UU = rb[7] * rk[7];
+                for (int r0 = 0; r0 < nab; r0 += FJ_FL_CH) {
+                    const int nr = min(FJ_FL_CH, nab - r0);
+                    for (int j = lane; j < nr * nq; j += 64) {
+                        const int r = j / nq, qi = j - r * nq;
+                        const int ab = r0 + r;
+                        const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
+                        const double* __restrict__ ex = eab + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
+                        const double* __restrict__ ey =
+                            eab + FJ_KL_AX + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
+                        const double* __restrict__ ez =
+                            eab + 2 * FJ_KL_AX + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
+                        double s = 0.0;
+                        for (int pi = 0; pi < np; ++pi) {
+                            const int cp = hc[pi];
+                            s += (ex[cp & 15] * ey[(cp >> 4) & 15] * ez[cp >> 8]) * R0[pqi[pi * nq + qi]];
+                        }
+                        W[r * FJ_KL_NP + qi] = s;
+                    }
+                    __syncthreads();
+                    for (int j = lane; j < nr * ncd; j += 64) {
+                        const int r = j / ncd, cdi = j - r * ncd;
+                        const int ab = r0 + r;
+                        const int ia = ab / nb, jb = ab % nb, kc = cdi / nd, ld_ = cdi % nd;
+                        const int cc = hc[hcc + kc], cdd = hc[hd + ld_];
+                        const double* __restrict__ fx = ecd + ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
+                        const double* __restrict__ fy =
+                            ecd + FJ_KL_AX + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
+                        const double* __restrict__ fz =
+                            ecd + 2 * FJ_KL_AX + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
+                        const double* __restrict__ wr = W + r * FJ_KL_NP;
+                        double s = 0.0;
+                        for (int qi = 0; qi < nq; ++qi) {
+                            const int cq = hc[qi];
+                            const double e = fx[cq & 15] * fy[(cq >> 4) & 15] * fz[cq >> 8];
+                            s += wr[qi] * (((cq & 15) + ((cq >> 4) & 15) + (cq >> 8)) & 1 ? -e : e);
+                        }
+                        const double x = UU * s;
+                        atomicAdd(&fab[ab], wcd[cdi] * x);
+                        atomicAdd(&fcd[cdi], wab[ab] * x);
+                        atomicAdd(&fac[ia * nc + kc], wbd[jb * nd + ld_] * x);
+                        atomicAdd(&fad[ia * nd + ld_], wbc[jb * nc + kc] * x);
+                        atomicAdd(&fbc[jb * nc + kc], wad[ia * nd + ld_] * x);
+                        atomicAdd(&fbd[jb * nd + ld_], wac[ia * nc + kc] * x);
+                    }
+                    __syncthreads();   // W and the E tables are rewritten next
+                }
+            }
+        }
+        // the ket's six blocks into F (the caller closes F + F^T)
+        for (int j = lane; j < nab; j += 64)
+            if (fab[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nb) * n_ao + ob + j % nb, fab[j]);
+        for (int j = lane; j < ncd; j += 64)
+            if (fcd[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oc + j / nd) * n_ao + od + j % nd, fcd[j]);
+        for (int j = lane; j < na * nc; j += 64)
+            if (fac[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nc) * n_ao + oc + j % nc, fac[j]);
+        for (int j = lane; j < na * nd; j += 64)
+            if (fad[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nd) * n_ao + od + j % nd, fad[j]);
+        for (int j = lane; j < nb * nc; j += 64)
+            if (fbc[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(ob + j / nc) * n_ao + oc + j % nc, fbc[j]);
+        for (int j = lane; j < nb * nd; j += 64)
+            if (fbd[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(ob + j / nd) * n_ao + od + j % nd, fbd[j]);
+        __syncthreads();   // before the next ket rewrites the blocks
+    }
+}

@@ -57,12 +57,14 @@ __device__ __forceinline__ int fj_herm_decode(int j) {
 // zero where t > i + j:  E^{00}_0 = exp(zeta X_PA X_PB),
 // E^{i+1,j}_t = E^{ij}_{t-1} / (2 zeta) + X_PA E^{ij}_t + (t + 1) E^{ij}_{t+1},
 // the j raise with X_PB.
-__device__ __forceinline__ void fj_kl_eaxis(double* e, int l1, int l2, double zeta, double pa,
-                                            double pb) {
+// The seed E^{00}_0 is e00 (fj_kl_eaxis: exp(zeta X_PA X_PB); the F records
+// carry that factor in U, so their tables start from 1).
+__device__ __forceinline__ void fj_kl_eaxis_from(double* e, int l1, int l2, double zeta, double pa,
+                                                 double pb, double e00) {
     const int nj = l2 + 1, nt = l1 + l2 + 1;
     for (int k = 0; k < (l1 + 1) * nj * nt; ++k) e[k] = 0.0;
     const double h = 0.5 / zeta;
-    e[0] = exp(zeta * pa * pb);
+    e[0] = e00;
     for (int i = 0; i <= l1; ++i) {
         for (int j = (i == 0 ? 1 : 0); j <= l2; ++j) {
             // (i, 0) from (i - 1, 0); (i, j > 0) from (i, j - 1)
@@ -79,6 +81,11 @@ __device__ __forceinline__ void fj_kl_eaxis(double* e, int l1, int l2, double ze
             }
         }
     }
+}
+
+__device__ __forceinline__ void fj_kl_eaxis(double* e, int l1, int l2, double zeta, double pa,
+                                            double pb) {
+    fj_kl_eaxis_from(e, l1, l2, zeta, pa, pb, exp(zeta * pa * pb));
 }
 
 // The tower's step table for every Hermite index j < nh: for t != 0 with

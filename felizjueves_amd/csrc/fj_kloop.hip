@@ -1,9 +1,11 @@
 // fj_kloop.hip -- launchers of the loop-form MMD kernels (fj_kloop.h): K
 // (the entries fj_k_<la><lc><lb><ld> of the classes with an f shell and the
 // generic fj_k_loop) and Schwarz (fj_schwarz_<la><lb> past la + lb = 4 and
-// the generic fj_schwarz_loop).
+// the generic fj_schwarz_loop) and the fused F kernel of the classes with
+// an f shell (their fj_f_ / fj_fx_ / fj_fxm_ entries, generated).
 #include "fj_device.h"
 #include "fj_kloop.h"
+#include "fj_floop.h"
 
 int fj_kloop_launch(hipStream_t s, int la, int lc, int lb, int ld, int threads, int n_bra, int n_ket,
                     int n_dc, double tau, double maxD, const double* P, const double* zeta_ac,
@@ -60,4 +62,20 @@ extern "C" int fj_schwarz_loop(hipStream_t s, int la, int lb, int threads, int n
                                double* schwarz_bound) {
     return fj_schwarz_loop_launch(s, la, lb, threads, n_pair, n_rec, n_ao, zeta_ab, P, XPA, XPB, s_ab,
                                   rec0, rec1, off_a, off_b, schwarz_bound);
+}
+
+int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, FjFArgs a, int n_blocks) {
+    if (la < 0 || la > FJ_KL_LMAX || lb < 0 || lb > FJ_KL_LMAX || lc < 0 || lc > FJ_KL_LMAX ||
+        ld < 0 || ld > FJ_KL_LMAX)
+        return FJ_ERR_ARG;
+    // threads is the item width (kets per work item) the plan was made for;
+    // the kernel takes an item's kets one at a time on one wavefront
+    if (threads <= 0) threads = FJ_DEFAULT_THREADS;
+    if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
+    if (n_blocks <= 0 || a.ket.n <= 0) return 0;
+    const dim3 grid((unsigned)n_blocks, a.work ? 1u : (unsigned)((a.ket.n + threads - 1) / threads));
+    a.n_bx = (int)grid.x;
+    a.n_items = (int)(grid.x * grid.y);
+    hipLaunchKernelGGL(fj_floop_k, grid, dim3(64), 0, s, la, lb, lc, ld, threads, a);
+    return (int)hipGetLastError();
 }

@@ -56,7 +56,7 @@ def fp32_amplification(basis, p) -> float:
     d = float(np.linalg.norm(basis.shell_centre[p.a] - basis.shell_centre[p.b]))
     z = p.rec[:, 6]
     return (1.0 + d) ** (p.la + p.lb) * math.sqrt(z.max() / z.min())
-FOCK_LMAX = 2  # the top class built: (dd|dd)
+FOCK_LMAX = 3  # through (ff|ff): HGP kernels through (dd|dd), the f classes in loop form
 
 # The primitive-loop split (see FockBuilder._lane_flops): a lane's share of
 # a quartet's primitive loop is at most LANE_FLOPS_MAX algorithmic FLOPs, and

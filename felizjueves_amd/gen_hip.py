@@ -647,6 +647,69 @@ def header_decls(classes) -> str:
     return "\n".join(out) + "\n"
 
 
+def emit_loop_entries(classes) -> str:
+    """fj_f_ (canonical classes), fj_fx_ and fj_fxm_ of the classes run by the
+    loop-form fused kernel (csrc/fj_floop.h): the generated kernels' entry
+    checks and argument records, one FP64 kernel for every tier."""
+    L = ["// generated by felizjueves_amd/gen_hip.py -- F classes on the loop-form kernel",
+         '#include "../fj_device.h"', "",
+         "int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, FjFArgs a,",
+         "                    int n_blocks);", ""]
+    for cls in classes:
+        la, lb, lc, ld = cls
+        n = cls_name(cls)
+        call = f"fj_floop_launch(s, {la}, {lb}, {lc}, {ld}, threads, a"
+        if canonical(cls):
+            L += [
+                f'extern "C" int fj_f_{n}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,',
+                "                        int n_sh, int n_ao, int diag, double tau,",
+                f"                        const double* bra_prim,{' const double* AB,' if lb else ''}"
+                " const double* G_ab, const int* bra_sh, const int* bra_ao,",
+                f"                        const double* ket_prim,{' const double* CD,' if ld else ''}"
+                " const double* G_cd, const int* ket_sh, const int* ket_ao,",
+                "                        const double* Dsh, const double* D, double* F) {",
+                "    if (k_ab < 1 || k_ab > FJ_MAX_PRIM_PAIRS || k_cd < 1 || k_cd > FJ_MAX_PRIM_PAIRS) return FJ_ERR_DEGREE;",
+                "    if (n_bra < 0 || n_ket < 0 || (diag && n_bra != n_ket)) return FJ_ERR_SHAPE;",
+                "    FjFArgs a;",
+                f"    a.bra = FjSide{{bra_prim, nullptr, k_ab, {'AB' if lb else 'nullptr'}, G_ab, bra_sh, bra_ao, n_bra}};",
+                f"    a.ket = FjSide{{ket_prim, nullptr, k_cd, {'CD' if ld else 'nullptr'}, G_cd, ket_sh, ket_ao, n_ket}};",
+                "    a.work = nullptr; a.ket_list = nullptr; a.sys = nullptr;",
+                "    a.one = FjSys{D, Dsh, F, n_ao, n_sh};",
+                "    a.diag = diag; a.tau = tau;",
+                "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0;",
+                f"    return {call}, n_bra);", "}", ""]
+        L += [
+            f'extern "C" int fj_fx_{n}(hipStream_t s, int threads, int precision,',
+            "                         const FjSide* bra, const FjSide* ket, const int* work, int n_work,",
+            "                         const int* ket_list, const FjSys* sys, int diag, double tau,",
+            "                         double* ws, long long ws_bytes) {",
+            "    if (!bra || !ket || !work || !sys || ws_bytes < 0) return FJ_ERR_ARG;",
+            "    if (precision != 0 && precision != 1) return FJ_ERR_ARG;",
+            "    FjFArgs a;",
+            "    a.bra = *bra; a.ket = *ket;",
+            "    a.work = reinterpret_cast<const int4*>(work);",
+            "    a.ket_list = ket_list; a.sys = sys;",
+            "    a.one = FjSys{nullptr, nullptr, nullptr, 0, 0};",
+            "    a.diag = diag; a.tau = tau;",
+            "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0;",
+            "    // the loop-form kernel computes in FP64 at either precision",
+            f"    return {call}, n_work);", "}", "",
+            f'extern "C" int fj_fxm_{n}(hipStream_t s, int threads, const FjSide* bra, const FjSide* ket,',
+            "                          const int* work, int n_work, int n_fp64, const int* ket_list,",
+            "                          const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes) {",
+            "    if (!bra || !ket || !work || !sys || ws_bytes < 0) return FJ_ERR_ARG;",
+            "    if (n_fp64 < 0 || n_fp64 > n_work) return FJ_ERR_ARG;",
+            "    FjFArgs a;",
+            "    a.bra = *bra; a.ket = *ket;",
+            "    a.work = reinterpret_cast<const int4*>(work);",
+            "    a.ket_list = ket_list; a.sys = sys;",
+            "    a.one = FjSys{nullptr, nullptr, nullptr, 0, 0};",
+            "    a.diag = diag; a.tau = tau;",
+            "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = n_fp64;",
+            f"    return {call}, n_work);", "}", ""]
+    return "\n".join(L)
+
+
 HEADER = HERE.parent / "include" / "fj_mi355x.h"
 BEGIN, END = ("/* BEGIN F CLASSES (written by felizjueves_amd/gen_hip.py) */",
               "/* END F CLASSES */")

@@ -182,6 +182,17 @@ int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0002(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0003(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0003(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0003(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0010(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -210,6 +221,17 @@ int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0012(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0013(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0013(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0013(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -233,6 +255,46 @@ int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0023(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0023(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0023(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0033(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -253,6 +315,17 @@ int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0102(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0103(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0103(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0103(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0110(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
@@ -283,6 +356,17 @@ int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0113(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0113(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0120(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -306,6 +390,46 @@ int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0123(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0130(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0130(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -315,6 +439,17 @@ int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0202(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0203(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0203(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0203(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0210(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
@@ -345,6 +480,17 @@ int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0213(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0213(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0220(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -368,6 +514,159 @@ int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_0222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0223(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0230(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0230(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0231(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0231(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0303(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0303(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0303(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0310(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0310(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0311(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0311(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0311(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0312(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0312(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0312(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0313(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0313(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0320(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0320(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0321(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0321(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0322(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0322(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0323(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0330(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0330(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0331(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0331(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_0332(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0332(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_0333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_0333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_0333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1010(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -386,6 +685,12 @@ int fj_fx_1012(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_1012(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1013(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1013(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -402,6 +707,36 @@ int fj_fx_1022(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1023(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1023(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
@@ -424,6 +759,17 @@ int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1113(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1113(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1120(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
@@ -449,6 +795,46 @@ int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_1122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1123(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1130(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1130(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -458,6 +844,17 @@ int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1213(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1213(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1220(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
@@ -483,6 +880,120 @@ int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_1222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1223(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1230(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1230(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1231(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1231(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1313(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1313(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1320(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1320(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1321(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1321(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1322(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1322(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1323(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1330(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1330(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1331(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1331(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_1332(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1332(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_1333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_1333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_1333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -501,6 +1012,36 @@ int fj_fx_2022(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_2022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2023(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2023(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2121(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
@@ -513,6 +1054,36 @@ int fj_fx_2122(fj_stream_t stream, int threads, int precision, const FjSide* bra
 int fj_fxm_2122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2123(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2130(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2130(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -522,6 +1093,151 @@ int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_2223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2223(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2223(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2230(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2230(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2231(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2231(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_2233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_2323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2323(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2323(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2330(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2330(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2331(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2331(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_2332(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2332(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_2333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_2333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_2333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_fx_3233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
+    int diag, double tau, double* ws, long long ws_bytes);
+int fj_f_3333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int diag, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* F);
+int fj_fx_3333(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
+    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
+    double tau, double* ws, long long ws_bytes);
+int fj_fxm_3333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
     int diag, double tau, double* ws, long long ws_bytes);
 /* END F CLASSES */

@@ -53,6 +53,7 @@ inline double __shfl_xor(double v, int o, int w) {
     return r;
 }
 inline void unsafeAtomicAdd(double* p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
+inline double atomicAdd(double* p, double v) { return std::atomic_ref<double>(*p).fetch_add(v); }
 inline void fj_side_range(const FjSide& s, int i, int& k0, int& k) {
     if (s.off) { k0 = s.off[i]; k = s.off[i + 1] - k0; } else { k0 = i * s.k; k = s.k; }
 }

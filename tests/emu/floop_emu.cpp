@@ -1,0 +1,79 @@
+// Emulate the loop-form fused F kernel (felizjueves_amd/csrc/fj_floop.h) on
+// the CPU over walker launches (upstream ABI: no work list, workgroup
+// (bra, ket block)) read from a binary file; writes F after every launch.
+// 64 threads, one per lane, one workgroup at a time. TEST INFRASTRUCTURE.
+#include "fj_device.h"   // the emulation header, copied under this name
+#include KLOOP_HEADER
+#include FLOOP_HEADER
+#include <fstream>
+#include <thread>
+#include <vector>
+thread_local uint3e threadIdx;
+uint3e blockIdx, blockDim, gridDim;
+std::barrier<>* g_bar;
+double g_shfl[256];
+
+template <typename T> T* rd(std::ifstream& f) {
+    long long n;
+    f.read((char*)&n, 8);
+    T* p = (T*)malloc(n * sizeof(T) + 1);   // exactly sized, so ASan sees an overrun
+    f.read((char*)p, n * sizeof(T));
+    return p;
+}
+
+int main(int argc, char** argv) {
+    std::ifstream f(argv[1], std::ios::binary);
+    int hdr[2];   // n_launch n_ao
+    f.read((char*)hdr, sizeof hdr);
+    const int n_ao = hdr[1], span = 64;
+    struct L { int c[4]; FjFArgs a; };
+    std::vector<L> Ls(hdr[0]);
+    double* F = (double*)calloc((size_t)n_ao * n_ao, 8);
+    for (auto& l : Ls) {
+        int h[11];   // la lb lc ld n_bra n_ket k_ab k_cd n_sh n_ao diag
+        f.read((char*)h, sizeof h);
+        double tau;
+        f.read((char*)&tau, 8);
+        for (int i = 0; i < 4; ++i) l.c[i] = h[i];
+        double* bp = rd<double>(f); double* AB = h[1] ? rd<double>(f) : nullptr;
+        double* Gab = rd<double>(f); int* bsh = rd<int>(f); int* bao = rd<int>(f);
+        double* kp = rd<double>(f); double* CD = h[3] ? rd<double>(f) : nullptr;
+        double* Gcd = rd<double>(f); int* ksh = rd<int>(f); int* kao = rd<int>(f);
+        double* Dsh = rd<double>(f); double* D = rd<double>(f);
+        FjFArgs& a = l.a;
+        a.bra = FjSide{bp, nullptr, h[6], AB, Gab, bsh, bao, h[4]};
+        a.ket = FjSide{kp, nullptr, h[7], CD, Gcd, ksh, kao, h[5]};
+        a.work = nullptr; a.ket_list = nullptr; a.sys = nullptr;
+        a.one = FjSys{D, Dsh, F, n_ao, h[8]};
+        a.diag = h[10]; a.tau = tau;
+        a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0;
+        a.n_bx = h[4]; a.n_items = h[4] * ((h[5] + span - 1) / span);
+    }
+    const int threads = 64;
+    std::barrier<> bar(threads);
+    g_bar = &bar;
+    blockDim = {(unsigned)threads, 1, 1};
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+        ts.emplace_back([&, t] {
+            threadIdx = {(unsigned)t, 0, 0};
+            for (auto& l : Ls) {
+                const unsigned gx = l.a.bra.n, gy = (l.a.ket.n + span - 1) / span;
+                for (unsigned y = 0; y < gy; ++y)
+                    for (unsigned x = 0; x < gx; ++x) {
+                        if (t == 0) {
+                            blockIdx = {x, y, 0};
+                            gridDim = {gx, gy, 1};
+                        }
+                        bar.arrive_and_wait();
+                        fj_floop_k(l.c[0], l.c[1], l.c[2], l.c[3], span, l.a);
+                        bar.arrive_and_wait();
+                    }
+            }
+        });
+    for (auto& t : ts) t.join();
+    std::ofstream o(argv[2], std::ios::binary);
+    o.write((char*)F, 8LL * n_ao * n_ao);
+    fprintf(stderr, "done\n");
+    return 0;
+}

@@ -27,12 +27,13 @@ def test_header_declares_entries():
     for cls in fock_classes(2):
         n = "".join(map(str, cls))
         assert f"fj_f_{n}" in names and f"fj_fx_{n}" in names
-    # the builder's classes (both pair orders) have the builder entries;
-    # the upstream-ABI entry exists for the canonical classes only
-    for cls in builder_classes(2):
+    # the builder's classes (both pair orders, through (ff|ff): the f classes
+    # on the loop-form kernel) have the builder entries; the upstream-ABI
+    # entry exists for the canonical classes only
+    for cls in builder_classes(3):
         n = "".join(map(str, cls))
         assert f"fj_fx_{n}" in names and f"fj_fxm_{n}" in names
-        assert (f"fj_f_{n}" in names) == (cls in fock_classes(2))
+        assert (f"fj_f_{n}" in names) == (cls in fock_classes(3))
 
 
 def test_header_matches_generator():
@@ -53,7 +54,7 @@ def test_upstream_parameter_lists():
     """fj_f_<cls> takes AB only when lb > 0 and CD only when ld > 0, in the
     upstream order (eri/fock/levers/seams.py:228-235)."""
     text = HEADER.read_text()
-    for cls in fock_classes(2):
+    for cls in fock_classes(3):
         n = "".join(map(str, cls))
         m = re.search(rf"int fj_f_{n}\((.*?)\);", text, flags=re.S)
         params = [p.strip().split()[-1].lstrip("*") for p in m.group(1).split(",")]

@@ -35,11 +35,20 @@ def test_walker_stream_refuses_no_magnitude_bins():
         P.f_launches(s.basis, s.D, tau=1e-12, n=0)
 
 
-def test_builder_refuses_shells_above_the_built_classes():
+def test_builder_plans_f_classes_onto_built_entries():
+    """An f shell is planned (no refusal), and every class a launch names has
+    its fj_fx_ and fj_fxm_ entries in the header (the f classes on the
+    loop-form kernel)."""
+    from pathlib import Path
     b = Basis([Shell(3, (0.0, 0.0, 0.0), (1.0,), (1.0,)), Shell(0, (0.0, 0.0, 1.0), (1.0,), (1.0,))])
     D = np.eye(b.n_ao) * 0.1
-    with pytest.raises(ValueError, match="built through"):
-        FockBuilder(b, D)
+    fb = FockBuilder(b, D, tau=1e-14)
+    classes = {tuple(L.cls) for L in fb.launches}
+    assert any(3 in c for c in classes)
+    hdr = (Path(__file__).resolve().parent.parent / "include" / "fj_mi355x.h").read_text()
+    for c in classes:
+        n = "".join(map(str, c))
+        assert f"int fj_fx_{n}(" in hdr and f"int fj_fxm_{n}(" in hdr, c
 
 
 def test_builder_refuses_more_primitive_pairs_than_the_staging_holds():

@@ -19,11 +19,12 @@ from test_builder_oracle import oracle_build, oracle_build_mmd
 
 pytestmark = pytest.mark.gpu
 
-F_OFF = ("f classes are switched off on the GPU: the first launch of the (ss|ff) workspace kernel "
-         "(upstream-ABI walker, two-atom s,s,p,d,f system) ended in a memory aperture violation; the "
-         "same generated code emulated on the CPU under AddressSanitizer (tests/test_kernel_emulation.py) "
-         "stays in bounds and matches the oracle on those inputs, so the cause is not shown and the "
-         "library is built through (dd|dd) only")
+F_OFF = ("the HGP f-class kernels are switched off on the GPU: the first launch of the (ss|ff) "
+         "workspace kernel (upstream-ABI walker, two-atom s,s,p,d,f system) ended in a memory aperture "
+         "violation; the same generated code emulated on the CPU under AddressSanitizer "
+         "(tests/test_kernel_emulation.py) stays in bounds and matches the oracle on those inputs, so "
+         "the cause is not shown and the library carries HGP kernels through (dd|dd) only. The f "
+         "classes now run on the loop-form kernel instead, tested by the *_loop_f tests")
 
 
 def _print(*a):
@@ -48,8 +49,12 @@ def test_boys_device_vs_oracle():
         assert rel.max() < 2e-14, (M, T[np.argmax(rel.max(axis=1))])
 
 
-def _launch_errors(system_basis, D, tau, n):
-    """Every launch of the upstream stream, device vs oracle, per class."""
+def _launch_errors(system_basis, D, tau, n, mmd_for_f: bool = False):
+    """Every launch of the upstream stream, device vs oracle, per class. With
+    mmd_for_f the classes with an f shell (the loop-form kernel, integrals by
+    McMurchie-Davidson) are checked against the oracle's MMD witness of the
+    same launch (same screen and digestion): the HGP witness's contracted
+    HRR loses digits on tight-diffuse f pairs."""
     plan = P.f_launches(system_basis, D, tau=tau, n=n)
     Dsh = P.density_blocks(system_basis, D)
     gpu = GpuRunner()
@@ -57,7 +62,12 @@ def _launch_errors(system_basis, D, tau, n):
     for L in plan.launches:
         args = P.launch_args(system_basis, plan, L, D, Dsh)
         Fo = np.zeros((system_basis.n_ao,) * 2)
-        O.f_launch(L.cls, args, Fo)
+        if mmd_for_f and 3 in L.cls:
+            side = lambda g, sh, ao: {"G": args[g], "sh": args[sh], "ao": args[ao]}  # noqa: E731
+            O.fx_launch_mmd(L.cls, side("G_ab", "bra_sh", "bra_ao"), side("G_cd", "ket_sh", "ket_ao"),
+                            args["diag"], system_basis, args["tau"], Dsh, D, Fo)
+        else:
+            O.f_launch(L.cls, args, Fo)
         Fg = gpu.zeros(system_basis.n_ao)
         gpu(L.cls, args["n_bra"], (64, 1), gpu.put(args), Fg)
         err = np.abs(gpu.host(Fg) - Fo).max()
@@ -286,6 +296,59 @@ def test_builder_co_ccpvtz_vs_libcint():
     _print("co_ccpvtz gpu-libcint", np.abs(G - ref).max(), "gpu-oracle", np.abs(G - Go).max())
     assert np.abs(G - Go).max() < 1e-10
     assert np.abs(G - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+
+
+def test_every_class_through_f_launches_loop_f():
+    """Every walker launch of the two-atom s,s,p,d,f system through the
+    upstream-ABI entries, the f classes on the loop-form kernel: each
+    launch's F against the oracle's witness (MMD for the f classes)."""
+    basis, D = _two_atom_spdf()
+    worst = _launch_errors(basis, D, 0.0, 1, mmd_for_f=True)
+    _print("per-class max |dF| (f, loop form):", {k: f"{v:.1e}" for k, v in sorted(worst.items()) if 3 in k})
+    assert len(worst) == 55
+    assert max(worst.values()) < 1e-11 * max(1.0, np.abs(D).max())
+
+
+def test_builder_spdf_vs_oracle_loop_f():
+    """The native builder on the two-atom s,s,p,d,f system (oriented pairs,
+    every class through (ff|ff); the f classes on the loop-form kernel):
+    FP64 bars against the oracle's MMD build."""
+    import torch
+    basis, D = _two_atom_spdf()
+    fb = FockBuilder(basis, D, tau=1e-12)
+    assert any(3 in L.cls for L in fb.launches)
+    G = fb.build(native.to_dev(D))
+    torch.cuda.synchronize()
+    Gg = G.cpu().numpy()
+    Go = oracle_build_mmd(fb, D)
+    dG = np.abs(Gg - Go).max()
+    dE = abs(energy(D, Gg) - energy(D, Go))
+    _print("spdf (loop f) max|dG|", dG, "|dE|", dE, "launches", len(fb.launches))
+    assert dG <= 1e-10 and dE <= 1e-9
+
+
+def test_builder_co_ccpvtz_vs_libcint_loop_f():
+    """CO/cc-pVTZ (f on both atoms): the native builder, f classes on the
+    loop-form kernel, against libcint's vj - vk/2 and the oracle; the mixed
+    build within the mixed bars."""
+    import torch
+    fx = load_fixture("co_ccpvtz")
+    fb = FockBuilder(fx.basis, fx.density, tau=1e-12)
+    G = fb.build(native.to_dev(fx.density))
+    torch.cuda.synchronize()
+    G = G.cpu().numpy()
+    ref = fx.arrays["vj"] - 0.5 * fx.arrays["vk"]
+    Go = oracle_build_mmd(fb, fx.density)
+    fm = FockBuilder(fx.basis, fx.density, tau=1e-12, mixed=True)
+    Gm = fm.build(native.to_dev(fx.density))
+    torch.cuda.synchronize()
+    Gm = Gm.cpu().numpy()
+    dEm = abs(energy(fx.density, Gm) - energy(fx.density, G))
+    _print("co_ccpvtz (loop f) gpu-libcint", np.abs(G - ref).max(), "gpu-oracle", np.abs(G - Go).max(),
+           "mixed max|dG|", np.abs(Gm - G).max(), "|dE|", dEm)
+    assert np.abs(G - Go).max() < 1e-10
+    assert np.abs(G - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+    assert np.abs(Gm - G).max() <= 1e-8 and dEm <= 1e-6
 
 
 def test_graph_replays_match_eager():

@@ -146,3 +146,61 @@ def test_loop_k_kernel_on_cpu_vs_oracle():
     err = np.abs(K - Ko).max()
     print("loop-form K vs oracle", err, "scale", np.abs(Ko).max(), flush=True)
     assert err <= 1e-12 * max(1.0, np.abs(Ko).max())
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
+def test_loop_f_kernel_on_cpu_vs_oracle():
+    """The loop-form fused F kernel on every walker launch of the s,p,d,f
+    basis (every canonical class through (ff|ff)), F against the oracle's
+    launch-by-launch witness (the same screen and digestion, integrals by
+    McMurchie-Davidson)."""
+    from felizjueves_amd import pairs as P
+    basis, D = _system()
+    plan = P.f_launches(basis, D, tau=0.0, n=1)
+    Dsh = P.density_blocks(basis, D)
+    launches = plan.launches
+    assert max(max(L.cls) for L in launches) == 3
+    Fo = np.zeros((basis.n_ao, basis.n_ao))
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "gen").mkdir(exist_ok=True)
+        shutil.copy(EMU / "fj_device_emu.h", td / "fj_device.h")
+        shutil.copy(CSRC / "gen" / "fj_boys_table.h", td / "gen" / "fj_boys_table.h")
+        exe = td / "floop_emu"
+        cmd = ["g++", "-std=c++20", "-O1", "-g0", "-fsanitize=address", f"-I{ROOT / 'include'}",
+               f"-I{td}", f'-DKLOOP_HEADER="{CSRC / "fj_kloop.h"}"',
+               f'-DFLOOP_HEADER="{CSRC / "fj_floop.h"}"', str(EMU / "floop_emu.cpp"),
+               "-o", str(exe), "-lpthread"]
+        r = subprocess.run(cmd, cwd=td, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-3000:]
+        with open(td / "case.bin", "wb") as f:
+            np.array([len(launches), basis.n_ao], dtype=np.int32).tofile(f)
+
+            def w(x, dt):
+                x = np.ascontiguousarray(x, dtype=dt).ravel()
+                np.array([x.size], dtype=np.int64).tofile(f)
+                x.tofile(f)
+            for L in launches:
+                a = P.launch_args(basis, plan, L, D, Dsh)
+                O.f_launch(L.cls, a, Fo)
+                np.array([*L.cls, a["n_bra"], a["n_ket"], a["k_ab"], a["k_cd"], a["n_sh"],
+                          a["n_ao"], a["diag"]], dtype=np.int32).tofile(f)
+                np.array([a["tau"]], dtype=np.float64).tofile(f)
+                w(a["bra_prim"], np.float64)
+                if L.cls[1]:
+                    w(a["AB"], np.float64)
+                w(a["G_ab"], np.float64)
+                w(a["bra_sh"], np.int32)
+                w(a["bra_ao"], np.int32)
+                w(a["ket_prim"], np.float64)
+                if L.cls[3]:
+                    w(a["CD"], np.float64)
+                w(a["G_cd"], np.float64)
+                w(a["ket_sh"], np.int32)
+                w(a["ket_ao"], np.int32)
+                w(a["Dsh"], np.float64)
+                w(a["D"], np.float64)
+        F = _run(exe, td, "F.bin").reshape(basis.n_ao, basis.n_ao)
+    err = np.abs(F - Fo).max()
+    print("loop-form F vs oracle", err, "scale", np.abs(Fo).max(), "launches", len(launches), flush=True)
+    assert err <= 1e-12 * max(1.0, np.abs(Fo).max())
