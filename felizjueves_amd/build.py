@@ -21,7 +21,7 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libfj_mi355x.so"
 ARCH = "gfx950"
-LMAX = 2
+LMAX = 2         # generated HGP fused F kernels through (dd|dd)
 MMD_LMAX = 3     # Schwarz, DQ, [p|q] and JAB through f
 LOOP_LMAX = 3    # fused F classes past LMAX run on the loop-form kernel (fj_floop.h)
 MMD_K_LMAX = 2   # straight-line K kernels through d; the f classes in loop form (fj_kloop.h)
