@@ -62,6 +62,8 @@ def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, fl
     hdr = src.parent / (src.stem.replace("_r32", "") + ".h")
     if hdr.exists():
         deps.append(hdr)
+    if src.name == "fj_kloop.hip":
+        deps.append(CSRC / "fj_floop.h")
     if not force and obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime
                                           for d in deps if d.exists()):
         return obj, 0.0, ""
