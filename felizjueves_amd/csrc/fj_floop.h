@@ -144,23 +144,34 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
             wbd[j] = s_k * D[(size_t)(ob + j / nd) * n_ao + od + j % nd];
             fbd[j] = 0.0;
         }
-        for (int ib = kb0; ib < kb0 + kab; ++ib) {
-            const double* __restrict__ rb = a.bra.prim + 8 * (size_t)ib;
-            for (int ik = kc0; ik < kc0 + kcd; ++ik) {
+        // the side of higher angular momentum in the outer loop: its E tables
+        // are built once per outer primitive pair, the other side's per pair
+        // of primitive pairs
+        const bool ket_outer = lq > lp;
+        const int n_out = ket_outer ? kcd : kab, n_in = ket_outer ? kab : kcd;
+        for (int io = 0; io < n_out; ++io) {
+            for (int ii = 0; ii < n_in; ++ii) {
+                const int ib = kb0 + (ket_outer ? ii : io), ik = kc0 + (ket_outer ? io : ii);
+                const double* __restrict__ rb = a.bra.prim + 8 * (size_t)ib;
                 const double* __restrict__ rk = a.ket.prim + 8 * (size_t)ik;
                 const double p = rb[6], q = rk[6];
+                const bool bra_new = ket_outer || ii == 0, ket_new = !ket_outer || ii == 0;
                 if (lane < 3) {
-                    const double pa = rb[3 + lane], ab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
-                    fj_kl_eaxis_from(eab + lane * FJ_KL_AX, la, lb, p, pa, pa + ab, 1.0);
+                    if (bra_new) {
+                        const double pa = rb[3 + lane], ab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
+                        fj_kl_eaxis_from(eab + lane * FJ_KL_AX, la, lb, p, pa, pa + ab, 1.0);
+                    }
                 } else if (lane < 6) {
-                    const int x = lane - 3;
-                    const double qc = rk[3 + x], cd = x == 0 ? CDx : (x == 1 ? CDy : CDz);
-                    fj_kl_eaxis_from(ecd + x * FJ_KL_AX, lc, ld, q, qc, qc + cd, 1.0);
+                    if (ket_new) {
+                        const int x = lane - 3;
+                        const double qc = rk[3 + x], cd = x == 0 ? CDx : (x == 1 ? CDy : CDz);
+                        fj_kl_eaxis_from(ecd + x * FJ_KL_AX, lc, ld, q, qc, qc + cd, 1.0);
+                    }
                 }
                 const double PQx = rb[0] - rk[0], PQy = rb[1] - rk[1], PQz = rb[2] - rk[2];
                 const double rho = p * q / (p + q);
                 if (lane == 0) {
-                    fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz));
+                    fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), L);
                     R[L & 1][0] = wf[L];
                 }
                 __syncthreads();

@@ -110,16 +110,37 @@ __device__ __forceinline__ void fj_kl_steps(int* tw, int* hc, int nh, int lane) 
     }
 }
 
-// wf[m] = [0]^(m) = (2 / sqrt(pi)) sqrt(rho) (-2 rho)^m F_m(T), m <= 12.
-__device__ __forceinline__ void fj_kl_seed(double* wf, double rho, double T) {
-    double Fm[FJ_KL_MMAX + 1];
-    fj_boys<FJ_KL_MMAX>(T, Fm);
+// wf[m] = [0]^(m) = (2 / sqrt(pi)) sqrt(rho) (-2 rho)^m F_m(T), m <= M.
+template <int M>
+__device__ __forceinline__ void fj_kl_seed_m(double* wf, double rho, double T) {
+    double Fm[M + 1];
+    fj_boys<M>(T, Fm);
     const double m2r = -2.0 * rho;
     double wm = FJ_TWO_OVER_SQRT_PI * sqrt(rho);
 #pragma unroll
-    for (int m = 0; m <= FJ_KL_MMAX; ++m) {
+    for (int m = 0; m <= M; ++m) {
         wf[m] = wm * Fm[m];
         wm *= m2r;
+    }
+}
+
+// The same for m <= L (the tower's top order, 0 .. 12): the Boys function
+// evaluated at that order, not at the largest one.
+__device__ __forceinline__ void fj_kl_seed(double* wf, double rho, double T, int L) {
+    switch (L) {
+    case 0: fj_kl_seed_m<0>(wf, rho, T); break;
+    case 1: fj_kl_seed_m<1>(wf, rho, T); break;
+    case 2: fj_kl_seed_m<2>(wf, rho, T); break;
+    case 3: fj_kl_seed_m<3>(wf, rho, T); break;
+    case 4: fj_kl_seed_m<4>(wf, rho, T); break;
+    case 5: fj_kl_seed_m<5>(wf, rho, T); break;
+    case 6: fj_kl_seed_m<6>(wf, rho, T); break;
+    case 7: fj_kl_seed_m<7>(wf, rho, T); break;
+    case 8: fj_kl_seed_m<8>(wf, rho, T); break;
+    case 9: fj_kl_seed_m<9>(wf, rho, T); break;
+    case 10: fj_kl_seed_m<10>(wf, rho, T); break;
+    case 11: fj_kl_seed_m<11>(wf, rho, T); break;
+    default: fj_kl_seed_m<FJ_KL_MMAX>(wf, rho, T); break;
     }
 }
 
@@ -210,7 +231,7 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
         const double* __restrict__ dm = Dcd + (size_t)blk * (nc * nd);
         for (int j = lane; j < nc * nd; j += 64) dsh[j] = dm[j];
         if (lane == 0) {
-            fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz));
+            fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), L);
             R[L & 1][0] = wf[L];
         }
         __syncthreads();
@@ -306,7 +327,7 @@ __global__ void __launch_bounds__(64) fj_schwarz_loop_k(
                          PQz = P[3 * i + 2] - P[3 * k + 2];
             const double rho = zi * zk / (zi + zk);
             if (lane == 0) {
-                fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz));
+                fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), L);
                 R[L & 1][0] = wf[L];
             }
             __syncthreads();
