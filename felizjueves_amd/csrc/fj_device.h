@@ -43,6 +43,9 @@ struct FjFArgs {
 // (allocated on first use, grown when asked for more; NULL on failure).
 double* fj_library_workspace(long long bytes);
 
+// A kernel's dynamic LDS block (its size given at launch).
+#define FJ_DYN_SHARED(name) extern __shared__ double name[]
+
 // An empty asm statement that the compiler must assume redefines the
 // per-lane pointer p (a VGPR pair) and the uniform stride s (an SGPR pair):
 // values derived from them are recomputed after it, not hoisted above it.

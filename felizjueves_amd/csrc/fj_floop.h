@@ -24,19 +24,41 @@
 
 #define FJ_FL_CH 16   // bra rows per W chunk
 
-__global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
-    __shared__ double R[2][FJ_KL_NH];
-    __shared__ double eab[3 * FJ_KL_AX], ecd[3 * FJ_KL_AX];
-    __shared__ double W[FJ_FL_CH * FJ_KL_NP];
-    __shared__ double wab[FJ_KL_NC * FJ_KL_NC], wcd[FJ_KL_NC * FJ_KL_NC], wac[FJ_KL_NC * FJ_KL_NC],
-        wad[FJ_KL_NC * FJ_KL_NC], wbc[FJ_KL_NC * FJ_KL_NC], wbd[FJ_KL_NC * FJ_KL_NC];
-    __shared__ double fab[FJ_KL_NC * FJ_KL_NC], fcd[FJ_KL_NC * FJ_KL_NC], fac[FJ_KL_NC * FJ_KL_NC],
-        fad[FJ_KL_NC * FJ_KL_NC], fbc[FJ_KL_NC * FJ_KL_NC], fbd[FJ_KL_NC * FJ_KL_NC];
-    __shared__ double wf[FJ_KL_MMAX + 1];
-    __shared__ int hc[FJ_KL_NH];
-    __shared__ int tw[FJ_KL_NH];
-    __shared__ unsigned short pqi[FJ_KL_NP * FJ_KL_NP];
+// The kernel's LDS, sized for its class and carved from one dynamic block:
+// byte offsets of each array (doubles, then ints, then the u16 index table)
+// and the total. The launcher and the kernel both use this layout.
+struct FjFlLds {
+    int R0, R1, eab, ecd, W, w[6], f[6], wf, hc, tw, pqi, bytes, axa, axc;
+};
 
+__host__ __device__ inline FjFlLds fj_floop_lds(int la, int lb, int lc, int ld) {
+    auto herm = [](int n) { return n * (n + 1) * (n + 2) / 6; };
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+    const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
+    const int np = herm(la + lb + 1), nq = herm(lc + ld + 1), nh = herm(la + lb + lc + ld + 1);
+    FjFlLds o;
+    o.axa = (la + 1) * (lb + 1) * (la + lb + 1);
+    o.axc = (lc + 1) * (ld + 1) * (lc + ld + 1);
+    const int blk[6] = {na * nb, nc * nd, na * nc, na * nd, nb * nc, nb * nd};
+    int at = 0;
+    auto take = [&](int n, int size) { const int r = at; at += ((n * size + 7) / 8) * 8; return r; };
+    o.R0 = take(nh, 8);
+    o.R1 = take(nh, 8);
+    o.eab = take(3 * o.axa, 8);
+    o.ecd = take(3 * o.axc, 8);
+    o.W = take(FJ_FL_CH * nq, 8);
+    for (int i = 0; i < 6; ++i) o.w[i] = take(blk[i], 8);
+    for (int i = 0; i < 6; ++i) o.f[i] = take(blk[i], 8);
+    o.wf = take(FJ_KL_MMAX + 1, 8);
+    o.hc = take(nh, 4);
+    o.tw = take(nh, 4);
+    o.pqi = take(np * nq, 2);
+    o.bytes = at;
+    return o;
+}
+
+__global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
+    FJ_DYN_SHARED(fj_floop_lds_block);
     const int lane = (int)threadIdx.x;
     // the work item: (bra pair, first ket, system | bra chunk, ket end), or
     // (x, y * span, the single system, n_ket) without a work list
@@ -72,6 +94,31 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
               hd = fj_herm_base(ld);
     const int ta = lp + 1, tc = lq + 1;   // t extents of the two E tables
 
+    // the class's LDS arrays
+    const FjFlLds lay = fj_floop_lds(la, lb, lc, ld);
+    char* const lds = reinterpret_cast<char*>(fj_floop_lds_block);
+    double* const Rr0 = reinterpret_cast<double*>(lds + lay.R0);
+    double* const Rr1 = reinterpret_cast<double*>(lds + lay.R1);
+    double* const eab = reinterpret_cast<double*>(lds + lay.eab);
+    double* const ecd = reinterpret_cast<double*>(lds + lay.ecd);
+    double* const W = reinterpret_cast<double*>(lds + lay.W);
+    double* const wab = reinterpret_cast<double*>(lds + lay.w[0]);
+    double* const wcd = reinterpret_cast<double*>(lds + lay.w[1]);
+    double* const wac = reinterpret_cast<double*>(lds + lay.w[2]);
+    double* const wad = reinterpret_cast<double*>(lds + lay.w[3]);
+    double* const wbc = reinterpret_cast<double*>(lds + lay.w[4]);
+    double* const wbd = reinterpret_cast<double*>(lds + lay.w[5]);
+    double* const fab = reinterpret_cast<double*>(lds + lay.f[0]);
+    double* const fcd = reinterpret_cast<double*>(lds + lay.f[1]);
+    double* const fac = reinterpret_cast<double*>(lds + lay.f[2]);
+    double* const fad = reinterpret_cast<double*>(lds + lay.f[3]);
+    double* const fbc = reinterpret_cast<double*>(lds + lay.f[4]);
+    double* const fbd = reinterpret_cast<double*>(lds + lay.f[5]);
+    double* const wf = reinterpret_cast<double*>(lds + lay.wf);
+    int* const hc = reinterpret_cast<int*>(lds + lay.hc);
+    int* const tw = reinterpret_cast<int*>(lds + lay.tw);
+    unsigned short* const pqi = reinterpret_cast<unsigned short*>(lds + lay.pqi);
+    const int axa = lay.axa, axc = lay.axc;
     fj_kl_steps(tw, hc, nh, lane);
     for (int j = lane; j < np * nq; j += 64) {
         const int p = j / nq, q = j - p * nq;
@@ -159,24 +206,24 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
                 if (lane < 3) {
                     if (bra_new) {
                         const double pa = rb[3 + lane], ab = lane == 0 ? ABx : (lane == 1 ? ABy : ABz);
-                        fj_kl_eaxis_from(eab + lane * FJ_KL_AX, la, lb, p, pa, pa + ab, 1.0);
+                        fj_kl_eaxis_from(eab + lane * axa, la, lb, p, pa, pa + ab, 1.0);
                     }
                 } else if (lane < 6) {
                     if (ket_new) {
                         const int x = lane - 3;
                         const double qc = rk[3 + x], cd = x == 0 ? CDx : (x == 1 ? CDy : CDz);
-                        fj_kl_eaxis_from(ecd + x * FJ_KL_AX, lc, ld, q, qc, qc + cd, 1.0);
+                        fj_kl_eaxis_from(ecd + x * axc, lc, ld, q, qc, qc + cd, 1.0);
                     }
                 }
                 const double PQx = rb[0] - rk[0], PQy = rb[1] - rk[1], PQz = rb[2] - rk[2];
                 const double rho = p * q / (p + q);
                 if (lane == 0) {
                     fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), L);
-                    R[L & 1][0] = wf[L];
+                    ((L & 1) ? Rr1 : Rr0)[0] = wf[L];
                 }
                 __syncthreads();
-                fj_kl_tower(R, wf, tw, L, PQx, PQy, PQz, lane);
-                const double* __restrict__ R0 = R[0];
+                fj_kl_tower(Rr0, Rr1, wf, tw, L, PQx, PQy, PQz, lane);
+                const double* __restrict__ R0 = Rr0;
                 const double UU = rb[7] * rk[7];
                 for (int r0 = 0; r0 < nab; r0 += FJ_FL_CH) {
                     const int nr = min(FJ_FL_CH, nab - r0);
@@ -186,15 +233,15 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
                         const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
                         const double* __restrict__ ex = eab + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
                         const double* __restrict__ ey =
-                            eab + FJ_KL_AX + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
+                            eab + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
                         const double* __restrict__ ez =
-                            eab + 2 * FJ_KL_AX + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
+                            eab + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
                         double s = 0.0;
                         for (int pi = 0; pi < np; ++pi) {
                             const int cp = hc[pi];
                             s += (ex[cp & 15] * ey[(cp >> 4) & 15] * ez[cp >> 8]) * R0[pqi[pi * nq + qi]];
                         }
-                        W[r * FJ_KL_NP + qi] = s;
+                        W[r * nq + qi] = s;
                     }
                     __syncthreads();
                     for (int j = lane; j < nr * ncd; j += 64) {
@@ -204,10 +251,10 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
                         const int cc = hc[hcc + kc], cdd = hc[hd + ld_];
                         const double* __restrict__ fx = ecd + ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
                         const double* __restrict__ fy =
-                            ecd + FJ_KL_AX + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
+                            ecd + axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
                         const double* __restrict__ fz =
-                            ecd + 2 * FJ_KL_AX + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
-                        const double* __restrict__ wr = W + r * FJ_KL_NP;
+                            ecd + 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
+                        const double* __restrict__ wr = W + r * nq;
                         double s = 0.0;
                         for (int qi = 0; qi < nq; ++qi) {
                             const int cq = hc[qi];

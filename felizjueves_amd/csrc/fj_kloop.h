@@ -145,14 +145,14 @@ __device__ __forceinline__ void fj_kl_seed(double* wf, double rho, double T, int
 }
 
 // Levels m = L - 1 .. 0 of the tower, lanes over t: level m holds [t]^(m)
-// for |t| <= L - m in R[m & 1], so [t]^(0) ends in R[0]. The caller has
-// written wf and R[L & 1][0] = wf[L] and passed a barrier; every lane of the
-// workgroup calls this (a barrier after each level).
-__device__ __forceinline__ void fj_kl_tower(double (*R)[FJ_KL_NH], const double* wf, const int* tw,
+// for |t| <= L - m in row m & 1 (R0 or R1), so [t]^(0) ends in R0. The
+// caller has written wf and row L & 1's [0] = wf[L] and passed a barrier;
+// every lane of the workgroup calls this (a barrier after each level).
+__device__ __forceinline__ void fj_kl_tower(double* R0, double* R1, const double* wf, const int* tw,
                                             int L, double PQx, double PQy, double PQz, int lane) {
     for (int m = L - 1; m >= 0; --m) {
-        double* __restrict__ cur = R[m & 1];
-        const double* __restrict__ prv = R[(m + 1) & 1];
+        double* __restrict__ cur = (m & 1) ? R1 : R0;
+        const double* __restrict__ prv = (m & 1) ? R0 : R1;
         const int n_m = fj_herm_base(L - m + 1);
         for (int j = lane; j < n_m; j += 64) {
             double v;
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
             R[L & 1][0] = wf[L];
         }
         __syncthreads();
-        fj_kl_tower(R, wf, tw, L, PQx, PQy, PQz, lane);
+        fj_kl_tower(R[0], R[1], wf, tw, L, PQx, PQy, PQz, lane);
         const double* __restrict__ R0 = R[0];
         for (int ib = 0; ib < nb; ++ib) {
             const int cb = hc[ob + ib];
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(64) fj_schwarz_loop_k(
                 R[L & 1][0] = wf[L];
             }
             __syncthreads();
-            fj_kl_tower(R, wf, tw, L, PQx, PQy, PQz, lane);
+            fj_kl_tower(R[0], R[1], wf, tw, L, PQx, PQy, PQz, lane);
             const double sik = s_ab[i] * s_ab[k];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {

@@ -76,6 +76,7 @@ int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, 
     const dim3 grid((unsigned)n_blocks, a.work ? 1u : (unsigned)((a.ket.n + threads - 1) / threads));
     a.n_bx = (int)grid.x;
     a.n_items = (int)(grid.x * grid.y);
-    hipLaunchKernelGGL(fj_floop_k, grid, dim3(64), 0, s, la, lb, lc, ld, threads, a);
+    const FjFlLds lay = fj_floop_lds(la, lb, lc, ld);
+    hipLaunchKernelGGL(fj_floop_k, grid, dim3(64), (size_t)lay.bytes, s, la, lb, lc, ld, threads, a);
     return (int)hipGetLastError();
 }

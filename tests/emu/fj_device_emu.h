@@ -43,6 +43,9 @@ struct FjFArgs {
 extern std::barrier<>* g_bar;
 extern double g_shfl[256];
 #define FJ_LAUNDER(p, s) ((void)0)
+// the dynamic LDS block: a heap buffer of the launch's exact size, set by the driver
+extern double* g_dyn_lds;
+#define FJ_DYN_SHARED(name) double* const name = g_dyn_lds
 inline void __syncthreads() { g_bar->arrive_and_wait(); }
 inline double __shfl_xor(double v, int o, int w) {
     g_bar->arrive_and_wait();

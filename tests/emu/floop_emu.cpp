@@ -12,6 +12,7 @@ thread_local uint3e threadIdx;
 uint3e blockIdx, blockDim, gridDim;
 std::barrier<>* g_bar;
 double g_shfl[256];
+double* g_dyn_lds;
 
 template <typename T> T* rd(std::ifstream& f) {
     long long n;
@@ -26,7 +27,7 @@ int main(int argc, char** argv) {
     int hdr[2];   // n_launch n_ao
     f.read((char*)hdr, sizeof hdr);
     const int n_ao = hdr[1], span = 64;
-    struct L { int c[4]; FjFArgs a; };
+    struct L { int c[4]; FjFArgs a; double* lds; };
     std::vector<L> Ls(hdr[0]);
     double* F = (double*)calloc((size_t)n_ao * n_ao, 8);
     for (auto& l : Ls) {
@@ -35,6 +36,8 @@ int main(int argc, char** argv) {
         double tau;
         f.read((char*)&tau, 8);
         for (int i = 0; i < 4; ++i) l.c[i] = h[i];
+        // the launch's dynamic LDS at its exact size, so ASan sees an overrun
+        l.lds = (double*)malloc(fj_floop_lds(h[0], h[1], h[2], h[3]).bytes);
         double* bp = rd<double>(f); double* AB = h[1] ? rd<double>(f) : nullptr;
         double* Gab = rd<double>(f); int* bsh = rd<int>(f); int* bao = rd<int>(f);
         double* kp = rd<double>(f); double* CD = h[3] ? rd<double>(f) : nullptr;
@@ -64,6 +67,7 @@ int main(int argc, char** argv) {
                         if (t == 0) {
                             blockIdx = {x, y, 0};
                             gridDim = {gx, gy, 1};
+                            g_dyn_lds = l.lds;
                         }
                         bar.arrive_and_wait();
                         fj_floop_k(l.c[0], l.c[1], l.c[2], l.c[3], span, l.a);

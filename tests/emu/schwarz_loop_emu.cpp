@@ -10,6 +10,7 @@ thread_local uint3e threadIdx;
 uint3e blockIdx, blockDim, gridDim;
 std::barrier<>* g_bar;
 double g_shfl[256];
+double* g_dyn_lds;
 
 template <typename T> T* rd(std::ifstream& f) {
     long long n;
