@@ -171,6 +171,39 @@ __device__ __forceinline__ void fj_kl_tower(double* R0, double* R1, const double
     }
 }
 
+// The K kernel's LDS, sized for its class and carved from one dynamic block:
+// byte offsets of each array (doubles, then ints, then the u16 index table)
+// and the total. The launcher and the kernel both use this layout.
+struct FjKlLds {
+    int R0, R1, ea, eb, X, Y, kac, dsh, wf, hc, tw, pqi, bytes, axa, axb;
+};
+
+__host__ __device__ inline FjKlLds fj_kloop_lds(int la, int lc, int lb, int ld) {
+    auto herm = [](int n) { return n * (n + 1) * (n + 2) / 6; };
+    const int na = (la + 1) * (la + 2) / 2, nc = (lc + 1) * (lc + 2) / 2;
+    const int nb = (lb + 1) * (lb + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
+    const int np = herm(la + lc + 1), nq = herm(lb + ld + 1), nh = herm(la + lc + lb + ld + 1);
+    FjKlLds o;
+    o.axa = (la + 1) * (lc + 1) * (la + lc + 1);
+    o.axb = (lb + 1) * (ld + 1) * (lb + ld + 1);
+    int at = 0;
+    auto take = [&](int n, int size) { const int r = at; at += ((n * size + 7) / 8) * 8; return r; };
+    o.R0 = take(nh, 8);
+    o.R1 = take(nh, 8);
+    o.ea = take(3 * o.axa, 8);
+    o.eb = take(3 * o.axb, 8);
+    o.X = take(nc * nq, 8);
+    o.Y = take(nc * np, 8);
+    o.kac = take(na * nc * nb, 8);
+    o.dsh = take(nc * nd, 8);
+    o.wf = take(FJ_KL_MMAX + 1, 8);
+    o.hc = take(nh, 4);
+    o.tw = take(nh, 4);
+    o.pqi = take(np * nq, 2);
+    o.bytes = at;
+    return o;
+}
+
 __global__ void __launch_bounds__(64) fj_kloop_k(
     int la, int lc, int lb, int ld, int n_bra, int n_ket, int n_dc, double tau, double maxD,
     const double* __restrict__ P, const double* __restrict__ zeta_ac,
@@ -181,15 +214,7 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
     const double* __restrict__ XQD, const double* __restrict__ s_bd,
     const double* __restrict__ Iq, const int* __restrict__ dd,
     const double* __restrict__ Dcd, double* __restrict__ K) {
-    __shared__ double R[2][FJ_KL_NH];
-    __shared__ double ea[3 * FJ_KL_AX], eb[3 * FJ_KL_AX];
-    __shared__ double X[FJ_KL_NC * FJ_KL_NP], Y[FJ_KL_NC * FJ_KL_NP];
-    __shared__ double kac[FJ_KL_NC * FJ_KL_NC * FJ_KL_NC];
-    __shared__ double dsh[FJ_KL_NC * FJ_KL_NC];
-    __shared__ double wf[FJ_KL_MMAX + 1];
-    __shared__ int hc[FJ_KL_NH];
-    __shared__ int tw[FJ_KL_NH];
-    __shared__ unsigned short pqi[FJ_KL_NP * FJ_KL_NP];
+    FJ_DYN_SHARED(fj_kloop_lds_block);
 
     const int bra = (int)blockIdx.x, lane = (int)threadIdx.x;
     if (bra >= n_bra) return;   // uniform per workgroup
@@ -200,6 +225,22 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
     const int oa = fj_herm_base(la), ob = fj_herm_base(lb), oc = fj_herm_base(lc),
               od = fj_herm_base(ld);
     const int ta = lp + 1, tb = lq + 1;   // t extents of the two E tables
+    // the class's LDS arrays
+    const FjKlLds lay = fj_kloop_lds(la, lc, lb, ld);
+    char* const lds = reinterpret_cast<char*>(fj_kloop_lds_block);
+    double* const Rr0 = reinterpret_cast<double*>(lds + lay.R0);
+    double* const Rr1 = reinterpret_cast<double*>(lds + lay.R1);
+    double* const ea = reinterpret_cast<double*>(lds + lay.ea);
+    double* const eb = reinterpret_cast<double*>(lds + lay.eb);
+    double* const X = reinterpret_cast<double*>(lds + lay.X);
+    double* const Y = reinterpret_cast<double*>(lds + lay.Y);
+    double* const kac = reinterpret_cast<double*>(lds + lay.kac);
+    double* const dsh = reinterpret_cast<double*>(lds + lay.dsh);
+    double* const wf = reinterpret_cast<double*>(lds + lay.wf);
+    int* const hc = reinterpret_cast<int*>(lds + lay.hc);
+    int* const tw = reinterpret_cast<int*>(lds + lay.tw);
+    unsigned short* const pqi = reinterpret_cast<unsigned short*>(lds + lay.pqi);
+    const int axa = lay.axa, axb = lay.axb;
 
     // index tables: components of every Hermite index, the tower's step of
     // each t (first nonzero axis d: [t] = PQ_d [t - 1_d] + (t_d - 1) [t - 2_d]),
@@ -215,7 +256,7 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
     for (int j = lane; j < na * nc * nb; j += 64) kac[j] = 0.0;
     const double z = zeta_ac[bra];
     const double Px = P[3 * bra], Py = P[3 * bra + 1], Pz = P[3 * bra + 2];
-    if (lane < 3) fj_kl_eaxis(ea + lane * FJ_KL_AX, la, lc, z, XPA[3 * bra + lane], XPC[3 * bra + lane]);
+    if (lane < 3) fj_kl_eaxis(ea + lane * axa, la, lc, z, XPA[3 * bra + lane], XPC[3 * bra + lane]);
     const double ip = Ip[bra];
     const int dcb = dc[bra];
     __syncthreads();
@@ -227,16 +268,16 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
         const double e = eta[k], sb = s_bd[k];
         const double PQx = Px - Q[3 * k], PQy = Py - Q[3 * k + 1], PQz = Pz - Q[3 * k + 2];
         const double rho = z * e / (z + e);
-        if (lane < 3) fj_kl_eaxis(eb + lane * FJ_KL_AX, lb, ld, e, XQB[3 * k + lane], XQD[3 * k + lane]);
+        if (lane < 3) fj_kl_eaxis(eb + lane * axb, lb, ld, e, XQB[3 * k + lane], XQD[3 * k + lane]);
         const double* __restrict__ dm = Dcd + (size_t)blk * (nc * nd);
         for (int j = lane; j < nc * nd; j += 64) dsh[j] = dm[j];
         if (lane == 0) {
             fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), L);
-            R[L & 1][0] = wf[L];
+            ((L & 1) ? Rr1 : Rr0)[0] = wf[L];
         }
         __syncthreads();
-        fj_kl_tower(R[0], R[1], wf, tw, L, PQx, PQy, PQz, lane);
-        const double* __restrict__ R0 = R[0];
+        fj_kl_tower(Rr0, Rr1, wf, tw, L, PQx, PQy, PQz, lane);
+        const double* __restrict__ R0 = Rr0;
         for (int ib = 0; ib < nb; ++ib) {
             const int cb = hc[ob + ib];
             const int bx = cb & 15, by = (cb >> 4) & 15, bz = cb >> 8;
@@ -248,8 +289,8 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
                 for (int d = 0; d < nd; ++d) {
                     const int cd = hc[od + d];
                     const double ex = eb[(bx * (ld + 1) + (cd & 15)) * tb + qx];
-                    const double ey = eb[FJ_KL_AX + (by * (ld + 1) + ((cd >> 4) & 15)) * tb + qy];
-                    const double ez = eb[2 * FJ_KL_AX + (bz * (ld + 1) + (cd >> 8)) * tb + qz];
+                    const double ey = eb[axb + (by * (ld + 1) + ((cd >> 4) & 15)) * tb + qy];
+                    const double ez = eb[2 * axb + (bz * (ld + 1) + (cd >> 8)) * tb + qz];
                     s += dsh[c * nd + d] * (ex * ey * ez);
                 }
                 X[j] = ((qx + qy + qz) & 1 ? -sb : sb) * s;
@@ -269,8 +310,8 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
                 const int a = j / nc, c = j - a * nc;
                 const int ca = hc[oa + a], cc = hc[oc + c];
                 const double* __restrict__ ex = ea + ((ca & 15) * (lc + 1) + (cc & 15)) * ta;
-                const double* __restrict__ ey = ea + FJ_KL_AX + (((ca >> 4) & 15) * (lc + 1) + ((cc >> 4) & 15)) * ta;
-                const double* __restrict__ ez = ea + 2 * FJ_KL_AX + ((ca >> 8) * (lc + 1) + (cc >> 8)) * ta;
+                const double* __restrict__ ey = ea + axa + (((ca >> 4) & 15) * (lc + 1) + ((cc >> 4) & 15)) * ta;
+                const double* __restrict__ ez = ea + 2 * axa + ((ca >> 8) * (lc + 1) + (cc >> 8)) * ta;
                 const double* __restrict__ yr = Y + c * np;
                 double s = 0.0;
                 for (int p = 0; p < np; ++p) {

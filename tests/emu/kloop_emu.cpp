@@ -25,7 +25,7 @@ template <typename T> T* rd(std::ifstream& f) {
 struct Launch {
     int h[7];   // la lc lb ld n_bra n_ket n_dc
     double tau, maxD;
-    double *P, *z, *XA, *XC, *s, *Ip, *Q, *eta, *XB, *XD, *sb, *Iq, *Dcd, *K;
+    double *P, *z, *XA, *XC, *s, *Ip, *Q, *eta, *XB, *XD, *sb, *Iq, *Dcd, *K, *lds;
     int *dc, *dd;
 };
 
@@ -44,6 +44,8 @@ int main(int argc, char** argv) {
         L.sb = rd<double>(f); L.Iq = rd<double>(f); L.dd = rd<int>(f); L.Dcd = rd<double>(f);
         const int na = (L.h[0] + 1) * (L.h[0] + 2) / 2, nb = (L.h[2] + 1) * (L.h[2] + 2) / 2;
         L.K = (double*)malloc(sizeof(double) * (size_t)L.h[4] * na * nb + 1);
+        // the launch's dynamic LDS at its exact size, so ASan sees an overrun
+        L.lds = (double*)malloc(fj_kloop_lds(L.h[0], L.h[1], L.h[2], L.h[3]).bytes);
     }
     const int threads = 64;
     std::barrier<> bar(threads);
@@ -58,6 +60,7 @@ int main(int argc, char** argv) {
                     if (t == 0) {
                         blockIdx = {(unsigned)b, 0, 0};
                         gridDim = {(unsigned)L.h[4], 1, 1};
+                        g_dyn_lds = L.lds;
                     }
                     bar.arrive_and_wait();
                     fj_kloop_k(L.h[0], L.h[1], L.h[2], L.h[3], L.h[4], L.h[5], L.h[6], L.tau, L.maxD,
