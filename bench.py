@@ -352,6 +352,48 @@ def cof_block(a) -> None:
     print(json.dumps(out), flush=True)
 
 
+def c4f_block(a) -> None:
+    """C4 molecules with their f shells, each built alone (FP64, one stream,
+    eager): one warm-up and `--steps` timed builds per molecule, beside the
+    same molecule built through d. The whole 64-molecule batch with f is not
+    timed: the f classes run on the loop-form kernel, about a thousand
+    times slower per build than the HGP classes."""
+    import torch
+    from felizjueves_amd import native
+    from felizjueves_amd.fock import FockBuilder
+    from felizjueves_amd.systems import batch_molecule
+    rows = []
+    for i in a.molecules:
+        row = {"molecule": i}
+        for lmax in (2, 3):
+            s = batch_molecule(i, lmax=lmax)
+            t0 = time.time()
+            fb = FockBuilder(s.basis, s.D, tau=1e-12)
+            plan_s = time.time() - t0
+            Dd = native.to_dev(s.D)
+            log(f"c4f: {s.name} lmax {lmax}: n_ao {s.basis.n_ao}, {len(fb.launches)} launches, "
+                f"plan {plan_s:.1f}s; building")
+            t = timed(lambda: fb.build(Dd), a.steps, 1)
+            torch.cuda.synchronize()
+            log(f"c4f: {s.name} lmax {lmax}: {1e3 * t / a.steps:.1f} ms per build")
+            row.update({"name": s.name, f"n_ao_l{lmax}": s.basis.n_ao,
+                        f"launches_l{lmax}": len(fb.launches),
+                        f"f_class_launches_l{lmax}": sum(1 for L in fb.launches if 3 in L.cls),
+                        f"quartets_l{lmax}": fb.shell_quartets(),
+                        f"ms_per_build_l{lmax}": 1e3 * t / a.steps})
+        rows.append(row)
+    out = {
+        "metric": "Fock build ms of single C4 molecules with and without their f shells",
+        "value": rows[0]["ms_per_build_l3"], "unit": "ms per build (first molecule, with f)",
+        "n_gpus": 1, "steps": a.steps, "warmup": 1, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c4-single-molecules-with-f", "molecules": a.molecules, "tau": 1e-12,
+                   "n_streams": 1, "graph": False},
+        "molecules": rows,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,10 +403,13 @@ def main() -> None:
     ap.add_argument("--selector", default=str(TUNED / "selector"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the c3 / c5 blocks")
-    ap.add_argument("--workload", choices=("c2", "c4d", "cof"), default="c2",
+    ap.add_argument("--molecules", type=int, nargs="+", default=[7],
+                    help="c4f: which C4 molecules (index i: seed 4 + i)")
+    ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f"), default="c2",
                     help="c4d: the 64-molecule batch (8-80 atoms, seeds 4+i) with s,s,p,d shells "
                          "(with its f shells one build would take hours on the loop-form f "
-                         "kernel); cof: CO/cc-pVTZ through f (the loop-form kernels)")
+                         "kernel); cof: CO/cc-pVTZ through f (the loop-form kernels); c4f: "
+                         "single C4 molecules with and without their f shells")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for HBM counter runs: one elementwise copy of 512 MiB (the counters' "
                          "scale) and one eager build of the default plan, then exit")
@@ -401,6 +446,9 @@ def main() -> None:
         return
     if a.workload == "cof":
         cof_block(a)
+        return
+    if a.workload == "c4f":
+        c4f_block(a)
         return
 
     selector = load_selector(a.selector)
