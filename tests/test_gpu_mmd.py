@@ -97,6 +97,20 @@ def test_k_vs_libcint(name):
     assert err < 1e-11 * max(1.0, np.abs(ref).max())
 
 
+def test_k_screened_through_f_vs_libcint():
+    """CO/cc-pVTZ with the K screen on (tau = 1e-12: each bra stops at its
+    first ket below the bound, the f classes in the loop form): within the
+    screen's reach of libcint's vk, and equal to the unscreened build where
+    nothing is cut."""
+    fx = load_fixture("co_ccpvtz")
+    K = _host(jk.ExchangeBuild(fx.basis, tau=1e-12).run(fx.density))
+    K0 = _host(jk.ExchangeBuild(fx.basis).run(fx.density))
+    ref = fx.arrays["vk"]
+    err, cut = np.abs(K - ref).max(), np.abs(K - K0).max()
+    print("co_ccpvtz screened K: max |K - vk|", err, "vs unscreened", cut, flush=True)
+    assert err < 1e-10 * max(1.0, np.abs(ref).max())
+
+
 def _k_rows(form: str, basis, D):
     """Every K launch's rows through the straight-line entries or through
     fj_k_loop, with the same tables."""
