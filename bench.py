@@ -394,6 +394,44 @@ def c4f_block(a) -> None:
     print(json.dumps(out), flush=True)
 
 
+def c4f_batch_block(a) -> None:
+    """Chosen C4 molecules with their f shells in one batch builder (one
+    launch per class pair for all of them, one stream, eager; the f classes
+    on the loop-form kernel): batched builds/s, and the first molecule
+    against its own single build."""
+    import torch
+    from felizjueves_amd import native
+    from felizjueves_amd.fock import FockBuilder
+    from felizjueves_amd.systems import batch_molecule
+    systems = [batch_molecule(i, lmax=3) for i in a.molecules]
+    t0 = time.time()
+    fb = FockBuilder.batch([(s.basis, s.D) for s in systems], tau=1e-12)
+    plan_s = time.time() - t0
+    Ds = [native.to_dev(s.D) for s in systems]
+    log(f"c4f-batch: {[s.name for s in systems]}: {len(fb.launches)} launches, "
+        f"quartets {fb.shell_quartets():.3e}, plan {plan_s:.1f}s; building")
+    t = timed(lambda: fb.build(Ds), a.steps, 1)
+    Gs = [G.cpu().numpy() for G in fb.build(Ds)]
+    torch.cuda.synchronize()
+    log(f"c4f-batch: {1e3 * t / a.steps:.0f} ms per batched build; checking molecule {a.molecules[0]}")
+    s0 = systems[0]
+    G0 = FockBuilder(s0.basis, s0.D, tau=1e-12).build(Ds[0]).cpu().numpy()
+    out = {
+        "metric": "batched Fock builds/s of chosen C4 molecules with their f shells",
+        "value": a.steps / t, "unit": "batched builds/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": 1, "ms_per_step": 1e3 * t / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c4-molecules-with-f-batched", "molecules": a.molecules,
+                   "atoms": [s.basis.n_sh // 5 for s in systems], "tau": 1e-12,
+                   "n_streams": 1, "graph": False},
+        "n_ao": [s.basis.n_ao for s in systems], "launches": len(fb.launches),
+        "f_class_launches": sum(1 for L in fb.launches if 3 in L.cls),
+        "quartets_per_build": fb.shell_quartets(), "plan_seconds": plan_s,
+        "max_dG_first_vs_alone": float(np.abs(Gs[0] - G0).max()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,11 +443,12 @@ def main() -> None:
     ap.add_argument("--no-extra", action="store_true", help="skip the c3 / c5 blocks")
     ap.add_argument("--molecules", type=int, nargs="+", default=[7],
                     help="c4f: which C4 molecules (index i: seed 4 + i)")
-    ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f", "c4f-batch"), default="c2",
                     help="c4d: the 64-molecule batch (8-80 atoms, seeds 4+i) with s,s,p,d shells "
                          "(with its f shells one build would take hours on the loop-form f "
                          "kernel); cof: CO/cc-pVTZ through f (the loop-form kernels); c4f: "
-                         "single C4 molecules with and without their f shells")
+                         "single C4 molecules with and without their f shells; c4f-batch: "
+                         "chosen C4 molecules with f in one batch builder")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for HBM counter runs: one elementwise copy of 512 MiB (the counters' "
                          "scale) and one eager build of the default plan, then exit")
@@ -449,6 +488,9 @@ def main() -> None:
         return
     if a.workload == "c4f":
         c4f_block(a)
+        return
+    if a.workload == "c4f-batch":
+        c4f_batch_block(a)
         return
 
     selector = load_selector(a.selector)
