@@ -528,7 +528,7 @@ def main() -> None:
 
     achieved = Lp.flops / (avg_ms * 1e-3) / 1e12
     roofline = {
-        "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
         "kernel": f"fj_f_{''.join(map(str, Lp.cls))}_k<double>",
         "note": ("compute-bound on FP64 VALU (no MFMA in this kernel); 78.6 TFLOP/s is MI355X's FP64 "

@@ -31,7 +31,13 @@ struct FjFArgs {
     int n_bx;              // work items: n_items = n_bx * (grid y of a walk launch)
     int n_items;
     int n_fp64;            // a mixed launch: items [0, n_fp64) FP64, the rest FP32
+    const int* tier;       // a mixed launch: per item 0 FP64, 1 FP32 (overrides n_fp64)
 };
+
+// The tier of work item i of a mixed launch.
+__device__ __forceinline__ bool fj_item_fp64(const FjFArgs& a, int i) {
+    return a.tier ? a.tier[i] == 0 : i < a.n_fp64;
+}
 
 // Workgroups of a workspace class: at most FJ_WS_MAX_BLOCKS in one launch;
 // the upstream-ABI entries (no workspace argument) use a library-owned one

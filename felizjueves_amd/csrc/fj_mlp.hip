@@ -219,3 +219,114 @@ extern "C" int fj_mlp_round(hipStream_t s, int np, const float* P, uint16_t* Wb)
     hipLaunchKernelGGL(fj_mlp_round_k, dim3((np + 255) / 256), dim3(256), 0, s, np, P, Wb);
     return (int)hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Per-build tier choice of a mixed launch, on the device, ahead of the ERI
+// launch that reads it: one wavefront per work item. The lanes take the
+// item's kets, evaluate the kernel's own six-block density screen with the
+// CURRENT density block maxima and keep, over the kets that pass it, the
+// largest bound (or the bound rule's amplified error estimate) and the
+// largest six-block weight. The bound rule sends the item to FP32 when that
+// estimate is below the error budget. Otherwise, with the learned selector,
+// the item's density-free features (standardised once at plan time) get the
+// density feature log10(max weight) and the network runs in the same
+// wavefront, one hidden unit per lane, with the fj_mlp_eval arithmetic in
+// the same order: FP32 iff the predicted log10 error plus the margin is
+// below log10(budget) and the features lie in the training range.
+__device__ __forceinline__ double fj_wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ void __launch_bounds__(256) fj_tier_select_k(FjSide bra, FjSide ket, const FjSys* sys,
+                                                        FjTierSel t, FjMlpW W) {
+    extern __shared__ float smem[];
+    const bool net = t.feat != nullptr;
+    const int np = net ? fj_mlp_nparam(t.n_feat) : 0;
+    float* sW = smem;
+    float* hx = smem + np + (threadIdx.x >> 6) * (2 * FJ_MLP_H + FJ_MLP_FMAX);  // per wavefront
+    if (net) fj_mlp_stage(t.n_feat, W, sW);   // ends with a barrier
+    const int lane = threadIdx.x & 63;
+    const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+    for (int it = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); it < t.n_items;
+         it += waves) {
+        const int4 w = reinterpret_cast<const int4*>(t.work)[it];
+        const FjSys S = sys[w.z & 0xffff];
+        const int b = w.x, sa = bra.sh[2 * b], sb = bra.sh[2 * b + 1], ns = S.n_sh;
+        const double Gb = bra.G[b];
+        const double amp_b = t.amp_bra[b];
+        const double dab = 2.0 * S.Dsh[sa * ns + sb];
+        double est = -1.0, dm_max = 0.0;
+        for (int e = w.y + lane; e < w.w; e += 64) {
+            const int k = t.ket_list[e] & 0x1ffff;
+            if (t.diag && k < b) continue;
+            const int sc = ket.sh[2 * k], sd = ket.sh[2 * k + 1];
+            double dm = fmax(dab, 2.0 * S.Dsh[sc * ns + sd]);
+            dm = fmax(dm, 0.5 * S.Dsh[sa * ns + sc]);
+            dm = fmax(dm, 0.5 * S.Dsh[sa * ns + sd]);
+            dm = fmax(dm, 0.5 * S.Dsh[sb * ns + sc]);
+            dm = fmax(dm, 0.5 * S.Dsh[sb * ns + sd]);
+            const double bound = Gb * (ket.G[k] * dm);
+            if (bound < t.tau) continue;   // the ERI kernel's screen skips this quartet
+            const double e_k = t.fp32_bound >= 0.0 ? bound : bound * amp_b * t.amp_ket[k] * t.est_scale;
+            est = fmax(est, e_k);
+            dm_max = fmax(dm_max, dm);
+        }
+        est = fj_wave_max(est);
+        dm_max = fj_wave_max(dm_max);
+        int fp32 = 0;
+        if (est >= 0.0) {   // some quartet of the item passes the screen
+            fp32 = t.fp32_bound >= 0.0 ? (est < t.fp32_bound) : (est < t.fp32_error);
+            if (!fp32 && net && t.inside[it]) {
+                const double xd = log10(dm_max);
+                if (xd >= t.d_lo && xd <= t.d_hi) {
+                    const int F = t.n_feat;
+                    float* x = hx + 2 * FJ_MLP_H;
+                    if (lane < F - 1) x[lane] = t.feat[(size_t)it * F + lane];
+                    if (lane == 0) x[F - 1] = (float)((xd - t.d_mean) / t.d_std);
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    const float* W1 = sW;
+                    const float* b1 = W1 + FJ_MLP_H * F;
+                    const float* W2 = b1 + FJ_MLP_H;
+                    const float* b2 = W2 + FJ_MLP_H * FJ_MLP_H;
+                    const float* w3 = b2 + FJ_MLP_H;
+                    float a = b1[lane];
+                    for (int k = 0; k < F; ++k) a = fmaf(W1[lane * F + k], x[k], a);
+                    hx[lane] = fmaxf(a, 0.0f);
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    a = b2[lane];
+#pragma unroll 8
+                    for (int k = 0; k < FJ_MLP_H; ++k) a = fmaf(W2[lane * FJ_MLP_H + k], hx[k], a);
+                    hx[FJ_MLP_H + lane] = fmaxf(a, 0.0f);
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    float y = w3[FJ_MLP_H];
+                    for (int j = 0; j < FJ_MLP_H; ++j) y = fmaf(w3[j], hx[FJ_MLP_H + j], y);
+                    fp32 = (double)y + t.margin < t.log_budget;
+                    __builtin_amdgcn_wave_barrier();   // hx is reused by the next item
+                }
+            }
+        }
+        if (lane == 0) t.tier[it] = fp32;
+    }
+}
+
+extern "C" int fj_tier_select(hipStream_t s, const FjSide* bra, const FjSide* ket, const FjSys* sys,
+                              const FjTierSel* sel) {
+    if (!bra || !ket || !sys || !sel || !sel->work || !sel->ket_list || !sel->tier ||
+        !sel->amp_bra || !sel->amp_ket || sel->n_items < 0)
+        return FJ_ERR_ARG;
+    if (sel->feat && (sel->n_feat < 2 || sel->n_feat > FJ_MLP_FMAX || !sel->inside || !sel->Wb ||
+                      !sel->P || !(sel->d_std > 0.0)))
+        return FJ_ERR_ARG;
+    if (sel->n_items == 0) return 0;
+    FjMlpW W{sel->Wb, sel->P};
+    const int np = sel->feat ? fj_mlp_nparam(sel->n_feat) : 0;
+    const size_t lds = sizeof(float) * (np + 4 * (2 * FJ_MLP_H + FJ_MLP_FMAX));
+    const int blocks = (sel->n_items + 3) / 4 < 1024 ? (sel->n_items + 3) / 4 : 1024;
+    hipLaunchKernelGGL(fj_tier_select_k, dim3(blocks), dim3(256), lds, s, *bra, *ket, sys, *sel, W);
+    return (int)hipGetLastError();
+}

@@ -30,6 +30,7 @@ struct Tier {
     std::vector<long long> cost;
     std::vector<int> kets;
     double n_quartets = 0, n_prim = 0, flops = 0;
+    long long n_cold = 0;
 };
 
 inline double dsh(const double* const* Dsh, const int* n_sh, int m, int i, int j) {
@@ -54,7 +55,7 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
         const int b0 = (int)((long long)bra->n * ci / n_chunk);
         const int b1 = (int)((long long)bra->n * (ci + 1) / n_chunk);
         Tier* tier = &part[2 * (size_t)ci];
-        std::vector<int> kl[2];
+        std::vector<int> kl[3];
         std::vector<long long> cnt;
         std::vector<int> order, lk, llo;
         // stable sort of the lanes by primitive count, descending
@@ -81,8 +82,13 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
             const double dab = 2.0 * dsh(Dsh, n_sh, m, sa, sb);
             kl[0].clear();
             kl[1].clear();
+            kl[2].clear();
             const int k_first = diag ? std::max(lo, b) : lo;
             for (int k = k_first; k < hi; ++k) {
+                // the density-free screen decides which quartets the plan
+                // covers; the planning density only sorts them (cold: the
+                // ones its six-block screen drops)
+                if (!(Gb * ket->G[k] >= tau)) continue;
                 const int sc = ket->sh[2 * k], sd = ket->sh[2 * k + 1];
                 double dm = dab;
                 dm = std::max(dm, 2.0 * dsh(Dsh, n_sh, m, sc, sd));
@@ -91,7 +97,10 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
                 dm = std::max(dm, 0.5 * dsh(Dsh, n_sh, m, sb, sc));
                 dm = std::max(dm, 0.5 * dsh(Dsh, n_sh, m, sb, sd));
                 const double bound = Gb * (ket->G[k] * dm);
-                if (!(bound >= tau)) continue;
+                if (!(bound >= tau)) {
+                    kl[2].push_back(k);
+                    continue;
+                }
                 int t = 0;
                 if (mixed) {
                     if (fp32_bound >= 0.0) {
@@ -177,6 +186,23 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
                     }
                 }
             }
+            // the cold kets: FP64 items of whole kets and the whole bra, no
+            // cost (listed last) and no counted work
+            if (!kl[2].empty() && lane_flops > 0.0) {
+                Tier& T = tier[0];
+                const long long st = (long long)T.kets.size();
+                const size_t n = kl[2].size();
+                for (int k : kl[2]) T.kets.push_back(k);
+                for (size_t t0 = 0; t0 < n; t0 += threads) {
+                    const size_t t1 = std::min(n, t0 + (size_t)threads);
+                    T.work.push_back(b);
+                    T.work.push_back((int)(st + t0));
+                    T.work.push_back(m);
+                    T.work.push_back((int)(st + t1));
+                    T.cost.push_back(-1);   // below every counted item: listed last
+                    ++T.n_cold;
+                }
+            }
         }
     };
     std::atomic<int> next{0};
@@ -205,6 +231,7 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
             dst.n_quartets += src.n_quartets;
             dst.n_prim += src.n_prim;
             dst.flops += src.flops;
+            dst.n_cold += src.n_cold;
         }
     }
     for (int t = 0; t < 2; ++t) {
@@ -215,6 +242,7 @@ extern "C" int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const
         const long long n = (long long)T.cost.size();
         out->n_work[t] = n;
         out->n_kets[t] = (long long)T.kets.size();
+        if (t == 0) out->n_cold = T.n_cold;
         if (lane_flops <= 0.0 || n == 0) continue;
         // costliest work items first (stable)
         std::vector<long long> idx(n);

@@ -9,18 +9,23 @@ Host planning, once per basis (and density screen):
   array, no degree-uniform batching (so a class pair is one launch, not one
   per (degree, magnitude) bin pair);
 * per class pair (X <= Y) a work list of tiles (bra pair, first ket): one
-  workgroup each, ``threads`` consecutive kets; tiles whose every quartet
-  fails the six-block screen ``G_ab G_cd dmax < tau`` are dropped here, and
-  (mixed mode) a tile whose largest bound is below the FP32 threshold goes
-  to the FP32-tier kernel instead of the FP64 one.
+  workgroup each, ``threads`` consecutive kets. Which quartets the plan
+  covers depends on the pair bounds alone: those with G_ab G_cd < tau are
+  dropped (the upstream pre-launch screen, `eri/fock/launch_f.py:448-466`,
+  per quartet). The density the builder is planned with only orders the
+  rest: kets whose six-block screen G_ab G_cd dmax < tau fails under it are
+  tiled apart (cold items, listed last), and in mixed mode the bound rule
+  (or the learned selector) groups the likely FP32 kets together.
 
-Per build (device only): zero F, the density block maxima, one launch per
-(class pair, tier) on one stream, close F <- F + F^T.
-
-The screen, weights and degeneracy are the upstream ones
-(`eri/fock/kernel_f.py:1-165`), so with the same pruning the result equals
-the upstream pair-stream walk up to the eq. 29 pre-launch screen, which
-only removes quartets with G_ab G_cd < tau.
+Per build (device only): zero F, the density block maxima, (mixed mode)
+the tier of every work item from the CURRENT density (`fj_tier_select`:
+the bound rule and the selector's network), one launch per class pair,
+close F <- F + F^T. The kernels apply the six-block density screen per
+quartet with the current density (`eri/fock/kernel_f.py:150-182`), so a
+plan stays exact for any density: the screen, weights and degeneracy are
+the upstream ones (`eri/fock/kernel_f.py:1-165`), and the result equals the
+upstream pair-stream walk up to its binned pre-launch screen (eq. 29),
+which drops a subset of the quartets with G_ab G_cd < tau.
 """
 
 from __future__ import annotations
@@ -34,11 +39,18 @@ import numpy as np
 from . import native
 from . import pairs as P
 from .basis import Basis
-from .features import side_arrays, tile_features
+from .features import N_FEATURES, items_features, side_arrays, tile_features
 from .hgp import cost
 
 FP64, FP32 = 0, 1
 MIXED = 2   # one launch for both tiers: items [0, n_fp64) FP64, the rest FP32
+# Planning only: kets that pass the density-free screen G_ab G_cd >= tau but
+# not the planning density's six-block screen. They are tiled apart from the
+# rest (whole kets per lane, FP64 work items listed last, counted as no
+# work): under the planning density every lane of such an item fails the
+# kernel's density screen at once, and under a density that has grown
+# there, they compute what the kernel's screen lets through.
+COLD = 3
 
 # The FP32 tier's error rule (the fallback of the learned selector): the
 # FP32 VRR's error is not bounded by the final integrals' Schwarz bound
@@ -107,8 +119,10 @@ class Launch:
     n_quartets: int = 0       # computed (post-screen) contracted quartets
     n_prim: int = 0           # primitive quartets of those
     flops: float = 0.0        # algorithmic FLOPs of those
-    n_fp64: int = 0           # MIXED: work items [0, n_fp64) run FP64, the rest FP32
-    n_quartets32: int = 0     # MIXED: the FP32 items' quartets
+    n_fp64: int = 0           # MIXED: work items [0, n_fp64) run FP64, the rest FP32 (as planned)
+    n_quartets32: int = 0     # MIXED: the FP32 items' quartets (as planned)
+    n_cold: int = 0           # the last n_cold FP64 items hold the cold kets (see COLD)
+    tier_dev: object = None   # MIXED: per-item tiers chosen on the device per build
 
 
 WAVE = 64             # lanes of a wavefront
@@ -217,16 +231,27 @@ def orient_pairs(basis: Basis, pairs: list) -> list:
     return out
 
 
-def merge_tier_launches(launches: list) -> list:
+def merge_tier_launches(launches: list, every: bool = False) -> list:
     """Each class pair's FP64 and FP32 launches as one MIXED launch: the
     FP64 work items (in their order) then the FP32 ones, the FP32 items'
-    ket-list ranges moved past the FP64 list; biggest launches first."""
+    ket-list ranges moved past the FP64 list; biggest launches first.
+    every: a class pair with items of one tier only is a MIXED launch too
+    (so that every launch of a mixed build takes per-build tiers)."""
     by = {}
     for L in launches:
         by.setdefault(L.cls, {})[L.precision] = L
     out = []
     for cls, d in by.items():
-        if FP64 in d and FP32 in d:
+        if every and len(d) == 1:
+            (tier, a), = d.items()
+            M = Launch(cls, a.bra, a.ket, a.diag, MIXED, a.work)
+            M.threads, M.ket_list = a.threads, a.ket_list
+            M.n_fp64 = a.work.shape[0] if tier == FP64 else 0
+            M.n_cold = a.n_cold
+            M.n_quartets, M.n_prim, M.flops = a.n_quartets, a.n_prim, a.flops
+            M.n_quartets32 = a.n_quartets if tier == FP32 else 0
+            out.append(M)
+        elif FP64 in d and FP32 in d:
             a, b = d[FP64], d[FP32]
             wb = b.work.copy()
             off = len(a.ket_list)
@@ -236,6 +261,7 @@ def merge_tier_launches(launches: list) -> list:
             M.threads = a.threads
             M.ket_list = np.concatenate([a.ket_list, b.ket_list]).astype(np.int32)
             M.n_fp64 = a.work.shape[0]
+            M.n_cold = a.n_cold
             M.n_quartets = a.n_quartets + b.n_quartets
             M.n_quartets32 = b.n_quartets
             M.n_prim = a.n_prim + b.n_prim
@@ -362,7 +388,7 @@ class FockBuilder:
         self.planner = "python" if (planner == "python" or (mixed and selector is not None)) else "native"
         self.launches = self._plan() if self.planner == "python" else self._plan_native()
         if self.merge_tiers:
-            self.launches = merge_tier_launches(self.launches)
+            self.launches = merge_tier_launches(self.launches, every=self.mixed)
         self.plan_seconds = time.time() - t0
         self._dev = False
 
@@ -409,12 +435,17 @@ class FockBuilder:
                 tiles, feats = [], []
                 for b in range(bra.n):
                     lo, bound, dmax = self._bounds_range(bra, ket, b)
-                    live = bound >= self.tau
+                    # the density-free screen decides which quartets the
+                    # plan covers; the planning density only sorts them
+                    live = bra.G[b] * ket.G[lo:lo + len(bound)] >= self.tau
                     if diag:
                         live[:max(0, b - lo)] = False
                     kl = np.nonzero(live)[0]
                     if not len(kl):
                         continue
+                    hot = bound[kl] >= self.tau
+                    cold = kl[~hot] + lo
+                    kl = kl[hot]
                     ks = kl + lo
                     bound, dmax = bound[kl], dmax[kl]
                     if self.mixed:
@@ -423,9 +454,10 @@ class FockBuilder:
                         else:
                             est = bound * bra.amp[b] * ket.amp[ks] * FP32_EST_SCALE
                             small = est < self.fp32_error
-                        parts = ((FP64, ks[~small], dmax[~small]), (FP32, ks[small], dmax[small]))
+                        parts = ((FP64, ks[~small], dmax[~small]), (FP32, ks[small], dmax[small]),
+                                 (COLD, cold, None))
                     else:
-                        parts = ((FP64, ks, dmax),)
+                        parts = ((FP64, ks, dmax), (COLD, cold, None))
                     for tier, kt, dm in parts:
                         if not len(kt):
                             continue
@@ -455,9 +487,9 @@ class FockBuilder:
             total = 0.0
             for X, Y in pairs:
                 c = cost((*X, *Y))
-                r = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems, diag=int(X == Y),
-                                  threads=64, lane_flops=0.0, per_prim=c.per_prim,
-                                  per_quartet=c.per_quartet, **common)
+                r, _ = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems, diag=int(X == Y),
+                                     threads=64, lane_flops=0.0, per_prim=c.per_prim,
+                                     per_quartet=c.per_quartet, **common)
                 total += r[0][2][2] + r[1][2][2]
             auto_lf = LANE_FLOPS_MAX if total >= SPLIT_MAX_FLOPS else \
                 float(min(LANE_FLOPS_MAX, max(1.0, total / LANES_TARGET)))
@@ -470,9 +502,9 @@ class FockBuilder:
             c = cost(cls)
             T = self.class_threads.get(cls, self.threads)
             lf = self.class_lane_flops.get(cls, auto_lf)
-            r = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems, diag=int(X == Y),
-                              threads=T, lane_flops=lf, per_prim=c.per_prim,
-                              per_quartet=c.per_quartet, **common)
+            r, n_cold = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems,
+                                      diag=int(X == Y), threads=T, lane_flops=lf,
+                                      per_prim=c.per_prim, per_quartet=c.per_quartet, **common)
             for tier, (w, k, stats) in zip((FP64, FP32), r):
                 if w is None:
                     continue
@@ -480,6 +512,7 @@ class FockBuilder:
                 L.threads = T
                 L.ket_list = k
                 L.n_quartets, L.n_prim, L.flops = stats
+                L.n_cold = n_cold if tier == FP64 else 0
                 out.append(L)
         out.sort(key=lambda L: -L.flops)
         return out
@@ -496,8 +529,9 @@ class FockBuilder:
         for X, Y, tiles, _ in groups:
             c = cost((*X, *Y))
             bra, ket = self.tables[X], self.tables[Y]
-            for b, seg, _t in tiles:
-                total += c.per_prim * bra.deg[b] * ket.deg[seg].sum() + c.per_quartet * len(seg)
+            for b, seg, t in tiles:
+                if t != COLD:
+                    total += c.per_prim * bra.deg[b] * ket.deg[seg].sum() + c.per_quartet * len(seg)
         if total >= SPLIT_MAX_FLOPS:
             return LANE_FLOPS_MAX
         return float(min(LANE_FLOPS_MAX, max(1.0, total / LANES_TARGET)))
@@ -543,12 +577,25 @@ class FockBuilder:
             # size, lanes sorted by primitive count so that each work item's
             # lanes loop the same number of times
             by_bra: dict = {}
+            n_cold = 0
             for b, seg, tr in tiles:
                 by_bra.setdefault((b, tr), []).append(seg)
             for (b, tr), segs in by_bra.items():
                 seg = np.concatenate(segs)
                 kab = int(bra.deg[b])
                 kcd = ket.deg[seg].astype(np.int64)
+                if tr == COLD:
+                    # whole kets, whole bra, no cost: under the planning
+                    # density every lane exits at the kernel's screen
+                    st = n_list[FP64]
+                    lists[FP64].append(seg.astype(np.int64))
+                    n_list[FP64] += len(seg)
+                    m = int(bra.mol[b])
+                    for t0 in range(0, len(seg), T):
+                        work[FP64].append((b, st + t0, m, st + min(len(seg), t0 + T)))
+                        cost_w[FP64].append(-1)   # below every counted item: listed last
+                        n_cold += 1
+                    continue
                 # split the primitive loop so that no lane's share exceeds
                 # lane_flops: bra primitive pairs per work item (cb), then
                 # ket primitive pairs per lane (ck)
@@ -590,6 +637,7 @@ class FockBuilder:
                     L.threads = T
                     L.ket_list = np.concatenate(lists[tier]).astype(np.int32)
                     L.n_quartets, L.n_prim, L.flops = stats[tier]
+                    L.n_cold = n_cold if tier == FP64 else 0
                     out.append(L)
         # biggest launches first: their tails overlap the small ones'
         out.sort(key=lambda L: -L.flops)
@@ -615,9 +663,13 @@ class FockBuilder:
             t.side = native.FjSide(d["prim"].data_ptr(), d["off"].data_ptr(), 0,
                                    d["sep"].data_ptr(), d["G"].data_ptr(), d["sh"].data_ptr(),
                                    d["ao"].data_ptr(), t.n)
+        for t in self.tables.values():
+            t.dev["amp"] = native.to_dev(np.asarray(t.amp, dtype=np.float64))
         for L in self.launches:
             L.work_dev = native.to_dev(L.work)
             L.ket_list_dev = native.to_dev(L.ket_list)
+            if self.mixed and L.precision == MIXED:
+                self._tier_setup(L)
         # per system: shell offsets/sizes, Dsh and F (views of flat buffers)
         self.shell_ao, self.shell_size = [], []
         n_ao = [b.n_ao for b, _ in self.systems]
@@ -636,6 +688,42 @@ class FockBuilder:
         self.F, self.Dsh = self.Fs[0], self.Dshs[0]
         self._sys_for = None
         self._dev = True
+
+    def _tier_setup(self, L: "Launch") -> None:
+        """The device side of a mixed launch's per-build tier choice: the
+        tier array (initially the plan's), and the fj_tier_select record:
+        the bound rule's parameters and, with a learned selector, every work
+        item's density-free features, standardised, and whether they lie in
+        the network's training range."""
+        n = L.work.shape[0]
+        plan = np.zeros(n, dtype=np.int32)
+        plan[L.n_fp64:] = 1
+        L.tier_dev = native.to_dev(plan)
+        sel = native.FjTierSel()
+        sel.work, sel.ket_list = L.work_dev.data_ptr(), L.ket_list_dev.data_ptr()
+        sel.n_items, sel.diag, sel.tau = n, int(L.diag), self.tau
+        sel.amp_bra, sel.amp_ket = L.bra.dev["amp"].data_ptr(), L.ket.dev["amp"].data_ptr()
+        sel.est_scale, sel.fp32_error = FP32_EST_SCALE, self.fp32_error
+        sel.fp32_bound = -1.0 if self.fp32_bound is None else float(self.fp32_bound)
+        keep = [L.tier_dev]
+        ls = self.selector if hasattr(self.selector, "net") else None
+        if ls is not None:
+            net = ls.net
+            X = items_features(L.cls, L.bra.feat, L.ket.feat, L.work, L.ket_list)
+            inside = np.all((X[:, 4:13] >= ls.lo[4:13]) & (X[:, 4:13] <= ls.hi[4:13]), axis=1)
+            inside &= tuple(int(v) for v in L.cls) in ls.classes
+            inside[L.n_fp64 - L.n_cold:L.n_fp64] = False      # cold items
+            feat = native.to_dev(net.normalise(X))
+            ins = native.to_dev(inside.astype(np.uint8))
+            keep += [feat, ins]
+            sel.feat, sel.inside, sel.n_feat = feat.data_ptr(), ins.data_ptr(), N_FEATURES
+            sel.d_mean, sel.d_std = float(net.mean[N_FEATURES - 1]), float(net.std[N_FEATURES - 1])
+            sel.d_lo, sel.d_hi = float(ls.lo[N_FEATURES - 1]), float(ls.hi[N_FEATURES - 1])
+            sel.Wb, sel.P = net.Wb.data_ptr(), net.P.data_ptr()
+            sel.margin, sel.log_budget = float(ls.margin), float(np.log10(ls.budget))
+            keep.append(net)
+        sel.tier = L.tier_dev.data_ptr()
+        L.tier_sel, L.tier_keep = sel, keep
 
     def _sys(self, D_devs):
         """The system table for the device densities D_devs (rebuilt when a
@@ -665,7 +753,8 @@ class FockBuilder:
     def launch(self, L: "Launch", sys, stream=None, ws=None) -> None:
         if L.precision == MIXED:
             native.call_fxm(L.cls, L.threads, L.bra.side, L.ket.side, L.work_dev, L.work.shape[0],
-                            L.n_fp64, sys, L.diag, self.tau, stream, L.ket_list_dev, ws)
+                            L.n_fp64, sys, L.diag, self.tau, stream, L.ket_list_dev, ws,
+                            tier=L.tier_dev)
             return
         native.call_fx(L.cls, L.threads, L.precision, L.bra.side, L.ket.side, L.work_dev,
                        L.work.shape[0], sys, L.diag, self.tau, stream, L.ket_list_dev, ws)
@@ -696,6 +785,9 @@ class FockBuilder:
         so launches too small to fill the chip overlap. Returns the F tensor
         (the list of them for a batch; overwritten by the next build)."""
         import torch
+        if self.mixed and not self.merge_tiers:
+            raise ValueError("a mixed build chooses its tiers per build only with merged tiers "
+                             "(merge_tiers=True); an unmerged mixed plan is for inspection")
         if not self._dev:
             self.to_device()
         D_devs = list(D_dev) if isinstance(D_dev, (list, tuple)) else [D_dev]
@@ -707,6 +799,12 @@ class FockBuilder:
             self._F_flat.zero_()
         for m, D in enumerate(D_devs):
             native.density_blocks(self.shell_ao[m], self.shell_size[m], D, self.Dshs[m], main)
+        if self.mixed:
+            # every mixed launch's tiers, from this density
+            for L in self.launches:
+                if L.tier_dev is not None:
+                    native.tier_select(L.bra.side, L.ket.side, sys, L.tier_sel, main)
+        self._built = True
         if n_streams <= 1:
             ws = self._workspace(0)
             for L in self.launches:
@@ -727,11 +825,13 @@ class FockBuilder:
 
     # -- graph capture --------------------------------------------------------
     def capture(self, D_dev, stream=None, n_streams: int = 1):
-        """Capture one whole build (zero F, density block maxima, every
-        launch, close) into a HIP graph on a side stream; `replay()` then
-        reruns it with one launch. `D_dev` must keep its address: to build
-        for a new density, copy it into `D_dev` first. F is cleared by
-        torch's fill kernel (no memset node in the graph)."""
+        """Capture one whole build (zero F, density block maxima, the
+        per-build tiers of a mixed build, every launch, close) into a HIP
+        graph on a side stream; `replay()` then reruns it with one launch.
+        The graph reads the density from D_dev's address: to build for a new
+        density, copy it into `D_dev` and replay (the plan covers every
+        quartet the pair bounds keep, so any density is built exactly). F
+        is cleared by torch's fill kernel (no memset node in the graph)."""
         import torch
         if not self._dev:
             self.to_device()
@@ -777,10 +877,38 @@ class FockBuilder:
             self._shell_quartets = int(one.n_quartets)
         return self._shell_quartets
 
+    @staticmethod
+    def item_quartets(L: "Launch") -> np.ndarray:
+        """Per work item, the contracted quartets it starts (the lanes that
+        hold a ket's first primitive chunk, in its first bra chunk; cold
+        items count none), so that the counts of a launch sum to its
+        n_quartets."""
+        first = ((L.ket_list.astype(np.int64) >> 17) & 0x7f) == 0
+        cs = np.concatenate([[0], np.cumsum(first)])
+        q = (cs[L.work[:, 3]] - cs[L.work[:, 1]]).astype(np.int64)
+        q[((L.work[:, 2].astype(np.int64) >> 16) & 0xff) != 0] = 0
+        if L.n_cold:
+            end = L.n_fp64 if L.precision == MIXED else L.work.shape[0]
+            q[end - L.n_cold:end] = 0
+        return q
+
     def fp32_share(self) -> float:
-        q = self.n_quartets
-        return sum(L.n_quartets if L.precision == FP32 else L.n_quartets32 if L.precision == MIXED
-                   else 0 for L in self.launches) / q if q else 0.0
+        """The share of the computed quartets in the FP32 tier: with the
+        tiers the device chose in the last build (mixed launches), else as
+        planned."""
+        tot = f32 = 0
+        for L in self.launches:
+            q = self.item_quartets(L)
+            if L.precision == MIXED:
+                if L.tier_dev is not None and getattr(self, "_built", False):
+                    t = L.tier_dev.cpu().numpy()
+                else:
+                    t = (np.arange(len(q)) >= L.n_fp64).astype(np.int32)
+                f32 += int(q[t == 1].sum())
+            elif L.precision == FP32:
+                f32 += int(q.sum())
+            tot += int(q.sum())
+        return f32 / tot if tot else 0.0
 
 
 def energy(D: np.ndarray, G: np.ndarray) -> float:

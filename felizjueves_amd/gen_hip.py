@@ -292,16 +292,17 @@ def emit_class(cls) -> str:
         kernel_body = "    fj_item_" + name + "<R>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);"
         launch_ws = ""
         f_ws = ""
-    # the mixed kernel: the same body, each work item in its tier (items
-    # [0, n_fp64) FP64, the rest FP32; uniform over the workgroup)
+    # the mixed kernel: the same body, each work item in its tier (the
+    # per-item tier array, else items [0, n_fp64) FP64 and the rest FP32;
+    # uniform over the workgroup)
     mixed_body = (kernel_body
                   .replace(f"        fj_item_{name}<R>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n",
-                           f"        if (it % a.n_bx < a.n_fp64)\n"
+                           f"        if (fj_item_fp64(a, it % a.n_bx))\n"
                            f"            fj_item_{name}<double>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n"
                            f"        else\n"
                            f"            fj_item_{name}<float>(a, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n")
                   .replace("    fj_item_" + name + "<R>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);",
-                           "    if ((int)blockIdx.x < a.n_fp64)\n"
+                           "    if (fj_item_fp64(a, (int)blockIdx.x))\n"
                            "        fj_item_" + name + "<double>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);\n"
                            "    else\n"
                            "        fj_item_" + name + "<float>(a, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);"))
@@ -451,11 +452,16 @@ static int fj_launch_{name}(hipStream_t s, int threads, int precision, FjFArgs a
 }}
 
 //@@UPSTREAM@@
-extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
-                        int n_sh, int n_ao, int diag, double tau,
-                        const double* bra_prim,{" const double* AB," if lb else ""} const double* G_ab, const int* bra_sh, const int* bra_ao,
-                        const double* ket_prim,{" const double* CD," if ld else ""} const double* G_cd, const int* ket_sh, const int* ket_ao,
-                        const double* Dsh, const double* D, double* F) {{
+// The upstream-ABI launch with every argument (AB / CD may be NULL when lb /
+// ld is 0) and an optional workspace: ws == NULL takes the library-owned one
+// (allocated on first use), else the persistent grid is cut to the slots ws
+// holds and nothing is allocated.
+int fj_f_any_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+                    int n_sh, int n_ao, int diag, double tau,
+                    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh,
+                    const int* bra_ao, const double* ket_prim, const double* CD, const double* G_cd,
+                    const int* ket_sh, const int* ket_ao, const double* Dsh, const double* D, double* F,
+                    double* ws, long long ws_bytes) {{
     if (k_ab < 1 || k_ab > FJ_MAX_PRIM_PAIRS || k_cd < 1 || k_cd > FJ_MAX_PRIM_PAIRS) return FJ_ERR_DEGREE;
     if (n_bra < 0 || n_ket < 0 || (diag && n_bra != n_ket)) return FJ_ERR_SHAPE;
     FjFArgs a;
@@ -466,10 +472,22 @@ extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int
     a.sys = nullptr;
     a.one = FjSys{{D, Dsh, F, n_ao, n_sh}};
     a.diag = diag; a.tau = tau;
-    a.ws = nullptr; a.ws_doubles = 0;
-    a.n_fp64 = 0;
+    a.ws = ws; a.ws_doubles = ws ? ws_bytes / 8 : 0;
+    a.n_fp64 = 0; a.tier = nullptr;
+    if (!ws) {{
 {f_ws}
+    }}
     return fj_launch_{name}(s, threads, 0, a, n_bra);
+}}
+
+extern "C" int fj_f_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+                        int n_sh, int n_ao, int diag, double tau,
+                        const double* bra_prim,{" const double* AB," if lb else ""} const double* G_ab, const int* bra_sh, const int* bra_ao,
+                        const double* ket_prim,{" const double* CD," if ld else ""} const double* G_cd, const int* ket_sh, const int* ket_ao,
+                        const double* Dsh, const double* D, double* F) {{
+    return fj_f_any_{name}(s, threads, n_bra, n_ket, k_ab, k_cd, n_sh, n_ao, diag, tau, bra_prim,
+                           {"AB" if lb else "nullptr"}, G_ab, bra_sh, bra_ao, ket_prim, {"CD" if ld else "nullptr"}, G_cd, ket_sh, ket_ao,
+                           Dsh, D, F, nullptr, 0);
 }}
 //@@END_UPSTREAM@@
 
@@ -486,14 +504,15 @@ extern "C" int fj_fx_{name}(hipStream_t s, int threads, int precision,
     a.one = FjSys{{nullptr, nullptr, nullptr, 0, 0}};
     a.diag = diag; a.tau = tau;
     a.ws = ws; a.ws_doubles = ws ? ws_bytes / 8 : 0;
-    a.n_fp64 = 0;
+    a.n_fp64 = 0; a.tier = nullptr;
     if (precision != 0 && precision != 1) return FJ_ERR_ARG;
     return fj_launch_{name}(s, threads, precision, a, n_work);
 }}
 
 extern "C" int fj_fxm_{name}(hipStream_t s, int threads, const FjSide* bra, const FjSide* ket,
-                          const int* work, int n_work, int n_fp64, const int* ket_list,
-                          const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes) {{
+                          const int* work, int n_work, int n_fp64, const int* tier,
+                          const int* ket_list, const FjSys* sys, int diag, double tau, double* ws,
+                          long long ws_bytes) {{
     if (!bra || !ket || !work || !sys || ws_bytes < 0) return FJ_ERR_ARG;
     if (n_fp64 < 0 || n_fp64 > n_work) return FJ_ERR_ARG;
     FjFArgs a;
@@ -504,7 +523,7 @@ extern "C" int fj_fxm_{name}(hipStream_t s, int threads, const FjSide* bra, cons
     a.one = FjSys{{nullptr, nullptr, nullptr, 0, 0}};
     a.diag = diag; a.tau = tau;
     a.ws = ws; a.ws_doubles = ws ? ws_bytes / 8 : 0;
-    a.n_fp64 = n_fp64;
+    a.n_fp64 = n_fp64; a.tier = tier;
     return fj_launch_{name}(s, threads, 2, a, n_work);
 }}
 """)
@@ -642,8 +661,8 @@ def header_decls(classes) -> str:
             f"    const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,\n"
             f"    double tau, double* ws, long long ws_bytes);\n"
             f"int fj_fxm_{n}(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,\n"
-            f"    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,\n"
-            f"    int diag, double tau, double* ws, long long ws_bytes);")
+            f"    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,\n"
+            f"    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);")
     return "\n".join(out) + "\n"
 
 
@@ -661,13 +680,16 @@ def emit_loop_entries(classes) -> str:
         call = f"fj_floop_launch(s, {la}, {lb}, {lc}, {ld}, threads, a"
         if canonical(cls):
             L += [
-                f'extern "C" int fj_f_{n}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,',
-                "                        int n_sh, int n_ao, int diag, double tau,",
-                f"                        const double* bra_prim,{' const double* AB,' if lb else ''}"
-                " const double* G_ab, const int* bra_sh, const int* bra_ao,",
-                f"                        const double* ket_prim,{' const double* CD,' if ld else ''}"
-                " const double* G_cd, const int* ket_sh, const int* ket_ao,",
-                "                        const double* Dsh, const double* D, double* F) {",
+                "// the upstream-ABI launch with every argument; the loop-form kernel",
+                "// uses no workspace, so ws is ignored",
+                f"int fj_f_any_{n}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,",
+                "                    int n_sh, int n_ao, int diag, double tau,",
+                "                    const double* bra_prim, const double* AB, const double* G_ab,",
+                "                    const int* bra_sh, const int* bra_ao, const double* ket_prim,",
+                "                    const double* CD, const double* G_cd, const int* ket_sh,",
+                "                    const int* ket_ao, const double* Dsh, const double* D, double* F,",
+                "                    double* ws, long long ws_bytes) {",
+                "    (void)ws; (void)ws_bytes;",
                 "    if (k_ab < 1 || k_ab > FJ_MAX_PRIM_PAIRS || k_cd < 1 || k_cd > FJ_MAX_PRIM_PAIRS) return FJ_ERR_DEGREE;",
                 "    if (n_bra < 0 || n_ket < 0 || (diag && n_bra != n_ket)) return FJ_ERR_SHAPE;",
                 "    FjFArgs a;",
@@ -676,8 +698,19 @@ def emit_loop_entries(classes) -> str:
                 "    a.work = nullptr; a.ket_list = nullptr; a.sys = nullptr;",
                 "    a.one = FjSys{D, Dsh, F, n_ao, n_sh};",
                 "    a.diag = diag; a.tau = tau;",
-                "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0;",
-                f"    return {call}, n_bra);", "}", ""]
+                "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0; a.tier = nullptr;",
+                f"    return {call}, n_bra);", "}", "",
+                f'extern "C" int fj_f_{n}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,',
+                "                        int n_sh, int n_ao, int diag, double tau,",
+                f"                        const double* bra_prim,{' const double* AB,' if lb else ''}"
+                " const double* G_ab, const int* bra_sh, const int* bra_ao,",
+                f"                        const double* ket_prim,{' const double* CD,' if ld else ''}"
+                " const double* G_cd, const int* ket_sh, const int* ket_ao,",
+                "                        const double* Dsh, const double* D, double* F) {",
+                f"    return fj_f_any_{n}(s, threads, n_bra, n_ket, k_ab, k_cd, n_sh, n_ao, diag, tau, bra_prim,",
+                f"                        {'AB' if lb else 'nullptr'}, G_ab, bra_sh, bra_ao, ket_prim,"
+                f" {'CD' if ld else 'nullptr'}, G_cd, ket_sh, ket_ao,",
+                "                        Dsh, D, F, nullptr, 0);", "}", ""]
         L += [
             f'extern "C" int fj_fx_{n}(hipStream_t s, int threads, int precision,',
             "                         const FjSide* bra, const FjSide* ket, const int* work, int n_work,",
@@ -691,12 +724,13 @@ def emit_loop_entries(classes) -> str:
             "    a.ket_list = ket_list; a.sys = sys;",
             "    a.one = FjSys{nullptr, nullptr, nullptr, 0, 0};",
             "    a.diag = diag; a.tau = tau;",
-            "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0;",
+            "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0; a.tier = nullptr;",
             "    // the loop-form kernel computes in FP64 at either precision",
             f"    return {call}, n_work);", "}", "",
             f'extern "C" int fj_fxm_{n}(hipStream_t s, int threads, const FjSide* bra, const FjSide* ket,',
-            "                          const int* work, int n_work, int n_fp64, const int* ket_list,",
-            "                          const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes) {",
+            "                          const int* work, int n_work, int n_fp64, const int* tier,",
+            "                          const int* ket_list, const FjSys* sys, int diag, double tau,",
+            "                          double* ws, long long ws_bytes) {",
             "    if (!bra || !ket || !work || !sys || ws_bytes < 0) return FJ_ERR_ARG;",
             "    if (n_fp64 < 0 || n_fp64 > n_work) return FJ_ERR_ARG;",
             "    FjFArgs a;",
@@ -705,8 +739,32 @@ def emit_loop_entries(classes) -> str:
             "    a.ket_list = ket_list; a.sys = sys;",
             "    a.one = FjSys{nullptr, nullptr, nullptr, 0, 0};",
             "    a.diag = diag; a.tau = tau;",
-            "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = n_fp64;",
+            "    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = n_fp64; a.tier = tier;",
             f"    return {call}, n_work);", "}", ""]
+    return "\n".join(L)
+
+
+def emit_any_table(classes) -> str:
+    """gen/f_any_table.h: the declarations of fj_f_any_<cls> of the built
+    canonical classes and fj_f_any_for(la, lb, lc, ld), their dispatch."""
+    sig = ("(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_sh, "
+           "int n_ao, int diag, double tau, const double* bra_prim, const double* AB, "
+           "const double* G_ab, const int* bra_sh, const int* bra_ao, const double* ket_prim, "
+           "const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao, "
+           "const double* Dsh, const double* D, double* F, double* ws, long long ws_bytes)")
+    can = [c for c in classes if canonical(c)]
+    L = ["// generated by felizjueves_amd/gen_hip.py -- the upstream-ABI launches by class",
+         "#pragma once", '#include "../fj_device.h"', "",
+         f"typedef int (*fj_f_any_t){sig};"]
+    L += [f"int fj_f_any_{cls_name(c)}{sig};" for c in can]
+    L += ["", "static inline fj_f_any_t fj_f_any_for(int la, int lb, int lc, int ld) {",
+          "    if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > 3 || lb > 3 || lc > 3 || ld > 3)",
+          "        return nullptr;",
+          "    switch (((la * 4 + lb) * 4 + lc) * 4 + ld) {"]
+    for c in can:
+        la, lb, lc, ld = c
+        L.append(f"        case {((la * 4 + lb) * 4 + lc) * 4 + ld}: return fj_f_any_{cls_name(c)};")
+    L += ["        default: return nullptr;", "    }", "}", ""]
     return "\n".join(L)
 
 

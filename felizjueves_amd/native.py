@@ -144,17 +144,38 @@ def call_fx(cls, threads: int, precision: int, bra: FjSide, ket: FjSide, work, n
 
 
 def call_fxm(cls, threads: int, bra: FjSide, ket: FjSide, work, n_work: int, n_fp64: int,
-             sys, diag: int, tau: float, stream=None, ket_list=None, ws=None) -> None:
-    """One mixed-tier launch of fj_fxm_<cls>: work items [0, n_fp64) run in
-    FP64, the rest in FP32 (otherwise as call_fx)."""
+             sys, diag: int, tau: float, stream=None, ket_list=None, ws=None, tier=None) -> None:
+    """One mixed-tier launch of fj_fxm_<cls>: with `tier` (a device int32
+    array, 0 FP64 / 1 FP32 per work item) each item in its tier, else items
+    [0, n_fp64) in FP64 and the rest in FP32 (otherwise as call_fx)."""
     fn = getattr(lib(), "fj_fxm_" + "".join(map(str, cls)))
     fn.restype = C.c_int
     ws_bytes = 0 if ws is None else ws.numel() * ws.element_size()
     rc = fn(C.c_void_p(stream_handle(stream)), C.c_int(threads), C.byref(bra), C.byref(ket),
-            C.c_void_p(ptr(work)), C.c_int(n_work), C.c_int(n_fp64), C.c_void_p(ptr(ket_list)),
-            C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau), C.c_void_p(ptr(ws)),
-            C.c_longlong(ws_bytes))
+            C.c_void_p(ptr(work)), C.c_int(n_work), C.c_int(n_fp64), C.c_void_p(ptr(tier)),
+            C.c_void_p(ptr(ket_list)), C.c_void_p(ptr(sys)), C.c_int(diag), C.c_double(tau),
+            C.c_void_p(ptr(ws)), C.c_longlong(ws_bytes))
     check(rc, fn.__name__)
+
+
+class FjTierSel(C.Structure):
+    """Mirror of ``FjTierSel`` in include/fj_mi355x.h (device pointers)."""
+
+    _fields_ = [("work", C.c_void_p), ("ket_list", C.c_void_p), ("n_items", C.c_int),
+                ("diag", C.c_int), ("tau", C.c_double), ("amp_bra", C.c_void_p),
+                ("amp_ket", C.c_void_p), ("est_scale", C.c_double), ("fp32_error", C.c_double),
+                ("fp32_bound", C.c_double), ("feat", C.c_void_p), ("inside", C.c_void_p),
+                ("n_feat", C.c_int), ("d_mean", C.c_double), ("d_std", C.c_double),
+                ("d_lo", C.c_double), ("d_hi", C.c_double), ("Wb", C.c_void_p), ("P", C.c_void_p),
+                ("margin", C.c_double), ("log_budget", C.c_double), ("tier", C.c_void_p)]
+
+
+def tier_select(bra: FjSide, ket: FjSide, sys, sel: FjTierSel, stream=None) -> None:
+    """The per-build tiers of one mixed launch, on the device (fj_tier_select)."""
+    fn = lib().fj_tier_select
+    fn.restype = C.c_int
+    check(fn(C.c_void_p(stream_handle(stream)), C.byref(bra), C.byref(ket), C.c_void_p(ptr(sys)),
+             C.byref(sel)), "fj_tier_select")
 
 
 def workspace_bytes(cls, threads: int, n_work: int) -> int:

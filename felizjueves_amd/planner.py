@@ -20,7 +20,7 @@ class FjPlanResult(C.Structure):
     _fields_ = [("work", C.POINTER(C.c_int) * 2), ("n_work", C.c_longlong * 2),
                 ("kets", C.POINTER(C.c_int) * 2), ("n_kets", C.c_longlong * 2),
                 ("n_quartets", C.c_double * 2), ("n_prim", C.c_double * 2),
-                ("flops", C.c_double * 2)]
+                ("flops", C.c_double * 2), ("n_cold", C.c_longlong)]
 
 
 class Side:
@@ -50,7 +50,8 @@ def plan_pairs(bra: Side, ket: Side, ket_table, systems: Systems, *, diag: int, 
                lane_flops: float, per_prim: float, per_quartet: float, max_prim: int):
     """One class pair. lane_flops <= 0: counting only. Returns, per tier,
     (work [n, 4] int32 or None, ket list int32 or None, (n_quartets,
-    n_prim, flops))."""
+    n_prim, flops)), and the number of cold items (the last ones of the
+    FP64 tier's work)."""
     n_mol = len(systems.Dsh)
     lo = np.zeros(n_mol, dtype=np.int32)
     hi = np.zeros(n_mol, dtype=np.int32)
@@ -76,6 +77,7 @@ def plan_pairs(bra: Side, ket: Side, ket_table, systems: Systems, *, diag: int, 
                 out.append((w, k, stats))
             else:
                 out.append((None, None, stats))
+        n_cold = int(res.n_cold)
     finally:
         native.lib().fj_plan_free(C.byref(res))
-    return out
+    return out, n_cold

@@ -117,9 +117,14 @@ long long fj_fx_workspace_bytes(int la, int lb, int lc, int ld, int threads, int
 /* The native planner of one class pair (host code; no upstream counterpart:
  * the upstream plans in Python, eri/fock/launch_f.py:390-470, per
  * (degree, magnitude) bin).  Per bra pair of `bra`, the kets of the same
- * system (ket pairs mol_lo[m] .. mol_hi[m]) that pass the six-block screen
- * G_ab G_cd dmax >= tau (dmax from Dsh[m], n_sh[m] x n_sh[m]; with `diag`,
- * kets k >= b only), split by the FP32 rule when `mixed` (fp32_bound >= 0:
+ * system (ket pairs mol_lo[m] .. mol_hi[m]; with `diag`, kets k >= b only)
+ * that pass the density-free screen G_ab G_cd >= tau (the upstream
+ * pre-launch screen, launch_f.py:448-466, per quartet).  Those that fail
+ * the planning density's six-block screen G_ab G_cd dmax >= tau (dmax from
+ * Dsh[m], n_sh[m] x n_sh[m]) are the cold kets: FP64 work items of whole
+ * kets, listed last (the kernel's own screen drops them while the density
+ * stays as planned; they are not counted as work).  The others are split
+ * by the FP32 rule when `mixed` (fp32_bound >= 0:
  * bound < fp32_bound; else bound amp_b amp_k est_scale < fp32_error), are
  * expanded into lanes (ket pair, chunk of <= lane_flops / (cb per_prim)
  * primitive pairs, equal chunks per ket) sorted by primitive count, cut
@@ -141,6 +146,7 @@ typedef struct FjPlanResult {
     int* kets[2];         /* per tier: ket-list entries */
     long long n_kets[2];
     double n_quartets[2], n_prim[2], flops[2];
+    long long n_cold;     /* the last n_cold FP64 items: the cold kets */
 } FjPlanResult;
 int fj_plan_pairs(const FjPlanSide* bra, const FjPlanSide* ket, const int* mol_lo,
     const int* mol_hi, const double* const* Dsh, const int* n_sh, int diag, double tau, int mixed,
@@ -158,8 +164,8 @@ int fj_fx_0000(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0000(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -169,8 +175,8 @@ int fj_fx_0001(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0001(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -180,8 +186,8 @@ int fj_fx_0002(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0002(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0003(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -191,14 +197,14 @@ int fj_fx_0003(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0003(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0010(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0010(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -208,8 +214,8 @@ int fj_fx_0011(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0011(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -219,8 +225,8 @@ int fj_fx_0012(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0012(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0013(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -230,20 +236,20 @@ int fj_fx_0013(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0013(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0020(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0021(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0021(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -253,8 +259,8 @@ int fj_fx_0022(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0023(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -264,26 +270,26 @@ int fj_fx_0023(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0023(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0033(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -293,8 +299,8 @@ int fj_fx_0033(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -304,8 +310,8 @@ int fj_fx_0101(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0101(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -315,8 +321,8 @@ int fj_fx_0102(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0102(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0103(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -326,14 +332,14 @@ int fj_fx_0103(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0103(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0110(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0110(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -343,8 +349,8 @@ int fj_fx_0111(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0111(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -354,8 +360,8 @@ int fj_fx_0112(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -365,20 +371,20 @@ int fj_fx_0113(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0113(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0120(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0120(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0121(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0121(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -388,8 +394,8 @@ int fj_fx_0122(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -399,26 +405,26 @@ int fj_fx_0123(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0123(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0130(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0130(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -428,8 +434,8 @@ int fj_fx_0133(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -439,8 +445,8 @@ int fj_fx_0202(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0202(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0203(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -450,14 +456,14 @@ int fj_fx_0203(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0203(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0210(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0210(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -467,8 +473,8 @@ int fj_fx_0211(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0211(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -478,8 +484,8 @@ int fj_fx_0212(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -489,20 +495,20 @@ int fj_fx_0213(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0213(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0220(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0220(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0221(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0221(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -512,8 +518,8 @@ int fj_fx_0222(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -523,26 +529,26 @@ int fj_fx_0223(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0223(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0230(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0230(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0231(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0231(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -552,8 +558,8 @@ int fj_fx_0233(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0303(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -563,14 +569,14 @@ int fj_fx_0303(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0303(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0310(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0310(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0311(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -580,8 +586,8 @@ int fj_fx_0311(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0311(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0312(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -591,8 +597,8 @@ int fj_fx_0312(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0312(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -602,20 +608,20 @@ int fj_fx_0313(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0313(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0320(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0320(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0321(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0321(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -625,8 +631,8 @@ int fj_fx_0322(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0322(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -636,26 +642,26 @@ int fj_fx_0323(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0323(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0330(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0330(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0331(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0331(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_0332(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0332(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_0333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -665,80 +671,80 @@ int fj_fx_0333(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_0333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1010(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1010(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1011(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1011(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1012(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1012(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1013(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1013(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1020(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1021(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1021(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1023(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1023(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -748,8 +754,8 @@ int fj_fx_1111(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1111(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -759,8 +765,8 @@ int fj_fx_1112(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1112(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1113(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -770,20 +776,20 @@ int fj_fx_1113(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1113(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1120(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1120(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1121(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1121(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -793,8 +799,8 @@ int fj_fx_1122(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1123(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -804,26 +810,26 @@ int fj_fx_1123(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1123(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1130(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1130(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1133(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -833,8 +839,8 @@ int fj_fx_1133(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -844,8 +850,8 @@ int fj_fx_1212(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1212(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1213(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -855,20 +861,20 @@ int fj_fx_1213(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1213(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1220(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1220(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1221(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1221(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -878,8 +884,8 @@ int fj_fx_1222(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -889,26 +895,26 @@ int fj_fx_1223(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1223(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1230(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1230(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1231(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1231(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -918,8 +924,8 @@ int fj_fx_1233(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1313(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -929,20 +935,20 @@ int fj_fx_1313(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1313(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1320(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1320(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1321(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1321(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1322(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -952,8 +958,8 @@ int fj_fx_1322(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1322(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -963,26 +969,26 @@ int fj_fx_1323(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1323(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1330(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1330(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1331(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1331(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_1332(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1332(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_1333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -992,98 +998,98 @@ int fj_fx_1333(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_1333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2020(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2020(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2021(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2021(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2022(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2022(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2023(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2023(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2121(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2121(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2122(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2122(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2123(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2123(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2130(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2130(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -1093,8 +1099,8 @@ int fj_fx_2222(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2222(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2223(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -1104,26 +1110,26 @@ int fj_fx_2223(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2223(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2230(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2230(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2231(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2231(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2233(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -1133,8 +1139,8 @@ int fj_fx_2233(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2323(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -1144,26 +1150,26 @@ int fj_fx_2323(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2323(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2330(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2330(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2331(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2331(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_2332(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2332(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_2333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -1173,62 +1179,62 @@ int fj_fx_2333(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_2333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3030(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3030(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3031(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3031(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3032(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3032(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3033(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3033(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3131(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3131(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3132(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3132(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3133(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3133(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3232(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3232(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_fx_3233(fj_stream_t stream, int threads, int precision, const FjSide* bra, const FjSide* ket,
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3233(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 int fj_f_3333(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
     int n_sh, int n_ao, int diag, double tau,
     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
@@ -1238,8 +1244,8 @@ int fj_fx_3333(fj_stream_t stream, int threads, int precision, const FjSide* bra
     const int* work, int n_work, const int* ket_list, const FjSys* sys, int diag,
     double tau, double* ws, long long ws_bytes);
 int fj_fxm_3333(fj_stream_t stream, int threads, const FjSide* bra, const FjSide* ket,
-    const int* work, int n_work, int n_fp64, const int* ket_list, const FjSys* sys,
-    int diag, double tau, double* ws, long long ws_bytes);
+    const int* work, int n_work, int n_fp64, const int* tier, const int* ket_list,
+    const FjSys* sys, int diag, double tau, double* ws, long long ws_bytes);
 /* END F CLASSES */
 
 /* The MMD families, one entry per class, each with the upstream kernel's
@@ -1306,6 +1312,42 @@ int fj_mlp_adam(fj_stream_t stream, int np, float* P, const float* G, float* M, 
                 unsigned short* Wb, float lr, float beta1, float beta2, float eps, int t);
 /* Wb <- BF16(P), round to nearest even. */
 int fj_mlp_round(fj_stream_t stream, int np, const float* P, unsigned short* Wb);
+
+/* Per-build precision tiers of one mixed launch (fj_fxm_<cls> with a `tier`
+ * array), chosen on the device from the CURRENT density block maxima, so a
+ * plan made with one density stays correct for another (no upstream
+ * counterpart: the learned selector's inference, fused ahead of the ERI
+ * launch).  One wavefront per work item: over the item's kets that pass the
+ * ERI kernel's six-block screen (G_ab G_cd dmax >= tau), the largest bound
+ * (fp32_bound >= 0: FP32 iff it is < fp32_bound) or bound amp_ab amp_cd
+ * est_scale (FP32 iff < fp32_error).  With `feat` (the learned selector):
+ * an item the rule keeps in FP64 goes to FP32 iff inside[item], the density
+ * feature log10(max dmax) lies in [d_lo, d_hi], and the network's
+ * prediction (fj_mlp_forward's arithmetic) on feat[item] (n_feat floats,
+ * standardised; the last is replaced by (log10(max dmax) - d_mean) / d_std)
+ * plus margin is below log_budget.  tier[item] = 1 for FP32, 0 for FP64 (an
+ * item none of whose quartets passes the screen: 0).  `bra`, `ket`, `sel`
+ * are HOST pointers to records of device pointers; `sys` as fj_fx_. */
+typedef struct FjTierSel {
+    const int* work;              /* [n_items][4] the launch's work items */
+    const int* ket_list;          /* the launch's ket list */
+    int n_items;
+    int diag;
+    double tau;
+    const double* amp_bra;        /* [bra.n] FP32 error amplification per pair */
+    const double* amp_ket;        /* [ket.n] */
+    double est_scale, fp32_error, fp32_bound;
+    const float* feat;            /* NULL: the bound rule alone */
+    const unsigned char* inside;  /* [n_items] */
+    int n_feat;
+    double d_mean, d_std, d_lo, d_hi;
+    const unsigned short* Wb;     /* the network, as fj_mlp_forward */
+    const float* P;
+    double margin, log_budget;
+    int* tier;                    /* out: [n_items] */
+} FjTierSel;
+int fj_tier_select(fj_stream_t stream, const FjSide* bra, const FjSide* ket, const FjSys* sys,
+                   const FjTierSel* sel);
 
 /* BEGIN MMD CLASSES (written by felizjueves_amd/gen_mmd.py) */
 int fj_schwarz_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,

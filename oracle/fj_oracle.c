@@ -465,6 +465,9 @@ void oracle_eri_hgp(int sa, int sb, int sc, int sd, const int *shell_l,
  * seams.py:424-737): per bra pair and per ket pair k with k >= diag*bra
  * (the canonical half), dmax = max(2 Dsh_ab, 2 Dsh_cd, 1/2 Dsh_ac,
  * 1/2 Dsh_ad, 1/2 Dsh_bc, 1/2 Dsh_bd); skip when G_ab G_cd dmax < tau;
+ * with `pre` set, also skip when G_ab G_cd < tau (the density-free
+ * pre-launch screen of launch_f.py:448-466, applied per quartet, as the
+ * builder's plan applies it);
  * otherwise the K_CD x K_AB nest of HGP VRR, the HRR on the contracted
  * boundary, and the six-block digestion with weights (2, -1/2) times
  * deg = (1/2)^{[a=b]+[c=d]+[P=Q]} (kernel_f.py:439-487), ADDED into F
@@ -472,7 +475,7 @@ void oracle_eri_hgp(int sa, int sb, int sc, int sd, const int *shell_l,
  * Rows of bra_prim / ket_prim: (Px,Py,Pz, Xx,Xy,Xz, zeta, U).               */
 void oracle_fx_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
                       int k_ab, int k_cd, const int *bra_off, const int *ket_off,
-                      int n_sh, int n_ao, int diag, double tau,
+                      int n_sh, int n_ao, int diag, double tau, int pre,
                       const double *bra_prim, const double *AB, const double *G_ab,
                       const int *bra_sh, const int *bra_ao, const double *ket_prim,
                       const double *CD, const double *G_cd, const int *ket_sh,
@@ -510,6 +513,7 @@ void oracle_fx_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
                 if ((t = 0.5 * Dsh[sa * n_sh + sd]) > dm) dm = t;
                 if ((t = 0.5 * Dsh[sb * n_sh + sc]) > dm) dm = t;
                 if ((t = 0.5 * Dsh[sb * n_sh + sd]) > dm) dm = t;
+                if (pre && G_ab[bra] * G_cd[k] < tau) continue;
                 if (G_ab[bra] * (G_cd[k] * dm) < tau) continue;
                 memset(C, 0, sizeof(double) * ne * nf);
                 int kc0 = ket_off ? ket_off[k] : k * k_cd;
@@ -562,7 +566,8 @@ void oracle_fx_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
  * MMD has no HRR, so it keeps full FP64 accuracy where the HGP transfer
  * cancels (a very tight primitive paired with a distant diffuse shell). */
 void oracle_fx_launch_mmd(int la, int lb, int lc, int ld, int n_bra, int n_ket, int n_sh,
-                          int n_ao, int diag, double tau, const double *G_ab, const int *bra_sh,
+                          int n_ao, int diag, double tau, int pre, const double *G_ab,
+                          const int *bra_sh,
                           const int *bra_ao, const double *G_cd, const int *ket_sh,
                           const int *ket_ao, const double *Dsh, const double *D,
                           const int *shell_l, const double *centre, const int *nprim,
@@ -597,6 +602,7 @@ void oracle_fx_launch_mmd(int la, int lb, int lc, int ld, int n_bra, int n_ket, 
                 if ((t = 0.5 * Dsh[sa * n_sh + sd]) > dm) dm = t;
                 if ((t = 0.5 * Dsh[sb * n_sh + sc]) > dm) dm = t;
                 if ((t = 0.5 * Dsh[sb * n_sh + sd]) > dm) dm = t;
+                if (pre && G_ab[bra] * G_cd[k] < tau) continue;
                 if (G_ab[bra] * (G_cd[k] * dm) < tau) continue;
                 shell_t Cs = mkshell(sc, shell_l, centre, nprim, pstart, exps, coefs);
                 shell_t Ds = mkshell(sd, shell_l, centre, nprim, pstart, exps, coefs);
@@ -636,7 +642,7 @@ void oracle_f_launch(int la, int lb, int lc, int ld, int n_bra, int n_ket,
                      const double *CD, const double *G_cd, const int *ket_sh,
                      const int *ket_ao, const double *Dsh, const double *D, double *F) {
     oracle_fx_launch(la, lb, lc, ld, n_bra, n_ket, k_ab, k_cd, NULL, NULL, n_sh, n_ao, diag,
-                     tau, bra_prim, AB, G_ab, bra_sh, bra_ao, ket_prim, CD, G_cd, ket_sh,
+                     tau, 0, bra_prim, AB, G_ab, bra_sh, bra_ao, ket_prim, CD, G_cd, ket_sh,
                      ket_ao, Dsh, D, F);
 }
 

@@ -45,8 +45,8 @@ def lib():
         L.oracle_f_launch.argtypes = ([C.c_int] * 11 + [C.c_double] +
                                       [vp] * 13)
         L.oracle_fx_launch.argtypes = ([C.c_int] * 8 + [vp] * 2 + [C.c_int] * 3 +
-                                       [C.c_double] + [vp] * 13)
-        L.oracle_fx_launch_mmd.argtypes = [C.c_int] * 9 + [C.c_double] + [vp] * 15
+                                       [C.c_double, C.c_int] + [vp] * 13)
+        L.oracle_fx_launch_mmd.argtypes = [C.c_int] * 9 + [C.c_double, C.c_int] + [vp] * 15
         _lib = L
     return _lib
 
@@ -103,20 +103,23 @@ def f_launch(cls, args: dict, F: np.ndarray) -> None:
 
 
 def fx_launch(cls, bra: dict, ket: dict, diag: int, n_sh: int, n_ao: int, tau: float,
-              Dsh, D, F) -> None:
+              Dsh, D, F, pre: bool = False) -> None:
     """Ragged-degree launch (pair i's records prim[off[i]:off[i+1]]), adding
-    into F. Each side: prim [n_rec, 8], off [n+1], sep [n, 3], G, sh, ao."""
+    into F. Each side: prim [n_rec, 8], off [n+1], sep [n, 3], G, sh, ao.
+    pre: also skip the quartets with G_ab G_cd < tau (the density-free
+    pre-launch screen, per quartet)."""
     la, lb, lc, ld = cls
     c = lambda a: np.ascontiguousarray(a)  # noqa: E731
     b, k = {x: c(v) for x, v in bra.items()}, {x: c(v) for x, v in ket.items()}
     lib().oracle_fx_launch(
         la, lb, lc, ld, len(b["G"]), len(k["G"]), 0, 0, _ptr(b["off"]), _ptr(k["off"]),
-        n_sh, n_ao, diag, float(tau), _ptr(b["prim"]), _ptr(b["sep"]), _ptr(b["G"]),
+        n_sh, n_ao, diag, float(tau), int(bool(pre)), _ptr(b["prim"]), _ptr(b["sep"]), _ptr(b["G"]),
         _ptr(b["sh"]), _ptr(b["ao"]), _ptr(k["prim"]), _ptr(k["sep"]), _ptr(k["G"]),
         _ptr(k["sh"]), _ptr(k["ao"]), _ptr(c(Dsh)), _ptr(c(D)), _ptr(F))
 
 
-def fx_launch_mmd(cls, bra: dict, ket: dict, diag: int, basis, tau: float, Dsh, D, F) -> None:
+def fx_launch_mmd(cls, bra: dict, ket: dict, diag: int, basis, tau: float, Dsh, D, F,
+                  pre: bool = False) -> None:
     """fx_launch's screen, weights and digestion with every quartet's
     integrals by MMD over the shells' full primitive lists (no HRR: full
     FP64 accuracy for tight-diffuse pairs, where the HGP transfer cancels).
@@ -128,5 +131,5 @@ def fx_launch_mmd(cls, bra: dict, ket: dict, diag: int, basis, tau: float, Dsh, 
     arrs = [c(a) for a in _b(basis)]
     lib().oracle_fx_launch_mmd(
         la, lb, lc, ld, len(b["G"]), len(k["G"]), basis.n_sh, basis.n_ao, diag, tau,
-        _ptr(b["G"]), _ptr(b["sh"]), _ptr(b["ao"]), _ptr(k["G"]), _ptr(k["sh"]), _ptr(k["ao"]),
+        int(bool(pre)), _ptr(b["G"]), _ptr(b["sh"]), _ptr(b["ao"]), _ptr(k["G"]), _ptr(k["sh"]), _ptr(k["ao"]),
         _ptr(Dsh), _ptr(D), *[_ptr(a) for a in arrs], _ptr(F))

@@ -32,7 +32,7 @@ int main(int argc, char** argv) {
     FjFArgs a;
     a.bra = FjSide{H(bra_prim), nullptr, k_ab, HAS_AB ? H(AB) : nullptr, H(G_ab), H(bra_sh), H(bra_ao), n_bra};
     a.ket = FjSide{H(ket_prim), nullptr, k_cd, HAS_CD ? H(CD) : nullptr, H(G_cd), H(ket_sh), H(ket_ao), n_ket};
-    a.work = nullptr; a.ket_list = nullptr; a.sys = nullptr;
+    a.work = nullptr; a.ket_list = nullptr; a.sys = nullptr; a.tier = nullptr;
     a.one = FjSys{H(D), H(Dsh), F, n_ao, n_sh};
     a.diag = diag; a.tau = tau;
     const long long W = WS_DOUBLES;

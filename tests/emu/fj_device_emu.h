@@ -39,7 +39,9 @@ typedef struct ihipStream_t* hipStream_t;
 struct FjFArgs {
     FjSide bra, ket; const int4* work; const int* ket_list; const FjSys* sys; FjSys one;
     int diag; double tau; double* ws; long long ws_doubles; int n_bx; int n_items; int n_fp64;
+    const int* tier;
 };
+inline bool fj_item_fp64(const FjFArgs& a, int i) { return a.tier ? a.tier[i] == 0 : i < a.n_fp64; }
 extern std::barrier<>* g_bar;
 extern double g_shfl[256];
 #define FJ_LAUNDER(p, s) ((void)0)

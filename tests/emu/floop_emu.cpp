@@ -49,7 +49,7 @@ int main(int argc, char** argv) {
         a.work = nullptr; a.ket_list = nullptr; a.sys = nullptr;
         a.one = FjSys{D, Dsh, F, n_ao, h[8]};
         a.diag = h[10]; a.tau = tau;
-        a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0;
+        a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0; a.tier = nullptr;
         a.n_bx = h[4]; a.n_items = h[4] * ((h[5] + span - 1) / span);
     }
     const int threads = 64;

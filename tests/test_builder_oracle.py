@@ -27,7 +27,10 @@ def builder_tables_host(fb: FockBuilder):
 
 def oracle_build(fb: FockBuilder, D: np.ndarray) -> np.ndarray:
     """What the builder's launches compute, on the CPU: every class pair of
-    its tables through the oracle's ragged launch, then F + F^T."""
+    its tables through the oracle's ragged launch, then F + F^T. The
+    quartets are the builder's: the density-free pre-launch screen
+    (G_ab G_cd < tau, launch_f.py:448-466) and the six-block density screen
+    with D (kernel_f.py:150-182)."""
     b = fb.basis
     host = builder_tables_host(fb)
     Dsh = P.density_blocks(b, D)
@@ -35,7 +38,8 @@ def oracle_build(fb: FockBuilder, D: np.ndarray) -> np.ndarray:
     classes = sorted(host)
     for i, X in enumerate(classes):
         for Y in classes[i:]:
-            O.fx_launch((*X, *Y), host[X], host[Y], int(X == Y), b.n_sh, b.n_ao, fb.tau, Dsh, D, F)
+            O.fx_launch((*X, *Y), host[X], host[Y], int(X == Y), b.n_sh, b.n_ao, fb.tau, Dsh, D, F,
+                        pre=True)
     return F + F.T
 
 
@@ -53,7 +57,7 @@ def oracle_build_mmd(fb: FockBuilder, D: np.ndarray) -> np.ndarray:
     classes = sorted(host)
     for i, X in enumerate(classes):
         for Y in classes[i:]:
-            O.fx_launch_mmd((*X, *Y), host[X], host[Y], int(X == Y), b, fb.tau, Dsh, D, F)
+            O.fx_launch_mmd((*X, *Y), host[X], host[Y], int(X == Y), b, fb.tau, Dsh, D, F, pre=True)
     return F + F.T
 
 

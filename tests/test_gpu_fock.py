@@ -146,29 +146,6 @@ def test_builder_all_fp32_tier(which):
     assert err <= 1e-3 * scale
 
 
-def test_mixed_one_launch_per_class_matches_split():
-    """A class pair with tiles in both tiers runs as one launch (fj_fxm_*,
-    each work item in its tier): the same items in the same tiers as the
-    two per-tier launches, so the same G up to the atomics' order."""
-    import torch
-    from felizjueves_amd.fock import MIXED
-    s = config2(seed=11, n_atoms=10)
-    Dd = native.to_dev(s.D)
-    kw = dict(tau=1e-12, mixed=True, fp32_bound=1e-7)
-    merged = FockBuilder(s.basis, s.D, merge_tiers=True, **kw)
-    split = FockBuilder(s.basis, s.D, merge_tiers=False, **kw)
-    assert any(L.precision == MIXED for L in merged.launches)
-    assert len(merged.launches) < len(split.launches)
-    assert merged.fp32_share() == split.fp32_share() > 0.0
-    Gm = merged.build(Dd).clone()
-    Gs = split.build(Dd).clone()
-    torch.cuda.synchronize()
-    err = (Gm - Gs).abs().max().item()
-    _print("merged vs split max|dG|", err, "fp32 share", merged.fp32_share(),
-           "launches", len(merged.launches), len(split.launches))
-    assert err < 1e-12
-
-
 def test_builder_everything_screened_gives_zero():
     """tau above every bound: no launch at all; eager and graph builds give
     an exact zero G (F is still cleared and closed)."""

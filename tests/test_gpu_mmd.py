@@ -98,10 +98,12 @@ def test_k_vs_libcint(name):
 
 
 def test_k_screened_through_f_vs_libcint():
-    """CO/cc-pVTZ with the K screen on (tau = 1e-12: each bra stops at its
-    first ket below the bound, the f classes in the loop form): within the
-    screen's reach of libcint's vk, and equal to the unscreened build where
-    nothing is cut."""
+    """CO/cc-pVTZ with the K screen on (tau = 1e-12: each bra record stops
+    at its first ket record whose bound Ip Iq max|D| is below tau, the f
+    classes in the loop form): within the screen's reach of libcint's vk
+    and of the unscreened build (the skipped primitive quartets each
+    contribute less than tau per Hermite component, and few of them reach
+    any one element)."""
     fx = load_fixture("co_ccpvtz")
     K = _host(jk.ExchangeBuild(fx.basis, tau=1e-12).run(fx.density))
     K0 = _host(jk.ExchangeBuild(fx.basis).run(fx.density))
@@ -109,6 +111,7 @@ def test_k_screened_through_f_vs_libcint():
     err, cut = np.abs(K - ref).max(), np.abs(K - K0).max()
     print("co_ccpvtz screened K: max |K - vk|", err, "vs unscreened", cut, flush=True)
     assert err < 1e-10 * max(1.0, np.abs(ref).max())
+    assert cut < 1e-10 * max(1.0, np.abs(ref).max())
 
 
 def _k_rows(form: str, basis, D):

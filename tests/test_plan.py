@@ -1,7 +1,8 @@
-"""The builder's work lists (CPU): every live quartet's primitive loop is
-covered exactly once by the (work item bra chunk) x (ket-list lane chunk)
-products, for whole tiles, for the automatic split and for a forced fine
-split, in both tiers."""
+"""The builder's work lists (CPU): every quartet the density-free screen
+keeps (G_ab G_cd >= tau) has its primitive loop covered exactly once by the
+(work item bra chunk) x (ket-list lane chunk) products, for whole tiles,
+for the automatic split and for a forced fine split, in both tiers; the
+ones the planning density's screen drops sit in the cold items."""
 
 import numpy as np
 import pytest
@@ -34,19 +35,53 @@ def _decode_cover(fb):
     return cover
 
 
-def _expected(fb):
+def _expected(fb, density: bool = False):
+    """{(class, bra, ket)} the plan must cover: G_ab G_cd >= tau over the
+    kets of the bra's own system (density: also the planning density's
+    six-block screen)."""
     out = set()
     classes = sorted(fb.tables)
     for i, X in enumerate(classes):
         for Y in classes[i:]:
             bra, ket = fb.tables[X], fb.tables[Y]
             for b in range(bra.n):
-                live = fb._bounds(bra, ket, b) >= fb.tau
+                lo, hi = ket.mol_range.get(int(bra.mol[b]), (0, 0))
+                live = np.zeros(ket.n, dtype=bool)
+                live[lo:hi] = bra.G[b] * ket.G[lo:hi] >= fb.tau
+                if density:
+                    live &= fb._bounds(bra, ket, b) >= fb.tau
                 if X == Y:
                     live[:b] = False
                 for k in np.nonzero(live)[0]:
                     out.add(((*X, *Y), b, int(k)))
     return out
+
+
+def _cold(fb):
+    """{(class, bra, ket)} of the cold items (the last n_cold FP64 items)."""
+    out = set()
+    for L in fb.launches:
+        end = L.n_fp64 if L.precision == MIXED else L.work.shape[0]
+        for b, k0, z, k1 in L.work[end - L.n_cold:end]:
+            assert z >> 16 == 0                        # the whole bra
+            for v in L.ket_list[k0:k1]:
+                assert v >> 17 == 0                    # whole kets
+                out.add((L.cls, int(b), int(v)))
+    return out
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_cold_items_hold_exactly_the_density_screened_quartets(mixed):
+    """Cold items: the quartets G_ab G_cd >= tau that the planning density's
+    screen drops, not counted as work (the launch counts sum over the other
+    items)."""
+    s = config2(seed=11, n_atoms=10)
+    fb = FockBuilder(s.basis, s.D, tau=1e-12, mixed=mixed)
+    cold = _cold(fb)
+    assert cold and cold == _expected(fb) - _expected(fb, density=True)
+    for L in fb.launches:
+        assert FockBuilder.item_quartets(L).sum() == L.n_quartets, L.cls
+    assert sum(L.n_quartets for L in fb.launches) == len(_expected(fb, density=True))
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(lane_flops=3e4), dict(lane_flops=1e30),
