@@ -48,10 +48,14 @@ def sources(lmax: int = LMAX) -> list[Path]:
         files.append(p)
     built = set(classes) | set(loop)
     gen_hip.write_header(tuple(c for c in builder_classes(LOOP_LMAX) if c in built))
+    p = gen_hip.GEN / "f_any_table.h"
+    text = gen_hip.emit_any_table(tuple(c for c in builder_classes(LOOP_LMAX) if c in built))
+    if not p.exists() or p.read_text() != text:
+        p.write_text(text)
     mmd = gen_mmd.generate(MMD_LMAX, k_lmax=MMD_K_LMAX)
     gen_mmd.write_header(MMD_LMAX, MMD_K_LMAX)
     return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", CSRC / "fj_kloop.hip",
-            *files, *mmd]
+            CSRC / "fj_noalloc.hip", *files, *mmd]
 
 
 def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, float, str]:
@@ -64,6 +68,8 @@ def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, fl
         deps.append(hdr)
     if src.name == "fj_kloop.hip":
         deps.append(CSRC / "fj_floop.h")
+    if src.name == "fj_noalloc.hip":
+        deps.append(CSRC / "gen" / "f_any_table.h")
     if not force and obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime
                                           for d in deps if d.exists()):
         return obj, 0.0, ""

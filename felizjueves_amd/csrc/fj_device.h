@@ -44,6 +44,9 @@ __device__ __forceinline__ bool fj_item_fp64(const FjFArgs& a, int i) {
 // sized for FJ_LIB_WS_BLOCKS.
 #define FJ_WS_MAX_BLOCKS 1024
 #define FJ_LIB_WS_BLOCKS 256
+// The fixed device array of fj_f_noalloc, per GPU: 64 MiB (the (dd|dd)
+// kernel's 2,113 doubles per lane fit 62 workgroups of 64 lanes).
+#define FJ_NOALLOC_SCRATCH_BYTES (64LL << 20)
 
 // The library-owned workspace of at least `bytes` on the current device
 // (allocated on first use, grown when asked for more; NULL on failure).

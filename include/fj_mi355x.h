@@ -1313,6 +1313,25 @@ int fj_mlp_adam(fj_stream_t stream, int np, float* P, const float* G, float* M, 
 /* Wb <- BF16(P), round to nearest even. */
 int fj_mlp_round(fj_stream_t stream, int np, const float* P, unsigned short* Wb);
 
+/* fj_f_noalloc: one launch of fj_f_<la><lb><lc><ld> (same arguments and
+ * result; AB / CD may be NULL when lb / ld is 0) that allocates nothing.
+ * No upstream counterpart: it is for callers that cannot allocate (code
+ * under stream capture, serving loops, harnesses that forbid hipMalloc);
+ * fj_f_<cls> allocates its staged classes' workspace on first use.  The
+ * workspace here is a fixed device array of fj_f_noalloc_scratch_bytes()
+ * (64 MiB) per GPU, held until the process ends; the persistent kernel runs
+ * on as many workgroups as its slots fit (fewer than fj_f_<cls> uses for
+ * the largest classes: (dd|dd) gets 62 workgroups of 64 lanes).  A class
+ * whose one workgroup needs more than the array returns FJ_ERR_WORKSPACE.
+ * Calls that overlap in time on one GPU share the array and race. */
+int fj_f_noalloc(fj_stream_t stream, int la, int lb, int lc, int ld, int threads, int n_bra,
+                 int n_ket, int k_ab, int k_cd, int n_sh, int n_ao, int diag, double tau,
+                 const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh,
+                 const int* bra_ao, const double* ket_prim, const double* CD, const double* G_cd,
+                 const int* ket_sh, const int* ket_ao, const double* Dsh, const double* D,
+                 double* F);
+long long fj_f_noalloc_scratch_bytes(void);
+
 /* Per-build precision tiers of one mixed launch (fj_fxm_<cls> with a `tier`
  * array), chosen on the device from the CURRENT density block maxima, so a
  * plan made with one density stays correct for another (no upstream
