@@ -25,6 +25,7 @@ LMAX = 2         # generated HGP fused F kernels through (dd|dd)
 MMD_LMAX = 3     # Schwarz, DQ, [p|q] and JAB through f
 LOOP_LMAX = 3    # fused F classes past LMAX run on the loop-form kernel (fj_floop.h)
 MMD_K_LMAX = 2   # straight-line K kernels through d; the f classes in loop form (fj_kloop.h)
+SPLIT_LMAX = 2   # split HGP J / K kernels (one per orbit) through (dd|dd)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -52,6 +53,12 @@ def sources(lmax: int = LMAX) -> list[Path]:
     text = gen_hip.emit_any_table(tuple(c for c in builder_classes(LOOP_LMAX) if c in built))
     if not p.exists() or p.read_text() != text:
         p.write_text(text)
+    # the split J / K kernels: one per orbit representative through d
+    from . import orbits
+    reps = orbits.canonical_classes(4 * SPLIT_LMAX, SPLIT_LMAX)
+    split = gen_hip.generate_split([(r, sum(1 << g for g in orbits.orientations(r))) for r in reps])
+    gen_hip.write_split_header(reps)
+    files += split
     mmd = gen_mmd.generate(MMD_LMAX, k_lmax=MMD_K_LMAX)
     gen_mmd.write_header(MMD_LMAX, MMD_K_LMAX)
     return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", CSRC / "fj_kloop.hip",

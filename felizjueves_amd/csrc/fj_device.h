@@ -34,6 +34,24 @@ struct FjFArgs {
     const int* tier;       // a mixed launch: per item 0 FP64, 1 FP32 (overrides n_fp64)
 };
 
+// The digestion of a split J or K launch (fj_hj_ / fj_hk_): the weight of
+// each output block of the built quartet (ab, cd, ac, ad, bc, bd; one is 1,
+// the others 0), the strides of the density element (r, c) read, D[r * dr +
+// c * dc], and of the output element written, F[r * fr + c * fc] (a block
+// transposed by the launch's orientation), and the built positions of the
+// density pair whose block maximum screens a quartet.
+struct FjDig {
+    double w[6];
+    int dr, dc, fr, fc;
+    int q0, q1;
+};
+
+// The FjDig of a split launch: screen 0 (J: the output on the orientation's
+// positions (g^-1[0], g^-1[1]), the density on (g^-1[2], g^-1[3])) or 1 (K:
+// output (g^-1[0], g^-1[2]), density (g^-1[1], g^-1[3])), g the orientation
+// GROUP[perm] (host code).
+int fj_split_dig(int screen, int perm, int n_ao, FjDig* dg);
+
 // The tier of work item i of a mixed launch.
 __device__ __forceinline__ bool fj_item_fp64(const FjFArgs& a, int i) {
     return a.tier ? a.tier[i] == 0 : i < a.n_fp64;

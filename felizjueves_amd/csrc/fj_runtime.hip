@@ -33,6 +33,41 @@ double* fj_library_workspace(long long bytes) {
     return buf[dev];
 }
 
+// The 8-element ERI symmetry group as shell-position maps g: the class g.cls
+// has cls[g[j]] in position j (upstream hgp/orbits.py:78-87, the same order,
+// so a perm index means the same orientation on both sides).
+static const int FJ_GROUP[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                   {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
+
+int fj_split_dig(int screen, int perm, int n_ao, FjDig* dg) {
+    if (!dg || perm < 0 || perm > 7 || (screen != 0 && screen != 1) || n_ao <= 0) return FJ_ERR_ARG;
+    int inv[4];
+    for (int j = 0; j < 4; ++j) inv[FJ_GROUP[perm][j]] = j;
+    // J: K_ab += D_cd (ab|cd) on the ordered quartet's positions (0,1)/(2,3);
+    // K: K_ac += D_bd (ab|cd), positions (0,2)/(1,3); each carried to the
+    // built quartet's positions by g^-1
+    const int o0 = inv[0], o1 = screen ? inv[2] : inv[1];
+    const int q0 = screen ? inv[1] : inv[2], q1 = inv[3];
+    const int lo = o0 < o1 ? o0 : o1, hi = o0 < o1 ? o1 : o0;
+    int blk = -1;
+    if (lo == 0 && hi == 1) blk = 0;
+    else if (lo == 2 && hi == 3) blk = 1;
+    else if (lo == 0 && hi == 2) blk = 2;
+    else if (lo == 0 && hi == 3) blk = 3;
+    else if (lo == 1 && hi == 2) blk = 4;
+    else if (lo == 1 && hi == 3) blk = 5;
+    if (blk < 0) return FJ_ERR_ARG;
+    for (int i = 0; i < 6; ++i) dg->w[i] = i == blk ? 1.0 : 0.0;
+    // the kernels read and write every block in (lower, higher) position order
+    dg->fr = o0 < o1 ? n_ao : 1;
+    dg->fc = o0 < o1 ? 1 : n_ao;
+    dg->dr = q0 < q1 ? n_ao : 1;
+    dg->dc = q0 < q1 ? 1 : n_ao;
+    dg->q0 = q0;
+    dg->q1 = q1;
+    return 0;
+}
+
 // F <- F + F^T: each thread owns one (i, j), i < j, pair and the diagonal.
 __global__ void fj_close_k(int n, double* __restrict__ F) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -74,8 +74,37 @@ def _hname(node) -> str:
 AX = "xyz"
 
 
-def emit_quartet(cls) -> str:
-    """The device function for one contracted quartet of class `cls`."""
+# The split digestion (fj_hj_ / fj_hk_): the output block each density block
+# serves in the six-block form, and the output blocks' weights in the split
+# form (s_<block>, one of them 1 per launch, the others 0).
+SERVES = {"cd": "ab", "ab": "cd", "ac": "bd", "ad": "bc", "bc": "ad", "bd": "ac"}
+SPLIT_WEIGHTS = ("    const double s_ab = dg.w[0], s_cd = dg.w[1], s_ac = dg.w[2], s_ad = dg.w[3],\n"
+                 "                 s_bc = dg.w[4], s_bd = dg.w[5];")
+
+
+def _dw(split: bool, dens: str) -> str:
+    """The weight a density block is multiplied by: s_j (Coulomb) or s_k
+    (exchange) in the fused form, its output block's s_<block> in the split
+    form."""
+    if split:
+        return f"s_{SERVES[dens]}"
+    return "s_j" if dens in ("cd", "ab") else "s_k"
+
+
+def _didx(split: bool, r: str, c: str) -> str:
+    """D's element (r, c): row-major, or through the split launch's strides
+    (a transposed density block)."""
+    return f"({r}) * dg.dr + ({c}) * dg.dc" if split else f"({r}) * n_ao + {c}"
+
+
+def _fidx_out(split: bool, r: str, c: str) -> str:
+    """F's element (r, c), likewise for the split launch's output block."""
+    return f"({r}) * dg.fr + ({c}) * dg.fc" if split else f"({r}) * n_ao + {c}"
+
+
+def emit_quartet(cls, split: bool = False) -> str:
+    """The device function for one contracted quartet of class `cls`
+    (split: the one-block digestion of the split J / K kernels, fj_qs_)."""
     la, lb, lc, ld = cls
     L = sum(cls)
     v = vrr_dag(tuple(cls))
@@ -99,13 +128,16 @@ def emit_quartet(cls) -> str:
     E(f"// ({la}{lb}|{lc}{ld}): {v.n_nodes} VRR nodes, {len(v.targets)} boundary "
       f"values, {len(order)} HRR statements, {len(targets)} ERIs", 0)
     E("template <typename R>", 0)
-    E(f"__device__ __forceinline__ void fj_q_{name}(", 0)
+    E(f"__device__ __forceinline__ void fj_q{'s' if split else ''}_{name}(", 0)
     E("const double* __restrict__ bp, int k_ab,       // bra records, LDS, FJ_SREC doubles each")
     E("const double* __restrict__ kp, int k_cd,       // ket records, global, 8 doubles each")
     E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
     E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
-    E("double s_j, double s_k, const double* __restrict__ wab,")
+    E("const FjDig& dg, const double* __restrict__ wab," if split else
+      "double s_j, double s_k, const double* __restrict__ wab,")
     E("double* __restrict__ fab, double* __restrict__ F, double* __restrict__ ws, size_t st) {", 1)
+    if split:
+        E(SPLIT_WEIGHTS, 0)
     for i in range(len(v.targets)):
         E(f"R c{i} = R(0);")
     E("for (int kc = 0; kc < k_cd; ++kc) {")
@@ -189,10 +221,11 @@ def emit_quartet(cls) -> str:
     E("// the weighted densities (weight x degeneracy folded in once)")
     for blk in ("cd", "ac", "ad", "bc", "bd"):
         p0, p1 = blocks[blk]
-        sc = "s_j" if blk == "cd" else "s_k"
+        sc = _dw(split, blk)
         for i in range(sizes[p0]):
             for j in range(sizes[p1]):
-                E(f"const double w{blk}{i}_{j} = {sc} * D[({aov[p0]} + {i}) * n_ao + {aov[p1]} + {j}];")
+                E(f"const double w{blk}{i}_{j} = {sc} * "
+                  f"D[{_didx(split, f'{aov[p0]} + {i}', f'{aov[p1]} + {j}')}];")
     for blk in ("cd", "ac", "ad", "bc", "bd"):
         p0, p1 = blocks[blk]
         for i in range(sizes[p0]):
@@ -215,9 +248,14 @@ def emit_quartet(cls) -> str:
         E("}")
     for blk in ("cd", "ac", "ad", "bc", "bd"):
         p0, p1 = blocks[blk]
+        if split:
+            E(f"if (s_{blk} != 0.0) {{")
         for i in range(sizes[p0]):
             for j in range(sizes[p1]):
-                E(f"unsafeAtomicAdd(F + ({aov[p0]} + {i}) * n_ao + {aov[p1]} + {j}, f{blk}{i}_{j});")
+                E(f"unsafeAtomicAdd(F + {_fidx_out(split, f'{aov[p0]} + {i}', f'{aov[p1]} + {j}')}, "
+                  f"f{blk}{i}_{j});", 2 if split else 1)
+        if split:
+            E("}")
     E("}", 0)
     return E.text()
 
@@ -533,6 +571,216 @@ extern "C" int fj_fxm_{name}(hipStream_t s, int threads, const FjSide* bra, cons
         # the upstream-ABI entry (walker launches use canonical classes only)
         return text[:i] + text[i + len("//@@UPSTREAM@@\n"):j] + text[j + len("//@@END_UPSTREAM@@\n"):]
     return text[:i] + text[j + len("//@@END_UPSTREAM@@\n"):]
+
+
+# ---------------------------------------------------------------------------
+# The split J and K kernels (upstream eri/hjk/kernel_hj.py, kernel_hk.py): one
+# kernel per orbit of the 8-fold ERI symmetry, built on the orbit's
+# representative (orbits.representative). A launch is one ordered batch pair
+# of the split stream routed to the representative through its orientation
+# `perm` (an index into orbits.GROUP): the kernel runs the representative's
+# VRR and HRR (the fused kernels' bodies) and deposits ONE block, J_ab +=
+# D_cd (ab|cd) or K_ac += D_bd (ab|cd) on the ordered quartet's positions,
+# carried to the built quartet's by the orientation (fj_split_dig): no
+# weights, no degeneracy, no canonical half, one-block density screen.
+
+def emit_split_class(rep, mask: int) -> str:
+    """hs_<rep>.hip: the split kernel of representative class `rep` and its
+    entries fj_hj_<rep> and fj_hk_<rep>; `mask` has bit g set for every
+    orientation the orbit's members reach (other perms deposit nothing)."""
+    la, lb, lc, ld = rep
+    name = cls_name(rep)
+    na, nb = n_cart(la), n_cart(lb)
+    nab = na * nb
+    W = ws_doubles(rep)
+    body = emit_quartet_staged(rep, split=True) if staged(rep) else emit_quartet(rep, split=True)
+    abx, aby, abz = (("a.bra.sep[3 * bra]", "a.bra.sep[3 * bra + 1]", "a.bra.sep[3 * bra + 2]")
+                     if lb else ("0.0",) * 3)
+    cdx, cdy, cdz = (("a.ket.sep[3 * k]", "a.ket.sep[3 * k + 1]", "a.ket.sep[3 * k + 2]")
+                     if ld else ("0.0",) * 3)
+    if W:
+        kernel_body = (
+            "    __shared__ double tb[FJ_BOYS_NROW * FJ_BOYS_LROW];\n"
+            f"    fj_boys_stage<{sum(rep)}>(tb);\n"
+            "    __syncthreads();\n"
+            "    const size_t st = (size_t)gridDim.x * blockDim.x;\n"
+            "    double* ws = a.ws + (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
+            "    for (int it = (int)blockIdx.x; it < a.n_items; it += (int)gridDim.x) {\n"
+            "        double* wsi = ws;\n"
+            "        size_t sti = st;\n"
+            "        FJ_LAUNDER(wsi, sti);\n"
+            f"        fj_hsitem_{name}(a, dg, it % a.n_bx, it / a.n_bx, bsh, wsi, sti, tb);\n"
+            "        __syncthreads();\n"
+            "    }")
+        launch_ws = (
+            f"    {{\n"
+            f"        const long long need = (long long)8 * {W} * threads * FJ_LIB_WS_BLOCKS;\n"
+            f"        a.ws = fj_library_workspace(need);\n"
+            f"        if (!a.ws) return FJ_ERR_WORKSPACE;\n"
+            f"        a.ws_doubles = need / 8;\n"
+            f"        const long long blocks = a.ws_doubles / {W} / threads;\n"
+            f"        grid = dim3((unsigned)(blocks < (long long)a.n_items ? blocks : a.n_items));\n"
+            f"    }}")
+        call_tail = "S.D, n_ao, oc, od, oa, ob, dg, wab, fab, S.F, ws, st, tb"
+    else:
+        kernel_body = f"    fj_hsitem_{name}(a, dg, (int)blockIdx.x, (int)blockIdx.y, bsh, nullptr, 0, nullptr);"
+        launch_ws = ""
+        call_tail = "S.D, n_ao, oc, od, oa, ob, dg, wab, fab, S.F, ws, st"
+    out = [f"// generated by felizjueves_amd/gen_hip.py -- split J / K kernel, orbit of ({la}{lb}|{lc}{ld})",
+           '#include "../fj_device.h"', "", body]
+
+    def entry(kind: str, screen: int) -> str:
+        return f"""
+extern "C" int fj_h{kind}_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+                         int n_sh, int n_ao, int perm, double tau,
+                         const double* bra_prim,{" const double* AB," if lb else ""} const double* G_ab, const int* bra_sh, const int* bra_ao,
+                         const double* ket_prim,{" const double* CD," if ld else ""} const double* G_cd, const int* ket_sh, const int* ket_ao,
+                         const double* Dsh, const double* D, double* {kind.upper()}) {{
+    if (k_ab < 1 || k_ab > FJ_MAX_PRIM_PAIRS || k_cd < 1 || k_cd > FJ_MAX_PRIM_PAIRS) return FJ_ERR_DEGREE;
+    if (n_bra < 0 || n_ket < 0) return FJ_ERR_SHAPE;
+    if (perm < 0 || perm > 7) return FJ_ERR_ARG;
+    if (!((FJ_HS_MASK_{name} >> perm) & 1)) return 0;   // an orientation outside the orbit: no deposit
+    FjDig dg;
+    const int rc = fj_split_dig({screen}, perm, n_ao, &dg);
+    if (rc) return rc;
+    FjFArgs a;
+    a.bra = FjSide{{bra_prim, nullptr, k_ab, {"AB" if lb else "nullptr"}, G_ab, bra_sh, bra_ao, n_bra}};
+    a.ket = FjSide{{ket_prim, nullptr, k_cd, {"CD" if ld else "nullptr"}, G_cd, ket_sh, ket_ao, n_ket}};
+    a.work = nullptr; a.ket_list = nullptr; a.sys = nullptr;
+    a.one = FjSys{{D, Dsh, {kind.upper()}, n_ao, n_sh}};
+    a.diag = 0; a.tau = tau;
+    a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0; a.tier = nullptr;
+    return fj_hs_launch_{name}(s, threads, a, dg, n_bra);
+}}
+"""
+    out.append(f"""
+#define FJ_HS_MASK_{name} {mask}
+
+// One ordered quartet per lane: the bra pair's records staged in LDS, the
+// one-block density screen, the representative's quartet body with the
+// launch's one-block digestion, the bra block reduced across the wavefront.
+__device__ __forceinline__ void fj_hsitem_{name}(const FjFArgs& a, const FjDig& dg, int bx, int by,
+                                              double* bsh, double* __restrict__ ws, size_t st,
+                                              const double* __restrict__ tb) {{
+    const int bra = bx;
+    const int k0 = by * (int)blockDim.x;
+    const FjSys S = a.one;
+    const int k1 = a.ket.n;
+    if (k0 >= k1) return;   // uniform
+    int kb0, kab;
+    fj_side_range(a.bra, bra, kb0, kab);
+    if (kab > FJ_MAX_PRIM_PAIRS) return;
+    for (int i = threadIdx.x; i < kab; i += blockDim.x) {{
+        const double* __restrict__ r = a.bra.prim + 8 * (size_t)(kb0 + i);
+        double* s = bsh + FJ_SREC * i;
+#pragma unroll
+        for (int f = 0; f < 8; ++f) s[f] = r[f];
+        s[8] = 1.0 / r[6];
+    }}
+    __syncthreads();
+    const int sa = a.bra.sh[2 * bra], sb = a.bra.sh[2 * bra + 1];
+    const int oa = a.bra.ao[2 * bra], ob = a.bra.ao[2 * bra + 1];
+    const double Gb = a.bra.G[bra];
+    const double ABx = {abx}, ABy = {aby}, ABz = {abz};
+    const int n_ao = S.n_ao;
+    const int k = k0 + (int)threadIdx.x;
+    double fab[{nab}];
+    double wab[{nab}];
+#pragma unroll
+    for (int i = 0; i < {nab}; ++i) fab[i] = 0.0;
+    bool live = k < k1;
+    if (live) {{
+        const int sc = a.ket.sh[2 * k], sd = a.ket.sh[2 * k + 1];
+        const int s0 = dg.q0 == 0 ? sa : dg.q0 == 1 ? sb : dg.q0 == 2 ? sc : sd;
+        const int s1 = dg.q1 == 0 ? sa : dg.q1 == 1 ? sb : dg.q1 == 2 ? sc : sd;
+        live = !(Gb * (a.ket.G[k] * S.Dsh[s0 * S.n_sh + s1]) < a.tau);
+    }}
+    if (live) {{
+        const int oc = a.ket.ao[2 * k], od = a.ket.ao[2 * k + 1];
+#pragma unroll
+        for (int i = 0; i < {na}; ++i)
+#pragma unroll
+            for (int j = 0; j < {nb}; ++j)
+                wab[i * {nb} + j] = dg.w[1] * S.D[(oa + i) * dg.dr + (ob + j) * dg.dc];
+        int kc0, kcd;
+        fj_side_range(a.ket, k, kc0, kcd);
+        const double CDx = {cdx}, CDy = {cdy}, CDz = {cdz};
+        fj_qs_{name}<double>(bsh, kab, a.ket.prim + 8 * (size_t)kc0, kcd, ABx, ABy, ABz, CDx, CDy, CDz,
+                         {call_tail});
+    }}
+    if (dg.w[0] != 0.0) {{   // uniform
+#pragma unroll
+        for (int i = 0; i < {nab}; ++i) {{
+            const double s = fj_wave_sum(fab[i]);
+            if ((threadIdx.x & 63) == 0 && s != 0.0)
+                unsafeAtomicAdd(S.F + (oa + i / {nb}) * dg.fr + (ob + i % {nb}) * dg.fc, s);
+        }}
+    }}
+}}
+
+__global__ void __launch_bounds__(FJ_MAX_THREADS) fj_hs_{name}_k(FjFArgs a, FjDig dg) {{
+    __shared__ double bsh[FJ_MAX_PRIM_PAIRS * FJ_SREC];
+{kernel_body}
+}}
+
+static int fj_hs_launch_{name}(hipStream_t s, int threads, FjFArgs a, FjDig dg, int n_blocks) {{
+    if (threads <= 0) threads = FJ_DEFAULT_THREADS;
+    if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
+    if (n_blocks <= 0 || a.ket.n <= 0) return 0;
+    dim3 grid((unsigned)n_blocks, (unsigned)((a.ket.n + threads - 1) / threads));
+    a.n_bx = (int)grid.x;
+    a.n_items = (int)(grid.x * grid.y);
+{launch_ws}
+    hipLaunchKernelGGL(fj_hs_{name}_k, grid, dim3(threads), 0, s, a, dg);
+    return (int)hipGetLastError();
+}}
+""")
+    out.append(entry("j", 0))
+    out.append(entry("k", 1))
+    return "\n".join(out)
+
+
+def split_header_decls(reps) -> str:
+    """The split J / K declarations of include/fj_mi355x.h."""
+    out = []
+    for rep in reps:
+        la, lb, lc, ld = rep
+        n = cls_name(rep)
+        for kind in ("j", "k"):
+            out.append(
+                f"int fj_h{kind}_{n}(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab,"
+                f" int k_cd,\n    int n_sh, int n_ao, int perm, double tau,\n"
+                f"    const double* bra_prim,{' const double* AB,' if lb else ''} const double* G_ab,"
+                f" const int* bra_sh, const int* bra_ao,\n"
+                f"    const double* ket_prim,{' const double* CD,' if ld else ''} const double* G_cd,"
+                f" const int* ket_sh, const int* ket_ao,\n"
+                f"    const double* Dsh, const double* D, double* {kind.upper()});")
+    return "\n".join(out) + "\n"
+
+
+def generate_split(reps_masks, out: Path = GEN) -> list[Path]:
+    """hs_<rep>.hip for every (representative, orientation mask)."""
+    out.mkdir(parents=True, exist_ok=True)
+    files = []
+    for rep, mask in reps_masks:
+        p = out / f"hs_{cls_name(rep)}.hip"
+        text = emit_split_class(rep, mask)
+        if not p.exists() or p.read_text() != text:
+            p.write_text(text)
+        files.append(p)
+    return files
+
+
+SPLIT_BEGIN, SPLIT_END = ("/* BEGIN SPLIT CLASSES (written by felizjueves_amd/gen_hip.py) */",
+                          "/* END SPLIT CLASSES */")
+
+
+def write_split_header(reps) -> None:
+    cur = HEADER.read_text()
+    i, j = cur.index(SPLIT_BEGIN), cur.index(SPLIT_END)
+    new = cur[:i + len(SPLIT_BEGIN)] + "\n" + split_header_decls(reps) + cur[j:]
+    if new != cur:
+        HEADER.write_text(new)
 
 
 # ---------------------------------------------------------------------------
@@ -906,7 +1154,7 @@ def _emit_side_hrr(E: _Emit, la: int, lb: int, sep: str, src, dst, ind: int) -> 
         dst(j, name[t])
 
 
-def emit_quartet_staged(cls) -> str:
+def emit_quartet_staged(cls, split: bool = False) -> str:
     from .hgp import closure, e_groups
     la, lb, lc, ld = cls
     L = sum(cls)
@@ -1029,14 +1277,17 @@ def emit_quartet_staged(cls) -> str:
             group_body(tg, 1)
             E("}", 0)
     E("template <typename R>", 0)
-    E(f"__device__ __forceinline__ void fj_q_{name}(", 0)
+    E(f"__device__ __forceinline__ void fj_q{'s' if split else ''}_{name}(", 0)
     E("const double* __restrict__ bp, int k_ab,")
     E("const double* __restrict__ kp, int k_cd,")
     E("double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,")
     E("const double* __restrict__ D, int n_ao, int oc, int od, int oa, int ob,")
-    E("double s_j, double s_k, const double* __restrict__ wab,")
+    E("const FjDig& dg, const double* __restrict__ wab," if split else
+      "double s_j, double s_k, const double* __restrict__ wab,")
     E("double* __restrict__ fab, double* __restrict__ F, double* __restrict__ ws, size_t st,", 1)
     E("const double* __restrict__ tb) {", 1)
+    if split:
+        E(SPLIT_WEIGHTS, 0)
     if W:
         E(f"// (e0|f0) [f][e], (ab|f0) [ab][f] and F_cd in the workspace ({W} doubles per lane)")
     if W and fns:
@@ -1074,18 +1325,27 @@ def emit_quartet_staged(cls) -> str:
     else:
         for c in range(nc):
             for d in range(nd):
-                E(f"const double wcd{c}_{d} = s_j * D[(oc + {c}) * n_ao + od + {d}];")
+                E(f"const double wcd{c}_{d} = {_dw(split, 'cd')} * "
+                  f"D[{_didx(split, f'oc + {c}', f'od + {d}')}];")
                 E(f"double fcd{c}_{d} = 0.0;")
     E("#pragma unroll 1", 0)
     E(f"for (int ab = 0; ab < {NAB}; ++ab) {{")
     E(f"const int a = ab / {nb}, b = ab % {nb};", 2)
-    E("const double w_ab = s_j * D[(oa + a) * n_ao + ob + b];", 2)
-    E("const double* __restrict__ Db = D + (size_t)(ob + b) * n_ao;", 2)
-    E("const double* __restrict__ Da = D + (size_t)(oa + a) * n_ao;", 2)
-    for c in range(nc):
-        E(f"const double dbc{c} = s_k * Db[oc + {c}], dac{c} = s_k * Da[oc + {c}];", 2)
-    for d in range(nd):
-        E(f"const double dbd{d} = s_k * Db[od + {d}], dad{d} = s_k * Da[od + {d}];", 2)
+    E(f"const double w_ab = {_dw(split, 'ab')} * D[{_didx(split, 'oa + a', 'ob + b')}];", 2)
+    if split:
+        for c in range(nc):
+            E(f"const double dbc{c} = s_ad * D[{_didx(True, 'ob + b', f'oc + {c}')}], "
+              f"dac{c} = s_bd * D[{_didx(True, 'oa + a', f'oc + {c}')}];", 2)
+        for d in range(nd):
+            E(f"const double dbd{d} = s_ac * D[{_didx(True, 'ob + b', f'od + {d}')}], "
+              f"dad{d} = s_bc * D[{_didx(True, 'oa + a', f'od + {d}')}];", 2)
+    else:
+        E("const double* __restrict__ Db = D + (size_t)(ob + b) * n_ao;", 2)
+        E("const double* __restrict__ Da = D + (size_t)(oa + a) * n_ao;", 2)
+        for c in range(nc):
+            E(f"const double dbc{c} = s_k * Db[oc + {c}], dac{c} = s_k * Da[oc + {c}];", 2)
+        for d in range(nd):
+            E(f"const double dbd{d} = s_k * Db[od + {d}], dad{d} = s_k * Da[od + {d}];", 2)
     E("double sab = 0.0;", 2)
     for c in range(nc):
         E(f"double fac{c} = 0.0, fbc{c} = 0.0;", 2)
@@ -1095,7 +1355,7 @@ def emit_quartet_staged(cls) -> str:
     def digest(j, x):
         c, d = divmod(j, nd)
         if fcd_ws:
-            E(f"sab += (s_j * D[(oc + {c}) * n_ao + od + {d}]) * {x}; "
+            E(f"sab += ({_dw(split, 'cd')} * D[{_didx(split, f'oc + {c}', f'od + {d}')}]) * {x}; "
               f"{fcd_ref(j)} += w_ab * {x};", 2)
         else:
             E(f"sab += wcd{c}_{d} * {x}; fcd{c}_{d} += w_ab * {x};", 2)
@@ -1105,19 +1365,37 @@ def emit_quartet_staged(cls) -> str:
         E, lc, ld, "CD", lambda i: mid_ref(f"ab * {NF} + {i}"), digest, 2)
     E("fab[ab] += sab;", 2)
     for c in range(nc):
-        E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + oc + {c}, fac{c});", 2)
-        E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + oc + {c}, fbc{c});", 2)
+        if split:
+            E(f"if (s_ac != 0.0) unsafeAtomicAdd(F + {_fidx_out(True, 'oa + a', f'oc + {c}')}, fac{c});", 2)
+            E(f"if (s_bc != 0.0) unsafeAtomicAdd(F + {_fidx_out(True, 'ob + b', f'oc + {c}')}, fbc{c});", 2)
+        else:
+            E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + oc + {c}, fac{c});", 2)
+            E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + oc + {c}, fbc{c});", 2)
     for d in range(nd):
-        E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + od + {d}, fad{d});", 2)
-        E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + od + {d}, fbd{d});", 2)
+        if split:
+            E(f"if (s_ad != 0.0) unsafeAtomicAdd(F + {_fidx_out(True, 'oa + a', f'od + {d}')}, fad{d});", 2)
+            E(f"if (s_bd != 0.0) unsafeAtomicAdd(F + {_fidx_out(True, 'ob + b', f'od + {d}')}, fbd{d});", 2)
+        else:
+            E(f"unsafeAtomicAdd(F + (size_t)(oa + a) * n_ao + od + {d}, fad{d});", 2)
+            E(f"unsafeAtomicAdd(F + (size_t)(ob + b) * n_ao + od + {d}, fbd{d});", 2)
     E("}")
     if fcd_ws:
         E("#pragma unroll 1", 0)
         E(f"for (int j = 0; j < {NCD}; ++j)")
-        E(f"unsafeAtomicAdd(F + (size_t)(oc + j / {nd}) * n_ao + od + j % {nd}, {fcd_ref('j')});", 2)
+        if split:
+            E(f"if (s_cd != 0.0) unsafeAtomicAdd(F + {_fidx_out(True, f'oc + j / {nd}', f'od + j % {nd}')}, "
+              f"{fcd_ref('j')});", 2)
+        else:
+            E(f"unsafeAtomicAdd(F + (size_t)(oc + j / {nd}) * n_ao + od + j % {nd}, {fcd_ref('j')});", 2)
     else:
+        if split:
+            E("if (s_cd != 0.0) {")
         for c in range(nc):
             for d in range(nd):
-                E(f"unsafeAtomicAdd(F + (size_t)(oc + {c}) * n_ao + od + {d}, fcd{c}_{d});")
+                E(f"unsafeAtomicAdd(F + {_fidx_out(True, f'oc + {c}', f'od + {d}')}, fcd{c}_{d});"
+                  if split else f"unsafeAtomicAdd(F + (size_t)(oc + {c}) * n_ao + od + {d}, fcd{c}_{d});",
+                  2 if split else 1)
+        if split:
+            E("}")
     E("}", 0)
     return E.text()

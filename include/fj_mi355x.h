@@ -1368,6 +1368,233 @@ typedef struct FjTierSel {
 int fj_tier_select(fj_stream_t stream, const FjSide* bra, const FjSide* ket, const FjSys* sys,
                    const FjTierSel* sel);
 
+/* The split HGP J and K kernels (upstream eri/hjk/kernel_hj.py:1-140 and
+ * kernel_hk.py:1-110, canonical class mode; parameter list params_of,
+ * kernel_hj.py:305-315): one kernel per orbit of the 8-fold ERI symmetry,
+ * named after its representative (orbits.representative), through (dd|dd).
+ * A launch is one ORDERED batch pair of the split stream (hjk.hjk_launches,
+ * upstream eri/hjk/launch_hjk.py:280-342) presented in the representative's
+ * positions, `perm` the orientation's index in the symmetry group (upstream
+ * hgp/orbits.py GROUP order).  fj_hj_: J[a_i][b_j] += D[c_k][d_l] (ab|cd)
+ * for the ordered quartet (a b|c d); fj_hk_: K[a_i][c_k] += D[b_j][d_l]
+ * (ab|cd) (K's Hermite pairing: the class is (la, lc | lb, ld) of K_ab +=
+ * D_cd (ac|bd)).  One block per quartet, no weights, no degeneracy, no
+ * canonical half; a quartet is skipped when G_ab G_cd Dsh[density pair] <
+ * tau; a perm outside the orbit's orientations deposits nothing.  J / K are
+ * accumulated, never closed or mirrored. */
+/* BEGIN SPLIT CLASSES (written by felizjueves_amd/gen_hip.py) */
+int fj_hj_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_0010(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_0010(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_0020(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_0020(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_0021(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_0021(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_1010(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_1010(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_1011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_1011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_1121(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_1121(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2010(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2010(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2020(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2020(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2021(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2021(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2110(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2110(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2121(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2121(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2210(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2210(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2220(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2220(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2221(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2221(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+int fj_hj_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* J);
+int fj_hk_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
+    int n_sh, int n_ao, int perm, double tau,
+    const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh, const int* bra_ao,
+    const double* ket_prim, const double* CD, const double* G_cd, const int* ket_sh, const int* ket_ao,
+    const double* Dsh, const double* D, double* K);
+/* END SPLIT CLASSES */
+
 /* BEGIN MMD CLASSES (written by felizjueves_amd/gen_mmd.py) */
 int fj_schwarz_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
     const double* zeta_ab, const double* P, const double* XPA, const double* XPB,
