@@ -102,62 +102,38 @@ def load_selector(path):
     return LearnedSelector.load(p)
 
 
-def cpu_baseline(fb, D, budget_s=15.0):
-    """The CPU oracle (OpenMP over the host cores) on a bounded sample of
-    the same workload: every `stride`-th bra pair of the costliest
-    off-diagonal class pair, all its kets; FLOPs of the sample counted by
-    the same screen and cost model as the device plan."""
+def cpu_baseline(fb, D, G_gpu=None):
+    """The CPU oracle timed on a whole build of the same workload: every
+    class pair of the builder's tables through the oracle's HGP launch
+    (OpenMP over the host cores; the same quartets, screen, weights and
+    digestion as the device plan), then F + F^T. The CPU G is checked
+    against the GPU's FP64 G when that is given."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle import oracle as O
-    from felizjueves_amd import pairs as P
-    from felizjueves_amd.hgp import cost
-    from test_builder_oracle import builder_tables_host
-    host = builder_tables_host(fb)
-    Ls = [L for L in fb.launches if not L.diag and L.precision == 0]
-    L = max(Ls, key=lambda L: L.flops)
-    X, Y = L.cls[:2], L.cls[2:]
-    bra, ket = fb.tables[X], fb.tables[Y]
-    c = cost(L.cls)
-    Dsh = P.density_blocks(fb.basis, D)
-
-    def sample(stride):
-        idx = np.arange(0, bra.n, stride)
-        fl, nq = 0.0, 0
-        for b in idx:
-            bound = fb._bounds(bra, ket, b)
-            live = bound >= fb.tau
-            nq += int(live.sum())
-            fl += c.per_prim * int(bra.deg[b]) * float(ket.deg[live].sum()) + c.per_quartet * live.sum()
-        hb = host[X]
-        off = hb["off"]
-        sub = {"prim": np.concatenate([hb["prim"][off[b]:off[b + 1]] for b in idx]),
-               "off": np.concatenate([[0], np.cumsum([off[b + 1] - off[b] for b in idx])]).astype(np.int32),
-               "sep": hb["sep"][idx], "G": hb["G"][idx], "sh": hb["sh"][idx], "ao": hb["ao"][idx]}
-        F = np.zeros((fb.basis.n_ao,) * 2)
-        t = time.perf_counter()
-        O.fx_launch(L.cls, sub, host[Y], 0, fb.basis.n_sh, fb.basis.n_ao, fb.tau, Dsh, D, F)
-        return time.perf_counter() - t, fl, nq, len(idx)
-
-    # grow the sample (halving the stride) until it takes a third of the
-    # budget, then size the measured one to the budget
-    stride = max(1, bra.n // 8)
-    t, fl, nq, nb = sample(stride)
-    while t < budget_s / 3 and stride > 1:
-        stride = max(1, stride // 2)
-        t, fl, nq, nb = sample(stride)
-    if t < budget_s and stride > 1:
-        stride = max(1, int(stride * t / budget_s))
-        t, fl, nq, nb = sample(stride)
-    gflops = fl / t / 1e9
-    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    return {
-        "value": (fl / t) / fb.flops, "unit": "Fock builds/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle f-launch, class {''.join(map(str, L.cls))}, every {stride}th bra pair "
-                   f"({nb} of {bra.n}) x all {ket.n} ket pairs: {nq} quartets, "
-                   f"{fl / 1e9:.1f} GFLOP in {t:.1f} s = {gflops:.2f} GFLOP/s; builds/s = that rate / "
-                   f"{fb.flops / 1e9:.1f} GFLOP per build"),
-        "quartets_per_s": nq / t, "gflops": gflops,
+    from test_builder_oracle import oracle_build
+    O.lib()
+    t = time.perf_counter()
+    G = oracle_build(fb, D)
+    t = time.perf_counter() - t
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    out = {
+        "value": 1.0 / t, "unit": "Fock builds/s", "cores": O.threads(), "kind": "port",
+        "sample": (f"one whole build of the workload ({fb.basis.n_ao} AOs, {fb.n_quartets} computed "
+                   f"quartets, {fb.flops / 1e9:.1f} GFLOP): every class pair through the oracle's HGP "
+                   f"launch, {t:.1f} s"),
+        "cpu_model": model, "seconds_per_build": t, "gflops": fb.flops / t / 1e9,
+        "quartets_per_s": fb.n_quartets / t,
     }
+    if G_gpu is not None:
+        out["max_dG_vs_gpu_fp64"] = float(np.abs(G - G_gpu).max())
+    return out
 
 
 def accuracy(D, G, Gref):
@@ -545,12 +521,20 @@ def main() -> None:
             roofline["traffic"] = t["traffic_bytes"]
             roofline["traffic_note"] = (t["source"] + f", scaled by a 512 MiB copy (fetch x{t['fetch_scale']:.3f}, "
                                         f"write x{t['write_scale']:.3f}); mostly the launch workspace")
+    va = TUNED / f"valu_c2_{a.atoms}atom.json"
+    if va.exists():
+        t = json.loads(va.read_text())
+        if t.get("kernel") == roofline["kernel"]:
+            roofline["executed_over_algorithmic"] = t["executed_over_algorithmic"]
+            roofline["executed_tflops"] = achieved * t["executed_over_algorithmic"]
+            roofline["valu_active_frac"] = t["valu_active_frac"]
+            roofline["counters_note"] = t["source"]
     extra = {}
     if not a.no_extra:
         extra["c3"] = c3_block(max(10, a.steps), a.warmup, selector)
         extra["c5"] = c5_block(max(5, a.steps // 2), a.warmup, selector)
         extra["jk"] = jk_block(3, 1)
-    cpu = None if a.no_cpu else cpu_baseline(plain, D)
+    cpu = None if a.no_cpu else cpu_baseline(plain, D, G64)
     out = {
         "metric": METRIC, "value": a.steps / t_prod, "unit": "Fock builds/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * t_prod / a.steps,

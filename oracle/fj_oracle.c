@@ -726,3 +726,12 @@ void oracle_schwarz(int n_sh, const int *shell_l, const double *centre,
 }
 
 int oracle_version(void) { return 1; }
+
+/* The OpenMP threads the oracle's parallel loops use. */
+int oracle_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -78,6 +78,11 @@ def fock_ordered(basis, D) -> tuple[np.ndarray, np.ndarray]:
     return J, K
 
 
+def threads() -> int:
+    """The OpenMP threads of the oracle's parallel loops."""
+    return int(lib().oracle_threads())
+
+
 def schwarz(basis) -> np.ndarray:
     n = basis.n_ao
     Q = np.zeros((n, n))
