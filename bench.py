@@ -85,13 +85,18 @@ def launch_times(fb, Dd, reps=1):
     return out
 
 
-def load_tuning(name):
+def load_tuning(name, mixed: bool = False):
+    """(builder keywords, stream count) of a tuned configuration; with
+    `mixed`, also the class pairs where the FP32 tier was measured to pay."""
     from felizjueves_amd import autotune
     p = TUNED / name
     if not p.exists():
         return {}, 1
     cfg = autotune.load(p)
-    return autotune.builder_kwargs(cfg), int(cfg.get("n_streams", 1))
+    kw = autotune.builder_kwargs(cfg)
+    if mixed:
+        kw.update(autotune.mixed_kwargs(cfg))
+    return kw, int(cfg.get("n_streams", 1))
 
 
 def load_selector(path):
@@ -162,7 +167,7 @@ def c3_block(steps, warmup, selector):
     plain = FockBuilder(s.basis, s.D, tau=1e-12, lane_flops=1e6)
     t_plain = timed(lambda: plain.build(Dd), steps, warmup)
     G64 = plain.F.cpu().numpy()
-    tuning, ns = load_tuning("c1_3atom.json")
+    tuning, ns = load_tuning("c1_3atom.json", mixed=True)
     fb, t_prod = production(s, Dd, tuning, ns, selector, steps, warmup)
     Gp = fb.replay().cpu().numpy()
     torch.cuda.synchronize()
@@ -495,7 +500,8 @@ def main() -> None:
     t_t64 = timed(fb64.replay, a.steps, a.warmup)
     log(f"tuned fp64 graph: {a.steps / t_t64:.3f} builds/s")
 
-    fbp, t_prod = production(s, Dd, tuning, ns, selector, a.steps, a.warmup)
+    tuning_mx = (load_tuning("c2_40atom.json", mixed=True)[0] if a.atoms == 40 else tuning)
+    fbp, t_prod = production(s, Dd, tuning_mx, ns, selector, a.steps, a.warmup)
     Gp = fbp.replay().cpu().numpy()
     torch.cuda.synchronize()
     acc = accuracy(D, Gp, G64)

@@ -97,6 +97,24 @@ def tune(systems, threads=THREADS, lane_flops=LANE_FLOPS, streams=STREAMS, log=N
             "launch_ms": table}
 
 
+def tier_pays(systems, tuned: dict, margin: float = 0.02, log=None, **kw) -> list:
+    """The class pairs whose mixed launch (per-item FP32 / FP64 tiers, kw
+    the mixed builder's keywords, e.g. the selector) is measured faster than
+    their FP64 launch at the tuned threads and split, by more than
+    `margin`: a mixed build sends work to the FP32 tier only there (on
+    MI355X, FP32 VALU without packed math issues at the FP64 rate, so the
+    tier pays only where its smaller register footprint does)."""
+    D_devs = [native.to_dev(D) for _, D in systems]
+    kwt = builder_kwargs(tuned)
+    t64 = launch_times(FockBuilder.batch(systems, **kwt), D_devs, reps=5)
+    tmx = launch_times(FockBuilder.batch(systems, mixed=True, **kwt, **kw), D_devs, reps=5)
+    out = sorted(c for c in tmx if c in t64 and tmx[c] < (1.0 - margin) * t64[c])
+    if log:
+        log({"".join(map(str, c)): (round(t64[c], 3), round(tmx.get(c, float("nan")), 3))
+             for c in sorted(t64)})
+    return ["".join(map(str, c)) for c in out]
+
+
 def save(cfg: dict, path: Path) -> None:
     Path(path).write_text(json.dumps(cfg, indent=1, sort_keys=True))
 
@@ -108,5 +126,43 @@ def load(path: Path) -> dict:
 def builder_kwargs(cfg: dict) -> dict:
     """FockBuilder keywords of a tuned configuration."""
     key = lambda s: tuple(int(ch) for ch in s)  # noqa: E731
-    return {"threads": {key(k): v for k, v in cfg["threads"].items()},
-            "lane_flops": {key(k): v for k, v in cfg["lane_flops"].items()}}
+    out = {"threads": {key(k): v for k, v in cfg["threads"].items()},
+           "lane_flops": {key(k): v for k, v in cfg["lane_flops"].items()}}
+    return out
+
+
+def mixed_kwargs(cfg: dict) -> dict:
+    """The mixed builder's extra keywords of a tuned configuration: the class
+    pairs where the FP32 tier was measured to pay (none recorded: every
+    class pair)."""
+    if "fp32_classes" not in cfg:
+        return {}
+    return {"fp32_classes": [tuple(int(ch) for ch in c) for c in cfg["fp32_classes"]]}
+
+
+def main() -> None:
+    """Measure where the FP32 tier pays for a tuned configuration and record
+    it in the configuration file (`fp32_classes`)."""
+    import argparse
+    import sys
+    ap = argparse.ArgumentParser()
+    ap.add_argument("config", help="tuned configuration file (JSON) to update")
+    ap.add_argument("--workload", choices=("c1", "c2"), required=True)
+    ap.add_argument("--selector", default=str(Path(__file__).resolve().parent / "tuned" / "selector"))
+    ap.add_argument("--out", default=None, help="write the updated configuration here instead")
+    a = ap.parse_args()
+    from .selector import LearnedSelector
+    from .systems import config1, config2
+    s = config1() if a.workload == "c1" else config2()
+    sel = LearnedSelector.load(a.selector)
+    cfg = load(a.config)
+    pays = tier_pays([(s.basis, s.D)], cfg, selector=sel,
+                     log=lambda m: print("[tier_pays] fp64 ms, mixed ms:", m, flush=True))
+    cfg["fp32_classes"] = pays
+    save(cfg, a.out or a.config)
+    print(f"[tier_pays] {len(pays)} class pairs: {pays}", flush=True)
+    sys.exit(0)
+
+
+if __name__ == "__main__":
+    main()

@@ -239,78 +239,135 @@ __device__ __forceinline__ double fj_wave_max(double v) {
     return v;
 }
 
+// The network staged in LDS for one wavefront per sample: W1 and W2
+// transposed (W1T[k][j], W2T[k][j]), so that lane j's reads of row k are
+// consecutive words (no bank conflicts), then b1, b2, w3, b3 -- the same
+// values fj_mlp_stage stages, so the arithmetic and its order are
+// fj_mlp_eval's.
+__device__ void fj_mlp_stage_t(int F, FjMlpW W, float* sW) {
+    const int n = fj_mlp_nparam(F);
+    const int o_b1 = FJ_MLP_H * F, o_W2 = o_b1 + FJ_MLP_H, o_b2 = o_W2 + FJ_MLP_H * FJ_MLP_H;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int dst = i;
+        float v;
+        if (i < o_b1) {                      // W1[j][k] -> W1T[k][j]
+            const int j = i / F, k = i - j * F;
+            dst = k * FJ_MLP_H + j;
+            v = fj_bf(W.w[i]);
+        } else if (i < o_W2) {               // b1
+            v = W.p[i];
+        } else if (i < o_b2) {               // W2[j][k] -> W2T[k][j]
+            const int r = i - o_W2, j = r / FJ_MLP_H, k = r - j * FJ_MLP_H;
+            dst = o_W2 + k * FJ_MLP_H + j;
+            v = fj_bf(W.w[i]);
+        } else if (i < o_b2 + FJ_MLP_H || i == n - 1) {   // b2, b3
+            v = W.p[i];
+        } else {                             // w3
+            v = fj_bf(W.w[i]);
+        }
+        sW[dst] = v;
+    }
+    __syncthreads();
+}
+
+// One work item's tier, by one wavefront (hx: the wavefront's LDS slots,
+// 2 * 64 + 32 floats; sW: the network staged by fj_mlp_stage_t).
+__device__ __forceinline__ void fj_tier_item(const FjSide& bra, const FjSide& ket, const FjSys* sys,
+                                             const FjTierSel& t, int it, const float* sW, float* hx) {
+    const int lane = threadIdx.x & 63;
+    const int4 w = reinterpret_cast<const int4*>(t.work)[it];
+    const FjSys S = sys[w.z & 0xffff];
+    const int b = w.x, sa = bra.sh[2 * b], sb = bra.sh[2 * b + 1], ns = S.n_sh;
+    const double Gb = bra.G[b];
+    const double amp_b = t.amp_bra[b];
+    const double dab = 2.0 * S.Dsh[sa * ns + sb];
+    double est = -1.0, dm_max = 0.0;
+    for (int e = w.y + lane; e < w.w; e += 64) {
+        const int k = t.ket_list[e] & 0x1ffff;
+        if (t.diag && k < b) continue;
+        const int sc = ket.sh[2 * k], sd = ket.sh[2 * k + 1];
+        double dm = fmax(dab, 2.0 * S.Dsh[sc * ns + sd]);
+        dm = fmax(dm, 0.5 * S.Dsh[sa * ns + sc]);
+        dm = fmax(dm, 0.5 * S.Dsh[sa * ns + sd]);
+        dm = fmax(dm, 0.5 * S.Dsh[sb * ns + sc]);
+        dm = fmax(dm, 0.5 * S.Dsh[sb * ns + sd]);
+        const double bound = Gb * (ket.G[k] * dm);
+        if (bound < t.tau) continue;   // the ERI kernel's screen skips this quartet
+        const double e_k = t.fp32_bound >= 0.0 ? bound : bound * amp_b * t.amp_ket[k] * t.est_scale;
+        est = fmax(est, e_k);
+        dm_max = fmax(dm_max, dm);
+    }
+    est = fj_wave_max(est);
+    dm_max = fj_wave_max(dm_max);
+    int fp32 = 0;
+    if (est >= 0.0) {   // some quartet of the item passes the screen
+        fp32 = t.fp32_bound >= 0.0 ? (est < t.fp32_bound) : (est < t.fp32_error);
+        if (!fp32 && t.feat && t.inside[it]) {
+            const double xd = log10(dm_max);
+            if (xd >= t.d_lo && xd <= t.d_hi) {
+                const int F = t.n_feat;
+                float* x = hx + 2 * FJ_MLP_H;
+                if (lane < F - 1) x[lane] = t.feat[(size_t)it * F + lane];
+                if (lane == 0) x[F - 1] = (float)((xd - t.d_mean) / t.d_std);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                const float* W1T = sW;
+                const float* b1 = W1T + FJ_MLP_H * F;
+                const float* W2T = b1 + FJ_MLP_H;
+                const float* b2 = W2T + FJ_MLP_H * FJ_MLP_H;
+                const float* w3 = b2 + FJ_MLP_H;
+                float a = b1[lane];
+                for (int k = 0; k < F; ++k) a = fmaf(W1T[k * FJ_MLP_H + lane], x[k], a);
+                hx[lane] = fmaxf(a, 0.0f);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                a = b2[lane];
+#pragma unroll 8
+                for (int k = 0; k < FJ_MLP_H; ++k) a = fmaf(W2T[k * FJ_MLP_H + lane], hx[k], a);
+                hx[FJ_MLP_H + lane] = fmaxf(a, 0.0f);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                float y = w3[FJ_MLP_H];
+                for (int j = 0; j < FJ_MLP_H; ++j) y = fmaf(w3[j], hx[FJ_MLP_H + j], y);
+                fp32 = (double)y + t.margin < t.log_budget;
+                __builtin_amdgcn_wave_barrier();   // hx is reused by the next item
+            }
+        }
+    }
+    if (lane == 0) t.tier[it] = fp32;
+}
+
 __global__ void __launch_bounds__(256) fj_tier_select_k(FjSide bra, FjSide ket, const FjSys* sys,
                                                         FjTierSel t, FjMlpW W) {
     extern __shared__ float smem[];
-    const bool net = t.feat != nullptr;
-    const int np = net ? fj_mlp_nparam(t.n_feat) : 0;
-    float* sW = smem;
+    const int np = t.feat ? fj_mlp_nparam(t.n_feat) : 0;
     float* hx = smem + np + (threadIdx.x >> 6) * (2 * FJ_MLP_H + FJ_MLP_FMAX);  // per wavefront
-    if (net) fj_mlp_stage(t.n_feat, W, sW);   // ends with a barrier
-    const int lane = threadIdx.x & 63;
+    if (t.feat) fj_mlp_stage_t(t.n_feat, W, smem);   // ends with a barrier
     const int waves = (int)(gridDim.x * (blockDim.x >> 6));
     for (int it = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); it < t.n_items;
-         it += waves) {
-        const int4 w = reinterpret_cast<const int4*>(t.work)[it];
-        const FjSys S = sys[w.z & 0xffff];
-        const int b = w.x, sa = bra.sh[2 * b], sb = bra.sh[2 * b + 1], ns = S.n_sh;
-        const double Gb = bra.G[b];
-        const double amp_b = t.amp_bra[b];
-        const double dab = 2.0 * S.Dsh[sa * ns + sb];
-        double est = -1.0, dm_max = 0.0;
-        for (int e = w.y + lane; e < w.w; e += 64) {
-            const int k = t.ket_list[e] & 0x1ffff;
-            if (t.diag && k < b) continue;
-            const int sc = ket.sh[2 * k], sd = ket.sh[2 * k + 1];
-            double dm = fmax(dab, 2.0 * S.Dsh[sc * ns + sd]);
-            dm = fmax(dm, 0.5 * S.Dsh[sa * ns + sc]);
-            dm = fmax(dm, 0.5 * S.Dsh[sa * ns + sd]);
-            dm = fmax(dm, 0.5 * S.Dsh[sb * ns + sc]);
-            dm = fmax(dm, 0.5 * S.Dsh[sb * ns + sd]);
-            const double bound = Gb * (ket.G[k] * dm);
-            if (bound < t.tau) continue;   // the ERI kernel's screen skips this quartet
-            const double e_k = t.fp32_bound >= 0.0 ? bound : bound * amp_b * t.amp_ket[k] * t.est_scale;
-            est = fmax(est, e_k);
-            dm_max = fmax(dm_max, dm);
+         it += waves)
+        fj_tier_item(bra, ket, sys, t, it, smem, hx);
+}
+
+// Every mixed launch of a build in one launch: job j's items are the global
+// items [first[j], first[j + 1]); one network (W, n_feat) for every job.
+__global__ void __launch_bounds__(256) fj_tier_select_jobs_k(const FjTierJob* __restrict__ jobs,
+                                                             int n_jobs, int n_items, const FjSys* sys,
+                                                             int n_feat, FjMlpW W) {
+    extern __shared__ float smem[];
+    const int np = n_feat ? fj_mlp_nparam(n_feat) : 0;
+    float* hx = smem + np + (threadIdx.x >> 6) * (2 * FJ_MLP_H + FJ_MLP_FMAX);
+    if (n_feat) fj_mlp_stage_t(n_feat, W, smem);
+    const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+    for (int g = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); g < n_items; g += waves) {
+        int lo = 0, hi = n_jobs - 1;   // the last job whose first item is <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].first <= g) lo = mid;
+            else hi = mid - 1;
         }
-        est = fj_wave_max(est);
-        dm_max = fj_wave_max(dm_max);
-        int fp32 = 0;
-        if (est >= 0.0) {   // some quartet of the item passes the screen
-            fp32 = t.fp32_bound >= 0.0 ? (est < t.fp32_bound) : (est < t.fp32_error);
-            if (!fp32 && net && t.inside[it]) {
-                const double xd = log10(dm_max);
-                if (xd >= t.d_lo && xd <= t.d_hi) {
-                    const int F = t.n_feat;
-                    float* x = hx + 2 * FJ_MLP_H;
-                    if (lane < F - 1) x[lane] = t.feat[(size_t)it * F + lane];
-                    if (lane == 0) x[F - 1] = (float)((xd - t.d_mean) / t.d_std);
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    const float* W1 = sW;
-                    const float* b1 = W1 + FJ_MLP_H * F;
-                    const float* W2 = b1 + FJ_MLP_H;
-                    const float* b2 = W2 + FJ_MLP_H * FJ_MLP_H;
-                    const float* w3 = b2 + FJ_MLP_H;
-                    float a = b1[lane];
-                    for (int k = 0; k < F; ++k) a = fmaf(W1[lane * F + k], x[k], a);
-                    hx[lane] = fmaxf(a, 0.0f);
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    a = b2[lane];
-#pragma unroll 8
-                    for (int k = 0; k < FJ_MLP_H; ++k) a = fmaf(W2[lane * FJ_MLP_H + k], hx[k], a);
-                    hx[FJ_MLP_H + lane] = fmaxf(a, 0.0f);
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    float y = w3[FJ_MLP_H];
-                    for (int j = 0; j < FJ_MLP_H; ++j) y = fmaf(w3[j], hx[FJ_MLP_H + j], y);
-                    fp32 = (double)y + t.margin < t.log_budget;
-                    __builtin_amdgcn_wave_barrier();   // hx is reused by the next item
-                }
-            }
-        }
-        if (lane == 0) t.tier[it] = fp32;
+        const FjTierJob& J = jobs[lo];
+        fj_tier_item(J.bra, J.ket, sys, J.sel, g - J.first, smem, hx);
     }
 }
 
@@ -328,5 +385,20 @@ extern "C" int fj_tier_select(hipStream_t s, const FjSide* bra, const FjSide* ke
     const size_t lds = sizeof(float) * (np + 4 * (2 * FJ_MLP_H + FJ_MLP_FMAX));
     const int blocks = (sel->n_items + 3) / 4 < 1024 ? (sel->n_items + 3) / 4 : 1024;
     hipLaunchKernelGGL(fj_tier_select_k, dim3(blocks), dim3(256), lds, s, *bra, *ket, sys, *sel, W);
+    return (int)hipGetLastError();
+}
+
+extern "C" int fj_tier_select_jobs(hipStream_t s, int n_jobs, const FjTierJob* jobs, int n_items,
+                                   const FjSys* sys, int n_feat, const unsigned short* Wb, const float* P) {
+    if (n_jobs < 0 || n_items < 0 || !sys || (n_jobs && !jobs) || n_feat < 0 || n_feat > FJ_MLP_FMAX ||
+        n_feat == 1 || (n_feat && (!Wb || !P)))
+        return FJ_ERR_ARG;
+    if (n_jobs == 0 || n_items == 0) return 0;
+    FjMlpW W{reinterpret_cast<const uint16_t*>(Wb), P};
+    const int np = n_feat ? fj_mlp_nparam(n_feat) : 0;
+    const size_t lds = sizeof(float) * (np + 4 * (2 * FJ_MLP_H + FJ_MLP_FMAX));
+    const int blocks = (n_items + 3) / 4 < 1024 ? (n_items + 3) / 4 : 1024;
+    hipLaunchKernelGGL(fj_tier_select_jobs_k, dim3(blocks), dim3(256), lds, s, jobs, n_jobs, n_items, sys,
+                       n_feat, W);
     return (int)hipGetLastError();
 }

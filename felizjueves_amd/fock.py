@@ -231,18 +231,19 @@ def orient_pairs(basis: Basis, pairs: list) -> list:
     return out
 
 
-def merge_tier_launches(launches: list, every: bool = False) -> list:
+def merge_tier_launches(launches: list, every=False) -> list:
     """Each class pair's FP64 and FP32 launches as one MIXED launch: the
     FP64 work items (in their order) then the FP32 ones, the FP32 items'
     ket-list ranges moved past the FP64 list; biggest launches first.
     every: a class pair with items of one tier only is a MIXED launch too
-    (so that every launch of a mixed build takes per-build tiers)."""
+    (so that every launch of a mixed build takes per-build tiers); a
+    predicate of the class pair limits that to the class pairs it accepts."""
     by = {}
     for L in launches:
         by.setdefault(L.cls, {})[L.precision] = L
     out = []
     for cls, d in by.items():
-        if every and len(d) == 1:
+        if (every(cls) if callable(every) else every) and len(d) == 1:
             (tier, a), = d.items()
             M = Launch(cls, a.bra, a.ket, a.diag, MIXED, a.work)
             M.threads, M.ket_list = a.threads, a.ket_list
@@ -281,10 +282,11 @@ class FockBuilder:
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads=64,
                  mixed: bool = False, fp32_error: float = 1e-11, selector=None,
                  lane_flops=None, fp32_bound: float | None = None, planner: str = "native",
-                 merge_tiers: bool = True, orient: bool = True):
+                 merge_tiers: bool = True, orient: bool = True, fp32_classes=None):
         self._setup([(basis, D)], tau=tau, threads=threads, mixed=mixed, fp32_error=fp32_error,
                      selector=selector, lane_flops=lane_flops, fp32_bound=fp32_bound,
-                     planner=planner, merge_tiers=merge_tiers, orient=orient)
+                     planner=planner, merge_tiers=merge_tiers, orient=orient,
+                     fp32_classes=fp32_classes)
 
     @classmethod
     def batch(cls, systems, **kw) -> "FockBuilder":
@@ -298,7 +300,7 @@ class FockBuilder:
     def _setup(self, systems, *, tau: float = 1e-12, threads=64, mixed: bool = False,
                fp32_error: float = 1e-11, selector=None, lane_flops=None,
                fp32_bound: float | None = None, planner: str = "native",
-               merge_tiers: bool = True, orient: bool = True):
+               merge_tiers: bool = True, orient: bool = True, fp32_classes=None):
         """threads, lane_flops: one value for every class pair, or a dict
         {(la, lb, lc, ld): value} (the autotuner's choice per class pair;
         missing ones take the default 64 and the automatic split).
@@ -313,9 +315,15 @@ class FockBuilder:
         nearer their product centre (`orient_pairs`): a pair may run as
         (b, a), in class (lb, la), and a mixed pair as two partial pairs.
         n_quartets then counts the partial quartets; shell_quartets() the
-        contracted shell quartets."""
+        contracted shell quartets.
+        fp32_classes: the class pairs (la, lb, lc, ld) a mixed build may
+        send to the FP32 tier (the autotuner's choice: where the mixed
+        launch is measured faster than the FP64 one); the others run FP64
+        launches. None: every class pair."""
         self.merge_tiers = bool(merge_tiers)
         self.orient = bool(orient)
+        self.fp32_classes = None if fp32_classes is None else {tuple(int(x) for x in c)
+                                                               for c in fp32_classes}
         self.fp32_bound = fp32_bound
         if not systems:
             raise ValueError("no systems")
@@ -388,11 +396,15 @@ class FockBuilder:
         self.planner = "python" if (planner == "python" or (mixed and selector is not None)) else "native"
         self.launches = self._plan() if self.planner == "python" else self._plan_native()
         if self.merge_tiers:
-            self.launches = merge_tier_launches(self.launches, every=self.mixed)
+            self.launches = merge_tier_launches(self.launches, every=self.mixed_for)
         self.plan_seconds = time.time() - t0
         self._dev = False
 
     # -- planning ---------------------------------------------------------
+    def mixed_for(self, cls) -> bool:
+        """Whether class pair cls may put work in the FP32 tier."""
+        return self.mixed and (self.fp32_classes is None or tuple(cls) in self.fp32_classes)
+
     def _bounds(self, bra: SideTable, ket: SideTable, b: int) -> np.ndarray:
         return self._bounds_dmax(bra, ket, b)[0]
 
@@ -426,12 +438,12 @@ class FockBuilder:
         tiles or None)]."""
         T = self.threads
         classes = sorted(self.tables)
-        want_feats = self.mixed and self.selector is not None
         out = []
         for i, X in enumerate(classes):
             for Y in classes[i:]:
                 bra, ket = self.tables[X], self.tables[Y]
                 diag = X == Y
+                want_feats = self.mixed_for((*X, *Y)) and self.selector is not None
                 tiles, feats = [], []
                 for b in range(bra.n):
                     lo, bound, dmax = self._bounds_range(bra, ket, b)
@@ -448,7 +460,7 @@ class FockBuilder:
                     kl = kl[hot]
                     ks = kl + lo
                     bound, dmax = bound[kl], dmax[kl]
-                    if self.mixed:
+                    if self.mixed_for((*X, *Y)):
                         if self.fp32_bound is not None:
                             small = bound < self.fp32_bound
                         else:
@@ -480,7 +492,7 @@ class FockBuilder:
         systems = NP.Systems(self.Dsh_plans)
         classes = sorted(self.tables)
         pairs = [(X, Y) for i, X in enumerate(classes) for Y in classes[i:]]
-        common = dict(tau=self.tau, mixed=self.mixed, fp32_error=self.fp32_error,
+        common = dict(tau=self.tau, fp32_error=self.fp32_error,
                       fp32_bound=self.fp32_bound, est_scale=FP32_EST_SCALE,
                       max_prim=P.MAX_PRIM_PAIRS)
         if self.lane_flops is None:
@@ -489,7 +501,8 @@ class FockBuilder:
                 c = cost((*X, *Y))
                 r, _ = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems, diag=int(X == Y),
                                      threads=64, lane_flops=0.0, per_prim=c.per_prim,
-                                     per_quartet=c.per_quartet, **common)
+                                     per_quartet=c.per_quartet, mixed=self.mixed_for((*X, *Y)),
+                                     **common)
                 total += r[0][2][2] + r[1][2][2]
             auto_lf = LANE_FLOPS_MAX if total >= SPLIT_MAX_FLOPS else \
                 float(min(LANE_FLOPS_MAX, max(1.0, total / LANES_TARGET)))
@@ -504,7 +517,8 @@ class FockBuilder:
             lf = self.class_lane_flops.get(cls, auto_lf)
             r, n_cold = NP.plan_pairs(sides[X], sides[Y], self.tables[Y], systems,
                                       diag=int(X == Y), threads=T, lane_flops=lf,
-                                      per_prim=c.per_prim, per_quartet=c.per_quartet, **common)
+                                      per_prim=c.per_prim, per_quartet=c.per_quartet,
+                                      mixed=self.mixed_for(cls), **common)
             for tier, (w, k, stats) in zip((FP64, FP32), r):
                 if w is None:
                     continue
@@ -725,6 +739,22 @@ class FockBuilder:
         sel.tier = L.tier_dev.data_ptr()
         L.tier_sel, L.tier_keep = sel, keep
 
+    def _tier_jobs(self):
+        """(device job table, n_jobs, n_items, network) of the mixed
+        launches' per-build tier choice (built once), or None."""
+        if getattr(self, "_tjobs", None) is None:
+            jobs, first = [], 0
+            for L in self.launches:
+                if L.tier_dev is not None:
+                    jobs.append((L.bra.side, L.ket.side, L.tier_sel, first))
+                    first += L.work.shape[0]
+            if not jobs:
+                self._tjobs = ()
+            else:
+                net = self.selector.net if hasattr(self.selector, "net") else None
+                self._tjobs = (native.tier_jobs(jobs), len(jobs), first, net)
+        return self._tjobs or None
+
     def _sys(self, D_devs):
         """The system table for the device densities D_devs (rebuilt when a
         density buffer changes)."""
@@ -800,10 +830,10 @@ class FockBuilder:
         for m, D in enumerate(D_devs):
             native.density_blocks(self.shell_ao[m], self.shell_size[m], D, self.Dshs[m], main)
         if self.mixed:
-            # every mixed launch's tiers, from this density
-            for L in self.launches:
-                if L.tier_dev is not None:
-                    native.tier_select(L.bra.side, L.ket.side, sys, L.tier_sel, main)
+            # every mixed launch's tiers, from this density, in one launch
+            jobs = self._tier_jobs()
+            if jobs is not None:
+                native.tier_select_jobs(jobs[0], jobs[1], jobs[2], sys, jobs[3], main)
         self._built = True
         if n_streams <= 1:
             ws = self._workspace(0)

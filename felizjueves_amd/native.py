@@ -170,6 +170,32 @@ class FjTierSel(C.Structure):
                 ("margin", C.c_double), ("log_budget", C.c_double), ("tier", C.c_void_p)]
 
 
+class FjTierJob(C.Structure):
+    """Mirror of ``FjTierJob`` in include/fj_mi355x.h."""
+
+    _fields_ = [("bra", FjSide), ("ket", FjSide), ("sel", FjTierSel), ("first", C.c_int),
+                ("pad", C.c_int)]
+
+
+def tier_jobs(jobs) -> "torch.Tensor":
+    """A device array of FjTierJob from (bra FjSide, ket FjSide, FjTierSel,
+    first item) tuples."""
+    arr = (FjTierJob * len(jobs))()
+    for i, (b, k, sel, first) in enumerate(jobs):
+        arr[i] = FjTierJob(b, k, sel, first, 0)
+    return to_dev(np.frombuffer(bytes(arr), dtype=np.uint8).copy())
+
+
+def tier_select_jobs(jobs_dev, n_jobs: int, n_items: int, sys, net=None, stream=None) -> None:
+    """Every mixed launch's per-build tiers in one launch (fj_tier_select_jobs)."""
+    fn = lib().fj_tier_select_jobs
+    fn.restype = C.c_int
+    check(fn(C.c_void_p(stream_handle(stream)), C.c_int(n_jobs), C.c_void_p(ptr(jobs_dev)),
+             C.c_int(n_items), C.c_void_p(ptr(sys)), C.c_int(net.F if net is not None else 0),
+             C.c_void_p(ptr(net.Wb) if net is not None else None),
+             C.c_void_p(ptr(net.P) if net is not None else None)), "fj_tier_select_jobs")
+
+
 def tier_select(bra: FjSide, ket: FjSide, sys, sel: FjTierSel, stream=None) -> None:
     """The per-build tiers of one mixed launch, on the device (fj_tier_select)."""
     fn = lib().fj_tier_select

@@ -1367,6 +1367,19 @@ typedef struct FjTierSel {
 } FjTierSel;
 int fj_tier_select(fj_stream_t stream, const FjSide* bra, const FjSide* ket, const FjSys* sys,
                    const FjTierSel* sel);
+/* Every mixed launch of a build in ONE launch: `jobs` is a DEVICE array of
+ * n_jobs records (the two sides and the FjTierSel of one mixed launch, its
+ * items the global items [first, next job's first)), n_items the total;
+ * every job with `feat` set uses the one network Wb / P of n_feat inputs
+ * (n_feat 0: no job has a network). */
+typedef struct FjTierJob {
+    FjSide bra, ket;
+    FjTierSel sel;
+    int first;
+    int pad;
+} FjTierJob;
+int fj_tier_select_jobs(fj_stream_t stream, int n_jobs, const FjTierJob* jobs, int n_items,
+                        const FjSys* sys, int n_feat, const unsigned short* Wb, const float* P);
 
 /* The split HGP J and K kernels (upstream eri/hjk/kernel_hj.py:1-140 and
  * kernel_hk.py:1-110, canonical class mode; parameter list params_of,
