@@ -171,6 +171,10 @@ __device__ __forceinline__ void fj_kl_tower(double* R0, double* R1, const double
     }
 }
 
+// An includer that needs only the helpers above (the E tables, the tower)
+// defines FJ_KLOOP_HELPERS_ONLY first; the kernels below stay in one unit.
+#ifndef FJ_KLOOP_HELPERS_ONLY
+
 // The K kernel's LDS, sized for its class and carved from one dynamic block:
 // byte offsets of each array (doubles, then ints, then the u16 index table)
 // and the total. The launcher and the kernel both use this layout.
@@ -420,3 +424,5 @@ __global__ void __launch_bounds__(64) fj_schwarz_loop_k(
         }
     }
 }
+
+#endif  // FJ_KLOOP_HELPERS_ONLY

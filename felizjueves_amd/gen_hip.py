@@ -771,6 +771,77 @@ def generate_split(reps_masks, out: Path = GEN) -> list[Path]:
     return files
 
 
+# ---------------------------------------------------------------------------
+# The density-fitting classes (upstream eri/density_fit/kernel_metric.py,
+# kernel_eri3.py): per-class entries with the upstream parameter lists onto
+# the loop-form kernel (csrc/fj_dfloop.hip).
+
+DF_AUX_LMAX = 6      # aux shells through i
+DF_AO_LMAX = 3       # AO shells through f
+
+
+def df_classes():
+    """(metric classes (lP <= lQ), three-center classes (la <= lb, lP))."""
+    met = [(p, q) for p in range(DF_AUX_LMAX + 1) for q in range(p, DF_AUX_LMAX + 1)]
+    e3 = [(a, b, p) for a in range(DF_AO_LMAX + 1) for b in range(a, DF_AO_LMAX + 1)
+          for p in range(DF_AUX_LMAX + 1)]
+    return met, e3
+
+
+METRIC_SIG = ("int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha,"
+              " const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0,"
+              " const int* q_rec1, const int* off_p, const int* off_q, double* metric")
+ERI3_SIG = ("int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux,"
+            " const double* zeta_ab, const double* P, const double* XPA, const double* XPB,"
+            " const double* s_ab, const double* A, const double* alpha, const double* s_aux,"
+            " const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1,"
+            " const int* off_a, const int* off_b, const int* off_p, double* eri3")
+
+
+def emit_df_entries() -> str:
+    met, e3 = df_classes()
+    L = ["// generated by felizjueves_amd/gen_hip.py -- density-fitting classes on the loop-form kernel",
+         '#include "../fj_device.h"', "",
+         'extern "C" int fj_metric_loop(hipStream_t s, int lP, int lQ, ' + METRIC_SIG + ");",
+         'extern "C" int fj_eri3_loop(hipStream_t s, int la, int lb, int lP, ' + ERI3_SIG + ");", ""]
+    for p, q in met:
+        L += [f'extern "C" int fj_metric_{p}{q}(hipStream_t s, {METRIC_SIG}) {{',
+              f"    return fj_metric_loop(s, {p}, {q}, threads, n_pair, n_rec, n_aux, A, alpha, s_aux,"
+              " p_rec0, p_rec1, q_rec0, q_rec1, off_p, off_q, metric);", "}", ""]
+    for a, b, p in e3:
+        L += [f'extern "C" int fj_eri3_{a}{b}{p}(hipStream_t s, {ERI3_SIG}) {{',
+              f"    return fj_eri3_loop(s, {a}, {b}, {p}, threads, n_cell, n_bra_rec, n_aux_rec, n_ao,"
+              " n_aux, zeta_ab, P, XPA, XPB, s_ab, A, alpha, s_aux, ab_rec0, ab_rec1, p_rec0, p_rec1,"
+              " off_a, off_b, off_p, eri3);", "}", ""]
+    return "\n".join(L)
+
+
+def df_header_decls() -> str:
+    met, e3 = df_classes()
+    out = [f"int fj_metric_{p}{q}(fj_stream_t stream, {METRIC_SIG});" for p, q in met]
+    out += [f"int fj_eri3_{a}{b}{p}(fj_stream_t stream, {ERI3_SIG});" for a, b, p in e3]
+    return "\n".join(out) + "\n"
+
+
+DF_BEGIN, DF_END = ("/* BEGIN DF CLASSES (written by felizjueves_amd/gen_hip.py) */",
+                    "/* END DF CLASSES */")
+
+
+def write_df(out: Path = GEN) -> Path:
+    """gen/df_entries.hip and the header's DF section."""
+    out.mkdir(parents=True, exist_ok=True)
+    p = out / "df_entries.hip"
+    text = emit_df_entries()
+    if not p.exists() or p.read_text() != text:
+        p.write_text(text)
+    cur = HEADER.read_text()
+    i, j = cur.index(DF_BEGIN), cur.index(DF_END)
+    new = cur[:i + len(DF_BEGIN)] + "\n" + df_header_decls() + cur[j:]
+    if new != cur:
+        HEADER.write_text(new)
+    return p
+
+
 SPLIT_BEGIN, SPLIT_END = ("/* BEGIN SPLIT CLASSES (written by felizjueves_amd/gen_hip.py) */",
                           "/* END SPLIT CLASSES */")
 

@@ -1608,6 +1608,133 @@ int fj_hk_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, 
     const double* Dsh, const double* D, double* K);
 /* END SPLIT CLASSES */
 
+/* Density fitting (upstream eri/density_fit/, the metric and three-center
+ * integrals; DF-J and DF-K are out of the upstream's scope too).
+ * fj_metric_loop / fj_metric_<lP><lQ>: (P|Q) of the aux shell pairs of class
+ * (lP <= lQ, each <= 6), the upstream kernel_metric parameter list
+ * (kernel_metric.py:460-480): aux records A[n_rec][3], alpha, s_aux = c
+ * (pi/alpha)^{3/2}; per pair the record ranges [p_rec0, p_rec1), [q_rec0,
+ * q_rec1) and the first aux functions off_p, off_q; metric [n_aux][n_aux]
+ * written (both triangles; a diagonal pair writes a >= b, mirrored).
+ * fj_eri3_loop / fj_eri3_<la><lb><lP>: (mn|P) of the cells (AO shell pair
+ * la <= lb <= 3, aux shell lP <= 6), the upstream kernel_eri3 parameter
+ * list (kernel_eri3.py:805-838): bra records (zeta_ab, P, XPA, XPB, s_ab =
+ * c_m c_n (pi/zeta)^{3/2}), aux records, per cell the two record ranges and
+ * off_a, off_b, off_p; eri3 [n_ao][n_ao][n_aux] written, mirrored in (m, n)
+ * (a diagonal shell pair writes a >= b).  Plain stores, no accumulation:
+ * every element belongs to one cell.  The loop entries take the class as
+ * arguments (FJ_ERR_CLASS past it). */
+int fj_metric_loop(fj_stream_t stream, int lP, int lQ, int threads, int n_pair, int n_rec,
+                   int n_aux, const double* A, const double* alpha, const double* s_aux,
+                   const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1,
+                   const int* off_p, const int* off_q, double* metric);
+int fj_eri3_loop(fj_stream_t stream, int la, int lb, int lP, int threads, int n_cell,
+                 int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab,
+                 const double* P, const double* XPA, const double* XPB, const double* s_ab,
+                 const double* A, const double* alpha, const double* s_aux, const int* ab_rec0,
+                 const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a,
+                 const int* off_b, const int* off_p, double* eri3);
+/* BEGIN DF CLASSES (written by felizjueves_amd/gen_hip.py) */
+int fj_metric_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_01(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_02(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_03(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_04(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_05(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_06(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_11(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_12(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_13(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_14(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_15(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_16(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_22(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_23(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_24(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_25(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_26(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_33(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_34(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_35(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_36(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_44(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_45(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_46(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_55(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_56(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_metric_66(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
+int fj_eri3_000(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_001(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_002(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_003(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_004(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_005(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_006(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_010(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_011(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_012(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_013(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_014(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_015(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_016(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_020(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_021(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_022(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_023(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_024(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_025(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_026(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_030(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_031(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_032(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_033(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_034(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_035(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_036(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_110(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_111(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_112(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_113(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_114(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_115(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_116(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_120(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_121(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_122(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_123(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_124(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_125(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_126(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_130(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_131(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_132(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_133(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_134(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_135(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_136(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_220(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_221(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_222(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_223(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_224(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_225(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_226(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_230(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_231(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_232(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_233(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_234(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_235(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_236(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_330(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_331(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_332(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_333(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_334(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_335(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+int fj_eri3_336(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
+/* END DF CLASSES */
+
 /* BEGIN MMD CLASSES (written by felizjueves_amd/gen_mmd.py) */
 int fj_schwarz_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
     const double* zeta_ab, const double* P, const double* XPA, const double* XPB,

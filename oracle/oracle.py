@@ -83,6 +83,64 @@ def threads() -> int:
     return int(lib().oracle_threads())
 
 
+class _Flat:
+    """Shell arrays in the oracle's layout (any l the MMD path takes)."""
+
+    def __init__(self, parts):
+        self.shell_l = np.concatenate([p[0] for p in parts]).astype(np.int32)
+        self.shell_centre = np.ascontiguousarray(np.concatenate([p[1] for p in parts]), dtype=np.float64)
+        self.shell_nprim = np.concatenate([p[2] for p in parts]).astype(np.int32)
+        self.exps = np.ascontiguousarray(np.concatenate([p[3] for p in parts]), dtype=np.float64)
+        self.coefs = np.ascontiguousarray(np.concatenate([p[4] for p in parts]), dtype=np.float64)
+        self.shell_pstart = np.concatenate([[0], np.cumsum(self.shell_nprim)[:-1]]).astype(np.int32)
+        self.shells = [type("S", (), {"n_cart": (int(l) + 1) * (int(l) + 2) // 2})()
+                       for l in self.shell_l]  # noqa: E741
+
+
+def _parts(b):
+    return (b.shell_l, b.shell_centre, b.shell_nprim, b.exps, b.coefs)
+
+
+# the unit s "shell": one primitive of exponent 0 and coefficient 1. Paired
+# with an aux shell P it is P's single-Gaussian Hermite pair: zeta = alpha,
+# the product centre A, no Gaussian factor (exp(0) = 1), dressing
+# c (pi/alpha)^{3/2} -- the upstream's aux record (metric.py:1-90) -- so the
+# oracle's four-center MMD integral (P unit|Q unit) is (P|Q) and (mn|P unit)
+# is (mn|P). Pinned to libcint's int2c2e / int3c2e in the tests.
+_UNIT = (np.array([0]), np.zeros((1, 3)), np.array([1]), np.array([0.0]), np.array([1.0]))
+
+
+def df_metric(aux) -> np.ndarray:
+    """(P|Q) of every aux function, [n_aux, n_aux]."""
+    f = _Flat([_parts(aux), _UNIT])
+    u = len(f.shell_l) - 1
+    off = np.concatenate([[0], np.cumsum([s.n_cart for s in f.shells[:u]])])
+    M = np.zeros((off[-1], off[-1]))
+    for p in range(u):
+        for q in range(p, u):
+            x = eri(f, (p, u, q, u), "mmd")[:, 0, :, 0]
+            M[off[p]:off[p + 1], off[q]:off[q + 1]] = x
+            M[off[q]:off[q + 1], off[p]:off[p + 1]] = x.T
+    return M
+
+
+def df_eri3(basis, aux) -> np.ndarray:
+    """(mn|P), [n_ao, n_ao, n_aux]."""
+    f = _Flat([_parts(basis), _parts(aux), _UNIT])
+    nb, u = len(basis.shell_l), len(f.shell_l) - 1
+    sz = [s.n_cart for s in f.shells]
+    ao = np.concatenate([[0], np.cumsum(sz[:nb])])
+    xo = np.concatenate([[0], np.cumsum(sz[nb:u])])
+    out = np.zeros((ao[-1], ao[-1], xo[-1]))
+    for m in range(nb):
+        for n in range(m, nb):
+            for p in range(u - nb):
+                x = eri(f, (m, n, nb + p, u), "mmd")[:, :, :, 0]
+                out[ao[m]:ao[m + 1], ao[n]:ao[n + 1], xo[p]:xo[p + 1]] = x
+                out[ao[n]:ao[n + 1], ao[m]:ao[m + 1], xo[p]:xo[p + 1]] = x.transpose(1, 0, 2)
+    return out
+
+
 def schwarz(basis) -> np.ndarray:
     n = basis.n_ao
     Q = np.zeros((n, n))

@@ -22,6 +22,11 @@ Each output `<name>.npz` holds:
     eri_keys [nblk,4] i4, eri_off [nblk+1] i8, eri_vals   int2e blocks, one
                                                shell quartet each, row-major
                                                (n_i, n_j, n_k, n_l)
+
+and each density-fitting set `df_<name>.npz` (DF_SETS) holds the auxiliary
+basis as aux_shell_l, aux_shell_centre, aux_shell_nprim, aux_exps,
+aux_coefs, with int2c2e and int1e_ovlp [n_aux, n_aux] (metric sets) or the
+AO basis and int3c2e [n_ao, n_ao, n_aux] (three-center sets).
 """
 
 from __future__ import annotations
@@ -91,9 +96,65 @@ def convert(name: str, sources: list[str]) -> dict[str, np.ndarray]:
     return out
 
 
+# density-fitting sets: output name -> (kind, source chunk files, the AO
+# basis's source fixture or None). Each chunk holds a block of rows (metric)
+# or aux columns (three-center) of libcint's int2c2e_cart / int3c2e_cart.
+DF_SETS = {
+    "df_h2o_sto3g_jkfit_3c": ("3c", ["h2o_sto3g_jkfit_3c_cols0_166"], "h2o_sto3g_j"),
+    "df_prim_spd_f_auxi_3c": ("3c", ["prim_spd_f_auxi_3c_cols0_35", "prim_spd_f_auxi_3c_cols35_84",
+                                     "prim_spd_f_auxi_3c_cols84_119",
+                                     "prim_spd_f_auxi_3c_cols119_140",
+                                     "prim_spd_f_auxi_3c_cols140_168"], "prim_spd_f"),
+    "df_prim_spd_f_auxi_metric": ("metric", ["prim_spd_f_auxi_metric_rows0_56",
+                                             "prim_spd_f_auxi_metric_rows56_104",
+                                             "prim_spd_f_auxi_metric_rows104_140",
+                                             "prim_spd_f_auxi_metric_rows140_168"], None),
+    "df_h2o_ccpvtz_jkfit_metric": ("metric", ["h2o_ccpvtz_jkfit_metric_rows0_43",
+                                              "h2o_ccpvtz_jkfit_metric_rows43_81",
+                                              "h2o_ccpvtz_jkfit_metric_rows81_121",
+                                              "h2o_ccpvtz_jkfit_metric_rows121_166"], None),
+}
+
+
+def convert_df(kind: str, sources: list[str], parent: str | None) -> dict[str, np.ndarray]:
+    """The auxiliary basis (aux_* arrays, the same converted convention as
+    the AO basis), the AO basis of the parent fixture (three-center sets)
+    and libcint's array assembled from its chunks: int2c2e [n_aux, n_aux]
+    and int1e_ovlp [n_aux, n_aux], or int3c2e [n_ao, n_ao, n_aux]."""
+    out: dict[str, np.ndarray] = {}
+    parts, ovl = [], []
+    for src in sources:
+        fx = json.loads((SRC / f"{src}.json").read_text())
+        sec = fx["metric" if kind == "metric" else "three_center"]
+        aux = {"aux_" + k: v for k, v in _basis(sec["converted"]).items()}
+        for k, v in aux.items():
+            if k in out and not np.array_equal(out[k], v):
+                raise ValueError(f"{src}: aux basis differs from the set's")
+            out[k] = v
+        arrays = fx["arrays"]
+        if kind == "metric":
+            parts.append(_arr(arrays["int2c2e_cart:rows"]))
+            ovl.append(_arr(arrays["int1e_ovlp:rows"]))
+        else:
+            (key,) = [k for k in arrays if k.startswith("int3c2e_cart")]
+            parts.append(_arr(arrays[key]))
+    if kind == "metric":
+        out["int2c2e"] = np.concatenate(parts, axis=0)
+        out["int1e_ovlp"] = np.concatenate(ovl, axis=0)
+    else:
+        out["int3c2e"] = np.concatenate(parts, axis=2)
+        fx = json.loads((SRC / f"{parent}.json").read_text())
+        out.update(_basis(fx["converted"]))
+    return out
+
+
 def main() -> None:
     for name, sources in SETS.items():
         data = convert(name, sources)
+        np.savez_compressed(OUT / f"{name}.npz", **data)
+        print(name, {k: v.shape for k, v in data.items()})
+    for name, (kind, sources, parent) in DF_SETS.items():
+        data = convert_df(kind, sources, parent)
         np.savez_compressed(OUT / f"{name}.npz", **data)
         print(name, {k: v.shape for k, v in data.items()})
 
