@@ -141,6 +141,31 @@ def cpu_baseline(fb, D, G_gpu=None):
     return out
 
 
+def stream_block(s) -> dict:
+    """The upstream pair stream (f_launches, eq. 29 with n = 3) of the
+    workload planned on the host (pairs.f_launches) and on the device
+    (stream.f_launches_device: bounds, prune, keys, bins, sort and
+    launches by the library's kernels): seconds per plan and whether the
+    two plans agree. An SCF caller of the walker re-plans once per density;
+    the native builder's plan does not depend on the density at all."""
+    from felizjueves_amd import pairs as P
+    from felizjueves_amd.stream import DeviceStream, f_launches_device
+    t = time.perf_counter()
+    host = P.f_launches(s.basis, s.D, tau=1e-12, n=3)
+    t_host = time.perf_counter() - t
+    ds = DeviceStream(s.basis)
+    f_launches_device(s.basis, s.D, tau=1e-12, n=3, ds=ds)      # warm
+    dev, tm = f_launches_device(s.basis, s.D, tau=1e-12, n=3, ds=ds)
+    same = ([(L.bra, L.ket) for L in dev.launches] == [(L.bra, L.ket) for L in host.launches]
+            and [(b.la, b.lb, b.degree, b.mag, len(b.pairs)) for b in dev.batches]
+            == [(b.la, b.lb, b.degree, b.mag, len(b.pairs)) for b in host.batches])
+    out = {"workload": s.name, "host_plan_s": t_host, "device_plan_s": tm["device_s"],
+           "device_plan_to_fplan_s": tm["host_s"], "batches": len(dev.batches),
+           "launches": len(dev.launches), "same_plan_as_host": bool(same)}
+    log("stream", json.dumps(out))
+    return out
+
+
 def accuracy(D, G, Gref):
     from felizjueves_amd.fock import energy
     d = G - Gref
@@ -540,6 +565,7 @@ def main() -> None:
         extra["c3"] = c3_block(max(10, a.steps), a.warmup, selector)
         extra["c5"] = c5_block(max(5, a.steps // 2), a.warmup, selector)
         extra["jk"] = jk_block(3, 1)
+        extra["pair_stream"] = stream_block(s)
     cpu = None if a.no_cpu else cpu_baseline(plain, D, G64)
     out = {
         "metric": METRIC, "value": a.steps / t_prod, "unit": "Fock builds/s", "n_gpus": 1,

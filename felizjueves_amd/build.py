@@ -63,7 +63,7 @@ def sources(lmax: int = LMAX) -> list[Path]:
     mmd = gen_mmd.generate(MMD_LMAX, k_lmax=MMD_K_LMAX)
     gen_mmd.write_header(MMD_LMAX, MMD_K_LMAX)
     return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", CSRC / "fj_kloop.hip",
-            CSRC / "fj_noalloc.hip", CSRC / "fj_dfloop.hip", *files, *mmd]
+            CSRC / "fj_noalloc.hip", CSRC / "fj_dfloop.hip", CSRC / "fj_stream.hip", *files, *mmd]
 
 
 def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, float, str]:

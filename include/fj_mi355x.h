@@ -1608,6 +1608,59 @@ int fj_hk_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, 
     const double* Dsh, const double* D, double* K);
 /* END SPLIT CLASSES */
 
+/* The fused-F pair stream on the device (upstream eri/fock/launch_f.py:
+ * 390-470, f_launches; ao/pairs.py:270-290,528-560 for the primitive-pair
+ * bound and the prune): for the unique shell pairs pair_ab (canonical, in
+ * the upstream's unique-pair order) with their primitive pairs at
+ * prim_off, the primitive bounds g, the (compensated) sums G, eps = tau /
+ * (2 max G), each pair's kept primitive pairs (local indices, bound-
+ * descending, stably; the longest tail with sum < eps dropped), dmax from
+ * Dsh, the key G dmax (G with ket_break), the eq. 29 magnitude index
+ * against mags[i] = tau^(i/n), and the batches: the kept pairs in bin order
+ * (la, lb, degree, magnitude) ascending, each bin by key descending (stable
+ * in pair order), with each batch's start, magnitude index and largest G.
+ * Everything is written to the DEVICE arrays of `out`; `ws` is a device
+ * workspace of fj_stream_workspace_bytes(n_pair, n_prim).
+ * fj_stream_launches then lists the batch pairs I <= J (I outer) whose
+ * magnitude indices sum below n: launches[k] = (I, J), n_launch of them
+ * (row_off [n_batch + 1], launches [n_batch (n_batch + 1) / 2][2]). */
+typedef struct FjStreamIn {
+    int n_pair, n_prim, n_sh, n, ket_break;
+    double tau;
+    const int* pair_ab;
+    const int* prim_off;
+    const int* shell_l;
+    const double* centre;
+    const int* nprim;
+    const int* pstart;
+    const double* exps;
+    const double* coefs;
+    const double* Dsh;
+    const double* mags;
+} FjStreamIn;
+typedef struct FjStreamOut {
+    double* g;
+    double* G;
+    double* dmax;
+    double* key;
+    int* degree;
+    int* mag;
+    int* keep;
+    int* order;
+    int* batch_start;
+    int* batch_mag;
+    unsigned long long* batch_gmax_bits;
+    unsigned long long* gmax_bits;
+    double* eps;
+    int* n_batch;
+    int* n_kept;
+} FjStreamOut;
+long long fj_stream_workspace_bytes(int n_pair, int n_prim);
+int fj_stream_plan(fj_stream_t stream, const FjStreamIn* in, FjStreamOut* out, void* ws,
+                   long long ws_bytes);
+int fj_stream_launches(fj_stream_t stream, int n_batch, int n, const int* batch_mag, int* row_off,
+                       int* launches, int* n_launch);
+
 /* Density fitting (upstream eri/density_fit/, the metric and three-center
  * integrals; DF-J and DF-K are out of the upstream's scope too).
  * fj_metric_loop / fj_metric_<lP><lQ>: (P|Q) of the aux shell pairs of class
