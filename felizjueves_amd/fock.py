@@ -87,6 +87,9 @@ SPLIT_MAX_FLOPS = 4.0e9
 # there): at most this many bytes.
 WS_MAX_BYTES = 8 << 30
 KET_INDEX_LIMIT = 1 << 17  # ket-list entries: pair index in bits 0-16
+# Below this many shell pairs a mixed builder with a learned selector plans
+# in Python, so the selector also groups its FP32 tiles at plan time.
+SELECTOR_PLAN_PAIRS = 50_000
 
 
 @dataclass
@@ -281,7 +284,7 @@ class FockBuilder:
 
     def __init__(self, basis: Basis, D: np.ndarray, *, tau: float = 1e-12, threads=64,
                  mixed: bool = False, fp32_error: float = 1e-11, selector=None,
-                 lane_flops=None, fp32_bound: float | None = None, planner: str = "native",
+                 lane_flops=None, fp32_bound: float | None = None, planner: str = "auto",
                  merge_tiers: bool = True, orient: bool = True, fp32_classes=None):
         self._setup([(basis, D)], tau=tau, threads=threads, mixed=mixed, fp32_error=fp32_error,
                      selector=selector, lane_flops=lane_flops, fp32_bound=fp32_bound,
@@ -299,7 +302,7 @@ class FockBuilder:
 
     def _setup(self, systems, *, tau: float = 1e-12, threads=64, mixed: bool = False,
                fp32_error: float = 1e-11, selector=None, lane_flops=None,
-               fp32_bound: float | None = None, planner: str = "native",
+               fp32_bound: float | None = None, planner: str = "auto",
                merge_tiers: bool = True, orient: bool = True, fp32_classes=None):
         """threads, lane_flops: one value for every class pair, or a dict
         {(la, lb, lc, ld): value} (the autotuner's choice per class pair;
@@ -391,9 +394,18 @@ class FockBuilder:
         # selector(features [n, N_FEATURES]) -> bool [n]: True moves an FP64
         # tile of the bound rule to the FP32 tier (mixed mode only)
         self.selector = selector
-        # the native planner unless the learned selector needs the tiles'
-        # features (the Python planner, which it reproduces exactly)
-        self.planner = "python" if (planner == "python" or (mixed and selector is not None)) else "native"
+        # the native planner, unless the learned selector should also group
+        # the tiles it sends to FP32 at plan time (the Python planner, the
+        # same plan otherwise; "auto" does that below SELECTOR_PLAN_PAIRS
+        # shell pairs). Either way the device picks every mixed work item's
+        # tier per build, with the network when there is one.
+        n_pairs = sum(t.n for t in self.tables.values())
+        if planner == "auto":
+            planner = ("python" if (mixed and selector is not None and n_pairs < SELECTOR_PLAN_PAIRS)
+                       else "native")
+        if planner not in ("python", "native"):
+            raise ValueError(f"planner = {planner!r}: 'native', 'python' or 'auto'")
+        self.planner = planner
         self.launches = self._plan() if self.planner == "python" else self._plan_native()
         if self.merge_tiers:
             self.launches = merge_tier_launches(self.launches, every=self.mixed_for)
