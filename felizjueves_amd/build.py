@@ -60,10 +60,11 @@ def sources(lmax: int = LMAX) -> list[Path]:
     gen_hip.write_split_header(reps)
     files += split
     files.append(gen_hip.write_df())   # density fitting: metric and three-center entries
+    files.append(gen_hip.write_gf())   # the fused two-electron gradient's entries
     mmd = gen_mmd.generate(MMD_LMAX, k_lmax=MMD_K_LMAX)
     gen_mmd.write_header(MMD_LMAX, MMD_K_LMAX)
     return [CSRC / "fj_runtime.hip", CSRC / "fj_mlp.hip", CSRC / "fj_plan.hip", CSRC / "fj_kloop.hip",
-            CSRC / "fj_noalloc.hip", CSRC / "fj_dfloop.hip", CSRC / "fj_stream.hip", *files, *mmd]
+            CSRC / "fj_noalloc.hip", CSRC / "fj_dfloop.hip", CSRC / "fj_stream.hip", CSRC / "fj_gfloop.hip", *files, *mmd]
 
 
 def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, float, str]:
@@ -74,8 +75,10 @@ def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, fl
     hdr = src.parent / (src.stem.replace("_r32", "") + ".h")
     if hdr.exists():
         deps.append(hdr)
-    if src.name in ("fj_kloop.hip", "fj_dfloop.hip"):
+    if src.name in ("fj_kloop.hip", "fj_dfloop.hip", "fj_gfloop.hip"):
         deps.append(CSRC / "fj_kloop.h")
+    if src.name == "fj_gfloop.hip":
+        deps.append(CSRC / "fj_gfloop.h")
     if src.name == "fj_kloop.hip":
         deps.append(CSRC / "fj_floop.h")
     if src.name == "fj_noalloc.hip":

@@ -842,6 +842,61 @@ def write_df(out: Path = GEN) -> Path:
     return p
 
 
+# ---------------------------------------------------------------------------
+# The fused two-electron gradient: one entry per class of the gradient
+# stream ((la lb | lc ld), la <= lb, lc <= ld, (la, lb) <= (lc, ld)) onto the
+# loop-form kernel fj_gf_loop, the upstream gf_ parameter list.
+
+GF_LMAX = 2
+
+GF_SIG = ("int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag,"
+          " const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao,"
+          " const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh,"
+          " const int* ket_ao, const int* ket_at, const double* D, double* G")
+GF_ARGS = ("threads, n_bra, n_ket, k_ab, k_cd, n_atom, n_ao, diag, bra_prim, AB, bra_sh, bra_ao,"
+           " bra_at, ket_prim, CD, ket_sh, ket_ao, ket_at, D, G")
+
+
+def gf_classes(lmax: int = GF_LMAX):
+    prs = [(a, b) for a in range(lmax + 1) for b in range(a, lmax + 1)]
+    return [(*p, *q) for i, p in enumerate(prs) for q in prs[i:]]
+
+
+def emit_gf_entries() -> str:
+    L = ["// generated by felizjueves_amd/gen_hip.py -- fused-gradient classes on the loop-form kernel",
+         '#include "../fj_device.h"', "",
+         'extern "C" int fj_gf_loop(hipStream_t s, int la, int lb, int lc, int ld, ' + GF_SIG + ");", ""]
+    for c in gf_classes():
+        name = "".join(map(str, c))
+        L += [f'extern "C" int fj_gf_{name}(hipStream_t s, {GF_SIG}) {{',
+              f"    return fj_gf_loop(s, {', '.join(map(str, c))}, {GF_ARGS});", "}", ""]
+    return "\n".join(L)
+
+
+def gf_header_decls() -> str:
+    return "\n".join(f"int fj_gf_{''.join(map(str, c))}(fj_stream_t stream, {GF_SIG});"
+                     for c in gf_classes()) + "\n"
+
+
+GF_BEGIN, GF_END = ("/* BEGIN GRADIENT CLASSES (written by felizjueves_amd/gen_hip.py) */",
+                    "/* END GRADIENT CLASSES */")
+
+
+def write_gf(out: Path = GEN) -> Path:
+    """gen/gf_entries.hip and the header's gradient section."""
+    out.mkdir(parents=True, exist_ok=True)
+    p = out / "gf_entries.hip"
+    text = emit_gf_entries()
+    if not p.exists() or p.read_text() != text:
+        p.write_text(text)
+    cur = HEADER.read_text()
+    i, j = cur.index(GF_BEGIN), cur.index(GF_END)
+    new = cur[:i + len(GF_BEGIN)] + "\n" + gf_header_decls() + cur[j:]
+    if new != cur:
+        HEADER.write_text(new)
+    return p
+
+
 SPLIT_BEGIN, SPLIT_END = ("/* BEGIN SPLIT CLASSES (written by felizjueves_amd/gen_hip.py) */",
                           "/* END SPLIT CLASSES */")
 

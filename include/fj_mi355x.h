@@ -1788,6 +1788,57 @@ int fj_eri3_335(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int 
 int fj_eri3_336(fj_stream_t stream, int threads, int n_cell, int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab, const double* P, const double* XPA, const double* XPB, const double* s_ab, const double* A, const double* alpha, const double* s_aux, const int* ab_rec0, const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a, const int* off_b, const int* off_p, double* eri3);
 /* END DF CLASSES */
 
+/* ---------------------------------------------------------------------------
+ * The fused two-electron nuclear gradient (replaces the upstream gf_ kernels,
+ * gradient/kernel_gf.py:523 build_gf_kernel, its parameter list
+ * kernel_gf.py:501-512 and host stream gradient/interpreted.py:293-351):
+ *
+ *   G[atom][k] += dE_2e / dR_{atom,k},  E_2e = 1/2 tr(D (J - K/2))
+ *
+ * over one launch's unique quartets, bra pair I x ket pair J (J >= I on the
+ * diagonal launch), with the weights w = deg (4 D_ab D_cd - D_ac D_bd -
+ * D_ad D_bc), all four centres differentiated. A side is packed as the
+ * upstream's gf_ side (kernel_gf.py:366-395): prim [n][k][10] = P(3),
+ * X = P - first centre (3), zeta, U = c1 c2 (pi/zeta)^{3/2}
+ * exp(-e1 e2/zeta |sep|^2), e1, e2 (the two primitive exponents); sep [n][3]
+ * = first centre - second (AB, CD); sh, ao, at [n][2] the shell indices,
+ * AO offsets and atoms. bra_sh is read only when la == lb and ket_sh only
+ * when lc == ld (the upstream omits them otherwise; NULL is accepted then).
+ * G [n_atom][3] is accumulated with atomics and zeroed by the caller before
+ * the first launch; there is no screen and no tau (the upstream's gf_ has
+ * none). Classes through (dd|dd); FJ_ERR_CLASS past d.
+ * ------------------------------------------------------------------------- */
+int fj_gf_loop(fj_stream_t stream, int la, int lb, int lc, int ld, int threads, int n_bra,
+               int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag,
+               const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao,
+               const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh,
+               const int* ket_ao, const int* ket_at, const double* D, double* G);
+/* The LDS bytes of one workgroup of the class (0 past d). */
+int fj_gf_lds_bytes(int la, int lb, int lc, int ld);
+/* BEGIN GRADIENT CLASSES (written by felizjueves_amd/gen_hip.py) */
+int fj_gf_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+int fj_gf_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
+/* END GRADIENT CLASSES */
+
 /* BEGIN MMD CLASSES (written by felizjueves_amd/gen_mmd.py) */
 int fj_schwarz_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_ao,
     const double* zeta_ab, const double* P, const double* XPA, const double* XPB,

@@ -23,6 +23,8 @@
  *   oracle_fock_ordered  J, K by EVERY ordered quartet, weight 1
  *                                                     eri/fock/fock.py:126-176
  *   oracle_schwarz       Q_mn = sqrt(max((mn|mn), 0))   eri/schwarz.py:64-106
+ *   oracle_grad_ordered  dE_2e/dR by EVERY ordered quartet, the first
+ *                        centre differentiated  gradient/kernel_gf.py:12-60
  *
  * Parity pins (tests/test_oracle_*.py): libcint int2e blocks, vj, vk and
  * int2e_diag of the tracked fixtures; Boys closed forms and a 50-digit
@@ -723,6 +725,98 @@ void oracle_schwarz(int n_sh, const int *shell_l, const double *centre,
             }
         free(x);
     }
+}
+
+/* dE_2e/dR, E_2e = 1/2 tr(D J) - 1/4 tr(D K) = sum_ijkl Gam_ijkl (ij|kl)
+ * (gradient/kernel_gf.py:12-24), by every ORDERED shell quartet with only
+ * its first function's centre differentiated: the four positions of an
+ * ordered sweep are equivalent under the ERI symmetry, so with Gam
+ * symmetrised over it,
+ *
+ *   dE/dR_{A,x} = sum_{ijkl, i on A} g_ijkl d_i,x (ij|kl),
+ *   g_ijkl = 2 D_ij D_kl - 1/2 (D_ik D_jl + D_il D_jk)
+ *
+ * and d_i,x (ij|kl) = (i+1_x j|kl)^{2 alpha} - i_x (i-1_x j|kl) (the
+ * derivative of a Cartesian Gaussian, gradient/kernel_gf.py:44-50): the
+ * raised block from the shell at l + 1 with coefficients c 2 alpha, the
+ * lowered one from the shell at l - 1. G [n_atom][3] is overwritten. This is
+ * another formulation than the unique-quartet digestion (no weights w, no
+ * degeneracy, one centre), pinned to PySCF's gradient terms and to finite
+ * differences of oracle_fock_ordered's energy in the tests. */
+void oracle_grad_ordered(int n_sh, const int *shell_l, const double *centre,
+                         const int *nprim, const int *pstart, const double *exps,
+                         const double *coefs, const int *shell_ao, const int *shell_atom,
+                         int n_ao, const double *D, int n_atom, double *G) {
+    memset(G, 0, sizeof(double) * n_atom * 3);
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    double *Gp = calloc((size_t)nthr * n_atom * 3, sizeof(double));
+    int np_max = 1;
+    for (int s = 0; s < n_sh; ++s)
+        if (nprim[s] > np_max) np_max = nprim[s];
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *Gt = Gp + (size_t)tid * n_atom * 3;
+        double *up = malloc(sizeof(double) * 15 * 15 * 15 * 15);
+        double *dn = malloc(sizeof(double) * 15 * 15 * 15 * 15);
+        double *cup = malloc(sizeof(double) * np_max);
+#pragma omp for schedule(dynamic, 1) collapse(2)
+        for (int a = 0; a < n_sh; ++a)
+            for (int b = 0; b < n_sh; ++b) {
+                shell_t A = mkshell(a, shell_l, centre, nprim, pstart, exps, coefs);
+                shell_t B = mkshell(b, shell_l, centre, nprim, pstart, exps, coefs);
+                shell_t Au = A, Ad = A;
+                for (int i = 0; i < A.np; ++i) cup[i] = 2.0 * A.ex[i] * A.co[i];
+                Au.l = A.l + 1;
+                Au.co = cup;
+                Ad.l = A.l - 1;
+                int la = A.l, na = ncart(la), nb = ncart(B.l);
+                int oa = shell_ao[a], ob = shell_ao[b], at = shell_atom[a];
+                for (int c = 0; c < n_sh; ++c)
+                    for (int d = 0; d < n_sh; ++d) {
+                        shell_t Cc = mkshell(c, shell_l, centre, nprim, pstart, exps, coefs);
+                        shell_t Dd = mkshell(d, shell_l, centre, nprim, pstart, exps, coefs);
+                        int nc = ncart(Cc.l), nd = ncart(Dd.l);
+                        int oc = shell_ao[c], od = shell_ao[d];
+                        eri_mmd_sh(&Au, &B, &Cc, &Dd, up);
+                        if (la > 0) eri_mmd_sh(&Ad, &B, &Cc, &Dd, dn);
+                        for (int i = 0; i < na; ++i) {
+                            cart3 ci = cart_of(la, i);
+                            for (int j = 0; j < nb; ++j)
+                                for (int k = 0; k < nc; ++k)
+                                    for (int l = 0; l < nd; ++l) {
+                                        double g = 2.0 * D[(oa + i) * n_ao + ob + j] * D[(oc + k) * n_ao + od + l]
+                                                 - 0.5 * (D[(oa + i) * n_ao + oc + k] * D[(ob + j) * n_ao + od + l]
+                                                        + D[(oa + i) * n_ao + od + l] * D[(ob + j) * n_ao + oc + k]);
+                                        for (int x = 0; x < 3; ++x) {
+                                            int e[3] = {ci.c[0], ci.c[1], ci.c[2]};
+                                            e[x] += 1;
+                                            int iu = cidx(e[0], e[1], e[2]);
+                                            double v = up[((iu * nb + j) * nc + k) * nd + l];
+                                            if (ci.c[x] > 0) {
+                                                e[x] -= 2;
+                                                int id = cidx(e[0], e[1], e[2]);
+                                                v -= (double)ci.c[x] * dn[((id * nb + j) * nc + k) * nd + l];
+                                            }
+                                            Gt[at * 3 + x] += g * v;
+                                        }
+                                    }
+                        }
+                    }
+            }
+        free(up);
+        free(dn);
+        free(cup);
+    }
+    for (int t = 0; t < nthr; ++t)
+        for (int i = 0; i < n_atom * 3; ++i) G[i] += Gp[(size_t)t * n_atom * 3 + i];
+    free(Gp);
 }
 
 int oracle_version(void) { return 1; }

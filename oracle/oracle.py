@@ -41,6 +41,8 @@ def lib():
         L.oracle_fock_ordered.argtypes = ([C.c_int] + basis_args +
                                           [i32p, C.c_int, f64p, f64p, f64p])
         L.oracle_schwarz.argtypes = [C.c_int] + basis_args + [i32p, C.c_int, f64p]
+        L.oracle_grad_ordered.argtypes = ([C.c_int] + basis_args +
+                                          [i32p, i32p, C.c_int, f64p, C.c_int, f64p])
         vp = C.c_void_p
         L.oracle_f_launch.argtypes = ([C.c_int] * 11 + [C.c_double] +
                                       [vp] * 13)
@@ -76,6 +78,22 @@ def fock_ordered(basis, D) -> tuple[np.ndarray, np.ndarray]:
     lib().oracle_fock_ordered(basis.n_sh, *_b(basis), basis.shell_ao, n,
                               np.ascontiguousarray(D, dtype=np.float64), J, K)
     return J, K
+
+
+def grad_ordered(basis, shell_atom, n_atom: int, D) -> np.ndarray:
+    """dE_2e/dR [n_atom, 3] by every ordered shell quartet, the first
+    centre differentiated (fj_oracle.c's oracle_grad_ordered)."""
+    G = np.zeros((n_atom, 3))
+    lib().oracle_grad_ordered(basis.n_sh, *_b(basis), basis.shell_ao,
+                              np.ascontiguousarray(shell_atom, dtype=np.int32), basis.n_ao,
+                              np.ascontiguousarray(D, dtype=np.float64), int(n_atom), G)
+    return G
+
+
+def energy_2e(basis, D) -> float:
+    """E_2e = 1/2 tr(D J) - 1/4 tr(D K) by the ordered sweep."""
+    J, K = fock_ordered(basis, D)
+    return float(0.5 * np.sum(D * J) - 0.25 * np.sum(D * K))
 
 
 def threads() -> int:

@@ -27,6 +27,12 @@ and each density-fitting set `df_<name>.npz` (DF_SETS) holds the auxiliary
 basis as aux_shell_l, aux_shell_centre, aux_shell_nprim, aux_exps,
 aux_coefs, with int2c2e and int1e_ovlp [n_aux, n_aux] (metric sets) or the
 AO basis and int3c2e [n_ao, n_ao, n_aux] (three-center sets).
+
+Each gradient set `grad_<name>.npz` (GRAD_SETS) holds the converted basis,
+shell_atom [n_sh] i4 (the atom each shell sits on), atom_coord [n_atom,3],
+the converged density PySCF differentiated at, and its two-electron
+gradient terms grad_j, grad_k [n_atom,3] (PySCF's grad:j, grad:k), whose
+sum is dE_2e/dR for E_2e = 1/2 tr(D J) - 1/4 tr(D K).
 """
 
 from __future__ import annotations
@@ -148,6 +154,29 @@ def convert_df(kind: str, sources: list[str], parent: str | None) -> dict[str, n
     return out
 
 
+# gradient sets: output name -> the source fixture (PySCF 2.14.0 rhf.Gradients
+# at the fixture's converged density)
+GRAD_SETS = {
+    "grad_h2o_sto3g": "h2o_sto3g_grad",
+    "grad_h2o_sto3g_d": "h2o_sto3g_d_grad",
+    "grad_h2o_631g": "h2o_631g_grad",
+    "grad_h2o_ccpvdz": "h2o_ccpvdz_grad",
+    "grad_nh2f_631g": "nh2f_631g_grad",
+}
+
+
+def convert_grad(src: str) -> dict[str, np.ndarray]:
+    fx = json.loads((SRC / f"{src}.json").read_text())
+    out = _basis(fx["converted"])
+    out["shell_atom"] = np.array([s["atom"] for s in fx["converted"]["shells"]], dtype=np.int32)
+    out["atom_coord"] = np.array([a["coord"] for a in fx["atoms"]], dtype=np.float64)
+    arrays = fx["arrays"]
+    out["density"] = _arr(arrays["density"])
+    out["grad_j"] = _arr(arrays["grad:j"])
+    out["grad_k"] = _arr(arrays["grad:k"])
+    return out
+
+
 def main() -> None:
     for name, sources in SETS.items():
         data = convert(name, sources)
@@ -155,6 +184,10 @@ def main() -> None:
         print(name, {k: v.shape for k, v in data.items()})
     for name, (kind, sources, parent) in DF_SETS.items():
         data = convert_df(kind, sources, parent)
+        np.savez_compressed(OUT / f"{name}.npz", **data)
+        print(name, {k: v.shape for k, v in data.items()})
+    for name, src in GRAD_SETS.items():
+        data = convert_grad(src)
         np.savez_compressed(OUT / f"{name}.npz", **data)
         print(name, {k: v.shape for k, v in data.items()})
 
