@@ -17,6 +17,11 @@ Also reported:
   and the literal bound rule (FP32 iff Q_ab Q_cd max|D| < 1e-6) for
   comparison;
 * "c5": learned selector vs the bound rule on held-out systems;
+* "gradient": the fused two-electron nuclear gradient (dE_2e/dR, every
+  unique quartet, no screen) on the three-atom s,s,p,d system and a
+  10-atom cluster: gradients/s, its kernel's roofline (HIP events around
+  the merged per-class launches), max|dG| against the oracle and the
+  oracle's own time on the host cores for the three-atom system;
 * "roofline" of the dominant kernel (HIP events around its eager launches
   on the stream it runs on) and "cpu_baseline" (the oracle on the host
   cores, a bounded sample).
@@ -163,6 +168,64 @@ def stream_block(s) -> dict:
            "device_plan_to_fplan_s": tm["host_s"], "batches": len(dev.batches),
            "launches": len(dev.launches), "same_plan_as_host": bool(same)}
     log("stream", json.dumps(out))
+    return out
+
+
+def gradient_block(steps: int, cpu: bool = True) -> dict:
+    """The fused two-electron gradient (gradient.py, fj_gf_jobs: one merged
+    launch per class) on the three-atom s,s,p,d system and the 10-atom
+    cluster. Per system: seconds per gradient (HIP events on the stream
+    the launches go to, around whole gradients), the loop form's FLOPs
+    (gradient.flops) over that time against the FP64 peak, primitive
+    quartets/s; on the three-atom system max|dG| against the oracle
+    (oracle_grad_ordered) and, unless `cpu` is off, the oracle's time (the
+    whole gradient on the host cores)."""
+    import torch
+    from felizjueves_amd import gradient as GR
+    from felizjueves_amd import native
+    from felizjueves_amd.systems import config1, config2
+    out = {}
+    for key, s in (("c1_3atom", config1()), ("cluster_10atom", config2(n_atoms=10))):
+        at = GR.shell_atoms(s.basis, s.coords)
+        st = GR.gf_launches(s.basis, at)
+        run = GR.GpuGfRunner(st, len(s.coords), s.basis.n_ao)
+        Dd = native.to_dev(s.D)
+        stream = torch.cuda.current_stream()
+        run.run(Dd, stream)
+        torch.cuda.synchronize()
+        n = steps if key == "c1_3atom" else 3
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            run.run(Dd, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3 / n
+        fl = GR.flops(st)
+        r = {"workload": s.name, "n_ao": s.basis.n_ao, "quartets": GR.quartets(st),
+             "prim_quartets": GR.prim_quartets(st), "classes": len(run.classes),
+             "stream_launches": len(st.launches), "seconds_per_gradient": t,
+             "gradients_per_s": 1.0 / t, "prim_quartets_per_s": GR.prim_quartets(st) / t,
+             "roofline": {"bound": "fp64_valu", "achieved": fl / t / 1e12, "peak": FP64_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": fl / t / 1e12 / FP64_PEAK_TFLOPS,
+                          "gflop_per_gradient": fl / 1e9,
+                          "kernel": "fj_gfjobs_k (every class; its launches back to back)"}}
+        if key == "c1_3atom":
+            G = run.G.cpu().numpy()
+            sys.path.insert(0, str(ROOT / "tests"))
+            from oracle import oracle as O
+            tc = time.perf_counter()
+            Go = O.grad_ordered(s.basis, at, len(s.coords), s.D)
+            tc = time.perf_counter() - tc
+            r["max_dG_vs_oracle"] = float(np.abs(G - Go).max())
+            r["max_abs_G"] = float(np.abs(Go).max())
+            if cpu:
+                r["cpu_baseline"] = {"value": 1.0 / tc, "unit": "gradients/s", "cores": O.threads(),
+                                     "kind": "port",
+                                     "sample": ("one whole gradient of the three-atom system by the "
+                                                f"oracle's ordered sweep, {tc:.1f} s")}
+        log("gradient", key, json.dumps(r))
+        out[key] = r
     return out
 
 
@@ -484,12 +547,14 @@ def main() -> None:
     ap.add_argument("--no-extra", action="store_true", help="skip the c3 / c5 blocks")
     ap.add_argument("--molecules", type=int, nargs="+", default=[7],
                     help="c4f: which C4 molecules (index i: seed 4 + i)")
-    ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f", "c4f-batch"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f", "c4f-batch", "gradient"),
+                    default="c2",
                     help="c4d: the 64-molecule batch (8-80 atoms, seeds 4+i) with s,s,p,d shells "
                          "(with its f shells one build would take hours on the loop-form f "
                          "kernel); cof: CO/cc-pVTZ through f (the loop-form kernels); c4f: "
                          "single C4 molecules with and without their f shells; c4f-batch: "
-                         "chosen C4 molecules with f in one batch builder")
+                         "chosen C4 molecules with f in one batch builder; gradient: the fused "
+                         "two-electron gradient block alone")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for HBM counter runs: one elementwise copy of 512 MiB (the counters' "
                          "scale) and one eager build of the default plan, then exit")
@@ -532,6 +597,9 @@ def main() -> None:
         return
     if a.workload == "c4f-batch":
         c4f_batch_block(a)
+        return
+    if a.workload == "gradient":
+        print(json.dumps({"gradient": gradient_block(a.steps, cpu=not a.no_cpu)}), flush=True)
         return
 
     selector = load_selector(a.selector)
@@ -601,6 +669,7 @@ def main() -> None:
         extra["c5"] = c5_block(max(5, a.steps // 2), a.warmup, selector)
         extra["jk"] = jk_block(3, 1)
         extra["pair_stream"] = stream_block(s)
+        extra["gradient"] = gradient_block(max(5, a.steps // 2), cpu=not a.no_cpu)
     cpu = None if a.no_cpu else cpu_baseline(plain, D, G64)
     out = {
         "metric": METRIC, "value": a.steps / t_prod, "unit": "Fock builds/s", "n_gpus": 1,

@@ -26,10 +26,16 @@
 //   gA_k, gB_k += U_ab U_cd sum_ab sum_{|t| <= la+lb+1} dE^{ab}_t sum_u [t+u] Wk[ab][u]
 //   gC_k, gD_k += U_ab U_cd sum_cd sum_{|u| <= lc+ld+1} (-1)^|u| dE^{cd}_u sum_t [t+u] Wb[cd][t]
 //
-// One workgroup per unique quartet (threads = 64 .. 256 cooperating through
-// LDS): the per-axis E tables to l + 1 on every centre, the Hermite tower to
-// L + 1, then lanes over (ab, t) and (cd, u). The twelve sums are reduced over
-// the workgroup and added into G[atom][k] with twelve atomics per quartet.
+// A workgroup (threads = 64 .. 256 cooperating through LDS) takes unique
+// quartets in turn (grid-stride) and a quartet's primitive quartets in
+// batches of B: per batch the per-axis E tables to l + 1 on every centre and
+// the Hermite tower to L + 1 of each of its B primitive quartets, then lanes
+// over (b, ab, u), (b, cd, t), (b, ab, t) and (b, cd, u) -- L + 4 barriers
+// per batch, not per primitive quartet, so a highly contracted low class
+// fills the lanes and a (dd|dd) quartet (B = 1) has work for 256. B is the
+// largest that fits the LDS budget and the lanes (fj_gf_plan). The twelve
+// sums are reduced over the workgroup and added into G[atom][k] with twelve
+// atomics per quartet.
 // No screen (kernel_gf.py:82-85: the F screen bounds an integral, not its
 // derivative): every unique quartet of a launch is visited.
 //
@@ -48,6 +54,7 @@ constexpr int GF_MAXW = 4;    // waves per workgroup (threads <= 256)
 struct FjGfArgs {
     int la, lb, lc, ld;
     int n_bra, n_ket, k_ab, k_cd, n_atom, n_ao, diag;
+    int chunk;   // primitive quartets per work item (>= k_ab k_cd: whole quartets)
     const double* bra_prim;
     const double* AB;
     const int* bra_sh;
@@ -62,20 +69,28 @@ struct FjGfArgs {
     double* G;
 };
 
-// Byte offsets of the class's LDS arrays in one dynamic block (doubles, then
-// ints, then the u16 index table) and the total.
+// The class's launch plan: the batch B of primitive quartets, the byte
+// offsets of the LDS arrays in one dynamic block (per-quartet doubles, B
+// copies of the per-batch doubles, then ints and the u16 index table) and
+// the total.
 struct FjGfLds {
-    int R0, R1, eb, ek, wf, w, wk, wb, red, hc, tw, pab, pcd, add, bytes;
+    int B;
+    int w, red, R0, R1, eb, ek, wf, wk, wb, sc, hc, tw, pab, pcd, add, bytes;
     int axb, axk, nR, nu, nt, nu1, nt1;
 };
 
+constexpr int GF_LDS_BUDGET = 40 * 1024;   // bytes per workgroup
+constexpr int GF_BMAX = 64;                // primitive quartets per batch
+constexpr int GF_SC = 8;                   // per primitive quartet: U U, 2 x4 exponents, PQ
+
 __host__ __device__ inline int fj_gf_herm(int n) { return n * (n + 1) * (n + 2) / 6; }
 
-__host__ __device__ inline FjGfLds fj_gf_lds(int la, int lb, int lc, int ld) {
+__host__ __device__ inline FjGfLds fj_gf_layout(int la, int lb, int lc, int ld, int B) {
     const int L = la + lb + lc + ld, lab = la + lb, lcd = lc + ld;
     const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
     const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
     FjGfLds o;
+    o.B = B;
     o.nR = fj_gf_herm(L + 2);      // |t| <= L + 1
     o.nu = fj_gf_herm(lcd + 1);    // |u| <= lc + ld
     o.nt = fj_gf_herm(lab + 1);
@@ -85,15 +100,16 @@ __host__ __device__ inline FjGfLds fj_gf_lds(int la, int lb, int lc, int ld) {
     o.axk = (lc + 2) * (ld + 2) * (lcd + 3);
     int at = 0;
     auto take = [&](int n, int size) { const int r = at; at += ((n * size + 7) / 8) * 8; return r; };
-    o.R0 = take(o.nR, 8);
-    o.R1 = take(o.nR, 8);
-    o.eb = take(3 * o.axb, 8);
-    o.ek = take(3 * o.axk, 8);
-    o.wf = take(L + 2, 8);
     o.w = take(na * nb * nc * nd, 8);
-    o.wk = take(na * nb * o.nu, 8);
-    o.wb = take(nc * nd * o.nt, 8);
     o.red = take(12 * GF_MAXW, 8);
+    o.R0 = take(B * o.nR, 8);
+    o.R1 = take(B * o.nR, 8);
+    o.eb = take(B * 3 * o.axb, 8);
+    o.ek = take(B * 3 * o.axk, 8);
+    o.wf = take(B * (L + 2), 8);
+    o.wk = take(B * na * nb * o.nu, 8);
+    o.wb = take(B * nc * nd * o.nt, 8);
+    o.sc = take(B * GF_SC, 8);
     o.hc = take(o.nR, 4);
     o.tw = take(o.nR, 4);
     o.pab = take(na * nb, 4);
@@ -103,30 +119,25 @@ __host__ __device__ inline FjGfLds fj_gf_lds(int la, int lb, int lc, int ld) {
     return o;
 }
 
-// The Hermite tower (fj_kl_tower's levels, fj_kloop.h) with the
-// workgroup's lanes: threads tid, nthr of the block, a barrier per level.
-__device__ __forceinline__ void fj_gf_tower(double* R0, double* R1, const double* wf, const int* tw,
-                                            int L, double PQx, double PQy, double PQz, int tid,
-                                            int nthr) {
-    for (int m = L - 1; m >= 0; --m) {
-        double* __restrict__ cur = (m & 1) ? R1 : R0;
-        const double* __restrict__ prv = (m & 1) ? R0 : R1;
-        const int n_m = fj_herm_base(L - m + 1);
-        for (int j = tid; j < n_m; j += nthr) {
-            double v;
-            if (j == 0) {
-                v = wf[m];
-            } else {
-                const int w = tw[j];
-                const int d = (w >> 20) & 3, c1 = w >> 22;
-                const double pq = d == 0 ? PQx : (d == 1 ? PQy : PQz);
-                v = pq * prv[w & 1023];
-                if (c1 > 0) v += (double)c1 * prv[(w >> 10) & 1023];
-            }
-            cur[j] = v;
-        }
-        __syncthreads();
-    }
+// B for a launch: the largest that fits GF_LDS_BUDGET, at most GF_BMAX and
+// the quartet's n_pq primitive quartets, and no more than the lanes need
+// (about four elements per lane in the longest per-batch loop).
+__host__ __device__ inline FjGfLds fj_gf_plan(int la, int lb, int lc, int ld, int n_pq, int threads) {
+    const FjGfLds one = fj_gf_layout(la, lb, lc, ld, 1), two = fj_gf_layout(la, lb, lc, ld, 2);
+    const int per_b = two.bytes - one.bytes, fixed = one.bytes - per_b;
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+    const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
+    int el = na * nb * one.nt1;
+    if (nc * nd * one.nu1 > el) el = nc * nd * one.nu1;
+    if (na * nb * one.nu > el) el = na * nb * one.nu;
+    if (nc * nd * one.nt > el) el = nc * nd * one.nt;
+    int B = (GF_LDS_BUDGET - fixed) / per_b;
+    const int want = (4 * threads + el - 1) / el;
+    if (B > want) B = want;
+    if (B > GF_BMAX) B = GF_BMAX;
+    if (B > n_pq) B = n_pq;
+    if (B < 1) B = 1;
+    return fj_gf_layout(la, lb, lc, ld, B);
 }
 
 // Components of the i-th Cartesian of an l shell, packed x | y << 4 | z << 8.
@@ -148,18 +159,27 @@ __device__ __forceinline__ void fj_gf_axis(const double* __restrict__ e, int nj,
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) fj_gfloop_k(FjGfArgs g, FjGfLds o) {
+// One workgroup's share of a launch: work items blockIdx.x, + gridDim.x,
+// ... of n_items. A work item is one quartet's primitive quartets c chunk ..
+// (c + 1) chunk - 1 (a small launch's quartets split over several
+// workgroups, each adding its share into G). With a job table (several
+// launches of one class merged, jobs sorted by q0, each with its own chunk),
+// item i belongs to the last job whose q0 <= i and reads that job's sides
+// from g's arrays at the job's offsets; without one, g is the launch.
+__device__ __forceinline__ void fj_gf_body(const FjGfArgs& g0, const FjGfLds& o, const FjGfJob* jobs,
+                                           int n_jobs, long long n_items) {
     FJ_DYN_SHARED(fj_gfloop_lds_block);
     char* const lds = reinterpret_cast<char*>(fj_gfloop_lds_block);
+    double* const W = reinterpret_cast<double*>(lds + o.w);
+    double* const red = reinterpret_cast<double*>(lds + o.red);
     double* const R0 = reinterpret_cast<double*>(lds + o.R0);
     double* const R1 = reinterpret_cast<double*>(lds + o.R1);
     double* const eb = reinterpret_cast<double*>(lds + o.eb);
     double* const ek = reinterpret_cast<double*>(lds + o.ek);
     double* const wf = reinterpret_cast<double*>(lds + o.wf);
-    double* const W = reinterpret_cast<double*>(lds + o.w);
     double* const Wk = reinterpret_cast<double*>(lds + o.wk);
     double* const Wb = reinterpret_cast<double*>(lds + o.wb);
-    double* const red = reinterpret_cast<double*>(lds + o.red);
+    double* const sc = reinterpret_cast<double*>(lds + o.sc);
     int* const hc = reinterpret_cast<int*>(lds + o.hc);
     int* const tw = reinterpret_cast<int*>(lds + o.tw);
     int* const pab = reinterpret_cast<int*>(lds + o.pab);
@@ -167,149 +187,224 @@ __global__ void __launch_bounds__(256) fj_gfloop_k(FjGfArgs g, FjGfLds o) {
     unsigned short* const add = reinterpret_cast<unsigned short*>(lds + o.add);
 
     const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
-    const int la = g.la, lb = g.lb, lc = g.lc, ld = g.ld;
+    const int la = g0.la, lb = g0.lb, lc = g0.lc, ld = g0.ld;
     const int L = la + lb + lc + ld, Lt = L + 1;
     const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
     const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
     const int nab = na * nb, ncd = nc * nd;
-    const int nu = o.nu, nt = o.nt, nu1 = o.nu1, nt1 = o.nt1;
+    const int nu = o.nu, nt = o.nt, nu1 = o.nu1, nt1 = o.nt1, nR = o.nR, B = o.B;
+    const int axb3 = 3 * o.axb, axk3 = 3 * o.axk;
     const int njb = lb + 2, ntb = la + lb + 3, njk = ld + 2, ntk = lc + ld + 3;
 
-    // the quartet: (bra, ket), ket >= bra on the diagonal launch (its upper
-    // triangle, row-major), every pair otherwise
-    const long long q = (long long)blockIdx.x;
-    int bra, ket;
-    if (g.diag) {
-        const long long n = g.n_bra;
-        const double b2 = (double)(2 * n + 1);
-        long long b = (long long)((b2 - sqrt(b2 * b2 - 8.0 * (double)q)) * 0.5);
-        if (b < 0) b = 0;
-        if (b > n - 1) b = n - 1;
-        while (b > 0 && b * n - b * (b - 1) / 2 > q) --b;
-        while (b + 1 < n && (b + 1) * n - (b + 1) * b / 2 <= q) ++b;
-        bra = (int)b;
-        ket = (int)(b + (q - (b * n - b * (b - 1) / 2)));
-    } else {
-        bra = (int)(q / g.n_ket);
-        ket = (int)(q - (long long)bra * g.n_ket);
+    // the class's tables, once per workgroup: the tower's step table and the
+    // components of every Hermite index, the Cartesian pairs, [t + u]
+    for (int j = tid; j < nR; j += nthr) {
+        const int c = fj_herm_decode(j);
+        const int x = c & 15, y = (c >> 4) & 15, z = c >> 8;
+        hc[j] = c;
+        int w = 0;
+        if (j > 0) {
+            const int d = x ? 0 : (y ? 1 : 2);
+            const int td = d == 0 ? x : (d == 1 ? y : z);
+            const int i1 = fj_herm_index(x - (d == 0), y - (d == 1), z - (d == 2));
+            const int i2 = td >= 2 ? fj_herm_index(x - 2 * (d == 0), y - 2 * (d == 1), z - 2 * (d == 2)) : 0;
+            w = i1 | (i2 << 10) | (d << 20) | ((td - 1) << 22);
+        }
+        tw[j] = w;
+    }
+    for (int r = tid; r < nab; r += nthr) {
+        const int a = r / nb, b = r - a * nb;
+        pab[r] = fj_gf_cart(la, a) | (fj_gf_cart(lb, b) << 12);
+    }
+    for (int r = tid; r < ncd; r += nthr) {
+        const int c = r / nd, d = r - c * nd;
+        pcd[r] = fj_gf_cart(lc, c) | (fj_gf_cart(ld, d) << 12);
+    }
+    for (int r = tid; r < nt1 * nu1; r += nthr) {
+        const int t = r / nu1, u = r - t * nu1;
+        const int ct = fj_herm_decode(t), cu = fj_herm_decode(u);
+        add[r] = (unsigned short)fj_herm_index((ct & 15) + (cu & 15), ((ct >> 4) & 15) + ((cu >> 4) & 15),
+                                               (ct >> 8) + (cu >> 8));
     }
 
-    const int oa = g.bra_ao[2 * bra], ob = g.bra_ao[2 * bra + 1];
-    const int oc = g.ket_ao[2 * ket], od = g.ket_ao[2 * ket + 1];
-    // the degeneracy 1/|Stab| (kernel_gf.py:591-606): 1/2 for a = b, for
-    // c = d, and for the pair met with itself on the diagonal launch
-    double deg = 1.0;
-    if (la == lb && g.bra_sh[2 * bra] == g.bra_sh[2 * bra + 1]) deg *= 0.5;
-    if (lc == ld && g.ket_sh[2 * ket] == g.ket_sh[2 * ket + 1]) deg *= 0.5;
-    if (g.diag && ket == bra) deg *= 0.5;
-
-    // per quartet: the weights, the tower's step table, the components of
-    // every Hermite index, the Cartesian pairs and the index table [t + u]
-    {
-        const size_t n = (size_t)g.n_ao;
-        const double* __restrict__ D = g.D;
-        for (int r = tid; r < nab * ncd; r += nthr) {
-            const int ab = r / ncd, cd = r - ab * ncd;
-            const int a = ab / nb, b = ab - a * nb, c = cd / nd, d = cd - c * nd;
-            const double w = 4.0 * D[(oa + a) * n + ob + b] * D[(oc + c) * n + od + d] -
-                             D[(oa + a) * n + oc + c] * D[(ob + b) * n + od + d] -
-                             D[(oa + a) * n + od + d] * D[(ob + b) * n + oc + c];
-            W[r] = deg * w;
-        }
-        for (int j = tid; j < o.nR; j += nthr) {
-            const int c = fj_herm_decode(j);
-            const int x = c & 15, y = (c >> 4) & 15, z = c >> 8;
-            hc[j] = c;
-            int w = 0;
-            if (j > 0) {
-                const int d = x ? 0 : (y ? 1 : 2);
-                const int td = d == 0 ? x : (d == 1 ? y : z);
-                const int i1 = fj_herm_index(x - (d == 0), y - (d == 1), z - (d == 2));
-                const int i2 = td >= 2 ? fj_herm_index(x - 2 * (d == 0), y - 2 * (d == 1), z - 2 * (d == 2))
-                                       : 0;
-                w = i1 | (i2 << 10) | (d << 20) | ((td - 1) << 22);
+    for (long long q0 = (long long)blockIdx.x; q0 < n_items; q0 += (long long)gridDim.x) {
+        FjGfArgs g = g0;
+        long long q = q0;
+        if (jobs) {
+            int lo = 0, hi = n_jobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (jobs[mid].q0 <= q0) lo = mid;
+                else hi = mid - 1;
             }
-            tw[j] = w;
+            const FjGfJob J = jobs[lo];
+            q = q0 - J.q0;
+            g.n_bra = J.n_bra;
+            g.n_ket = J.n_ket;
+            g.k_ab = J.k_ab;
+            g.k_cd = J.k_cd;
+            g.diag = J.diag;
+            g.chunk = J.chunk;
+            g.bra_prim = g0.bra_prim + J.bra_prim;
+            g.ket_prim = g0.bra_prim + J.ket_prim;
+            g.AB = g0.AB + 3 * (size_t)J.bra_row;
+            g.CD = g0.AB + 3 * (size_t)J.ket_row;
+            g.bra_sh = g0.bra_sh + 2 * (size_t)J.bra_row;
+            g.ket_sh = g0.bra_sh + 2 * (size_t)J.ket_row;
+            g.bra_ao = g0.bra_ao + 2 * (size_t)J.bra_row;
+            g.ket_ao = g0.bra_ao + 2 * (size_t)J.ket_row;
+            g.bra_at = g0.bra_at + 2 * (size_t)J.bra_row;
+            g.ket_at = g0.bra_at + 2 * (size_t)J.ket_row;
         }
-        for (int r = tid; r < nab; r += nthr) {
-            const int a = r / nb, b = r - a * nb;
-            pab[r] = fj_gf_cart(la, a) | (fj_gf_cart(lb, b) << 12);
+        // the item: its quartet q and primitive quartets pq0 .. pq1 - 1
+        const int n_pq = g.k_ab * g.k_cd;
+        const int n_ch = (n_pq + g.chunk - 1) / g.chunk;
+        const long long qi = q / n_ch;
+        const int pq0 = (int)(q - qi * n_ch) * g.chunk;
+        const int pq1 = pq0 + g.chunk < n_pq ? pq0 + g.chunk : n_pq;
+        q = qi;
+        // the quartet: (bra, ket), ket >= bra on the diagonal launch (its
+        // upper triangle, row-major), every pair otherwise
+        int bra, ket;
+        if (g.diag) {
+            const long long n = g.n_bra;
+            const double b2 = (double)(2 * n + 1);
+            long long b = (long long)((b2 - sqrt(b2 * b2 - 8.0 * (double)q)) * 0.5);
+            if (b < 0) b = 0;
+            if (b > n - 1) b = n - 1;
+            while (b > 0 && b * n - b * (b - 1) / 2 > q) --b;
+            while (b + 1 < n && (b + 1) * n - (b + 1) * b / 2 <= q) ++b;
+            bra = (int)b;
+            ket = (int)(b + (q - (b * n - b * (b - 1) / 2)));
+        } else {
+            bra = (int)(q / g.n_ket);
+            ket = (int)(q - (long long)bra * g.n_ket);
         }
-        for (int r = tid; r < ncd; r += nthr) {
-            const int c = r / nd, d = r - c * nd;
-            pcd[r] = fj_gf_cart(lc, c) | (fj_gf_cart(ld, d) << 12);
-        }
-        for (int r = tid; r < nt1 * nu1; r += nthr) {
-            const int t = r / nu1, u = r - t * nu1;
-            const int ct = fj_herm_decode(t), cu = fj_herm_decode(u);
-            add[r] = (unsigned short)fj_herm_index((ct & 15) + (cu & 15), ((ct >> 4) & 15) + ((cu >> 4) & 15),
-                                                   (ct >> 8) + (cu >> 8));
-        }
-    }
+        const int oa = g.bra_ao[2 * bra], ob = g.bra_ao[2 * bra + 1];
+        const int oc = g.ket_ao[2 * ket], od = g.ket_ao[2 * ket + 1];
+        // the degeneracy 1/|Stab| (kernel_gf.py:591-606): 1/2 for a = b, for
+        // c = d, and for the pair met with itself on the diagonal launch
+        double deg = 1.0;
+        if (la == lb && g.bra_sh[2 * bra] == g.bra_sh[2 * bra + 1]) deg *= 0.5;
+        if (lc == ld && g.ket_sh[2 * ket] == g.ket_sh[2 * ket + 1]) deg *= 0.5;
+        if (g.diag && ket == bra) deg *= 0.5;
 
-    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-    double gC[3] = {0.0, 0.0, 0.0}, gD[3] = {0.0, 0.0, 0.0};
-    const double* __restrict__ AB = g.AB + 3 * bra;
-    const double* __restrict__ CD = g.CD + 3 * ket;
-    __syncthreads();
+        __syncthreads();   // the previous quartet's readers of W and red are done
+        {
+            const size_t n = (size_t)g.n_ao;
+            const double* __restrict__ D = g.D;
+            for (int r = tid; r < nab * ncd; r += nthr) {
+                const int ab = r / ncd, cd = r - ab * ncd;
+                const int a = ab / nb, b = ab - a * nb, c = cd / nd, d = cd - c * nd;
+                const double w = 4.0 * D[(oa + a) * n + ob + b] * D[(oc + c) * n + od + d] -
+                                 D[(oa + a) * n + oc + c] * D[(ob + b) * n + od + d] -
+                                 D[(oa + a) * n + od + d] * D[(ob + b) * n + oc + c];
+                W[r] = deg * w;
+            }
+        }
+        double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+        double gC[3] = {0.0, 0.0, 0.0}, gD[3] = {0.0, 0.0, 0.0};
+        const double* __restrict__ AB = g.AB + 3 * bra;
+        const double* __restrict__ CD = g.CD + 3 * ket;
+        const double* __restrict__ bp = g.bra_prim + (size_t)bra * g.k_ab * GF_REC;
+        const double* __restrict__ kp = g.ket_prim + (size_t)ket * g.k_cd * GF_REC;
+        __syncthreads();
 
-    for (int i = 0; i < g.k_ab; ++i) {
-        const double* __restrict__ rb = g.bra_prim + ((size_t)bra * g.k_ab + i) * GF_REC;
-        for (int k = 0; k < g.k_cd; ++k) {
-            const double* __restrict__ rk = g.ket_prim + ((size_t)ket * g.k_cd + k) * GF_REC;
-            const double p = rb[6], qz = rk[6];
-            const double PQx = rb[0] - rk[0], PQy = rb[1] - rk[1], PQz = rb[2] - rk[2];
-            if (tid < 3)
-                fj_kl_eaxis_from(eb + tid * o.axb, la + 1, lb + 1, p, rb[3 + tid], rb[3 + tid] + AB[tid], 1.0);
-            else if (tid < 6)
-                fj_kl_eaxis_from(ek + (tid - 3) * o.axk, lc + 1, ld + 1, qz, rk[tid], rk[tid] + CD[tid - 3],
-                                 1.0);
-            else if (tid == 6) {
-                const double rho = p * qz / (p + qz);
-                fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), Lt);
-                ((Lt & 1) ? R1 : R0)[0] = wf[Lt];
+        for (int base = pq0; base < pq1; base += B) {
+            const int nbq = pq1 - base < B ? pq1 - base : B;
+            // per primitive quartet b: its six E tables and its Boys seed
+            for (int r = tid; r < 7 * nbq; r += nthr) {
+                const int b = r / 7, s = r - 7 * b;
+                const int pq = base + b, i = pq / g.k_cd, k = pq - i * g.k_cd;
+                const double* __restrict__ rb = bp + i * GF_REC;
+                const double* __restrict__ rk = kp + k * GF_REC;
+                if (s < 3) {
+                    fj_kl_eaxis_from(eb + b * axb3 + s * o.axb, la + 1, lb + 1, rb[6], rb[3 + s],
+                                     rb[3 + s] + AB[s], 1.0);
+                } else if (s < 6) {
+                    fj_kl_eaxis_from(ek + b * axk3 + (s - 3) * o.axk, lc + 1, ld + 1, rk[6], rk[s],
+                                     rk[s] + CD[s - 3], 1.0);
+                } else {
+                    const double p = rb[6], qz = rk[6];
+                    const double PQx = rb[0] - rk[0], PQy = rb[1] - rk[1], PQz = rb[2] - rk[2];
+                    const double rho = p * qz / (p + qz);
+                    double* const wb_ = wf + b * (L + 2);
+                    fj_kl_seed(wb_, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), Lt);
+                    ((Lt & 1) ? R1 : R0)[b * nR] = wb_[Lt];
+                    double* const s_ = sc + b * GF_SC;
+                    s_[0] = rb[7] * rk[7];
+                    s_[1] = 2.0 * rb[8];
+                    s_[2] = 2.0 * rb[9];
+                    s_[3] = 2.0 * rk[8];
+                    s_[4] = 2.0 * rk[9];
+                    s_[5] = PQx;
+                    s_[6] = PQy;
+                    s_[7] = PQz;
+                }
             }
             __syncthreads();
-            fj_gf_tower(R0, R1, wf, tw, Lt, PQx, PQy, PQz, tid, nthr);
-
+            // the towers, levels m = Lt - 1 .. 0: lanes over (b, t)
+            for (int m = Lt - 1; m >= 0; --m) {
+                double* __restrict__ cur = (m & 1) ? R1 : R0;
+                const double* __restrict__ prv = (m & 1) ? R0 : R1;
+                const int n_m = fj_herm_base(Lt - m + 1);
+                for (int r = tid; r < nbq * n_m; r += nthr) {
+                    const int b = r / n_m, j = r - b * n_m;
+                    double v;
+                    if (j == 0) {
+                        v = wf[b * (L + 2) + m];
+                    } else {
+                        const int w = tw[j];
+                        const int d = (w >> 20) & 3, c1 = w >> 22;
+                        const double pq = sc[b * GF_SC + 5 + d];
+                        v = pq * prv[b * nR + (w & 1023)];
+                        if (c1 > 0) v += (double)c1 * prv[b * nR + ((w >> 10) & 1023)];
+                    }
+                    cur[b * nR + j] = v;
+                }
+                __syncthreads();
+            }
             // the weights in Hermite space
-            for (int r = tid; r < nab * nu; r += nthr) {
-                const int ab = r / nu, u = r - ab * nu;
+            for (int r = tid; r < nbq * nab * nu; r += nthr) {
+                const int b = r / (nab * nu), rr = r - b * (nab * nu);
+                const int ab = rr / nu, u = rr - ab * nu;
                 const int cu = hc[u];
                 const int ux = cu & 15, uy = (cu >> 4) & 15, uz = cu >> 8;
+                const double* __restrict__ e = ek + b * axk3;
                 double s = 0.0;
                 for (int cd = 0; cd < ncd; ++cd) {
                     const int pc = pcd[cd];
                     const int cx = pc & 15, cy = (pc >> 4) & 15, cz = (pc >> 8) & 15;
                     const int dx = (pc >> 12) & 15, dy = (pc >> 16) & 15, dz = pc >> 20;
-                    const double e = ek[(cx * njk + dx) * ntk + ux] * ek[o.axk + (cy * njk + dy) * ntk + uy] *
-                                     ek[2 * o.axk + (cz * njk + dz) * ntk + uz];
-                    s += W[ab * ncd + cd] * e;
+                    const double ecd = e[(cx * njk + dx) * ntk + ux] * e[o.axk + (cy * njk + dy) * ntk + uy] *
+                                       e[2 * o.axk + (cz * njk + dz) * ntk + uz];
+                    s += W[ab * ncd + cd] * ecd;
                 }
-                Wk[r] = ((ux + uy + uz) & 1) ? -s : s;
+                Wk[b * nab * nu + rr] = ((ux + uy + uz) & 1) ? -s : s;
             }
-            for (int r = tid; r < ncd * nt; r += nthr) {
-                const int cd = r / nt, t = r - cd * nt;
+            for (int r = tid; r < nbq * ncd * nt; r += nthr) {
+                const int b = r / (ncd * nt), rr = r - b * (ncd * nt);
+                const int cd = rr / nt, t = rr - cd * nt;
                 const int ct = hc[t];
                 const int tx = ct & 15, ty = (ct >> 4) & 15, tz = ct >> 8;
+                const double* __restrict__ e = eb + b * axb3;
                 double s = 0.0;
                 for (int ab = 0; ab < nab; ++ab) {
                     const int pa = pab[ab];
                     const int ax = pa & 15, ay = (pa >> 4) & 15, az = (pa >> 8) & 15;
                     const int bx = (pa >> 12) & 15, by = (pa >> 16) & 15, bz = pa >> 20;
-                    const double e = eb[(ax * njb + bx) * ntb + tx] * eb[o.axb + (ay * njb + by) * ntb + ty] *
-                                     eb[2 * o.axb + (az * njb + bz) * ntb + tz];
-                    s += W[ab * ncd + cd] * e;
+                    const double eab = e[(ax * njb + bx) * ntb + tx] * e[o.axb + (ay * njb + by) * ntb + ty] *
+                                       e[2 * o.axb + (az * njb + bz) * ntb + tz];
+                    s += W[ab * ncd + cd] * eab;
                 }
-                Wb[r] = s;
+                Wb[b * ncd * nt + rr] = s;
             }
             __syncthreads();
 
-            const double pref = rb[7] * rk[7];
-            const double a2 = 2.0 * rb[8], b2 = 2.0 * rb[9], c2 = 2.0 * rk[8], d2 = 2.0 * rk[9];
-            // the bra's centres: lanes over (ab, t), |t| <= la + lb + 1
-            for (int r = tid; r < nab * nt1; r += nthr) {
-                const int ab = r / nt1, t = r - ab * nt1;
+            // the bra's centres: lanes over (b, ab, t), |t| <= la + lb + 1
+            for (int r = tid; r < nbq * nab * nt1; r += nthr) {
+                const int b = r / (nab * nt1), rr = r - b * (nab * nt1);
+                const int ab = rr / nt1, t = rr - ab * nt1;
                 const int pa = pab[ab], ct = hc[t];
                 const int ax = pa & 15, ay = (pa >> 4) & 15, az = (pa >> 8) & 15;
                 const int bx = (pa >> 12) & 15, by = (pa >> 16) & 15, bz = pa >> 20;
@@ -317,15 +412,18 @@ __global__ void __launch_bounds__(256) fj_gfloop_k(FjGfArgs g, FjGfLds o) {
                 // outside the box t_k <= a_k + b_k (+1 on at most one axis) every term is zero
                 const int ox = tx - ax - bx, oy = ty - ay - by, oz = tz - az - bz;
                 if (ox > 1 || oy > 1 || oz > 1 || (ox == 1) + (oy == 1) + (oz == 1) > 1) continue;
-                double y = 0.0;
+                const double* __restrict__ s_ = sc + b * GF_SC;
                 const unsigned short* __restrict__ ar = add + t * nu1;
-                const double* __restrict__ wr = Wk + ab * nu;
-                for (int u = 0; u < nu; ++u) y += R0[ar[u]] * wr[u];
-                y *= pref;
+                const double* __restrict__ wr = Wk + b * nab * nu + ab * nu;
+                const double* __restrict__ Rb = R0 + b * nR;
+                double y = 0.0;
+                for (int u = 0; u < nu; ++u) y += Rb[ar[u]] * wr[u];
+                y *= s_[0];
+                const double* __restrict__ e = eb + b * axb3;
                 double ex, exa, exb, ey, eya, eyb, ez, eza, ezb;
-                fj_gf_axis(eb, njb, ntb, ax, bx, tx, a2, b2, ex, exa, exb);
-                fj_gf_axis(eb + o.axb, njb, ntb, ay, by, ty, a2, b2, ey, eya, eyb);
-                fj_gf_axis(eb + 2 * o.axb, njb, ntb, az, bz, tz, a2, b2, ez, eza, ezb);
+                fj_gf_axis(e, njb, ntb, ax, bx, tx, s_[1], s_[2], ex, exa, exb);
+                fj_gf_axis(e + o.axb, njb, ntb, ay, by, ty, s_[1], s_[2], ey, eya, eyb);
+                fj_gf_axis(e + 2 * o.axb, njb, ntb, az, bz, tz, s_[1], s_[2], ez, eza, ezb);
                 gA[0] += y * exa * ey * ez;
                 gA[1] += y * ex * eya * ez;
                 gA[2] += y * ex * ey * eza;
@@ -333,23 +431,27 @@ __global__ void __launch_bounds__(256) fj_gfloop_k(FjGfArgs g, FjGfLds o) {
                 gB[1] += y * ex * eyb * ez;
                 gB[2] += y * ex * ey * ezb;
             }
-            // the ket's centres: lanes over (cd, u), |u| <= lc + ld + 1
-            for (int r = tid; r < ncd * nu1; r += nthr) {
-                const int cd = r / nu1, u = r - cd * nu1;
+            // the ket's centres: lanes over (b, cd, u), |u| <= lc + ld + 1
+            for (int r = tid; r < nbq * ncd * nu1; r += nthr) {
+                const int b = r / (ncd * nu1), rr = r - b * (ncd * nu1);
+                const int cd = rr / nu1, u = rr - cd * nu1;
                 const int pc = pcd[cd], cu = hc[u];
                 const int cx = pc & 15, cy = (pc >> 4) & 15, cz = (pc >> 8) & 15;
                 const int dx = (pc >> 12) & 15, dy = (pc >> 16) & 15, dz = pc >> 20;
                 const int ux = cu & 15, uy = (cu >> 4) & 15, uz = cu >> 8;
                 const int ox = ux - cx - dx, oy = uy - cy - dy, oz = uz - cz - dz;
                 if (ox > 1 || oy > 1 || oz > 1 || (ox == 1) + (oy == 1) + (oz == 1) > 1) continue;
+                const double* __restrict__ s_ = sc + b * GF_SC;
+                const double* __restrict__ wr = Wb + b * ncd * nt + cd * nt;
+                const double* __restrict__ Rb = R0 + b * nR;
                 double x = 0.0;
-                const double* __restrict__ wr = Wb + cd * nt;
-                for (int t = 0; t < nt; ++t) x += R0[add[t * nu1 + u]] * wr[t];
-                x *= ((ux + uy + uz) & 1) ? -pref : pref;
+                for (int t = 0; t < nt; ++t) x += Rb[add[t * nu1 + u]] * wr[t];
+                x *= ((ux + uy + uz) & 1) ? -s_[0] : s_[0];
+                const double* __restrict__ e = ek + b * axk3;
                 double ex, exc, exd, ey, eyc, eyd, ez, ezc, ezd;
-                fj_gf_axis(ek, njk, ntk, cx, dx, ux, c2, d2, ex, exc, exd);
-                fj_gf_axis(ek + o.axk, njk, ntk, cy, dy, uy, c2, d2, ey, eyc, eyd);
-                fj_gf_axis(ek + 2 * o.axk, njk, ntk, cz, dz, uz, c2, d2, ez, ezc, ezd);
+                fj_gf_axis(e, njk, ntk, cx, dx, ux, s_[3], s_[4], ex, exc, exd);
+                fj_gf_axis(e + o.axk, njk, ntk, cy, dy, uy, s_[3], s_[4], ey, eyc, eyd);
+                fj_gf_axis(e + 2 * o.axk, njk, ntk, cz, dz, uz, s_[3], s_[4], ez, ezc, ezd);
                 gC[0] += x * exc * ey * ez;
                 gC[1] += x * ex * eyc * ez;
                 gC[2] += x * ex * ey * ezc;
@@ -357,26 +459,39 @@ __global__ void __launch_bounds__(256) fj_gfloop_k(FjGfArgs g, FjGfLds o) {
                 gD[1] += x * ex * eyd * ez;
                 gD[2] += x * ex * ey * ezd;
             }
-            __syncthreads();   // before the next primitive quartet rewrites the tables
+            __syncthreads();   // before the next batch rewrites the tables
         }
-    }
 
-    // the twelve sums over the workgroup, then into G
-    double v[12] = {gA[0], gA[1], gA[2], gB[0], gB[1], gB[2], gC[0], gC[1], gC[2], gD[0], gD[1], gD[2]};
-    const int wave = tid >> 6, lane = tid & 63;
+        // the twelve sums over the workgroup, then into G
+        double v[12] = {gA[0], gA[1], gA[2], gB[0], gB[1], gB[2], gC[0], gC[1], gC[2], gD[0], gD[1], gD[2]};
+        const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
-    for (int c = 0; c < 12; ++c) {
-        const double s = fj_wave_sum(v[c]);
-        if (lane == 0) red[wave * 12 + c] = s;
-    }
-    __syncthreads();
-    if (tid < 12) {
-        double s = 0.0;
-        for (int w = 0; w < (nthr >> 6); ++w) s += red[w * 12 + tid];
-        const int pos = tid / 3, k = tid - pos * 3;
-        const int atom = pos == 0 ? g.bra_at[2 * bra] : pos == 1 ? g.bra_at[2 * bra + 1]
-                       : pos == 2 ? g.ket_at[2 * ket] : g.ket_at[2 * ket + 1];
-        atomicAdd(g.G + 3 * atom + k, s);
+        for (int c = 0; c < 12; ++c) {
+            const double s = fj_wave_sum(v[c]);
+            if (lane == 0) red[wave * 12 + c] = s;
+        }
+        __syncthreads();
+        if (tid < 12) {
+            double s = 0.0;
+            for (int w = 0; w < (nthr >> 6); ++w) s += red[w * 12 + tid];
+            const int pos = tid / 3, k = tid - pos * 3;
+            const int atom = pos == 0 ? g.bra_at[2 * bra] : pos == 1 ? g.bra_at[2 * bra + 1]
+                           : pos == 2 ? g.ket_at[2 * ket] : g.ket_at[2 * ket + 1];
+            atomicAdd(g.G + 3 * atom + k, s);
+        }
     }
 }
 
+__global__ void __launch_bounds__(256) fj_gfloop_k(FjGfArgs g, FjGfLds o) {
+    const long long n_q = g.diag ? (long long)g.n_bra * (g.n_bra + 1) / 2 : (long long)g.n_bra * g.n_ket;
+    const int n_pq = g.k_ab * g.k_cd;
+    fj_gf_body(g, o, nullptr, 0, n_q * ((n_pq + g.chunk - 1) / g.chunk));
+}
+
+// Several launches of one class in one: g holds the class, n_atom, n_ao, D, G
+// and the base pointers of the concatenated sides (bra_prim, AB, bra_sh,
+// bra_ao, bra_at), which each job offsets for its bra and its ket.
+__global__ void __launch_bounds__(256) fj_gfjobs_k(FjGfArgs g, FjGfLds o, const FjGfJob* jobs, int n_jobs,
+                                                   long long n_items) {
+    fj_gf_body(g, o, jobs, n_jobs, n_items);
+}

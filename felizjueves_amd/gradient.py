@@ -5,9 +5,11 @@ kernel per F class (upstream `gradient/kernel_gf.py`, its stream
 `gf_gradient`), on the library's loop-form kernel (`fj_gf_loop` and its
 per-class entries `fj_gf_<la><lb><lc><ld>`).
 
-    gf_launches(basis, shell_atom)            -> [GfLaunch]  the stream
+    gf_launches(basis, shell_atom)            -> GfStream    the stream
     gf_gradient(basis, shell_atom, n_atom, D) -> G [n_atom, 3]
     GpuGfRunner                                the device side, reusable
+                                               (per class one merged launch,
+                                               or the per-launch entries)
 
 THE STREAM (`interpreted.py:293-310`): every unordered shell pair written
 canonically (la <= lb, the lower index first on a tie), batched by (la, lb,
@@ -36,6 +38,7 @@ from .basis import Basis
 
 GF_LMAX = 2
 GREC = 10
+ITEMS = 2048   # work items one merged class launch aims for
 
 
 @dataclass
@@ -136,33 +139,150 @@ def quartets(stream: GfStream) -> int:
     return n
 
 
+def _box(l1: int, l2: int) -> tuple[int, int]:
+    """(valid (ab, t) elements of the derivative loop, Hermite t with
+    |t| <= l1 + l2): t inside the box t_k <= a_k + b_k, +1 on at most one
+    axis, over every Cartesian pair (a, b)."""
+    from .basis import cartesians
+    herm = [(x, y, n - x - y) for n in range(l1 + l2 + 2) for x in range(n, -1, -1)
+            for y in range(n - x, -1, -1)]
+    n = 0
+    for a in cartesians(l1):
+        for b in cartesians(l2):
+            for t in herm:
+                o = [t[k] - a[k] - b[k] for k in range(3)]
+                if max(o) <= 1 and sum(1 for v in o if v == 1) <= 1:
+                    n += 1
+    return n, (l1 + l2 + 1) * (l1 + l2 + 2) * (l1 + l2 + 3) // 6
+
+
+def flops_per_prim_quartet(cls) -> int:
+    """The arithmetic of one primitive quartet in the loop form's
+    formulation: the Hermite-space weights 2 nab ncd (nu + nt) (the E
+    products counted once per (cd, u) / (ab, t)), the contraction with the
+    tower 2 nu per valid bra element and 2 nt per valid ket element, 13 per
+    valid element for the differentiated E and the six sums, and the tower
+    (3 per entry per level). No FMA credit beyond 2."""
+    la, lb, lc, ld = cls
+    nab = (la + 1) * (la + 2) // 2 * ((lb + 1) * (lb + 2) // 2)
+    ncd = (lc + 1) * (lc + 2) // 2 * ((ld + 1) * (ld + 2) // 2)
+    vb, nt = _box(la, lb)
+    vk, nu = _box(lc, ld)
+    Lt = sum(cls) + 1
+    herm = lambda n: n * (n + 1) * (n + 2) // 6  # noqa: E731
+    tower = sum(3 * herm(Lt - m + 1) for m in range(Lt))
+    return (2 * nab * ncd * (nu + nt) + 2 * ncd * nu * 2 + 2 * nab * nt * 2
+            + vb * (2 * nu + 13) + vk * (2 * nt + 13) + tower)
+
+
+def flops(stream: GfStream) -> int:
+    """The stream's arithmetic (flops_per_prim_quartet over every primitive
+    quartet of every launch)."""
+    tot = 0
+    cache: dict = {}
+    for L in stream.launches:
+        b, k = stream.sides[L.bra], stream.sides[L.ket]
+        nq = b.n * (b.n + 1) // 2 if L.diag else b.n * k.n
+        if L.cls not in cache:
+            cache[L.cls] = flops_per_prim_quartet(L.cls)
+        tot += nq * b.degree * k.degree * cache[L.cls]
+    return tot
+
+
+def prim_quartets(stream: GfStream) -> int:
+    n = 0
+    for L in stream.launches:
+        b, k = stream.sides[L.bra], stream.sides[L.ket]
+        n += (b.n * (b.n + 1) // 2 if L.diag else b.n * k.n) * b.degree * k.degree
+    return n
+
+
 def _p(t):
     return C.c_void_p(native.ptr(t))
 
 
+JOB_DTYPE = np.dtype([("bra_row", "<i4"), ("ket_row", "<i4"), ("n_bra", "<i4"), ("n_ket", "<i4"),
+                      ("k_ab", "<i4"), ("k_cd", "<i4"), ("diag", "<i4"), ("pad", "<i4"),
+                      ("bra_prim", "<i8"), ("ket_prim", "<i8"), ("q0", "<i8")])   # FjGfJob
+
+
 class GpuGfRunner:
     """The stream's sides on the device, reused across densities; `run`
-    launches every class into a zeroed G on `stream`."""
+    launches every class into a zeroed G on `stream`.
 
-    def __init__(self, stream_: GfStream, n_atom: int, n_ao: int, threads: dict | None = None):
+    mode "jobs" (default): every launch of a class in one kernel launch
+    (fj_gf_jobs over a job table: the stream's launches are many and small,
+    one per pair of contraction degrees, and one of them alone leaves the
+    chip idle); mode "launches": one fj_gf_<cls> call per launch of the
+    stream, the upstream gf_ parameter list."""
+
+    def __init__(self, stream_: GfStream, n_atom: int, n_ao: int, threads: dict | None = None,
+                 mode: str = "jobs"):
         import torch
-        self.s, self.n_atom, self.n_ao = stream_, int(n_atom), int(n_ao)
+        if mode not in ("jobs", "launches"):
+            raise ValueError(f"mode {mode!r} is not 'jobs' or 'launches'")
+        self.s, self.n_atom, self.n_ao, self.mode = stream_, int(n_atom), int(n_ao), mode
         self.threads = threads or {}
-        self.dev = [{k: native.to_dev(getattr(sd, k)) for k in ("prim", "sep", "sh", "ao", "at")}
-                    for sd in stream_.sides]
         self.G = torch.zeros((self.n_atom, 3), dtype=torch.float64, device=native.device())
         lib = native.lib()
         self.fns = {}
+        if mode == "launches":
+            self.dev = [{k: native.to_dev(getattr(sd, k)) for k in ("prim", "sep", "sh", "ao", "at")}
+                        for sd in stream_.sides]
+            for L in stream_.launches:
+                if L.cls not in self.fns:
+                    fn = getattr(lib, "fj_gf_" + "".join(map(str, L.cls)))
+                    fn.restype = C.c_int
+                    self.fns[L.cls] = fn
+            return
+        # the sides concatenated: rows and record offsets per side
+        rows = np.concatenate([[0], np.cumsum([sd.n for sd in stream_.sides])]).astype(np.int64)
+        recs = np.concatenate([[0], np.cumsum([sd.prim.size for sd in stream_.sides])]).astype(np.int64)
+        cat = lambda k: np.concatenate([getattr(sd, k).reshape(sd.n, -1) for sd in stream_.sides])  # noqa: E731
+        self.cat = {"prim": native.to_dev(np.concatenate([sd.prim.ravel() for sd in stream_.sides])),
+                    **{k: native.to_dev(np.ascontiguousarray(cat(k))) for k in ("sep", "sh", "ao", "at")}}
+        by: dict = {}
         for L in stream_.launches:
-            if L.cls not in self.fns:
-                fn = getattr(lib, "fj_gf_" + "".join(map(str, L.cls)))
-                fn.restype = C.c_int
-                self.fns[L.cls] = fn
+            by.setdefault(L.cls, []).append(L)
+        self.classes = []
+        lib.fj_gf_batch.restype = C.c_int
+        for cls, Ls in sorted(by.items()):
+            jobs = np.zeros(len(Ls), dtype=JOB_DTYPE)
+            nq = [stream_.sides[L.bra].n * (stream_.sides[L.bra].n + 1) // 2 if L.diag
+                  else stream_.sides[L.bra].n * stream_.sides[L.ket].n for L in Ls]
+            max_pq = max(stream_.sides[L.bra].degree * stream_.sides[L.ket].degree for L in Ls)
+            th = self.threads.get(cls, default_threads(cls))
+            B = int(lib.fj_gf_batch(*[C.c_int(x) for x in cls], C.c_int(max_pq), C.c_int(th)))
+            # a class of few quartets splits each quartet's primitive quartets
+            # into pieces (multiples of B) so that about ITEMS items fill the chip
+            pieces = max(1, -(-ITEMS // sum(nq)))
+            q0 = 0
+            for j, L in enumerate(Ls):
+                b, k = stream_.sides[L.bra], stream_.sides[L.ket]
+                n_pq = b.degree * k.degree
+                chunk = n_pq if pieces == 1 else min(n_pq, -(-(-(-n_pq // pieces)) // B) * B)
+                jobs[j] = (rows[L.bra], rows[L.ket], b.n, k.n, b.degree, k.degree, L.diag, chunk,
+                           recs[L.bra], recs[L.ket], q0)
+                q0 += nq[j] * -(-n_pq // chunk)
+            jd = native.to_dev(np.frombuffer(jobs.tobytes(), dtype=np.uint8))
+            self.classes.append((cls, jd, len(Ls), q0, max_pq))
+        self.fj_gf_jobs = lib.fj_gf_jobs
+        self.fj_gf_jobs.restype = C.c_int
 
     def run(self, D_dev, stream=None):
         """G [n_atom, 3] on the device (float64), zeroed then accumulated."""
         self.G.zero_()
         h = C.c_void_p(native.stream_handle(stream))
+        if self.mode == "jobs":
+            c = self.cat
+            for cls, jd, n_jobs, n_q, max_pq in self.classes:
+                th = self.threads.get(cls, default_threads(cls))
+                native.check(self.fj_gf_jobs(
+                    h, *[C.c_int(x) for x in cls], C.c_int(th), C.c_int(n_jobs), _p(jd),
+                    C.c_longlong(n_q), C.c_int(max_pq), C.c_int(self.n_atom), C.c_int(self.n_ao),
+                    _p(c["prim"]), _p(c["sep"]), _p(c["sh"]), _p(c["ao"]), _p(c["at"]), _p(D_dev),
+                    _p(self.G)), "fj_gf_jobs")
+            return self.G
         for L in self.s.launches:
             b, k = self.s.sides[L.bra], self.s.sides[L.ket]
             db, dk = self.dev[L.bra], self.dev[L.ket]

@@ -1813,8 +1813,34 @@ int fj_gf_loop(fj_stream_t stream, int la, int lb, int lc, int ld, int threads, 
                const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao,
                const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh,
                const int* ket_ao, const int* ket_at, const double* D, double* G);
-/* The LDS bytes of one workgroup of the class (0 past d). */
-int fj_gf_lds_bytes(int la, int lb, int lc, int ld);
+/* The LDS bytes of one workgroup of the class for quartets of n_pq primitive
+ * quartets at `threads` lanes (0 past d). */
+int fj_gf_lds_bytes(int la, int lb, int lc, int ld, int n_pq, int threads);
+/* The batch B of primitive quartets one workgroup of the class takes at once
+ * for quartets of n_pq primitive quartets at `threads` lanes (0 past d). */
+int fj_gf_batch(int la, int lb, int lc, int ld, int n_pq, int threads);
+/* Every launch of one class in one kernel launch (the stream's launches of a
+ * class are many and small: one per pair of contraction degrees). The sides
+ * of all launches are concatenated: prim (GREC doubles per primitive pair),
+ * sep [rows][3], sh / ao / at [rows][2]; job j reads its bra at row bra_row,
+ * record offset bra_prim (doubles), and its ket likewise. A job's work items
+ * are its quartets (in the order of one fj_gf_<cls> launch of those sides),
+ * each split into ceil(k_ab k_cd / chunk) items of `chunk` primitive
+ * quartets; job j owns items q0 .. q0 + (its count) - 1 of n_items. jobs
+ * (device memory) are sorted by q0, q0[0] = 0, chunk >= 1, and max_pq bounds
+ * every job's k_ab k_cd. The table is trusted as the host stream builds it
+ * (gradient.py); the per-class entries are the validated single launches.
+ * No upstream counterpart: the upstream launches each batch pair separately
+ * (gradient/interpreted.py:348-350). */
+typedef struct {
+    int bra_row, ket_row;
+    int n_bra, n_ket, k_ab, k_cd, diag, chunk;
+    long long bra_prim, ket_prim, q0;
+} FjGfJob;
+int fj_gf_jobs(fj_stream_t stream, int la, int lb, int lc, int ld, int threads, int n_jobs,
+               const FjGfJob* jobs, long long n_items, int max_pq, int n_atom, int n_ao,
+               const double* prim, const double* sep, const int* sh, const int* ao, const int* at,
+               const double* D, double* G);
 /* BEGIN GRADIENT CLASSES (written by felizjueves_amd/gen_hip.py) */
 int fj_gf_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);
 int fj_gf_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int k_ab, int k_cd, int n_atom, int n_ao, int diag, const double* bra_prim, const double* AB, const int* bra_sh, const int* bra_ao, const int* bra_at, const double* ket_prim, const double* CD, const int* ket_sh, const int* ket_ao, const int* ket_at, const double* D, double* G);

@@ -1,6 +1,7 @@
 // Emulate the fused-gradient kernel (felizjueves_amd/csrc/fj_gfloop.h) on
 // the CPU over a batch of launches read from a binary file (one std::thread
-// per lane, THREADS lanes per workgroup, one workgroup at a time); writes G.
+// per lane, THREADS lanes per workgroup, one workgroup at a time, a grid of
+// at most five workgroups so that each takes several quartets); writes G.
 // TEST INFRASTRUCTURE.
 #include "fj_device.h"   // the emulation header, copied under this name
 #include GFLOOP_HEADER
@@ -23,10 +24,10 @@ template <typename T> T* rd(std::ifstream& f) {
 }
 
 struct Launch {
-    int h[11];   // la lb lc ld n_bra n_ket k_ab k_cd n_atom n_ao diag
+    int h[12];   // la lb lc ld n_bra n_ket k_ab k_cd n_atom n_ao diag chunk
     double *bp, *AB, *kp, *CD, *lds;
     int *bs, *bao, *bat, *ks, *kao, *kat;
-    long long n_q;
+    long long n_q, grid;
 };
 
 int main(int argc, char** argv) {
@@ -46,8 +47,10 @@ int main(int argc, char** argv) {
         L.kp = rd<double>(f); L.CD = rd<double>(f); L.ks = rd<int>(f); L.kao = rd<int>(f);
         L.kat = rd<int>(f);
         L.n_q = L.h[10] ? (long long)L.h[4] * (L.h[4] + 1) / 2 : (long long)L.h[4] * L.h[5];
+        L.n_q *= (L.h[6] * L.h[7] + L.h[11] - 1) / L.h[11];   // work items
+        L.grid = L.n_q < 5 ? L.n_q : 5;
         // the launch's dynamic LDS at its exact size, so ASan sees an overrun
-        L.lds = (double*)malloc(fj_gf_lds(L.h[0], L.h[1], L.h[2], L.h[3]).bytes);
+        L.lds = (double*)malloc(fj_gf_plan(L.h[0], L.h[1], L.h[2], L.h[3], L.h[6] * L.h[7], threads).bytes);
     }
     std::barrier<> bar(threads);
     g_bar = &bar;
@@ -57,14 +60,14 @@ int main(int argc, char** argv) {
         ts.emplace_back([&, t] {
             threadIdx = {(unsigned)t, 0, 0};
             for (auto& L : Ls) {
-                const FjGfLds o = fj_gf_lds(L.h[0], L.h[1], L.h[2], L.h[3]);
+                const FjGfLds o = fj_gf_plan(L.h[0], L.h[1], L.h[2], L.h[3], L.h[6] * L.h[7], threads);
                 FjGfArgs g{L.h[0], L.h[1], L.h[2], L.h[3], L.h[4], L.h[5], L.h[6], L.h[7], L.h[8],
-                           L.h[9], L.h[10], L.bp, L.AB, L.bs, L.bao, L.bat, L.kp, L.CD, L.ks, L.kao,
+                           L.h[9], L.h[10], L.h[11], L.bp, L.AB, L.bs, L.bao, L.bat, L.kp, L.CD, L.ks, L.kao,
                            L.kat, D, G};
-                for (long long b = 0; b < L.n_q; ++b) {
+                for (long long b = 0; b < L.grid; ++b) {
                     if (t == 0) {
                         blockIdx = {(unsigned)b, 0, 0};
-                        gridDim = {(unsigned)L.n_q, 1, 1};
+                        gridDim = {(unsigned)L.grid, 1, 1};
                         g_dyn_lds = L.lds;
                     }
                     bar.arrive_and_wait();

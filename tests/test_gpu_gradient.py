@@ -1,5 +1,7 @@
-"""The fused two-electron gradient on the GPU (fj_gf_<cls> through the
-C-ABI, every launch of the stream into one zeroed G):
+"""The fused two-electron gradient on the GPU, every launch of the stream
+into one zeroed G: each class as one merged launch over its job table
+(fj_gf_jobs, the runner's default) and, separately, one fj_gf_<cls> call per
+launch (the upstream gf_ parameter list):
 
 - PySCF's grad:j + grad:k on the five upstream gradient fixtures within
   1e-10 absolute (the upstream's bound, test/test_kernel_gf.py ABS_TOL),
@@ -62,6 +64,23 @@ def test_gradient_three_atom_spd_vs_oracle(threads):
           GR.quartets(st), flush=True)
     assert err <= 1e-11 * np.abs(Go).max()
     assert np.abs(G.sum(axis=0)).max() <= 1e-11 * np.abs(G).max()
+
+
+def test_gradient_per_launch_entries_vs_merged():
+    """The per-class entries (one call per launch of the stream, the
+    upstream ABI) give the merged launches' G, both the oracle's."""
+    s = config1()
+    at = GR.shell_atoms(s.basis, s.coords)
+    st = GR.gf_launches(s.basis, at)
+    Gl = GR.gf_gradient(s.basis, at, len(s.coords), s.D,
+                        runner=GR.GpuGfRunner(st, len(s.coords), s.basis.n_ao, mode="launches"))
+    Gj = GR.gf_gradient(s.basis, at, len(s.coords), s.D,
+                        runner=GR.GpuGfRunner(st, len(s.coords), s.basis.n_ao))
+    Go = O.grad_ordered(s.basis, at, len(s.coords), s.D)
+    print("launches vs merged", np.abs(Gl - Gj).max(), "launches vs oracle", np.abs(Gl - Go).max(),
+          "calls", len(st.launches), flush=True)
+    assert np.abs(Gl - Gj).max() <= 1e-12 * np.abs(Go).max()
+    assert np.abs(Gl - Go).max() <= 1e-11 * np.abs(Go).max()
 
 
 def test_gradient_entry_refusals():

@@ -132,32 +132,44 @@ def test_refusals():
         GR.gf_launches(f, [0])
 
 
+def _emu(td: Path, driver: str) -> Path:
+    (td / "gen").mkdir(exist_ok=True)
+    shutil.copy(EMU / "fj_device_emu.h", td / "fj_device.h")
+    shutil.copy(CSRC / "gen" / "fj_boys_table.h", td / "gen" / "fj_boys_table.h")
+    exe = td / Path(driver).stem
+    r = subprocess.run(["g++", "-std=c++20", "-O1", "-g0", "-fsanitize=address", f"-I{ROOT / 'include'}",
+                        f"-I{td}", f'-DGFLOOP_HEADER="{CSRC / "fj_gfloop.h"}"', str(EMU / driver),
+                        "-o", str(exe), "-lpthread"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def _w(f, x, dt):
+    x = np.ascontiguousarray(x, dtype=dt).ravel()
+    np.array([x.size], dtype=np.int64).tofile(f)
+    x.tofile(f)
+
+
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
 def test_gradient_kernel_on_cpu_vs_oracle():
+    """Every launch of the stream through the single-launch kernel (the
+    per-class entries' path), each quartet split into work items of half its
+    primitive quartets, one wavefront per workgroup."""
     basis, at, D = small_system()
     st = GR.gf_launches(basis, at)
     assert {L.cls for L in st.launches} >= {(0, 0, 0, 0), (2, 2, 2, 2), (0, 2, 1, 2), (1, 2, 2, 2)}
     with tempfile.TemporaryDirectory() as td:
         td = Path(td)
-        (td / "gen").mkdir()
-        shutil.copy(EMU / "fj_device_emu.h", td / "fj_device.h")
-        shutil.copy(CSRC / "gen" / "fj_boys_table.h", td / "gen" / "fj_boys_table.h")
-        exe = td / "gfloop_emu"
-        r = subprocess.run(["g++", "-std=c++20", "-O1", "-g0", "-fsanitize=address", f"-I{ROOT / 'include'}",
-                            f"-I{td}", f'-DGFLOOP_HEADER="{CSRC / "fj_gfloop.h"}"', str(EMU / "gfloop_emu.cpp"),
-                            "-o", str(exe), "-lpthread"], capture_output=True, text=True)
-        assert r.returncode == 0, r.stderr[-3000:]
-
-        def w(f, x, dt):
-            x = np.ascontiguousarray(x, dtype=dt).ravel()
-            np.array([x.size], dtype=np.int64).tofile(f)
-            x.tofile(f)
+        exe = _emu(td, "gfloop_emu.cpp")
+        w = _w
         with open(td / "case.bin", "wb") as f:
             np.array([len(st.launches), 2, basis.n_ao], dtype=np.int32).tofile(f)
             w(f, D, np.float64)
             for L in st.launches:
                 b, k = st.sides[L.bra], st.sides[L.ket]
-                np.array([*L.cls, b.n, k.n, b.degree, k.degree, 2, basis.n_ao, L.diag],
+                # items of half a quartet's primitive quartets (uneven pieces)
+                chunk = max(1, b.degree * k.degree // 2)
+                np.array([*L.cls, b.n, k.n, b.degree, k.degree, 2, basis.n_ao, L.diag, chunk],
                          dtype=np.int32).tofile(f)
                 for s in (b, k):
                     w(f, s.prim, np.float64)
@@ -165,7 +177,7 @@ def test_gradient_kernel_on_cpu_vs_oracle():
                     w(f, s.sh, np.int32)
                     w(f, s.ao, np.int32)
                     w(f, s.at, np.int32)
-        r = subprocess.run([str(exe), str(td / "case.bin"), str(td / "G.bin"), "128"], cwd=td,
+        r = subprocess.run([str(exe), str(td / "case.bin"), str(td / "G.bin"), "64"], cwd=td,
                            env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"), capture_output=True,
                            text=True, timeout=900)
         assert r.returncode == 0, r.stderr[-3000:]
@@ -175,3 +187,52 @@ def test_gradient_kernel_on_cpu_vs_oracle():
     print("emulated gradient kernel vs oracle", err, "scale", np.abs(Go).max(), flush=True)
     assert err <= 1e-12 * np.abs(Go).max()
     assert np.abs(G.sum(axis=0)).max() <= 1e-12 * np.abs(G).max()
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
+def test_gradient_merged_launches_on_cpu_vs_oracle():
+    """Every class as one merged launch over its job table (fj_gfjobs_k, the
+    runner's default path): the job lookup and the concatenated sides' offsets
+    under AddressSanitizer, a grid of seven workgroups across the jobs, each
+    quartet split into items of half its primitive quartets, two wavefronts
+    per workgroup."""
+    basis, at, D = small_system()
+    st = GR.gf_launches(basis, at)
+    rows = np.concatenate([[0], np.cumsum([sd.n for sd in st.sides])])
+    recs = np.concatenate([[0], np.cumsum([sd.prim.size for sd in st.sides])])
+    by: dict = {}
+    for L in st.launches:
+        by.setdefault(L.cls, []).append(L)
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        exe = _emu(td, "gfjobs_emu.cpp")
+        with open(td / "case.bin", "wb") as f:
+            np.array([len(by), 2, basis.n_ao], dtype=np.int32).tofile(f)
+            _w(f, D, np.float64)
+            _w(f, np.concatenate([sd.prim.ravel() for sd in st.sides]), np.float64)
+            for k in ("sep", "sh", "ao", "at"):
+                _w(f, np.concatenate([getattr(sd, k) for sd in st.sides]),
+                   np.float64 if k == "sep" else np.int32)
+            for cls, Ls in sorted(by.items()):
+                jobs = np.zeros(len(Ls), dtype=GR.JOB_DTYPE)
+                q0 = 0
+                for j, L in enumerate(Ls):
+                    b, k = st.sides[L.bra], st.sides[L.ket]
+                    n_pq = b.degree * k.degree
+                    chunk = max(1, n_pq // 2)     # items of half a quartet's primitive quartets
+                    jobs[j] = (rows[L.bra], rows[L.ket], b.n, k.n, b.degree, k.degree, L.diag, chunk,
+                               recs[L.bra], recs[L.ket], q0)
+                    q0 += (b.n * (b.n + 1) // 2 if L.diag else b.n * k.n) * -(-n_pq // chunk)
+                np.array([*cls, len(Ls), int((jobs["k_ab"] * jobs["k_cd"]).max())], dtype=np.int32).tofile(f)
+                np.array([q0], dtype=np.int64).tofile(f)
+                np.array([len(Ls)], dtype=np.int64).tofile(f)
+                f.write(jobs.tobytes())
+        r = subprocess.run([str(exe), str(td / "case.bin"), str(td / "G.bin"), "128"], cwd=td,
+                           env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"), capture_output=True,
+                           text=True, timeout=900)
+        assert r.returncode == 0, r.stderr[-3000:]
+        G = np.fromfile(td / "G.bin").reshape(2, 3)
+    Go = O.grad_ordered(basis, at, 2, D)
+    err = np.abs(G - Go).max()
+    print("emulated merged launches vs oracle", err, "scale", np.abs(Go).max(), flush=True)
+    assert err <= 1e-12 * np.abs(Go).max()
