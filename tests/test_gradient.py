@@ -194,8 +194,8 @@ def test_gradient_merged_launches_on_cpu_vs_oracle():
     """Every class as one merged launch over its job table (fj_gfjobs_k, the
     runner's default path): the job lookup and the concatenated sides' offsets
     under AddressSanitizer, a grid of seven workgroups across the jobs, each
-    quartet split into items of half its primitive quartets, two wavefronts
-    per workgroup."""
+    quartet split into items of half its primitive quartets (the GPU tests
+    run it at two and four wavefronts per workgroup)."""
     basis, at, D = small_system()
     st = GR.gf_launches(basis, at)
     rows = np.concatenate([[0], np.cumsum([sd.n for sd in st.sides])])
@@ -227,7 +227,7 @@ def test_gradient_merged_launches_on_cpu_vs_oracle():
                 np.array([q0], dtype=np.int64).tofile(f)
                 np.array([len(Ls)], dtype=np.int64).tofile(f)
                 f.write(jobs.tobytes())
-        r = subprocess.run([str(exe), str(td / "case.bin"), str(td / "G.bin"), "128"], cwd=td,
+        r = subprocess.run([str(exe), str(td / "case.bin"), str(td / "G.bin"), "64"], cwd=td,
                            env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"), capture_output=True,
                            text=True, timeout=900)
         assert r.returncode == 0, r.stderr[-3000:]
