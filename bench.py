@@ -13,9 +13,10 @@ build's own rate (eager, one stream, default plan).
 Also reported:
 * "c3": the mixed configuration on the three-atom s,s,p,d system -- plain
   FP64 build vs the production build (tuned, streams, graph, mixed,
-  learned selector), |dE|, max|dG|, FP32 share, the J mixed lever's max|dJ|,
-  and the literal bound rule (FP32 iff Q_ab Q_cd max|D| < 1e-6) for
-  comparison;
+  learned selector), |dE|, max|dG|, FP32 share, the J and K mixed levers'
+  max|dJ| and max|dK| (FP32 Hermite towers in the MMD J and K engines) and
+  their G = J - K/2 against the FP64 build, and the literal bound rule
+  (FP32 iff Q_ab Q_cd max|D| < 1e-6) for comparison;
 * "c5": learned selector vs the bound rule on held-out systems;
 * "gradient": the fused two-electron nuclear gradient (dE_2e/dR, every
   unique quartet, no screen) on the three-atom s,s,p,d system and a
@@ -263,7 +264,10 @@ def c3_block(steps, warmup, selector):
     Gl = lit.build(Dd).cpu().numpy()
     J64 = jk.CoulombChain(s.basis, tau=1e-12).run(s.D)
     J32 = jk.CoulombChain(s.basis, tau=1e-12, mixed=True).run(s.D)
+    K64 = jk.ExchangeBuild(s.basis, tau=1e-12).run(s.D)
+    K32 = jk.ExchangeBuild(s.basis, tau=1e-12, mixed=True).run(s.D)
     torch.cuda.synchronize()
+    Gjk32 = (J32 - 0.5 * K32).cpu().numpy()
     out = {
         "workload": s.name, "n_ao": s.basis.n_ao, "quartets": plain.shell_quartets(),
         "plain_fp64_builds_per_s": steps / t_plain,
@@ -274,7 +278,8 @@ def c3_block(steps, warmup, selector):
                        **accuracy(s.D, Gp, G64)},
         "literal_bound_rule_1e-6": {"fp32_share": lit.fp32_share(), **accuracy(s.D, Gl, G64)},
         "j_mixed_lever_max_dJ": float((J32 - J64).abs().max().item()),
-        "k_mixed": "none: K runs in FP64 (no FP32 K kernel)",
+        "k_mixed_lever_max_dK": float((K32 - K64).abs().max().item()),
+        "jk_mixed_levers_vs_fused_fp64": accuracy(s.D, Gjk32, G64),
     }
     log("c3", json.dumps({k: v for k, v in out.items() if k not in ("production", "literal_bound_rule_1e-6")}))
     return out

@@ -438,17 +438,25 @@ def emit_jab(la: int, lb: int) -> str:
 # the per-bra K block wave-reduced and stored by lane 0.
 
 
-def emit_k(la: int, lc: int, lb: int, ld: int) -> str:
+def emit_k(la: int, lc: int, lb: int, ld: int, f32: bool = False) -> str:
+    """The K kernel of class (la lc | lb ld). f32: the K mixed lever's
+    compute arm (upstream eri/k_exchange/levers/mixed.py:1-60, `kmixed`): the
+    per-(bra, ket) region -- PQ, theta, T, the [0]^(m) and every [t]^(m) --
+    in float, F_m in double on the widened float T; the E blocks, D_cd, the
+    stage-1 I[p] and stage-2 folds and the K accumulation stay double (a
+    tower value enters them widened). Entry fj_k32_<cls>."""
     n = f"{la}{lc}{lb}{ld}"
+    kn = "k32" if f32 else "k"
     lp, lq = la + lc, lb + ld
     L = lp + lq
     ps, qs = hermite_order(lp), hermite_order(lq)
     na, nc, nb, nd = n_cart(la), n_cart(lc), n_cart(lb), n_cart(ld)
     E = _Emit()
-    E(f"// generated by felizjueves_amd/gen_mmd.py -- K class ({la}{lc}|{lb}{ld})", 0)
+    E(f"// generated by felizjueves_amd/gen_mmd.py -- K class ({la}{lc}|{lb}{ld})"
+      + (", the FP32 tower (K mixed lever)" if f32 else ""), 0)
     E('#include "../fj_device.h"', 0)
     E("", 0)
-    E(f"__global__ void __launch_bounds__(256) fj_k_{n}_k(int n_bra, int n_ket, double tau, double maxD,", 0)
+    E(f"__global__ void __launch_bounds__(256) fj_{kn}_{n}_k(int n_bra, int n_ket, double tau, double maxD,", 0)
     E("const double* __restrict__ P, const double* __restrict__ zeta_ac,")
     E("const double* __restrict__ XPA, const double* __restrict__ XPC,")
     E("const double* __restrict__ s_ac, const double* __restrict__ Ip,")
@@ -478,7 +486,7 @@ def emit_k(la: int, lc: int, lb: int, ld: int) -> str:
     E("const double PQx = Px - Q[3 * k], PQy = Py - Q[3 * k + 1], PQz = Pz - Q[3 * k + 2];", 2)
     E("const double rho = z * e / (z + e);", 2)
     E("const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);", 2)
-    R = emit_tower(E, L, 2)
+    R = emit_tower(E, L, 2, f32=f32)
     E(f"const double* __restrict__ dm = Dcd + (size_t)(dcb + dd[k]) * {nc * nd};", 2)
     for b_i, b in enumerate(cartesians(lb)):
         for d_i, d in enumerate(cartesians(ld)):
@@ -509,7 +517,7 @@ def emit_k(la: int, lc: int, lb: int, ld: int) -> str:
             E(f"{{ const double v = fj_wave_sum(k{a}_{b}); if (lane == 0) K[(size_t)bra * {na * nb} + {a * nb + b}] = v; }}")
     E("}", 0)
     E("", 0)
-    E(f'extern "C" int fj_k_{n}(hipStream_t s, int threads, int n_bra, int n_ket, int n_dc,', 0)
+    E(f'extern "C" int fj_{kn}_{n}(hipStream_t s, int threads, int n_bra, int n_ket, int n_dc,', 0)
     E("double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,")
     E("const double* XPC, const double* s_ac, const double* Ip, const int* dc,")
     E("const double* Q, const double* eta, const double* XQB, const double* XQD,")
@@ -519,7 +527,7 @@ def emit_k(la: int, lc: int, lb: int, ld: int) -> str:
     E("if (n_bra < 0 || n_ket < 0 || n_dc < 0) return FJ_ERR_SHAPE;")
     E("if (n_bra == 0) return 0;")
     E("const int waves = threads / 64;")
-    E(f"hipLaunchKernelGGL(fj_k_{n}_k, dim3((n_bra + waves - 1) / waves), dim3(threads), 0, s,")
+    E(f"hipLaunchKernelGGL(fj_{kn}_{n}_k, dim3((n_bra + waves - 1) / waves), dim3(threads), 0, s,")
     E("n_bra, n_ket, tau, maxD, P, zeta_ac, XPA, XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, s_bd, Iq, dd, Dcd, K);", 2)
     E("return (int)hipGetLastError();")
     E("}", 0)
@@ -565,6 +573,7 @@ def units(lmax: int, k_lmax: int | None = None) -> dict[str, str]:
             out[f"pq32_L{lp + lq}_p{lp}_q{lq}.hip"] = emit_pq(lp, lq, f32=True)
     for c in k_classes(k_lmax):
         out[f"k_{''.join(map(str, c))}.hip"] = emit_k(*c)
+        out[f"k32_{''.join(map(str, c))}.hip"] = emit_k(*c, f32=True)
     loop = [c for c in k_classes(lmax) if max(c) > k_lmax]
     if loop:
         out["k_loop_entries.hip"] = emit_k_loop_entries(loop)
@@ -664,6 +673,14 @@ def header_decls(lmax: int, k_lmax: int | None = None) -> str:
     for c in k_classes(lmax):
         out.append(
             f"int fj_k_{''.join(map(str, c))}(fj_stream_t stream, int threads, int n_bra, int n_ket,"
+            f" int n_dc,\n"
+            f"    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,\n"
+            f"    const double* XPC, const double* s_ac, const double* Ip, const int* dc,\n"
+            f"    const double* Q, const double* eta, const double* XQB, const double* XQD,\n"
+            f"    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);")
+    for c in k_classes(k_lmax):
+        out.append(
+            f"int fj_k32_{''.join(map(str, c))}(fj_stream_t stream, int threads, int n_bra, int n_ket,"
             f" int n_dc,\n"
             f"    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,\n"
             f"    const double* XPC, const double* s_ac, const double* Ip, const int* dc,\n"

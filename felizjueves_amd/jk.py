@@ -38,6 +38,7 @@ from .pairs import hermite_bound_sq, hermite_order
 
 
 K_LMAX = 3   # the K kernels through f (straight-line through d, the f classes in loop form)
+K_STRAIGHT_LMAX = 2   # the straight-line K kernels (and their FP32-tower twins fj_k32_*)
 
 
 def _records(basis: Basis, a: int, b: int):
@@ -306,11 +307,14 @@ class ExchangeBuild:
     """K_ab = sum_cd D_cd (ac|bd) through the K kernel; `run(D)` returns K
     on the device. The host tables (`exchange_tables`) are built and copied
     to the device once here; `run` gathers D's blocks on the device and
-    launches."""
+    launches. mixed: the K mixed lever (upstream k_exchange/levers/mixed.py,
+    `kmixed`): the straight-line classes' Hermite towers in FP32
+    (`fj_k32_*`), everything that accumulates in FP64."""
 
-    def __init__(self, basis: Basis, tau: float = 0.0):
+    def __init__(self, basis: Basis, tau: float = 0.0, mixed: bool = False):
         self.basis = basis
         self.tau = float(tau)
+        self.mixed = bool(mixed)
         host = exchange_tables(basis)
         self.by_l = host["by_l"]
         self.lmax = max(self.by_l)
@@ -339,7 +343,10 @@ class ExchangeBuild:
         for la, lc, lb, ld, bs in self.order:
             t, k = self.bras[(la, lc)], self.kets[(lb, ld, bs)]
             Kr = scratch[:t["n"] * n_cart(la) * n_cart(lb)]
-            _call(f"fj_k_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
+            # the K mixed lever: the FP32 tower on the straight-line classes;
+            # the f classes' loop-form kernel stays FP64
+            kn = "k32" if self.mixed and max(la, lc, lb, ld) <= K_STRAIGHT_LMAX else "k"
+            _call(f"fj_{kn}_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
                   C.c_int(self.dblk[(lc, ld)]["n"]), C.c_double(self.tau),
                   C.c_double(maxD), _c(t["P"]), _c(t["z"]), _c(t["XA"]), _c(t["XC"]),
                   _c(t["s"]), _c(t["Ip"]), _c(t[("dc", ld)]), _c(k["Q"]), _c(k["eta"]),
@@ -351,8 +358,9 @@ class ExchangeBuild:
         return K
 
 
-def fock_jk(basis: Basis, D: np.ndarray, tau: float = 0.0):
-    """G = J - K/2 through the MMD families (device tensors J, K, G)."""
-    J = CoulombChain(basis, tau).run(D)
-    K = ExchangeBuild(basis, tau).run(D)
+def fock_jk(basis: Basis, D: np.ndarray, tau: float = 0.0, mixed: bool = False):
+    """G = J - K/2 through the MMD families (device tensors J, K, G); mixed:
+    both levers (the FP32 Hermite towers of [p|q] and K)."""
+    J = CoulombChain(basis, tau, mixed=mixed).run(D)
+    K = ExchangeBuild(basis, tau, mixed=mixed).run(D)
     return J, K, J - 0.5 * K

@@ -3540,6 +3540,411 @@ int fj_k_3333(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
     const double* XPC, const double* s_ac, const double* Ip, const int* dc,
     const double* Q, const double* eta, const double* XQB, const double* XQD,
     const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_0222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_1222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2000(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2001(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2002(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2010(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2011(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2012(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2020(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2021(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2022(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2100(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2101(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2102(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2110(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2111(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2112(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2120(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2121(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2122(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2200(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2201(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2202(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2210(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2211(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2212(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2220(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2221(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
+int fj_k32_2222(fj_stream_t stream, int threads, int n_bra, int n_ket, int n_dc,
+    double tau, double maxD, const double* P, const double* zeta_ac, const double* XPA,
+    const double* XPC, const double* s_ac, const double* Ip, const int* dc,
+    const double* Q, const double* eta, const double* XQB, const double* XQD,
+    const double* s_bd, const double* Iq, const int* dd, const double* Dcd, double* K);
 /* END MMD CLASSES */
 
 #ifdef __cplusplus

@@ -97,6 +97,25 @@ def test_k_vs_libcint(name):
     assert err < 1e-11 * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("name", FIX_J)
+def test_k_mixed_lever_vs_libcint(name):
+    """The K mixed lever (upstream k_exchange/levers/mixed.py `kmixed`: the
+    straight-line classes' Hermite towers in FP32, the E blocks, D and every
+    accumulation FP64; the f classes' loop form stays FP64) against
+    libcint's vk: within the upstream FP32 tier bar, F32_REL = 1e-12 * 2^29
+    relative to the fixture's scale (test/oracle/tolerance.py:40), and not
+    the FP64 path's result (it really is the FP32 tower)."""
+    fx = load_fixture(name)
+    K = _host(jk.ExchangeBuild(fx.basis, mixed=True).run(fx.density))
+    K64 = _host(jk.ExchangeBuild(fx.basis).run(fx.density))
+    ref = fx.arrays["vk"]
+    scale = max(1.0, np.abs(ref).max())
+    err = np.abs(K - ref).max()
+    print(name, "mixed max |K - vk|", err, "vs fp64 path", np.abs(K - K64).max(), flush=True)
+    assert err < 1e-12 * 2.0 ** 29 * scale
+    assert np.abs(K - K64).max() > 0.0
+
+
 def test_k_screened_through_f_vs_libcint():
     """CO/cc-pVTZ with the K screen on (tau = 1e-12: each bra record stops
     at its first ket record whose bound Ip Iq max|D| is below tau, the f
