@@ -75,11 +75,12 @@ struct FjGfArgs {
 // the total.
 struct FjGfLds {
     int B;
-    int w, red, R0, R1, eb, ek, wf, wk, wb, sc, hc, tw, pab, pcd, add, bytes;
-    int axb, axk, nR, nu, nt, nu1, nt1;
+    int w, red, R0, R1, eb, ek, wf, wk, wb, sc, un, hc, tw, pab, pcd, add, bytes;
+    int axb, axk, nR, nu, nt, nu1, nt1, uw;
 };
 
-constexpr int GF_LDS_BUDGET = 40 * 1024;   // bytes per workgroup
+constexpr int GF_LDS_BUDGET = 64 * 1024;   // bytes per workgroup
+constexpr int GF_TM = 4, GF_TN = 4;        // register tile of the Hermite-space GEMMs
 constexpr int GF_BMAX = 64;                // primitive quartets per batch
 constexpr int GF_SC = 8;                   // per primitive quartet: U U, 2 x4 exponents, PQ
 
@@ -110,6 +111,11 @@ __host__ __device__ inline FjGfLds fj_gf_layout(int la, int lb, int lc, int ld, 
     o.wk = take(B * na * nb * o.nu, 8);
     o.wb = take(B * nc * nd * o.nt, 8);
     o.sc = take(B * GF_SC, 8);
+    // per primitive quartet one region holding first the E products Ecd
+    // [ncd][nu] and Eab [nab][nt], then the gathered tower H [nt1][nu1]
+    o.uw = nc * nd * o.nu + na * nb * o.nt;
+    if (o.nt1 * o.nu1 > o.uw) o.uw = o.nt1 * o.nu1;
+    o.un = take(B * o.uw, 8);
     o.hc = take(o.nR, 4);
     o.tw = take(o.nR, 4);
     o.pab = take(na * nb, 4);
@@ -121,18 +127,17 @@ __host__ __device__ inline FjGfLds fj_gf_layout(int la, int lb, int lc, int ld, 
 
 // B for a launch: the largest that fits GF_LDS_BUDGET, at most GF_BMAX and
 // the quartet's n_pq primitive quartets, and no more than the lanes need
-// (about four elements per lane in the longest per-batch loop).
+// (one register tile per lane in the last stage).
 __host__ __device__ inline FjGfLds fj_gf_plan(int la, int lb, int lc, int ld, int n_pq, int threads) {
     const FjGfLds one = fj_gf_layout(la, lb, lc, ld, 1), two = fj_gf_layout(la, lb, lc, ld, 2);
     const int per_b = two.bytes - one.bytes, fixed = one.bytes - per_b;
     const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
     const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
-    int el = na * nb * one.nt1;
-    if (nc * nd * one.nu1 > el) el = nc * nd * one.nu1;
-    if (na * nb * one.nu > el) el = na * nb * one.nu;
-    if (nc * nd * one.nt > el) el = nc * nd * one.nt;
+    auto tiles = [](int m, int n) { return ((m + GF_TM - 1) / GF_TM) * ((n + GF_TN - 1) / GF_TN); };
+    // register tiles of the last stage, one primitive quartet
+    const int el = tiles(na * nb, one.nt1) + tiles(nc * nd, one.nu1);
     int B = (GF_LDS_BUDGET - fixed) / per_b;
-    const int want = (4 * threads + el - 1) / el;
+    const int want = (threads + el - 1) / el;
     if (B > want) B = want;
     if (B > GF_BMAX) B = GF_BMAX;
     if (B > n_pq) B = n_pq;
@@ -155,6 +160,34 @@ __device__ __forceinline__ void fj_gf_axis(const double* __restrict__ e, int nj,
     if (i) ea -= (double)i * e[((i - 1) * nj + j) * nt + t];
     eb = x2 * e[(i * nj + j + 1) * nt + t];
     if (j) eb -= (double)j * e[(i * nj + j - 1) * nt + t];
+}
+
+// One register tile of C = A B: c[i][j] = sum_k A(m0 + i, k) B(k, n0 + j),
+// A(m, k) = A[m sam + k sak], B(k, n) = Bm[k sbk + n sbn], rows past M and
+// columns past N read row M - 1 / column N - 1 (their results are unused).
+__device__ __forceinline__ void fj_gf_tile(const double* __restrict__ A, int sam, int sak,
+                                           const double* __restrict__ Bm, int sbk, int sbn, int M,
+                                           int N, int K, int m0, int n0, double (&c)[GF_TM][GF_TN]) {
+    int ra[GF_TM], rb[GF_TN];
+#pragma unroll
+    for (int i = 0; i < GF_TM; ++i) ra[i] = (m0 + i < M ? m0 + i : M - 1) * sam;
+#pragma unroll
+    for (int j = 0; j < GF_TN; ++j) rb[j] = (n0 + j < N ? n0 + j : N - 1) * sbn;
+#pragma unroll
+    for (int i = 0; i < GF_TM; ++i)
+#pragma unroll
+        for (int j = 0; j < GF_TN; ++j) c[i][j] = 0.0;
+    for (int k = 0; k < K; ++k) {
+        double a[GF_TM], bv[GF_TN];
+#pragma unroll
+        for (int i = 0; i < GF_TM; ++i) a[i] = A[ra[i] + k * sak];
+#pragma unroll
+        for (int j = 0; j < GF_TN; ++j) bv[j] = Bm[k * sbk + rb[j]];
+#pragma unroll
+        for (int i = 0; i < GF_TM; ++i)
+#pragma unroll
+            for (int j = 0; j < GF_TN; ++j) c[i][j] += a[i] * bv[j];
+    }
 }
 
 }  // namespace
@@ -180,6 +213,11 @@ __device__ __forceinline__ void fj_gf_body(const FjGfArgs& g0, const FjGfLds& o,
     double* const Wk = reinterpret_cast<double*>(lds + o.wk);
     double* const Wb = reinterpret_cast<double*>(lds + o.wb);
     double* const sc = reinterpret_cast<double*>(lds + o.sc);
+    const int nc_ = (g0.lc + 1) * (g0.lc + 2) / 2, nd_ = (g0.ld + 1) * (g0.ld + 2) / 2;
+    double* const un = reinterpret_cast<double*>(lds + o.un);
+    double* const Ecd = un;                                   // [b][cd][u], then
+    double* const Eab = un + (size_t)(nc_ * nd_) * o.nu;      // [b][ab][t], then
+    double* const Hm = un;                                    // [b][t][u]
     int* const hc = reinterpret_cast<int*>(lds + o.hc);
     int* const tw = reinterpret_cast<int*>(lds + o.tw);
     int* const pab = reinterpret_cast<int*>(lds + o.pab);
@@ -364,100 +402,151 @@ __device__ __forceinline__ void fj_gf_body(const FjGfArgs& g0, const FjGfLds& o,
                 }
                 __syncthreads();
             }
-            // the weights in Hermite space
-            for (int r = tid; r < nbq * nab * nu; r += nthr) {
-                const int b = r / (nab * nu), rr = r - b * (nab * nu);
-                const int ab = rr / nu, u = rr - ab * nu;
-                const int cu = hc[u];
+            // the E products of every primitive quartet: Ecd[b][cd][u] with
+            // the ket's sign (-1)^|u| folded in, Eab[b][ab][t]
+            for (int r = tid; r < nbq * ncd * nu; r += nthr) {
+                const int b = r / (ncd * nu), rr = r - b * (ncd * nu);
+                const int cd = rr / nu, u = rr - cd * nu;
+                const int pc = pcd[cd], cu = hc[u];
                 const int ux = cu & 15, uy = (cu >> 4) & 15, uz = cu >> 8;
                 const double* __restrict__ e = ek + b * axk3;
-                double s = 0.0;
-                for (int cd = 0; cd < ncd; ++cd) {
-                    const int pc = pcd[cd];
-                    const int cx = pc & 15, cy = (pc >> 4) & 15, cz = (pc >> 8) & 15;
-                    const int dx = (pc >> 12) & 15, dy = (pc >> 16) & 15, dz = pc >> 20;
-                    const double ecd = e[(cx * njk + dx) * ntk + ux] * e[o.axk + (cy * njk + dy) * ntk + uy] *
-                                       e[2 * o.axk + (cz * njk + dz) * ntk + uz];
-                    s += W[ab * ncd + cd] * ecd;
-                }
-                Wk[b * nab * nu + rr] = ((ux + uy + uz) & 1) ? -s : s;
+                const double v = e[(((pc & 15) * njk + ((pc >> 12) & 15)) * ntk + ux)] *
+                                 e[o.axk + ((((pc >> 4) & 15) * njk + ((pc >> 16) & 15)) * ntk + uy)] *
+                                 e[2 * o.axk + ((((pc >> 8) & 15) * njk + (pc >> 20)) * ntk + uz)];
+                Ecd[b * o.uw + rr] = ((ux + uy + uz) & 1) ? -v : v;
             }
-            for (int r = tid; r < nbq * ncd * nt; r += nthr) {
-                const int b = r / (ncd * nt), rr = r - b * (ncd * nt);
-                const int cd = rr / nt, t = rr - cd * nt;
-                const int ct = hc[t];
-                const int tx = ct & 15, ty = (ct >> 4) & 15, tz = ct >> 8;
+            for (int r = tid; r < nbq * nab * nt; r += nthr) {
+                const int b = r / (nab * nt), rr = r - b * (nab * nt);
+                const int ab = rr / nt, t = rr - ab * nt;
+                const int pa = pab[ab], ct = hc[t];
                 const double* __restrict__ e = eb + b * axb3;
-                double s = 0.0;
-                for (int ab = 0; ab < nab; ++ab) {
-                    const int pa = pab[ab];
-                    const int ax = pa & 15, ay = (pa >> 4) & 15, az = (pa >> 8) & 15;
-                    const int bx = (pa >> 12) & 15, by = (pa >> 16) & 15, bz = pa >> 20;
-                    const double eab = e[(ax * njb + bx) * ntb + tx] * e[o.axb + (ay * njb + by) * ntb + ty] *
-                                       e[2 * o.axb + (az * njb + bz) * ntb + tz];
-                    s += W[ab * ncd + cd] * eab;
+                Eab[b * o.uw + rr] = e[(((pa & 15) * njb + ((pa >> 12) & 15)) * ntb + (ct & 15))] *
+                                     e[o.axb + ((((pa >> 4) & 15) * njb + ((pa >> 16) & 15)) * ntb + ((ct >> 4) & 15))] *
+                                     e[2 * o.axb + ((((pa >> 8) & 15) * njb + (pa >> 20)) * ntb + (ct >> 8))];
+            }
+            __syncthreads();
+            // the weights in Hermite space, two small GEMMs per primitive
+            // quartet in register tiles: Wk = W Ecd (nab x nu, over cd) and
+            // Wb = W^T Eab (ncd x nt, over ab)
+            {
+                const int tmk = (nab + GF_TM - 1) / GF_TM, tnk = (nu + GF_TN - 1) / GF_TN;
+                const int tmb = (ncd + GF_TM - 1) / GF_TM, tnb = (nt + GF_TN - 1) / GF_TN;
+                const int nk = tmk * tnk, nbt = tmb * tnb;
+                for (int r = tid; r < nbq * (nk + nbt); r += nthr) {
+                    const int b = r / (nk + nbt), rr = r - b * (nk + nbt);
+                    double c[GF_TM][GF_TN];
+                    if (rr < nk) {
+                        const int m0 = (rr / tnk) * GF_TM, n0 = (rr - (rr / tnk) * tnk) * GF_TN;
+                        fj_gf_tile(W, ncd, 1, Ecd + b * o.uw, nu, 1, nab, nu, ncd, m0, n0, c);
+                        double* __restrict__ out = Wk + b * nab * nu;
+#pragma unroll
+                        for (int i = 0; i < GF_TM; ++i)
+#pragma unroll
+                            for (int j = 0; j < GF_TN; ++j)
+                                if (m0 + i < nab && n0 + j < nu) out[(m0 + i) * nu + n0 + j] = c[i][j];
+                    } else {
+                        const int q_ = rr - nk;
+                        const int m0 = (q_ / tnb) * GF_TM, n0 = (q_ - (q_ / tnb) * tnb) * GF_TN;
+                        fj_gf_tile(W, 1, ncd, Eab + b * o.uw, nt, 1, ncd, nt, nab, m0, n0, c);
+                        double* __restrict__ out = Wb + b * ncd * nt;
+#pragma unroll
+                        for (int i = 0; i < GF_TM; ++i)
+#pragma unroll
+                            for (int j = 0; j < GF_TN; ++j)
+                                if (m0 + i < ncd && n0 + j < nt) out[(m0 + i) * nt + n0 + j] = c[i][j];
+                    }
                 }
-                Wb[b * ncd * nt + rr] = s;
+            }
+            __syncthreads();
+            // H[b][t][u] = [t + u] of each tower, t < nt1, u < nu1 (over the
+            // E products, which are done)
+            for (int r = tid; r < nbq * nt1 * nu1; r += nthr) {
+                const int b = r / (nt1 * nu1), rr = r - b * (nt1 * nu1);
+                // |t| + |u| = L + 2 (the corner no contraction reads) is past the tower
+                const int ix = add[rr];
+                Hm[b * o.uw + rr] = ix < nR ? R0[b * nR + ix] : 0.0;
             }
             __syncthreads();
 
-            // the bra's centres: lanes over (b, ab, t), |t| <= la + lb + 1
-            for (int r = tid; r < nbq * nab * nt1; r += nthr) {
-                const int b = r / (nab * nt1), rr = r - b * (nab * nt1);
-                const int ab = rr / nt1, t = rr - ab * nt1;
-                const int pa = pab[ab], ct = hc[t];
-                const int ax = pa & 15, ay = (pa >> 4) & 15, az = (pa >> 8) & 15;
-                const int bx = (pa >> 12) & 15, by = (pa >> 16) & 15, bz = pa >> 20;
-                const int tx = ct & 15, ty = (ct >> 4) & 15, tz = ct >> 8;
-                // outside the box t_k <= a_k + b_k (+1 on at most one axis) every term is zero
-                const int ox = tx - ax - bx, oy = ty - ay - by, oz = tz - az - bz;
-                if (ox > 1 || oy > 1 || oz > 1 || (ox == 1) + (oy == 1) + (oz == 1) > 1) continue;
-                const double* __restrict__ s_ = sc + b * GF_SC;
-                const unsigned short* __restrict__ ar = add + t * nu1;
-                const double* __restrict__ wr = Wk + b * nab * nu + ab * nu;
-                const double* __restrict__ Rb = R0 + b * nR;
-                double y = 0.0;
-                for (int u = 0; u < nu; ++u) y += Rb[ar[u]] * wr[u];
-                y *= s_[0];
-                const double* __restrict__ e = eb + b * axb3;
-                double ex, exa, exb, ey, eya, eyb, ez, eza, ezb;
-                fj_gf_axis(e, njb, ntb, ax, bx, tx, s_[1], s_[2], ex, exa, exb);
-                fj_gf_axis(e + o.axb, njb, ntb, ay, by, ty, s_[1], s_[2], ey, eya, eyb);
-                fj_gf_axis(e + 2 * o.axb, njb, ntb, az, bz, tz, s_[1], s_[2], ez, eza, ezb);
-                gA[0] += y * exa * ey * ez;
-                gA[1] += y * ex * eya * ez;
-                gA[2] += y * ex * ey * eza;
-                gB[0] += y * exb * ey * ez;
-                gB[1] += y * ex * eyb * ez;
-                gB[2] += y * ex * ey * ezb;
-            }
-            // the ket's centres: lanes over (b, cd, u), |u| <= lc + ld + 1
-            for (int r = tid; r < nbq * ncd * nu1; r += nthr) {
-                const int b = r / (ncd * nu1), rr = r - b * (ncd * nu1);
-                const int cd = rr / nu1, u = rr - cd * nu1;
-                const int pc = pcd[cd], cu = hc[u];
-                const int cx = pc & 15, cy = (pc >> 4) & 15, cz = (pc >> 8) & 15;
-                const int dx = (pc >> 12) & 15, dy = (pc >> 16) & 15, dz = pc >> 20;
-                const int ux = cu & 15, uy = (cu >> 4) & 15, uz = cu >> 8;
-                const int ox = ux - cx - dx, oy = uy - cy - dy, oz = uz - cz - dz;
-                if (ox > 1 || oy > 1 || oz > 1 || (ox == 1) + (oy == 1) + (oz == 1) > 1) continue;
-                const double* __restrict__ s_ = sc + b * GF_SC;
-                const double* __restrict__ wr = Wb + b * ncd * nt + cd * nt;
-                const double* __restrict__ Rb = R0 + b * nR;
-                double x = 0.0;
-                for (int t = 0; t < nt; ++t) x += Rb[add[t * nu1 + u]] * wr[t];
-                x *= ((ux + uy + uz) & 1) ? -s_[0] : s_[0];
-                const double* __restrict__ e = ek + b * axk3;
-                double ex, exc, exd, ey, eyc, eyd, ez, ezc, ezd;
-                fj_gf_axis(e, njk, ntk, cx, dx, ux, s_[3], s_[4], ex, exc, exd);
-                fj_gf_axis(e + o.axk, njk, ntk, cy, dy, uy, s_[3], s_[4], ey, eyc, eyd);
-                fj_gf_axis(e + 2 * o.axk, njk, ntk, cz, dz, uz, s_[3], s_[4], ez, ezc, ezd);
-                gC[0] += x * exc * ey * ez;
-                gC[1] += x * ex * eyc * ez;
-                gC[2] += x * ex * ey * ezc;
-                gD[0] += x * exd * ey * ez;
-                gD[1] += x * ex * eyd * ez;
-                gD[2] += x * ex * ey * ezd;
+            // the centres: Y = Wk H^T (nab x nt1, over u) and X = Wb H
+            // (ncd x nu1, over t) in register tiles, each tile's elements
+            // contracted at once with the differentiated E into the sums
+            {
+                const int tmy = (nab + GF_TM - 1) / GF_TM, tny = (nt1 + GF_TN - 1) / GF_TN;
+                const int tmx = (ncd + GF_TM - 1) / GF_TM, tnx = (nu1 + GF_TN - 1) / GF_TN;
+                const int ny = tmy * tny, nx = tmx * tnx;
+                for (int r = tid; r < nbq * (ny + nx); r += nthr) {
+                    const int b = r / (ny + nx), rr = r - b * (ny + nx);
+                    const double* __restrict__ s_ = sc + b * GF_SC;
+                    double c[GF_TM][GF_TN];
+                    if (rr < ny) {
+                        const int m0 = (rr / tny) * GF_TM, n0 = (rr - (rr / tny) * tny) * GF_TN;
+                        fj_gf_tile(Wk + b * nab * nu, nu, 1, Hm + b * o.uw, 1, nu1, nab, nt1, nu, m0, n0, c);
+                        const double* __restrict__ e = eb + b * axb3;
+#pragma unroll
+                        for (int i = 0; i < GF_TM; ++i) {
+                            const int ab = m0 + i;
+                            if (ab >= nab) continue;
+                            const int pa = pab[ab];
+                            const int ax = pa & 15, ay = (pa >> 4) & 15, az = (pa >> 8) & 15;
+                            const int bx = (pa >> 12) & 15, by = (pa >> 16) & 15, bz = pa >> 20;
+#pragma unroll
+                            for (int j = 0; j < GF_TN; ++j) {
+                                const int t = n0 + j;
+                                if (t >= nt1) continue;
+                                const int ct = hc[t];
+                                const int tx = ct & 15, ty = (ct >> 4) & 15, tz = ct >> 8;
+                                // outside the box t_k <= a_k + b_k (+1 on at most one axis) every term is zero
+                                const int ox = tx - ax - bx, oy = ty - ay - by, oz = tz - az - bz;
+                                if (ox > 1 || oy > 1 || oz > 1 || (ox == 1) + (oy == 1) + (oz == 1) > 1) continue;
+                                const double y = c[i][j] * s_[0];
+                                double ex, exa, exb, ey, eya, eyb, ez, eza, ezb;
+                                fj_gf_axis(e, njb, ntb, ax, bx, tx, s_[1], s_[2], ex, exa, exb);
+                                fj_gf_axis(e + o.axb, njb, ntb, ay, by, ty, s_[1], s_[2], ey, eya, eyb);
+                                fj_gf_axis(e + 2 * o.axb, njb, ntb, az, bz, tz, s_[1], s_[2], ez, eza, ezb);
+                                gA[0] += y * exa * ey * ez;
+                                gA[1] += y * ex * eya * ez;
+                                gA[2] += y * ex * ey * eza;
+                                gB[0] += y * exb * ey * ez;
+                                gB[1] += y * ex * eyb * ez;
+                                gB[2] += y * ex * ey * ezb;
+                            }
+                        }
+                    } else {
+                        const int q_ = rr - ny;
+                        const int m0 = (q_ / tnx) * GF_TM, n0 = (q_ - (q_ / tnx) * tnx) * GF_TN;
+                        fj_gf_tile(Wb + b * ncd * nt, nt, 1, Hm + b * o.uw, nu1, 1, ncd, nu1, nt, m0, n0, c);
+                        const double* __restrict__ e = ek + b * axk3;
+#pragma unroll
+                        for (int i = 0; i < GF_TM; ++i) {
+                            const int cd = m0 + i;
+                            if (cd >= ncd) continue;
+                            const int pc = pcd[cd];
+                            const int cx = pc & 15, cy = (pc >> 4) & 15, cz = (pc >> 8) & 15;
+                            const int dx = (pc >> 12) & 15, dy = (pc >> 16) & 15, dz = pc >> 20;
+#pragma unroll
+                            for (int j = 0; j < GF_TN; ++j) {
+                                const int u = n0 + j;
+                                if (u >= nu1) continue;
+                                const int cu = hc[u];
+                                const int ux = cu & 15, uy = (cu >> 4) & 15, uz = cu >> 8;
+                                const int ox = ux - cx - dx, oy = uy - cy - dy, oz = uz - cz - dz;
+                                if (ox > 1 || oy > 1 || oz > 1 || (ox == 1) + (oy == 1) + (oz == 1) > 1) continue;
+                                const double x = c[i][j] * (((ux + uy + uz) & 1) ? -s_[0] : s_[0]);
+                                double ex, exc, exd, ey, eyc, eyd, ez, ezc, ezd;
+                                fj_gf_axis(e, njk, ntk, cx, dx, ux, s_[3], s_[4], ex, exc, exd);
+                                fj_gf_axis(e + o.axk, njk, ntk, cy, dy, uy, s_[3], s_[4], ey, eyc, eyd);
+                                fj_gf_axis(e + 2 * o.axk, njk, ntk, cz, dz, uz, s_[3], s_[4], ez, ezc, ezd);
+                                gC[0] += x * exc * ey * ez;
+                                gC[1] += x * ex * eyc * ez;
+                                gC[2] += x * ex * ey * ezc;
+                                gD[0] += x * exd * ey * ez;
+                                gD[1] += x * ex * eyd * ez;
+                                gD[2] += x * ex * ey * ezd;
+                            }
+                        }
+                    }
+                }
             }
             __syncthreads();   // before the next batch rewrites the tables
         }

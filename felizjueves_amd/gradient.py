@@ -120,14 +120,12 @@ def gf_launches(basis: Basis, shell_atom) -> GfStream:
 
 
 def default_threads(cls) -> int:
-    """Workgroup size of a class: one wavefront for the small classes, more
-    lanes where the (ab, t) and (cd, u) loops are long."""
-    la, lb, lc, ld = cls
-    herm = lambda n: n * (n + 1) * (n + 2) // 6  # noqa: E731
-    nab = (la + 1) * (la + 2) // 2 * ((lb + 1) * (lb + 2) // 2)
-    ncd = (lc + 1) * (lc + 2) // 2 * ((ld + 1) * (ld + 2) // 2)
-    n = max(nab * herm(la + lb + 2), ncd * herm(lc + ld + 2))
-    return 64 if n <= 256 else (128 if n <= 1024 else 256)
+    """Workgroup size of a class: 256 lanes (four wavefronts) for every
+    class. Measured per class on C1 and the 10-atom cluster at 64, 128 and
+    256 lanes: 256 was fastest or within 15% of the fastest everywhere (the
+    register-tiled stages have few tiles per primitive quartet, so more
+    lanes take more primitive quartets per batch)."""
+    return 256
 
 
 def quartets(stream: GfStream) -> int:
