@@ -29,11 +29,13 @@
 // A workgroup (threads = 64 .. 256 cooperating through LDS) takes unique
 // quartets in turn (grid-stride) and a quartet's primitive quartets in
 // batches of B: per batch the per-axis E tables to l + 1 on every centre and
-// the Hermite tower to L + 1 of each of its B primitive quartets, then lanes
-// over (b, ab, u), (b, cd, t), (b, ab, t) and (b, cd, u) -- L + 4 barriers
-// per batch, not per primitive quartet, so a highly contracted low class
-// fills the lanes and a (dd|dd) quartet (B = 1) has work for 256. B is the
-// largest that fits the LDS budget and the lanes (fj_gf_plan). The twelve
+// the Hermite tower to L + 1 of each of its B primitive quartets, the E
+// products, the two weight GEMMs (Wk, Wb), the tower gathered into H[t][u],
+// and the two contraction GEMMs (Y, X) with the differentiated E in their
+// epilogue, lanes over (b, 4 x 4 register tile) -- L + 6 barriers per batch,
+// not per primitive quartet, so a highly contracted low class fills the
+// lanes and a (dd|dd) quartet (B = 1) has work for 256. B is the largest
+// that fits the LDS budget and the lanes (fj_gf_plan). The twelve
 // sums are reduced over the workgroup and added into G[atom][k] with twelve
 // atomics per quartet.
 // No screen (kernel_gf.py:82-85: the F screen bounds an integral, not its
