@@ -141,22 +141,45 @@ def mixed_kwargs(cfg: dict) -> dict:
 
 
 def main() -> None:
-    """Measure where the FP32 tier pays for a tuned configuration and record
-    it in the configuration file (`fp32_classes`)."""
+    """Tune a workload (--tune: per class pair threads and split, then the
+    stream count) and / or measure where the FP32 tier pays for the tuned
+    configuration (`fp32_classes`), writing the configuration file.
+
+        python -m felizjueves_amd.autotune tuned/c2_40atom.json --workload c2
+        python -m felizjueves_amd.autotune tuned/c4_64mol_d.json --workload c4 --tune --every 4
+
+    c4: the batched workload's molecules (shells through d), every `every`-th
+    of the 64 (a subset tunes the per-class choices in a fraction of the
+    time; the choices are per class pair and carry to the whole batch)."""
     import argparse
     import sys
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", help="tuned configuration file (JSON) to update")
-    ap.add_argument("--workload", choices=("c1", "c2"), required=True)
+    ap.add_argument("config", help="tuned configuration file (JSON) to update or create")
+    ap.add_argument("--workload", choices=("c1", "c2", "c4"), required=True)
+    ap.add_argument("--tune", action="store_true", help="run the threads / split / streams search first")
+    ap.add_argument("--every", type=int, default=4, help="c4: tune on every N-th molecule")
     ap.add_argument("--selector", default=str(Path(__file__).resolve().parent / "tuned" / "selector"))
     ap.add_argument("--out", default=None, help="write the updated configuration here instead")
     a = ap.parse_args()
     from .selector import LearnedSelector
-    from .systems import config1, config2
-    s = config1() if a.workload == "c1" else config2()
+    from .systems import batch_molecule, config1, config2
+    if a.workload == "c4":
+        systems = [(m.basis, m.D) for m in (batch_molecule(i, lmax=2) for i in range(0, 64, a.every))]
+    else:
+        s = config1() if a.workload == "c1" else config2()
+        systems = [(s.basis, s.D)]
+    log = lambda *m: print("[autotune]", *m, flush=True)  # noqa: E731
+    if a.tune:
+        t0 = time.perf_counter()
+        cfg = tune(systems, tau=1e-12, log=log)
+        cfg["tuned_on"] = (f"{a.workload}: {len(systems)} systems" +
+                           (f" (every {a.every}-th of the 64 batch molecules, shells through d)"
+                            if a.workload == "c4" else ""))
+        log(f"tuned in {time.perf_counter() - t0:.0f} s: streams {cfg['n_streams']}, graph ms {cfg['graph_ms']}")
+    else:
+        cfg = load(a.config)
     sel = LearnedSelector.load(a.selector)
-    cfg = load(a.config)
-    pays = tier_pays([(s.basis, s.D)], cfg, selector=sel,
+    pays = tier_pays(systems, cfg, selector=sel, tau=1e-12,
                      log=lambda m: print("[tier_pays] fp64 ms, mixed ms:", m, flush=True))
     cfg["fp32_classes"] = pays
     save(cfg, a.out or a.config)
