@@ -23,6 +23,13 @@ Also reported:
   10-atom cluster: gradients/s, its kernel's roofline (HIP events around
   the merged per-class launches), max|dG| against the oracle and the
   oracle's own time on the host cores for the three-atom system;
+* "split_jk": the split HGP J and K kernels (per orbit representative) on
+  the three-atom system (`--workload split-df` adds the 10-atom cluster),
+  ms per J / K build, their roofline, parity and the oracle's ordered sweep
+  beside them;
+* "density_fitting": the metric and three-center integrals on the upstream
+  fixtures (against libcint) and on the 10-atom cluster with a synthetic
+  aux set;
 * "roofline" of the dominant kernel (HIP events around its eager launches
   on the stream it runs on) and "cpu_baseline" (the oracle on the host
   cores, a bounded sample).
@@ -227,6 +234,183 @@ def gradient_block(steps: int, cpu: bool = True) -> dict:
                                                 f"oracle's ordered sweep, {tc:.1f} s")}
         log("gradient", key, json.dumps(r))
         out[key] = r
+    return out
+
+
+def split_jk_block(cpu: bool = True, cluster: bool = True) -> dict:
+    """The split HGP J and K kernels (hjk.py: one kernel per orbit
+    representative, every ordered quartet of the canonical stream, the
+    upstream's split design) on the three-atom system and the 10-atom
+    cluster: ms per J and per K build (HIP events around all launches of a
+    build, the launch arguments staged on the device beforehand), the
+    quartets passing the in-kernel screen and their algorithmic FLOPs
+    (hgp.cost of the kernel's class: VRR per primitive quartet, HRR and
+    digestion per quartet) over that time; on the three-atom system
+    max|dJ|, max|dK| against the oracle's ordered sweep, whose time (J and
+    K together, host cores) is the CPU baseline."""
+    import torch
+    from felizjueves_amd import hjk, native
+    from felizjueves_amd.hgp import cost
+    from felizjueves_amd.systems import config1, config2
+    out = {}
+    work = [("c1_3atom", config1())] + ([("cluster_10atom", config2(n_atoms=10))] if cluster else [])
+    for key, s in work:
+        r = {"workload": s.name, "n_ao": s.basis.n_ao}
+        run = hjk.GpuHJKRunner()
+        X = {}
+        for screen in ("j", "k"):
+            t = time.perf_counter()
+            plan = hjk.hjk_launches(s.basis, s.D, tau=1e-12, screen=screen)
+            t_plan = time.perf_counter() - t
+            D = np.asarray(s.D, dtype=np.float64)
+            # the launch arguments on the device: each (batch, orientation)
+            # side once, D and its block maxima once
+            dD = native.to_dev(np.ascontiguousarray(D.ravel()))
+            dDsh = native.to_dev(np.ascontiguousarray(plan.dmax.ravel()))
+            sides: dict = {}
+
+            def side(bi, sw):
+                if (bi, sw) not in sides:
+                    sides[(bi, sw)] = {k: (native.to_dev(v) if isinstance(v, np.ndarray) else v)
+                                       for k, v in hjk.side_records(s.basis, plan, plan.batches[bi], sw).items()}
+                return sides[(bi, sw)]
+            staged, n_q, fl = [], 0, 0.0
+            for L in plan.launches:
+                (bi, bs), (ki, ks) = hjk.kernel_sides(L)
+                b, k = side(bi, bs), side(ki, ks)
+                a = {"n_bra": len(b["G"]), "n_ket": len(k["G"]), "k_ab": b["k"], "k_cd": k["k"],
+                     "n_sh": s.basis.n_sh, "n_ao": s.basis.n_ao, "perm": L.orientation.perm,
+                     "tau": plan.tau, "bra_prim": b["prim"], "G_ab": b["G"], "bra_sh": b["sh"],
+                     "bra_ao": b["ao"], "ket_prim": k["prim"], "G_cd": k["G"], "ket_sh": k["sh"],
+                     "ket_ao": k["ao"], "Dsh": dDsh, "D": dD}
+                if b["lb"]:
+                    a["AB"] = b["sep"]
+                if k["lb"]:
+                    a["CD"] = k["sep"]
+                staged.append((("hj" if screen == "j" else "hk", L.cls), a["n_bra"], a))
+                bra, ket = plan.batches[L.bra], plan.batches[L.ket]
+                gb = np.array([p_.G for p_ in bra.pairs])
+                gk = np.array([q.G for q in ket.pairs])
+                # the screen's density block: D_cd (J) or D_bd (K) per quartet
+                if screen == "j":
+                    dm = np.array([plan.dmax[q.a, q.b] for q in ket.pairs])[None, :]
+                else:
+                    dm = plan.dmax[np.array([p_.b for p_ in bra.pairs])[:, None],
+                                   np.array([q.b for q in ket.pairs])[None, :]]
+                live = int(np.count_nonzero(gb[:, None] * gk[None, :] * dm >= 1e-12))
+                n_q += live
+                fl += live * cost(L.cls).flops(bra.degree, ket.degree)
+            Xd = run.zeros(s.basis.n_ao)
+            for kern, nb, a in staged:
+                run(kern, nb, (64, 1), a, Xd)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 3
+            e0.record()
+            for _ in range(reps):
+                Xd.zero_()
+                for kern, nb, a in staged:
+                    run(kern, nb, (64, 1), a, Xd)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            X[screen] = Xd.cpu().numpy()
+            r[screen] = {"ms_per_build": ms, "launches": len(plan.launches), "quartets_visited":
+                         plan.counts()["quartets"], "quartets_computed": n_q, "plan_s": t_plan,
+                         "roofline": {"bound": "fp64_valu", "achieved": fl / ms / 1e9,
+                                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                      "frac": fl / ms / 1e9 / FP64_PEAK_TFLOPS, "gflop": fl / 1e9}}
+        if key == "c1_3atom":
+            sys.path.insert(0, str(ROOT / "tests"))
+            from oracle import oracle as O
+            t = time.perf_counter()
+            Jo, Ko = O.fock_ordered(s.basis, s.D)
+            t = time.perf_counter() - t
+            r["max_dJ_vs_oracle"] = float(np.abs(X["j"] - Jo).max())
+            r["max_dK_vs_oracle"] = float(np.abs(X["k"] - Ko).max())
+            if cpu:
+                r["cpu_baseline"] = {"value": 1.0 / t, "unit": "J+K builds/s", "cores": O.threads(),
+                                     "kind": "port",
+                                     "sample": f"the oracle's ordered sweep (J and K together) of the "
+                                               f"three-atom system, {t:.1f} s"}
+        log("split_jk", key, json.dumps(r))
+        out[key] = r
+    return out
+
+
+def _aux_even_tempered(coords, ls=(0, 1, 2, 3, 4), n=4, a0=0.3, ratio=2.5):
+    """A synthetic auxiliary basis: per atom, per l, n even-tempered
+    single-primitive shells a0 ratio^k, unit-normalised primitives."""
+    from felizjueves_amd.basis import primitive_norm
+    from felizjueves_amd.df import AuxBasis
+    sl, sc, ex, co = [], [], [], []
+    for c in coords:
+        for l in ls:  # noqa: E741
+            for k in range(n):
+                a = a0 * ratio ** k
+                sl.append(l)
+                sc.append(c)
+                ex.append(a)
+                co.append(primitive_norm(l, a))
+    return AuxBasis(sl, sc, [1] * len(sl), ex, co)
+
+
+def df_block(cpu: bool = True) -> dict:
+    """The density-fitting integrals (df.py: the metric (P|Q) and the
+    three-center (mn|P) on the loop-form kernel) on the upstream fixtures
+    (H2O cc-pVTZ-JKFIT metric, 166 aux functions; H2O STO-3G x JKFIT
+    three-center) with max|d| against libcint, and on the 10-atom cluster
+    with a synthetic even-tempered s..g aux set (20 shells per atom,
+    1,400 aux functions): ms per call (HIP events), integrals per second;
+    the oracle's time on the metric fixture is the CPU baseline."""
+    import torch
+    from felizjueves_amd import df
+    from felizjueves_amd.basis import GOLDEN, Basis
+    from felizjueves_amd.systems import config2
+
+    def timed_call(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    z = np.load(GOLDEN / "df_h2o_ccpvtz_jkfit_metric.npz", allow_pickle=False)
+    aux = df.AuxBasis.from_npz(z)
+    M = df.metric_matrix(aux).cpu().numpy()
+    ms = timed_call(lambda: df.metric_matrix(aux))
+    r = {"n_aux": aux.n_aux, "ms": ms, "max_d_vs_libcint": float(np.abs(M - z["int2c2e"]).max()),
+         "integrals_per_s": aux.n_aux ** 2 / 2 / ms * 1e3}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle import oracle as O
+        t = time.perf_counter()
+        O.df_metric(aux)
+        t = time.perf_counter() - t
+        r["cpu_baseline"] = {"value": 1.0 / t, "unit": "metrics/s", "cores": O.threads(), "kind": "port",
+                             "sample": f"the oracle's metric of the same aux basis, {t:.2f} s"}
+    out["metric_h2o_ccpvtz_jkfit"] = r
+    z = np.load(GOLDEN / "df_h2o_sto3g_jkfit_3c.npz", allow_pickle=False)
+    aux = df.AuxBasis.from_npz(z)
+    b = Basis.from_arrays(z["shell_l"], z["shell_centre"], z["shell_nprim"], z["exps"], z["coefs"])
+    E3 = df.eri3(b, aux).cpu().numpy()
+    out["eri3_h2o_sto3g_jkfit"] = {"n_ao": b.n_ao, "n_aux": aux.n_aux,
+                                   "ms": timed_call(lambda: df.eri3(b, aux)),
+                                   "max_d_vs_libcint": float(np.abs(E3 - z["int3c2e"]).max())}
+    s = config2(n_atoms=10)
+    aux = _aux_even_tempered(s.coords)
+    ms_m = timed_call(lambda: df.metric_matrix(aux))
+    ms_3 = timed_call(lambda: df.eri3(s.basis, aux))
+    out["cluster_10atom"] = {"workload": s.name + " + even-tempered s..g aux", "n_ao": s.basis.n_ao,
+                             "n_aux": aux.n_aux, "metric_ms": ms_m, "eri3_ms": ms_3,
+                             "eri3_integrals_per_s": s.basis.n_ao * (s.basis.n_ao + 1) / 2 * aux.n_aux
+                             / ms_3 * 1e3}
+    log("df", json.dumps(out))
     return out
 
 
@@ -552,14 +736,16 @@ def main() -> None:
     ap.add_argument("--no-extra", action="store_true", help="skip the c3 / c5 blocks")
     ap.add_argument("--molecules", type=int, nargs="+", default=[7],
                     help="c4f: which C4 molecules (index i: seed 4 + i)")
-    ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f", "c4f-batch", "gradient"),
+    ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f", "c4f-batch", "gradient",
+                                           "split-df"),
                     default="c2",
                     help="c4d: the 64-molecule batch (8-80 atoms, seeds 4+i) with s,s,p,d shells "
                          "(with its f shells one build would take hours on the loop-form f "
                          "kernel); cof: CO/cc-pVTZ through f (the loop-form kernels); c4f: "
                          "single C4 molecules with and without their f shells; c4f-batch: "
                          "chosen C4 molecules with f in one batch builder; gradient: the fused "
-                         "two-electron gradient block alone")
+                         "two-electron gradient block alone; split-df: the split J / K and "
+                         "density-fitting blocks alone")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for HBM counter runs: one elementwise copy of 512 MiB (the counters' "
                          "scale) and one eager build of the default plan, then exit")
@@ -605,6 +791,10 @@ def main() -> None:
         return
     if a.workload == "gradient":
         print(json.dumps({"gradient": gradient_block(a.steps, cpu=not a.no_cpu)}), flush=True)
+        return
+    if a.workload == "split-df":
+        print(json.dumps({"split_jk": split_jk_block(cpu=not a.no_cpu),
+                          "density_fitting": df_block(cpu=not a.no_cpu)}), flush=True)
         return
 
     selector = load_selector(a.selector)
@@ -675,6 +865,8 @@ def main() -> None:
         extra["jk"] = jk_block(3, 1)
         extra["pair_stream"] = stream_block(s)
         extra["gradient"] = gradient_block(max(5, a.steps // 2), cpu=not a.no_cpu)
+        extra["split_jk"] = split_jk_block(cpu=not a.no_cpu, cluster=False)
+        extra["density_fitting"] = df_block(cpu=not a.no_cpu)
     cpu = None if a.no_cpu else cpu_baseline(plain, D, G64)
     out = {
         "metric": METRIC, "value": a.steps / t_prod, "unit": "Fock builds/s", "n_gpus": 1,
