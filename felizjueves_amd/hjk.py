@@ -90,6 +90,7 @@ class HPlan:
     launches: tuple
     G: dict                        # (a, b) -> G of the ordered pair
     dmax: np.ndarray               # [n_sh, n_sh] max |D| per shell block
+    prim_chunk: int = 0            # pack pairs as sub-pairs of <= this many primitive pairs (0: whole)
 
     def __iter__(self):
         return iter(self.launches)
@@ -134,9 +135,25 @@ def _density(basis: Basis, D) -> np.ndarray:
     return D
 
 
+# default sub-pair size: at most 2 x 2 primitive quartets per lane. Measured
+# on the GPU (J, C1 / the 10-atom cluster, ms): whole pairs 3,182 / 16,797;
+# 16: 1,042 / 7,539; 9: 483 / 3,721; 4: 180 / 1,399; 3: 157 / 1,103;
+# 2: 136 / 973; 1: 137 / 1,340 (every sub-quartet pays its own HRR).
+PRIM_CHUNK = 2
+
+
 def hjk_launches(basis: Basis, D, *, tau: float, mode: str = "canonical",
-                 screen: str = "j") -> HPlan:
-    """The ordered pair stream of `basis` and the symmetric density D."""
+                 screen: str = "j", prim_chunk: int = PRIM_CHUNK) -> HPlan:
+    """The ordered pair stream of `basis` and the symmetric density D.
+
+    prim_chunk: the kernels take one lane per contracted quartet and its
+    whole primitive loop, so a side's pairs are packed as sub-pairs of at
+    most `prim_chunk` primitive pairs (the largest divisor of the batch's
+    degree not above it; 0 packs whole pairs, the upstream's packing). A
+    sub-pair carries its pair's G, shells and AOs, so every sub-quartet of a
+    quartet passes or fails the screen together, and the HRR and digestion
+    are linear in the contracted boundary: the sub-quartets' J or K sum to
+    the quartet's (each through its own FP64 HRR and atomics)."""
     if mode not in MODES:
         raise ValueError(f"mode = {mode!r}: only the canonical class mode is built ({MODES})")
     if screen not in SCREENS:
@@ -175,7 +192,18 @@ def hjk_launches(basis: Basis, D, *, tau: float, mode: str = "canonical",
         for ki, ket in enumerate(batches):
             ordered = (bra.la, bra.lb, ket.la, ket.lb)
             launches.append(HLaunch(bi, ki, ordered, to_representative(ordered)))
-    return HPlan(float(tau), mode, screen, batches, tuple(launches), G, P.density_blocks(basis, D))
+    if not (isinstance(prim_chunk, int) and prim_chunk >= 0):
+        raise ValueError(f"prim_chunk = {prim_chunk!r} is not an int >= 0")
+    return HPlan(float(tau), mode, screen, batches, tuple(launches), G, P.density_blocks(basis, D),
+                 prim_chunk)
+
+
+def sub_degree(k: int, chunk: int) -> int:
+    """The sub-pair degree of a degree-k batch: the largest divisor of k not
+    above chunk (k itself when chunk is 0 or >= k)."""
+    if chunk <= 0 or chunk >= k:
+        return k
+    return max(d for d in range(1, chunk + 1) if k % d == 0)
 
 
 def kernel_sides(L: HLaunch):
@@ -192,7 +220,9 @@ def kernel_sides(L: HLaunch):
 def side_records(basis: Basis, plan: HPlan, batch: HBatch, swapped: bool) -> dict:
     """One kernel side: per pair (x, y) (= (b, a) when swapped) its staged
     records built on x, (P, P - X, zeta, U), G of the ordered pair (x, y),
-    sep = X - Y, shells and first AOs; degree-uniform."""
+    sep = X - Y, shells and first AOs; degree-uniform. With the plan's
+    prim_chunk, each pair as its sub-pairs (sub_degree records each, the
+    pair's G, sep, shells and AOs repeated)."""
     xy = [(p.b, p.a) if swapped else (p.a, p.b) for p in batch.pairs]
     la, lb = (batch.lb, batch.la) if swapped else (batch.la, batch.lb)
     pp = P.primitive_pairs(basis, xy)
@@ -205,12 +235,16 @@ def side_records(basis: Basis, plan: HPlan, batch: HBatch, swapped: bool) -> dic
         prim[i, :, 6] = pp.zeta[sel]
         prim[i, :, 7] = pp.U[sel]
     c = basis.shell_centre
-    return {"prim": prim.reshape(-1, 8), "k": k, "lb": lb,
-            "sep": np.array([c[x] - c[y] for x, y in xy], dtype=np.float64).reshape(-1, 3),
-            "G": np.array([plan.G[x, y] for x, y in xy]),
-            "sh": np.array(xy, dtype=np.int32).reshape(-1, 2),
-            "ao": np.array([(basis.shell_ao[x], basis.shell_ao[y]) for x, y in xy],
-                           dtype=np.int32).reshape(-1, 2)}
+    # sub-pairs: pair i's k records as m consecutive runs of d (d m = k)
+    d = sub_degree(k, plan.prim_chunk)
+    m = k // d
+    rep = lambda a: np.repeat(a, m, axis=0)  # noqa: E731
+    return {"prim": prim.reshape(-1, 8), "k": d, "lb": lb,
+            "sep": rep(np.array([c[x] - c[y] for x, y in xy], dtype=np.float64).reshape(-1, 3)),
+            "G": rep(np.array([plan.G[x, y] for x, y in xy])),
+            "sh": rep(np.array(xy, dtype=np.int32).reshape(-1, 2)),
+            "ao": rep(np.array([(basis.shell_ao[x], basis.shell_ao[y]) for x, y in xy],
+                               dtype=np.int32).reshape(-1, 2))}
 
 
 def launch_args(basis: Basis, plan: HPlan, L: HLaunch, D: np.ndarray, Dsh: np.ndarray) -> dict:

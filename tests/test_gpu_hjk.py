@@ -41,3 +41,18 @@ def test_split_vs_ordered_sweep_every_kernel(screen):
           len(perms), flush=True)
     assert len(w.kernels) == 21
     assert err < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("screen", ["j", "k"])
+def test_sub_pairs_match_whole_pairs(screen):
+    """The sub-pair packing (the default) and the upstream's whole-pair
+    packing give the same J / K on C1 (contractions up to 6 x 6 primitive
+    pairs per side): the sub-quartets sum to the quartet."""
+    s = config1()
+    X = {}
+    for pc in (0, hjk.PRIM_CHUNK):
+        plan = hjk.hjk_launches(s.basis, s.D, tau=1e-12, screen=screen, prim_chunk=pc)
+        X[pc] = hjk.walk_hjk(plan, s.basis, s.D, run=hjk.GpuHJKRunner()).X
+    d = np.abs(X[0] - X[hjk.PRIM_CHUNK]).max()
+    print(screen, "sub-pairs vs whole pairs", d, flush=True)
+    assert d <= 1e-12 * max(1.0, np.abs(X[0]).max())

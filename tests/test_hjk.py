@@ -116,6 +116,29 @@ def test_walk_with_oracle_eris_vs_ordered_sweep(screen):
     assert np.abs(X - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
 
 
+def test_sub_pairs_partition_each_pair():
+    """The sub-pair packing (prim_chunk) splits each pair's primitive records
+    into runs of sub_degree that concatenate to the whole pair's, and repeats
+    its G, separation, shells and AOs per run."""
+    s = config1()
+    whole = hjk.hjk_launches(s.basis, s.D, tau=1e-12, screen="j", prim_chunk=0)
+    sub = hjk.hjk_launches(s.basis, s.D, tau=1e-12, screen="j")
+    assert sub.prim_chunk == hjk.PRIM_CHUNK and whole.prim_chunk == 0
+    assert [hjk.sub_degree(k, 9) for k in (1, 5, 6, 7, 12, 18, 25, 36)] == [1, 5, 6, 7, 6, 9, 5, 9]
+    checked = 0
+    for bi, bt in enumerate(whole.batches):
+        for sw in (False, True):
+            a = hjk.side_records(s.basis, whole, bt, sw)
+            b = hjk.side_records(s.basis, sub, sub.batches[bi], sw)
+            m = a["k"] // b["k"]
+            assert a["k"] == bt.degree and b["k"] == hjk.sub_degree(bt.degree, hjk.PRIM_CHUNK)
+            assert np.array_equal(a["prim"], b["prim"])       # the same records, regrouped
+            for key in ("G", "sep", "sh", "ao"):
+                assert np.array_equal(np.repeat(a[key], m, axis=0), b[key])
+            checked += m > 1
+    assert checked > 0
+
+
 def test_refusals():
     fx = load_fixture("h2o_631g")
     with pytest.raises(ValueError):
@@ -127,3 +150,5 @@ def test_refusals():
     co = load_fixture("co_ccpvtz")
     with pytest.raises(ValueError):
         hjk.hjk_launches(co.basis, co.density, tau=1e-12)
+    with pytest.raises(ValueError):
+        hjk.hjk_launches(fx.basis, fx.density, tau=1e-12, prim_chunk=-1)
