@@ -13,9 +13,11 @@
 // live ket, the weighted density blocks and the six F accumulators sit in
 // LDS, and per primitive quartet the lanes build the E tables and the
 // Hermite tower, then, for the bra rows ab in chunks of FJ_FL_CH,
-//   W[ab][q] = sum_p E^{ab}_p [p + q]           (lanes over (ab, q))
-//   x(ab,cd) = U U sum_q W[ab][q] (-1)^|q| E^{cd}_q   (lanes over (ab, cd))
-// and digest x at once (LDS atomics into the six blocks). After the ket's
+//   W[ab][q] = sum_p E^{ab}_p [p + q]           (lanes over 4 x 4 tiles of (ab, q))
+//   x(ab,cd) = U U sum_q W[ab][q] (-1)^|q| E^{cd}_q   (lanes over 4 x 4 tiles of (ab, cd))
+// and digest x at once (LDS atomics into the six blocks). In a tile the
+// E products come from the per-axis tables and are shared across the tile's
+// row or column: eight operands feed sixteen multiply-adds. After the ket's
 // primitives the blocks go to F with one global atomic per element.
 // Every barrier sits in control flow that is uniform over the wavefront.
 //
@@ -227,47 +229,114 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
                 const double UU = rb[7] * rk[7];
                 for (int r0 = 0; r0 < nab; r0 += FJ_FL_CH) {
                     const int nr = min(FJ_FL_CH, nab - r0);
-                    for (int j = lane; j < nr * nq; j += 64) {
-                        const int r = j / nq, qi = j - r * nq;
-                        const int ab = r0 + r;
-                        const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
-                        const double* __restrict__ ex = eab + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
-                        const double* __restrict__ ey =
-                            eab + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
-                        const double* __restrict__ ez =
-                            eab + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
-                        double s = 0.0;
-                        for (int pi = 0; pi < np; ++pi) {
-                            const int cp = hc[pi];
-                            s += (ex[cp & 15] * ey[(cp >> 4) & 15] * ez[cp >> 8]) * R0[pqi[pi * nq + qi]];
+                    // W[r][q] = sum_p E^{ab}_p [p + q] in 4 x 4 register tiles of
+                    // (r, q): per p four E products (from the axis tables) and
+                    // four tower entries make sixteen multiply-adds
+                    {
+                        const int tq = (nq + 3) >> 2, nt = ((nr + 3) >> 2) * tq;
+                        for (int t = lane; t < nt; t += 64) {
+                            const int m0 = (t / tq) << 2, n0 = (t - (t / tq) * tq) << 2;
+                            const double* ex[4];
+                            const double* ey[4];
+                            const double* ez[4];
+                            int qq[4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                const int ab = r0 + min(m0 + i, nr - 1);
+                                const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
+                                ex[i] = eab + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
+                                ey[i] = eab + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
+                                ez[i] = eab + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
+                                qq[i] = min(n0 + i, nq - 1);
+                            }
+                            double c[4][4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) c[i][j] = 0.0;
+                            for (int pi = 0; pi < np; ++pi) {
+                                const int cp = hc[pi];
+                                const int px = cp & 15, py = (cp >> 4) & 15, pz = cp >> 8;
+                                const unsigned short* __restrict__ pr = pqi + pi * nq;
+                                double ea[4], rv[4];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) ea[i] = ex[i][px] * ey[i][py] * ez[i][pz];
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) rv[j] = R0[pr[qq[j]]];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                    for (int j = 0; j < 4; ++j) c[i][j] += ea[i] * rv[j];
+                            }
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                for (int j = 0; j < 4; ++j)
+                                    if (m0 + i < nr && n0 + j < nq) W[(m0 + i) * nq + n0 + j] = c[i][j];
                         }
-                        W[r * nq + qi] = s;
                     }
                     __syncthreads();
-                    for (int j = lane; j < nr * ncd; j += 64) {
-                        const int r = j / ncd, cdi = j - r * ncd;
-                        const int ab = r0 + r;
-                        const int ia = ab / nb, jb = ab % nb, kc = cdi / nd, ld_ = cdi % nd;
-                        const int cc = hc[hcc + kc], cdd = hc[hd + ld_];
-                        const double* __restrict__ fx = ecd + ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
-                        const double* __restrict__ fy =
-                            ecd + axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
-                        const double* __restrict__ fz =
-                            ecd + 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
-                        const double* __restrict__ wr = W + r * nq;
-                        double s = 0.0;
-                        for (int qi = 0; qi < nq; ++qi) {
-                            const int cq = hc[qi];
-                            const double e = fx[cq & 15] * fy[(cq >> 4) & 15] * fz[cq >> 8];
-                            s += wr[qi] * (((cq & 15) + ((cq >> 4) & 15) + (cq >> 8)) & 1 ? -e : e);
+                    // x(ab, cd) = U U sum_q W[r][q] (-1)^|q| E^{cd}_q in 4 x 4 tiles
+                    // of (r, cd), each element digested at once into the six blocks
+                    {
+                        const int tcd = (ncd + 3) >> 2, nt = ((nr + 3) >> 2) * tcd;
+                        for (int t = lane; t < nt; t += 64) {
+                            const int m0 = (t / tcd) << 2, n0 = (t - (t / tcd) * tcd) << 2;
+                            const double* fx[4];
+                            const double* fy[4];
+                            const double* fz[4];
+                            int wr[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int cdi = min(n0 + j, ncd - 1);
+                                const int cc = hc[hcc + cdi / nd], cdd = hc[hd + cdi % nd];
+                                fx[j] = ecd + ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
+                                fy[j] = ecd + axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
+                                fz[j] = ecd + 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
+                                wr[j] = min(m0 + j, nr - 1) * nq;
+                            }
+                            double c[4][4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) c[i][j] = 0.0;
+                            for (int qi = 0; qi < nq; ++qi) {
+                                const int cq = hc[qi];
+                                const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
+                                const bool neg = (qx + qy + qz) & 1;
+                                double w[4], e[4];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) w[i] = W[wr[i] + qi];
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    const double v = fx[j][qx] * fy[j][qy] * fz[j][qz];
+                                    e[j] = neg ? -v : v;
+                                }
+#pragma unroll
+                                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                    for (int j = 0; j < 4; ++j) c[i][j] += w[i] * e[j];
+                            }
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                if (m0 + i >= nr) continue;
+                                const int ab = r0 + m0 + i;
+                                const int ia = ab / nb, jb = ab % nb;
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    const int cdi = n0 + j;
+                                    if (cdi >= ncd) continue;
+                                    const int kc = cdi / nd, ld_ = cdi % nd;
+                                    const double x = UU * c[i][j];
+                                    atomicAdd(&fab[ab], wcd[cdi] * x);
+                                    atomicAdd(&fcd[cdi], wab[ab] * x);
+                                    atomicAdd(&fac[ia * nc + kc], wbd[jb * nd + ld_] * x);
+                                    atomicAdd(&fad[ia * nd + ld_], wbc[jb * nc + kc] * x);
+                                    atomicAdd(&fbc[jb * nc + kc], wad[ia * nd + ld_] * x);
+                                    atomicAdd(&fbd[jb * nd + ld_], wac[ia * nc + kc] * x);
+                                }
+                            }
                         }
-                        const double x = UU * s;
-                        atomicAdd(&fab[ab], wcd[cdi] * x);
-                        atomicAdd(&fcd[cdi], wab[ab] * x);
-                        atomicAdd(&fac[ia * nc + kc], wbd[jb * nd + ld_] * x);
-                        atomicAdd(&fad[ia * nd + ld_], wbc[jb * nc + kc] * x);
-                        atomicAdd(&fbc[jb * nc + kc], wad[ia * nd + ld_] * x);
-                        atomicAdd(&fbd[jb * nd + ld_], wac[ia * nc + kc] * x);
                     }
                     __syncthreads();   // W and the E tables are rewritten next
                 }
