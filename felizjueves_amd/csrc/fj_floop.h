@@ -24,7 +24,7 @@
 // The includer provides fj_device.h and fj_kloop.h.
 #pragma once
 
-#define FJ_FL_CH 16   // bra rows per W chunk
+#define FJ_FL_CH 32   // bra rows per W chunk
 
 // The kernel's LDS, sized for its class and carved from one dynamic block:
 // byte offsets of each array (doubles, then ints, then the u16 index table)
