@@ -9,7 +9,7 @@
 // (upstream mmd/kernel.py:232-487 for the Hermite form; the records' U
 // carries exp(-mu |AB|^2), so the E tables start from 1).
 //
-// One wavefront per work item. The item's kets are taken one at a time; per
+// One workgroup (FJ_FL_BLOCK lanes) per work item. The item's kets are taken one at a time; per
 // live ket, the weighted density blocks and the six F accumulators sit in
 // LDS, and per primitive quartet the lanes build the E tables and the
 // Hermite tower, then, for the bra rows ab in chunks of FJ_FL_CH,
@@ -25,6 +25,7 @@
 #pragma once
 
 #define FJ_FL_CH 32   // bra rows per W chunk
+#define FJ_FL_BLOCK 256   // lanes per workgroup
 
 // The kernel's LDS, sized for its class and carved from one dynamic block:
 // byte offsets of each array (doubles, then ints, then the u16 index table)
@@ -59,9 +60,9 @@ __host__ __device__ inline FjFlLds fj_floop_lds(int la, int lb, int lc, int ld) 
     return o;
 }
 
-__global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
+__global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
     FJ_DYN_SHARED(fj_floop_lds_block);
-    const int lane = (int)threadIdx.x;
+    const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
     // the work item: (bra pair, first ket, system | bra chunk, ket end), or
     // (x, y * span, the single system, n_ket) without a work list
     int bra, k0, k1, kb_lo = 0, kb_n = 0;
@@ -121,8 +122,8 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
     int* const tw = reinterpret_cast<int*>(lds + lay.tw);
     unsigned short* const pqi = reinterpret_cast<unsigned short*>(lds + lay.pqi);
     const int axa = lay.axa, axc = lay.axc;
-    fj_kl_steps(tw, hc, nh, lane);
-    for (int j = lane; j < np * nq; j += 64) {
+    fj_kl_steps_n(tw, hc, nh, lane, nthr);
+    for (int j = lane; j < np * nq; j += nthr) {
         const int p = j / nq, q = j - p * nq;
         const int cp = fj_herm_decode(p), cq = fj_herm_decode(q);
         pqi[j] = (unsigned short)fj_herm_index((cp & 15) + (cq & 15),
@@ -169,27 +170,27 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
         const int oc = a.ket.ao[2 * k], od = a.ket.ao[2 * k + 1];
         const double CDx = ld ? a.ket.sep[3 * k] : 0.0, CDy = ld ? a.ket.sep[3 * k + 1] : 0.0,
                      CDz = ld ? a.ket.sep[3 * k + 2] : 0.0;
-        for (int j = lane; j < nab; j += 64) {
+        for (int j = lane; j < nab; j += nthr) {
             wab[j] = s_j * D[(size_t)(oa + j / nb) * n_ao + ob + j % nb];
             fab[j] = 0.0;
         }
-        for (int j = lane; j < ncd; j += 64) {
+        for (int j = lane; j < ncd; j += nthr) {
             wcd[j] = s_j * D[(size_t)(oc + j / nd) * n_ao + od + j % nd];
             fcd[j] = 0.0;
         }
-        for (int j = lane; j < na * nc; j += 64) {
+        for (int j = lane; j < na * nc; j += nthr) {
             wac[j] = s_k * D[(size_t)(oa + j / nc) * n_ao + oc + j % nc];
             fac[j] = 0.0;
         }
-        for (int j = lane; j < na * nd; j += 64) {
+        for (int j = lane; j < na * nd; j += nthr) {
             wad[j] = s_k * D[(size_t)(oa + j / nd) * n_ao + od + j % nd];
             fad[j] = 0.0;
         }
-        for (int j = lane; j < nb * nc; j += 64) {
+        for (int j = lane; j < nb * nc; j += nthr) {
             wbc[j] = s_k * D[(size_t)(ob + j / nc) * n_ao + oc + j % nc];
             fbc[j] = 0.0;
         }
-        for (int j = lane; j < nb * nd; j += 64) {
+        for (int j = lane; j < nb * nd; j += nthr) {
             wbd[j] = s_k * D[(size_t)(ob + j / nd) * n_ao + od + j % nd];
             fbd[j] = 0.0;
         }
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
                     ((L & 1) ? Rr1 : Rr0)[0] = wf[L];
                 }
                 __syncthreads();
-                fj_kl_tower(Rr0, Rr1, wf, tw, L, PQx, PQy, PQz, lane);
+                fj_kl_tower_n(Rr0, Rr1, wf, tw, L, PQx, PQy, PQz, lane, nthr);
                 const double* __restrict__ R0 = Rr0;
                 const double UU = rb[7] * rk[7];
                 for (int r0 = 0; r0 < nab; r0 += FJ_FL_CH) {
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
                     // four tower entries make sixteen multiply-adds
                     {
                         const int tq = (nq + 3) >> 2, nt = ((nr + 3) >> 2) * tq;
-                        for (int t = lane; t < nt; t += 64) {
+                        for (int t = lane; t < nt; t += nthr) {
                             const int m0 = (t / tq) << 2, n0 = (t - (t / tq) * tq) << 2;
                             const double* ex[4];
                             const double* ey[4];
@@ -280,7 +281,7 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
                     // of (r, cd), each element digested at once into the six blocks
                     {
                         const int tcd = (ncd + 3) >> 2, nt = ((nr + 3) >> 2) * tcd;
-                        for (int t = lane; t < nt; t += 64) {
+                        for (int t = lane; t < nt; t += nthr) {
                             const int m0 = (t / tcd) << 2, n0 = (t - (t / tcd) * tcd) << 2;
                             const double* fx[4];
                             const double* fy[4];
@@ -343,17 +344,17 @@ __global__ void __launch_bounds__(64) fj_floop_k(int la, int lb, int lc, int ld,
             }
         }
         // the ket's six blocks into F (the caller closes F + F^T)
-        for (int j = lane; j < nab; j += 64)
+        for (int j = lane; j < nab; j += nthr)
             if (fab[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nb) * n_ao + ob + j % nb, fab[j]);
-        for (int j = lane; j < ncd; j += 64)
+        for (int j = lane; j < ncd; j += nthr)
             if (fcd[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oc + j / nd) * n_ao + od + j % nd, fcd[j]);
-        for (int j = lane; j < na * nc; j += 64)
+        for (int j = lane; j < na * nc; j += nthr)
             if (fac[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nc) * n_ao + oc + j % nc, fac[j]);
-        for (int j = lane; j < na * nd; j += 64)
+        for (int j = lane; j < na * nd; j += nthr)
             if (fad[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nd) * n_ao + od + j % nd, fad[j]);
-        for (int j = lane; j < nb * nc; j += 64)
+        for (int j = lane; j < nb * nc; j += nthr)
             if (fbc[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(ob + j / nc) * n_ao + oc + j % nc, fbc[j]);
-        for (int j = lane; j < nb * nd; j += 64)
+        for (int j = lane; j < nb * nd; j += nthr)
             if (fbd[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(ob + j / nd) * n_ao + od + j % nd, fbd[j]);
         __syncthreads();   // before the next ket rewrites the blocks
     }

@@ -92,8 +92,8 @@ __device__ __forceinline__ void fj_kl_eaxis(double* e, int l1, int l2, double ze
 // first nonzero axis d, [t] = PQ_d [t - 1_d] + (t_d - 1) [t - 2_d], packed as
 // index(t - 1_d) | index(t - 2_d) << 10 | d << 20 | (t_d - 1) << 22; with hc,
 // also the packed components of j.
-__device__ __forceinline__ void fj_kl_steps(int* tw, int* hc, int nh, int lane) {
-    for (int j = lane; j < nh; j += 64) {
+__device__ __forceinline__ void fj_kl_steps_n(int* tw, int* hc, int nh, int lane, int nthr) {
+    for (int j = lane; j < nh; j += nthr) {
         const int c = fj_herm_decode(j);
         const int x = c & 15, y = (c >> 4) & 15, z = c >> 8;
         if (hc) hc[j] = c;
@@ -108,6 +108,10 @@ __device__ __forceinline__ void fj_kl_steps(int* tw, int* hc, int nh, int lane) 
         }
         tw[j] = w;
     }
+}
+
+__device__ __forceinline__ void fj_kl_steps(int* tw, int* hc, int nh, int lane) {
+    fj_kl_steps_n(tw, hc, nh, lane, 64);
 }
 
 // wf[m] = [0]^(m) = (2 / sqrt(pi)) sqrt(rho) (-2 rho)^m F_m(T), m <= M.
@@ -144,17 +148,19 @@ __device__ __forceinline__ void fj_kl_seed(double* wf, double rho, double T, int
     }
 }
 
-// Levels m = L - 1 .. 0 of the tower, lanes over t: level m holds [t]^(m)
+// Levels m = L - 1 .. 0 of the tower, lanes over t (lane of nthr; the _n
+// forms take the workgroup's width, the plain ones one wavefront): level m holds [t]^(m)
 // for |t| <= L - m in row m & 1 (R0 or R1), so [t]^(0) ends in R0. The
 // caller has written wf and row L & 1's [0] = wf[L] and passed a barrier;
 // every lane of the workgroup calls this (a barrier after each level).
-__device__ __forceinline__ void fj_kl_tower(double* R0, double* R1, const double* wf, const int* tw,
-                                            int L, double PQx, double PQy, double PQz, int lane) {
+__device__ __forceinline__ void fj_kl_tower_n(double* R0, double* R1, const double* wf, const int* tw,
+                                              int L, double PQx, double PQy, double PQz, int lane,
+                                              int nthr) {
     for (int m = L - 1; m >= 0; --m) {
         double* __restrict__ cur = (m & 1) ? R1 : R0;
         const double* __restrict__ prv = (m & 1) ? R0 : R1;
         const int n_m = fj_herm_base(L - m + 1);
-        for (int j = lane; j < n_m; j += 64) {
+        for (int j = lane; j < n_m; j += nthr) {
             double v;
             if (j == 0) {
                 v = wf[m];
@@ -169,6 +175,11 @@ __device__ __forceinline__ void fj_kl_tower(double* R0, double* R1, const double
         }
         __syncthreads();
     }
+}
+
+__device__ __forceinline__ void fj_kl_tower(double* R0, double* R1, const double* wf, const int* tw,
+                                            int L, double PQx, double PQy, double PQz, int lane) {
+    fj_kl_tower_n(R0, R1, wf, tw, L, PQx, PQy, PQz, lane, 64);
 }
 
 // An includer that needs only the helpers above (the E tables, the tower)

@@ -70,7 +70,8 @@ int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, 
         ld < 0 || ld > FJ_KL_LMAX)
         return FJ_ERR_ARG;
     // threads is the item width (kets per work item) the plan was made for;
-    // the kernel takes an item's kets one at a time on one wavefront
+    // the kernel takes an item's kets one at a time on one workgroup of
+    // FJ_FL_BLOCK lanes
     if (threads <= 0) threads = FJ_DEFAULT_THREADS;
     if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
     if (n_blocks <= 0 || a.ket.n <= 0) return 0;
@@ -78,6 +79,6 @@ int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, 
     a.n_bx = (int)grid.x;
     a.n_items = (int)(grid.x * grid.y);
     const FjFlLds lay = fj_floop_lds(la, lb, lc, ld);
-    hipLaunchKernelGGL(fj_floop_k, grid, dim3(64), (size_t)lay.bytes, s, la, lb, lc, ld, threads, a);
+    hipLaunchKernelGGL(fj_floop_k, grid, dim3(FJ_FL_BLOCK), (size_t)lay.bytes, s, la, lb, lc, ld, threads, a);
     return (int)hipGetLastError();
 }

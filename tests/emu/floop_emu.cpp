@@ -52,7 +52,7 @@ int main(int argc, char** argv) {
         a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0; a.tier = nullptr;
         a.n_bx = h[4]; a.n_items = h[4] * ((h[5] + span - 1) / span);
     }
-    const int threads = 64;
+    const int threads = argc > 3 ? atoi(argv[3]) : 64;
     std::barrier<> bar(threads);
     g_bar = &bar;
     blockDim = {(unsigned)threads, 1, 1};

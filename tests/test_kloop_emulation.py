@@ -86,9 +86,9 @@ def _build(td: Path, driver: str) -> Path:
     return exe
 
 
-def _run(exe: Path, td: Path, out: str) -> np.ndarray:
+def _run(exe: Path, td: Path, out: str, *extra: str) -> np.ndarray:
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([str(exe), str(td / "case.bin"), str(td / out)], cwd=td, env=env,
+    r = subprocess.run([str(exe), str(td / "case.bin"), str(td / out), *extra], cwd=td, env=env,
                        capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     return np.fromfile(td / out)
@@ -200,7 +200,8 @@ def test_loop_f_kernel_on_cpu_vs_oracle():
                 w(a["ket_ao"], np.int32)
                 w(a["Dsh"], np.float64)
                 w(a["D"], np.float64)
-        F = _run(exe, td, "F.bin").reshape(basis.n_ao, basis.n_ao)
+        # two wavefronts per workgroup, as the library launches it
+        F = _run(exe, td, "F.bin", "128").reshape(basis.n_ao, basis.n_ao)
     err = np.abs(F - Fo).max()
     print("loop-form F vs oracle", err, "scale", np.abs(Fo).max(), "launches", len(launches), flush=True)
     assert err <= 1e-12 * max(1.0, np.abs(Fo).max())
