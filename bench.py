@@ -361,8 +361,10 @@ def df_block(cpu: bool = True) -> dict:
     (H2O cc-pVTZ-JKFIT metric, 166 aux functions; H2O STO-3G x JKFIT
     three-center) with max|d| against libcint, and on the 10-atom cluster
     with a synthetic even-tempered s..g aux set (20 shells per atom,
-    1,400 aux functions): ms per call (HIP events), integrals per second;
-    the oracle's time on the metric fixture is the CPU baseline."""
+    1,400 aux functions): ms per call of the launches (HIP events; the
+    tables staged on the device beforehand by df.DfIntegrals), integrals
+    per second and the kernel's roofline; the oracle's time on the metric
+    fixture is the CPU baseline."""
     import torch
     from felizjueves_amd import df
     from felizjueves_amd.basis import GOLDEN, Basis
@@ -382,8 +384,10 @@ def df_block(cpu: bool = True) -> dict:
     out = {}
     z = np.load(GOLDEN / "df_h2o_ccpvtz_jkfit_metric.npz", allow_pickle=False)
     aux = df.AuxBasis.from_npz(z)
-    M = df.metric_matrix(aux).cpu().numpy()
-    ms = timed_call(lambda: df.metric_matrix(aux))
+    st = df.DfIntegrals(aux)
+    M = st.metric()
+    ms = timed_call(lambda: st.metric(M))
+    M = M.cpu().numpy()
     r = {"n_aux": aux.n_aux, "ms": ms, "max_d_vs_libcint": float(np.abs(M - z["int2c2e"]).max()),
          "integrals_per_s": aux.n_aux ** 2 / 2 / ms * 1e3}
     if cpu:
@@ -398,18 +402,27 @@ def df_block(cpu: bool = True) -> dict:
     z = np.load(GOLDEN / "df_h2o_sto3g_jkfit_3c.npz", allow_pickle=False)
     aux = df.AuxBasis.from_npz(z)
     b = Basis.from_arrays(z["shell_l"], z["shell_centre"], z["shell_nprim"], z["exps"], z["coefs"])
-    E3 = df.eri3(b, aux).cpu().numpy()
-    out["eri3_h2o_sto3g_jkfit"] = {"n_ao": b.n_ao, "n_aux": aux.n_aux,
-                                   "ms": timed_call(lambda: df.eri3(b, aux)),
+    st = df.DfIntegrals(aux, b)
+    E3 = st.eri3()
+    ms = timed_call(lambda: st.eri3(E3))
+    E3 = E3.cpu().numpy()
+    out["eri3_h2o_sto3g_jkfit"] = {"n_ao": b.n_ao, "n_aux": aux.n_aux, "ms": ms,
                                    "max_d_vs_libcint": float(np.abs(E3 - z["int3c2e"]).max())}
     s = config2(n_atoms=10)
     aux = _aux_even_tempered(s.coords)
-    ms_m = timed_call(lambda: df.metric_matrix(aux))
-    ms_3 = timed_call(lambda: df.eri3(s.basis, aux))
+    st = df.DfIntegrals(aux, s.basis)
+    Mc, Ec = st.metric(), st.eri3()
+    ms_m = timed_call(lambda: st.metric(Mc))
+    ms_3 = timed_call(lambda: st.eri3(Ec))
+    fl3, flm = df.flops_eri3(s.basis, aux), df.flops_metric(aux)
     out["cluster_10atom"] = {"workload": s.name + " + even-tempered s..g aux", "n_ao": s.basis.n_ao,
                              "n_aux": aux.n_aux, "metric_ms": ms_m, "eri3_ms": ms_3,
                              "eri3_integrals_per_s": s.basis.n_ao * (s.basis.n_ao + 1) / 2 * aux.n_aux
-                             / ms_3 * 1e3}
+                             / ms_3 * 1e3,
+                             "roofline": {"bound": "fp64_valu", "kernel": "fj_dfloop_k (eri3)",
+                                          "achieved": fl3 / ms_3 / 1e9, "peak": FP64_PEAK_TFLOPS,
+                                          "unit": "TFLOP/s", "frac": fl3 / ms_3 / 1e9 / FP64_PEAK_TFLOPS,
+                                          "gflop": fl3 / 1e9, "metric_tflops": flm / ms_m / 1e9}}
     log("df", json.dumps(out))
     return out
 

@@ -79,6 +79,8 @@ def _compile(src: Path, extra: list[str], force: bool = False) -> tuple[Path, fl
         deps.append(CSRC / "fj_kloop.h")
     if src.name == "fj_gfloop.hip":
         deps.append(CSRC / "fj_gfloop.h")
+    if src.name == "fj_dfloop.hip":
+        deps.append(CSRC / "fj_dfloop.h")
     if src.name == "fj_kloop.hip":
         deps.append(CSRC / "fj_floop.h")
     if src.name == "fj_noalloc.hip":

@@ -7,6 +7,8 @@ kernel (`fj_metric_loop` / `fj_eri3_loop` and their per-class entries).
     AuxBasis.from_arrays(...)              an auxiliary basis (shells through i)
     metric_classes(aux) / metric_matrix(aux)          -> (P|Q) [n_aux, n_aux]
     eri3_classes(basis, aux) / eri3(basis, aux)       -> (mn|P) [n_ao, n_ao, n_aux]
+    flops_metric(aux) / flops_eri3(basis, aux)        -> the loop form's arithmetic
+    DfIntegrals(aux, basis).metric() / .eri3()         the tables staged once
 
 An aux primitive is a Hermite "pair" of one Gaussian: record (A, alpha, s =
 c (pi/alpha)^{3/2}) (upstream `metric.aux_dress`, the one place the dressing
@@ -149,52 +151,135 @@ def eri3_classes(basis: Basis, aux: AuxBasis) -> tuple[dict, dict]:
     return out, bra
 
 
+def _cell_flops(la: int, lb: int, lc: int) -> int:
+    """The loop-form kernel's arithmetic per (bra record, aux record) pair of
+    a class (la, lb | lc): the tower (3 per entry per level, |t| <= L), and
+    per output element ((a, b), c) for each Hermite index p of the bra box
+    (p_d <= a_d + b_d) the aux parity box (q_d = c_d, c_d - 2, ...): 4 per q
+    (the E product, the signed multiply-add with [p + q]), 4 per p (the bra E
+    product and its multiply-add), 2 per element (the accumulation)."""
+    from .basis import cartesians
+    L = la + lb + lc
+    herm = lambda n: n * (n + 1) * (n + 2) // 6  # noqa: E731
+    f = sum(3 * herm(L - m + 1) for m in range(L))
+    for a in cartesians(la):
+        for b in cartesians(lb):
+            nbox = (a[0] + b[0] + 1) * (a[1] + b[1] + 1) * (a[2] + b[2] + 1)
+            for c in cartesians(lc):
+                nq = (c[0] // 2 + 1) * (c[1] // 2 + 1) * (c[2] // 2 + 1)
+                f += nbox * (4 * nq + 4) + 2
+    return f
+
+
+def flops_metric(aux: AuxBasis) -> int:
+    """The arithmetic of one metric_matrix call (_cell_flops over every
+    record pair of every aux shell pair)."""
+    tot = 0
+    for (lp, lq), dc in metric_classes(aux).items():
+        a = dc.arrays
+        n_rp = (a["p_rec1"] - a["p_rec0"]).astype(np.int64) * (a["q_rec1"] - a["q_rec0"])
+        tot += int(n_rp.sum()) * _cell_flops(lp, 0, lq)
+    return tot
+
+
+def flops_eri3(basis: Basis, aux: AuxBasis) -> int:
+    """The arithmetic of one eri3 call (_cell_flops over every record pair
+    of every cell)."""
+    classes, _ = eri3_classes(basis, aux)
+    tot = 0
+    for (la, lb, lp), dc in classes.items():
+        a = dc.arrays
+        n_rp = (a["ab_rec1"] - a["ab_rec0"]).astype(np.int64) * (a["p_rec1"] - a["p_rec0"])
+        tot += int(n_rp.sum()) * _cell_flops(la, lb, lp)
+    return tot
+
+
 def _p(t):
     return C.c_void_p(native.ptr(t))
 
 
-def metric_matrix(aux: AuxBasis, stream=None, threads: int = 64):
+THREADS = 128   # lanes per workgroup of the DF kernel
+
+
+class DfIntegrals:
+    """The density-fitting tables of one (basis, aux) staged on the device
+    once; `metric(out)` and `eri3(out)` launch every class into device
+    arrays (the host work -- class tables, record arrays, uploads -- is paid
+    here, not per call)."""
+
+    def __init__(self, aux: AuxBasis, basis: Basis | None = None, threads: int = THREADS):
+        self.aux, self.basis, self.threads = aux, basis, threads
+        A, alpha, s, _, _ = aux.records()
+        self.n_rec = len(alpha)
+        self.d = {k: native.to_dev(v) for k, v in (("A", A), ("alpha", alpha), ("s", s))}
+        lib = native.lib()
+        self.fm = lib.fj_metric_loop_rp
+        self.fm.restype = C.c_int
+        self.fe = lib.fj_eri3_loop_rp
+        self.fe.restype = C.c_int
+        self.mcls = []
+        for (lp, lq), dc in metric_classes(aux).items():
+            a = dc.arrays
+            max_rp = int(((a["p_rec1"] - a["p_rec0"]) * (a["q_rec1"] - a["q_rec0"])).max())
+            self.mcls.append(((lp, lq), dc.n, max_rp, {k: native.to_dev(v) for k, v in a.items()}))
+        self.ecls = []
+        if basis is not None:
+            classes, bra = eri3_classes(basis, aux)
+            self.n_bra = len(bra["zeta_ab"])
+            self.b = {k: native.to_dev(v) for k, v in bra.items()}
+            for cls, dc in classes.items():
+                a = dc.arrays
+                max_rp = int(((a["ab_rec1"] - a["ab_rec0"]) * (a["p_rec1"] - a["p_rec0"])).max())
+                self.ecls.append((cls, dc.n, max_rp, {k: native.to_dev(v) for k, v in a.items()}))
+
+    def metric(self, out=None, stream=None):
+        """(P|Q) [n_aux, n_aux] into `out` (a new device array if None)."""
+        import torch
+        aux, d = self.aux, self.d
+        if out is None:
+            out = torch.empty((aux.n_aux, aux.n_aux), dtype=torch.float64, device=native.device())
+        h = C.c_void_p(native.stream_handle(stream))
+        for (lp, lq), n, max_rp, ar in self.mcls:
+            native.check(self.fm(h, C.c_int(lp), C.c_int(lq), C.c_int(self.threads), C.c_int(n),
+                                 C.c_int(self.n_rec), C.c_int(aux.n_aux), _p(d["A"]), _p(d["alpha"]),
+                                 _p(d["s"]), *[_p(ar[k]) for k in ("p_rec0", "p_rec1", "q_rec0", "q_rec1",
+                                                                   "off_p", "off_q")],
+                                 _p(out), C.c_int(max_rp)), "fj_metric_loop_rp")
+        return out
+
+    def eri3(self, out=None, stream=None):
+        """(mn|P) [n_ao, n_ao, n_aux] into `out` (a new device array if None)."""
+        import torch
+        if self.basis is None:
+            raise ValueError("no AO basis staged")
+        aux, d, b = self.aux, self.d, self.b
+        n_ao = self.basis.n_ao
+        if out is None:
+            out = torch.empty((n_ao, n_ao, aux.n_aux), dtype=torch.float64, device=native.device())
+        h = C.c_void_p(native.stream_handle(stream))
+        for (la, lb, lp), n, max_rp, ar in self.ecls:
+            native.check(self.fe(h, C.c_int(la), C.c_int(lb), C.c_int(lp), C.c_int(self.threads),
+                                 C.c_int(n), C.c_int(self.n_bra), C.c_int(self.n_rec), C.c_int(n_ao),
+                                 C.c_int(aux.n_aux),
+                                 *[_p(b[k]) for k in ("zeta_ab", "P", "XPA", "XPB", "s_ab")],
+                                 _p(d["A"]), _p(d["alpha"]), _p(d["s"]),
+                                 *[_p(ar[k]) for k in ("ab_rec0", "ab_rec1", "p_rec0", "p_rec1", "off_a",
+                                                       "off_b", "off_p")], _p(out), C.c_int(max_rp)),
+                         "fj_eri3_loop_rp")
+        return out
+
+
+def metric_matrix(aux: AuxBasis, stream=None, threads: int = THREADS):
     """(P|Q) of every aux function on the device, [n_aux, n_aux] (float64)."""
     import torch
-    A, alpha, s, _, _ = aux.records()
-    d = {k: native.to_dev(v) for k, v in (("A", A), ("alpha", alpha), ("s", s))}
-    out = torch.empty((aux.n_aux, aux.n_aux), dtype=torch.float64, device=native.device())
-    fn = native.lib().fj_metric_loop
-    fn.restype = C.c_int
-    keep = []
-    for (lp, lq), dc in metric_classes(aux).items():
-        ar = {k: native.to_dev(v) for k, v in dc.arrays.items()}
-        keep.append(ar)
-        native.check(fn(C.c_void_p(native.stream_handle(stream)), C.c_int(lp), C.c_int(lq),
-                        C.c_int(threads), C.c_int(dc.n), C.c_int(len(alpha)), C.c_int(aux.n_aux),
-                        _p(d["A"]), _p(d["alpha"]), _p(d["s"]),
-                        *[_p(ar[k]) for k in ("p_rec0", "p_rec1", "q_rec0", "q_rec1", "off_p",
-                                              "off_q")], _p(out)), "fj_metric_loop")
+    out = DfIntegrals(aux, threads=threads).metric(stream=stream)
     torch.cuda.synchronize()
     return out
 
 
-def eri3(basis: Basis, aux: AuxBasis, stream=None, threads: int = 64):
+def eri3(basis: Basis, aux: AuxBasis, stream=None, threads: int = THREADS):
     """(mn|P) on the device, [n_ao, n_ao, n_aux] (float64, aux fastest)."""
     import torch
-    classes, bra = eri3_classes(basis, aux)
-    A, alpha, s, _, _ = aux.records()
-    d = {k: native.to_dev(v) for k, v in (("A", A), ("alpha", alpha), ("s", s))}
-    b = {k: native.to_dev(v) for k, v in bra.items()}
-    out = torch.empty((basis.n_ao, basis.n_ao, aux.n_aux), dtype=torch.float64,
-                      device=native.device())
-    fn = native.lib().fj_eri3_loop
-    fn.restype = C.c_int
-    keep = []
-    for (la, lb, lp), dc in classes.items():
-        ar = {k: native.to_dev(v) for k, v in dc.arrays.items()}
-        keep.append(ar)
-        native.check(fn(C.c_void_p(native.stream_handle(stream)), C.c_int(la), C.c_int(lb),
-                        C.c_int(lp), C.c_int(threads), C.c_int(dc.n), C.c_int(len(bra["zeta_ab"])),
-                        C.c_int(len(alpha)), C.c_int(basis.n_ao), C.c_int(aux.n_aux),
-                        *[_p(b[k]) for k in ("zeta_ab", "P", "XPA", "XPB", "s_ab")],
-                        _p(d["A"]), _p(d["alpha"]), _p(d["s"]),
-                        *[_p(ar[k]) for k in ("ab_rec0", "ab_rec1", "p_rec0", "p_rec1", "off_a",
-                                              "off_b", "off_p")], _p(out)), "fj_eri3_loop")
+    out = DfIntegrals(aux, basis, threads=threads).eri3(stream=stream)
     torch.cuda.synchronize()
     return out

@@ -1687,6 +1687,19 @@ int fj_eri3_loop(fj_stream_t stream, int la, int lb, int lP, int threads, int n_
                  const double* A, const double* alpha, const double* s_aux, const int* ab_rec0,
                  const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a,
                  const int* off_b, const int* off_p, double* eri3);
+/* The same with max_rp, the largest record-pair count of a pair / cell (it
+ * sizes the batch of record pairs a workgroup takes at once; the entries
+ * above assume the most). No upstream counterpart. */
+int fj_metric_loop_rp(fj_stream_t stream, int lP, int lQ, int threads, int n_pair, int n_rec,
+                      int n_aux, const double* A, const double* alpha, const double* s_aux,
+                      const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1,
+                      const int* off_p, const int* off_q, double* metric, int max_rp);
+int fj_eri3_loop_rp(fj_stream_t stream, int la, int lb, int lP, int threads, int n_cell,
+                    int n_bra_rec, int n_aux_rec, int n_ao, int n_aux, const double* zeta_ab,
+                    const double* P, const double* XPA, const double* XPB, const double* s_ab,
+                    const double* A, const double* alpha, const double* s_aux, const int* ab_rec0,
+                    const int* ab_rec1, const int* p_rec0, const int* p_rec1, const int* off_a,
+                    const int* off_b, const int* off_p, double* eri3, int max_rp);
 /* BEGIN DF CLASSES (written by felizjueves_amd/gen_hip.py) */
 int fj_metric_00(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
 int fj_metric_01(fj_stream_t stream, int threads, int n_pair, int n_rec, int n_aux, const double* A, const double* alpha, const double* s_aux, const int* p_rec0, const int* p_rec1, const int* q_rec0, const int* q_rec1, const int* off_p, const int* off_q, double* metric);
