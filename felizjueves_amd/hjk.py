@@ -198,12 +198,7 @@ def hjk_launches(basis: Basis, D, *, tau: float, mode: str = "canonical",
                  prim_chunk)
 
 
-def sub_degree(k: int, chunk: int) -> int:
-    """The sub-pair degree of a degree-k batch: the largest divisor of k not
-    above chunk (k itself when chunk is 0 or >= k)."""
-    if chunk <= 0 or chunk >= k:
-        return k
-    return max(d for d in range(1, chunk + 1) if k % d == 0)
+sub_degree = P.sub_degree   # the sub-pair degree of a degree-k batch (pairs.sub_degree)
 
 
 def kernel_sides(L: HLaunch):

@@ -365,8 +365,23 @@ class Packed:
     ao: np.ndarray     # [n * 2] int32
 
 
-def pack(basis: Basis, pairs: list) -> Packed:
-    """One side of a launch (upstream kernel_f.py:572-606)."""
+def sub_degree(k: int, chunk: int) -> int:
+    """The sub-pair degree of a degree-k side: the largest divisor of k not
+    above chunk (k itself when chunk is 0 or >= k)."""
+    if chunk <= 0 or chunk >= k:
+        return k
+    return max(d for d in range(1, chunk + 1) if k % d == 0)
+
+
+def pack(basis: Basis, pairs: list, prim_chunk: int = 0) -> Packed:
+    """One side of a launch (upstream kernel_f.py:572-606). prim_chunk > 0
+    packs each pair as sub-pairs of sub_degree(degree, prim_chunk) of its
+    primitive records, each with the pair's G, separation, shells and AOs:
+    the per-class kernels take one lane per quartet and its whole primitive
+    loop, and the sub-quartets sum to the quartet (the HRR and digestion are
+    linear in the contracted boundary, the screen reads the pair's G and
+    shells, and on a diagonal launch the canonical half over sub-pairs with
+    the k == bra halving gives the pair with itself exactly once)."""
     if not pairs:
         raise ValueError("an empty side has no pairs to stage")
     degrees = {p.degree for p in pairs}
@@ -375,12 +390,15 @@ def pack(basis: Basis, pairs: list) -> Packed:
     (degree,) = degrees
     if not 1 <= degree <= MAX_PRIM_PAIRS:
         raise ValueError(f"degree {degree} outside 1..{MAX_PRIM_PAIRS}")
+    d = sub_degree(degree, prim_chunk)
+    m = degree // d
     prim = np.concatenate([p.rec.ravel() for p in pairs])
-    sep = np.concatenate([basis.shell_centre[p.a] - basis.shell_centre[p.b] for p in pairs])
-    G = np.array([p.G for p in pairs])
-    sh = np.array([(p.a, p.b) for p in pairs], dtype=np.int32).ravel()
-    ao = np.array([(basis.shell_ao[p.a], basis.shell_ao[p.b]) for p in pairs], dtype=np.int32).ravel()
-    return Packed(len(pairs), degree, np.ascontiguousarray(prim), np.ascontiguousarray(sep), G, sh, ao)
+    sep = np.repeat(np.array([basis.shell_centre[p.a] - basis.shell_centre[p.b] for p in pairs]), m, axis=0)
+    G = np.repeat(np.array([p.G for p in pairs]), m)
+    sh = np.repeat(np.array([(p.a, p.b) for p in pairs], dtype=np.int32), m, axis=0).ravel()
+    ao = np.repeat(np.array([(basis.shell_ao[p.a], basis.shell_ao[p.b]) for p in pairs], dtype=np.int32),
+                   m, axis=0).ravel()
+    return Packed(len(pairs) * m, d, np.ascontiguousarray(prim), np.ascontiguousarray(sep.ravel()), G, sh, ao)
 
 
 def density_blocks(basis: Basis, D: np.ndarray) -> np.ndarray:
@@ -398,11 +416,12 @@ def close(F: np.ndarray) -> np.ndarray:
     return F + F.T
 
 
-def launch_args(basis: Basis, plan: FPlan, L: FLaunch, D: np.ndarray, Dsh: np.ndarray) -> dict:
+def launch_args(basis: Basis, plan: FPlan, L: FLaunch, D: np.ndarray, Dsh: np.ndarray,
+                prim_chunk: int = 0) -> dict:
     """The per-class entry's arguments for launch L, by upstream name
-    (upstream walk_f.py:385-398), host numpy."""
-    pb = pack(basis, plan.batches[L.bra].pairs)
-    pk = pack(basis, plan.batches[L.ket].pairs)
+    (upstream walk_f.py:385-398), host numpy; prim_chunk as in pack."""
+    pb = pack(basis, plan.batches[L.bra].pairs, prim_chunk)
+    pk = pack(basis, plan.batches[L.ket].pairs, prim_chunk)
     _, lb, _, ld = L.cls
     a = {"n_bra": pb.n, "n_ket": pk.n, "k_ab": pb.degree, "k_cd": pk.degree,
          "n_sh": basis.n_sh, "n_ao": basis.n_ao, "diag": int(L.bra == L.ket),

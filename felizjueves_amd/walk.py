@@ -44,7 +44,15 @@ def _check_plan(plan: P.FPlan, basis: Basis) -> None:
                 raise ValueError(f"walk_f refuses the plan: pair {(p.a, p.b)} is not of its batch")
 
 
-def walk_f(plan: P.FPlan, basis: Basis, D: np.ndarray, *, run, threads: int = THREADS) -> FWalk:
+PRIM_CHUNK = 2   # the sub-pair packing the walker uses by default (pairs.pack)
+
+
+def walk_f(plan: P.FPlan, basis: Basis, D: np.ndarray, *, run, threads: int = THREADS,
+           prim_chunk: int = PRIM_CHUNK) -> FWalk:
+    """prim_chunk: each side's pairs packed as sub-pairs of at most that many
+    primitive pairs (pairs.pack; 0 packs whole pairs, the upstream's
+    packing): the same F to rounding, with the long primitive loops of
+    contracted pairs spread over lanes."""
     _check_plan(plan, basis)
     if not np.array_equal(D, D.T):
         raise ValueError("the density must be exactly symmetric")
@@ -52,10 +60,10 @@ def walk_f(plan: P.FPlan, basis: Basis, D: np.ndarray, *, run, threads: int = TH
     F = run.zeros(basis.n_ao)
     nq = 0
     for L in plan.launches:
-        args = P.launch_args(basis, plan, L, D, Dsh)
-        nb, nk = args["n_bra"], args["n_ket"]
+        args = P.launch_args(basis, plan, L, D, Dsh, prim_chunk)
+        nb, nk = len(plan.batches[L.bra].pairs), len(plan.batches[L.ket].pairs)
         nq += nb * (nb + 1) // 2 if args["diag"] else nb * nk
-        run(L.cls, nb, (threads, 1), run.put(args), F)
+        run(L.cls, args["n_bra"], (threads, 1), run.put(args), F)
     Fh = run.host(F)
     return FWalk(P.close(Fh), len(plan.launches), nq)
 
