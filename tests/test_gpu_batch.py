@@ -3,7 +3,7 @@ shells) in ONE batch builder, graph-captured -- each molecule's G against
 its own single build and the smallest against the oracle, FP64 bars --
 and the learned selector on the same batch (the network picks tiers per
 build on the device) against the bound rule, with the FP64 batch build as
-the judge of its error."""
+the reference for its error."""
 
 import numpy as np
 import pytest
