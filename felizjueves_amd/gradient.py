@@ -122,10 +122,10 @@ def gf_launches(basis: Basis, shell_atom) -> GfStream:
 def default_threads(cls) -> int:
     """Workgroup size of a class: 256 lanes (four wavefronts) for every
     class. Measured per class on the three-atom s,s,p,d system and the
-    10-atom cluster at 64, 128 and
-    256 lanes: 256 was fastest or within 15% of the fastest everywhere (the
-    register-tiled stages have few tiles per primitive quartet, so more
-    lanes take more primitive quartets per batch)."""
+    10-atom cluster at 64, 128 and 256 lanes: 256 was fastest or within 15%
+    of the fastest everywhere (the register-tiled stages have few tiles per
+    primitive quartet, so more lanes take more primitive quartets per
+    batch)."""
     return 256
 
 

@@ -137,9 +137,9 @@ def _density(basis: Basis, D) -> np.ndarray:
 
 # default sub-pair size: at most 2 x 2 primitive quartets per lane. Measured
 # on the GPU (J of the three-atom s,s,p,d system / the 10-atom cluster, ms):
-# whole pairs 3,182 / 16,797;
-# 16: 1,042 / 7,539; 9: 483 / 3,721; 4: 180 / 1,399; 3: 157 / 1,103;
-# 2: 136 / 973; 1: 137 / 1,340 (every sub-quartet pays its own HRR).
+# whole pairs 3,182 / 16,797; 16: 1,042 / 7,539; 9: 483 / 3,721;
+# 4: 180 / 1,399; 3: 157 / 1,103; 2: 136 / 973; 1: 137 / 1,340 (every
+# sub-quartet pays its own HRR).
 PRIM_CHUNK = 2
 
 
