@@ -158,6 +158,8 @@ def main() -> None:
     ap.add_argument("--workload", choices=("c1", "c2", "c4"), required=True)
     ap.add_argument("--tune", action="store_true", help="run the threads / split / streams search first")
     ap.add_argument("--every", type=int, default=4, help="c4: tune on every N-th molecule")
+    ap.add_argument("--lane-flops", type=float, nargs="+", default=list(LANE_FLOPS),
+                    help="the primitive-loop split candidates (FLOPs per lane)")
     ap.add_argument("--selector", default=str(Path(__file__).resolve().parent / "tuned" / "selector"))
     ap.add_argument("--out", default=None, help="write the updated configuration here instead")
     a = ap.parse_args()
@@ -171,7 +173,7 @@ def main() -> None:
     log = lambda *m: print("[autotune]", *m, flush=True)  # noqa: E731
     if a.tune:
         t0 = time.perf_counter()
-        cfg = tune(systems, tau=1e-12, log=log)
+        cfg = tune(systems, lane_flops=tuple(a.lane_flops), tau=1e-12, log=log)
         cfg["tuned_on"] = (f"{a.workload}: {len(systems)} systems" +
                            (f" (every {a.every}-th of the 64 batch molecules, shells through d)"
                             if a.workload == "c4" else ""))

@@ -7,16 +7,16 @@
 // (la lc | lb ld) as run-time arguments instead of straight-line code.
 //
 // The straight-line K kernels (gen_mmd.emit_k) grow as L^6 in source; this
-// one is a single kernel whose loops read small index tables. One wavefront
-// per workgroup and per bra record; the kets of a bra are taken one at a
-// time by the whole wavefront, cooperating through LDS:
+// one is a single kernel whose loops read small index tables. One workgroup
+// of two wavefronts (FJ_KL_BLOCK lanes) per bra record; the kets of a bra
+// are taken one at a time by the whole workgroup, cooperating through LDS:
 //
 //   per-axis E tables   E^{ij}_t of the bra (once) and of the ket, one lane
 //                       per axis (upstream mmd/hermite.py:45-79)
 //   Hermite tower       [t]^(m), m = L .. 0, one level per step, lanes over t
 //                       (upstream reference/pq.py:54-133), two LDS rows
 //   X[c][q]             s_bd (-1)^|q| sum_d D_cd E^{bd}_q   (per b)
-//   Y[c][p]             sum_q [p + q] X[c][q]
+//   Y[c][p]             sum_q [p + q] X[c][q]   (4 x 4 register tiles of (c, p))
 //   kac[a][c][b]       += sum_p E^{ac}_p Y[c][p]
 //
 // and K[bra][a][b] = s_ac sum_c kac[a][c][b] at the end. Every loop bound is
@@ -31,6 +31,7 @@
 #define FJ_KL_NP 84                        // |p| <= 6 (one side)
 #define FJ_KL_NC 10                        // Cartesians of an f shell
 #define FJ_KL_AX 112                       // one axis's E table: 4 x 4 x 7
+#define FJ_KL_BLOCK 128                    // lanes per workgroup of the K kernel
 
 // Hermite functions with |t| < n.
 __device__ __forceinline__ int fj_herm_base(int n) { return n * (n + 1) * (n + 2) / 6; }
@@ -219,7 +220,7 @@ __host__ __device__ inline FjKlLds fj_kloop_lds(int la, int lc, int lb, int ld) 
     return o;
 }
 
-__global__ void __launch_bounds__(64) fj_kloop_k(
+__global__ void __launch_bounds__(256) fj_kloop_k(
     int la, int lc, int lb, int ld, int n_bra, int n_ket, int n_dc, double tau, double maxD,
     const double* __restrict__ P, const double* __restrict__ zeta_ac,
     const double* __restrict__ XPA, const double* __restrict__ XPC,
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
     const double* __restrict__ Dcd, double* __restrict__ K) {
     FJ_DYN_SHARED(fj_kloop_lds_block);
 
-    const int bra = (int)blockIdx.x, lane = (int)threadIdx.x;
+    const int bra = (int)blockIdx.x, lane = (int)threadIdx.x, nthr = (int)blockDim.x;
     if (bra >= n_bra) return;   // uniform per workgroup
     const int lp = la + lc, lq = lb + ld, L = lp + lq;
     const int na = (la + 1) * (la + 2) / 2, nc = (lc + 1) * (lc + 2) / 2;
@@ -260,15 +261,15 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
     // index tables: components of every Hermite index, the tower's step of
     // each t (first nonzero axis d: [t] = PQ_d [t - 1_d] + (t_d - 1) [t - 2_d]),
     // and the index of p + q
-    fj_kl_steps(tw, hc, nh, lane);
-    for (int j = lane; j < np * nq; j += 64) {
+    fj_kl_steps_n(tw, hc, nh, lane, nthr);
+    for (int j = lane; j < np * nq; j += nthr) {
         const int p = j / nq, q = j - p * nq;
         const int cp = fj_herm_decode(p), cq = fj_herm_decode(q);
         pqi[j] = (unsigned short)fj_herm_index((cp & 15) + (cq & 15),
                                                ((cp >> 4) & 15) + ((cq >> 4) & 15),
                                                (cp >> 8) + (cq >> 8));
     }
-    for (int j = lane; j < na * nc * nb; j += 64) kac[j] = 0.0;
+    for (int j = lane; j < na * nc * nb; j += nthr) kac[j] = 0.0;
     const double z = zeta_ac[bra];
     const double Px = P[3 * bra], Py = P[3 * bra + 1], Pz = P[3 * bra + 2];
     if (lane < 3) fj_kl_eaxis(ea + lane * axa, la, lc, z, XPA[3 * bra + lane], XPC[3 * bra + lane]);
@@ -285,18 +286,18 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
         const double rho = z * e / (z + e);
         if (lane < 3) fj_kl_eaxis(eb + lane * axb, lb, ld, e, XQB[3 * k + lane], XQD[3 * k + lane]);
         const double* __restrict__ dm = Dcd + (size_t)blk * (nc * nd);
-        for (int j = lane; j < nc * nd; j += 64) dsh[j] = dm[j];
+        for (int j = lane; j < nc * nd; j += nthr) dsh[j] = dm[j];
         if (lane == 0) {
             fj_kl_seed(wf, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), L);
             ((L & 1) ? Rr1 : Rr0)[0] = wf[L];
         }
         __syncthreads();
-        fj_kl_tower(Rr0, Rr1, wf, tw, L, PQx, PQy, PQz, lane);
+        fj_kl_tower_n(Rr0, Rr1, wf, tw, L, PQx, PQy, PQz, lane, nthr);
         const double* __restrict__ R0 = Rr0;
         for (int ib = 0; ib < nb; ++ib) {
             const int cb = hc[ob + ib];
             const int bx = cb & 15, by = (cb >> 4) & 15, bz = cb >> 8;
-            for (int j = lane; j < nc * nq; j += 64) {
+            for (int j = lane; j < nc * nq; j += nthr) {
                 const int c = j / nq, q = j - c * nq;
                 const int cq = hc[q];
                 const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
@@ -311,17 +312,45 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
                 X[j] = ((qx + qy + qz) & 1 ? -sb : sb) * s;
             }
             __syncthreads();
-            for (int j = lane; j < nc * np; j += 64) {
-                const int c = j / np, p = j - c * np;
-                const double* __restrict__ xr = X + c * nq;
-                const unsigned short* __restrict__ ir = pqi + p * nq;
-                double s = 0.0;
-                for (int q = 0; q < nq; ++q) s += R0[ir[q]] * xr[q];
-                Y[j] = s;
+            // Y[c][p] = sum_q [p + q] X[c][q] in 4 x 4 register tiles of (c, p):
+            // per q four X and four tower entries feed sixteen multiply-adds
+            {
+                const int tp = (np + 3) >> 2, nt = ((nc + 3) >> 2) * tp;
+                for (int t = lane; t < nt; t += nthr) {
+                    const int c0 = (t / tp) << 2, p0 = (t - (t / tp) * tp) << 2;
+                    const double* xr[4];
+                    const unsigned short* ir[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        xr[i] = X + min(c0 + i, nc - 1) * nq;
+                        ir[i] = pqi + min(p0 + i, np - 1) * nq;
+                    }
+                    double acc[4][4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j2 = 0; j2 < 4; ++j2) acc[i][j2] = 0.0;
+                    for (int q = 0; q < nq; ++q) {
+                        double xv[4], rv[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) xv[i] = xr[i][q];
+#pragma unroll
+                        for (int j2 = 0; j2 < 4; ++j2) rv[j2] = R0[ir[j2][q]];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j2 = 0; j2 < 4; ++j2) acc[i][j2] += rv[j2] * xv[i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j2 = 0; j2 < 4; ++j2)
+                            if (c0 + i < nc && p0 + j2 < np) Y[(c0 + i) * np + p0 + j2] = acc[i][j2];
+                }
             }
             __syncthreads();
             // X of the next b is written before the next barrier, Y after it
-            for (int j = lane; j < na * nc; j += 64) {
+            for (int j = lane; j < na * nc; j += nthr) {
                 const int a = j / nc, c = j - a * nc;
                 const int ca = hc[oa + a], cc = hc[oc + c];
                 const double* __restrict__ ex = ea + ((ca & 15) * (lc + 1) + (cc & 15)) * ta;
@@ -339,7 +368,7 @@ __global__ void __launch_bounds__(64) fj_kloop_k(
     }
     __syncthreads();
     const double sa = s_ac[bra];
-    for (int j = lane; j < na * nb; j += 64) {
+    for (int j = lane; j < na * nb; j += nthr) {
         const int a = j / nb, b = j - a * nb;
         double s = 0.0;
         for (int c = 0; c < nc; ++c) s += kac[(a * nc + c) * nb + b];

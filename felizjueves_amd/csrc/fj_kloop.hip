@@ -23,7 +23,7 @@ int fj_kloop_launch(hipStream_t s, int la, int lc, int lb, int ld, int threads, 
     if (n_bra < 0 || n_ket < 0 || n_dc < 0) return FJ_ERR_SHAPE;
     if (n_bra == 0) return 0;
     const FjKlLds lay = fj_kloop_lds(la, lc, lb, ld);
-    hipLaunchKernelGGL(fj_kloop_k, dim3((unsigned)n_bra), dim3(64), (size_t)lay.bytes, s, la, lc, lb, ld, n_bra, n_ket,
+    hipLaunchKernelGGL(fj_kloop_k, dim3((unsigned)n_bra), dim3(FJ_KL_BLOCK), (size_t)lay.bytes, s, la, lc, lb, ld, n_bra, n_ket,
                        n_dc, tau, maxD, P, zeta_ac, XPA, XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, s_bd, Iq,
                        dd, Dcd, K);
     return (int)hipGetLastError();

@@ -47,7 +47,7 @@ int main(int argc, char** argv) {
         // the launch's dynamic LDS at its exact size, so ASan sees an overrun
         L.lds = (double*)malloc(fj_kloop_lds(L.h[0], L.h[1], L.h[2], L.h[3]).bytes);
     }
-    const int threads = 64;
+    const int threads = argc > 3 ? atoi(argv[3]) : 64;
     std::barrier<> bar(threads);
     g_bar = &bar;
     blockDim = {(unsigned)threads, 1, 1};
