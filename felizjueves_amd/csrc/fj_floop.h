@@ -26,6 +26,7 @@
 
 #define FJ_FL_CH 32   // bra rows per W chunk
 #define FJ_FL_BLOCK 256   // lanes per workgroup
+#define FJ_FL_WORKGROUPS 16384   // workgroups a work-list launch is cut into, at most
 
 // The kernel's LDS, sized for its class and carved from one dynamic block:
 // byte offsets of each array (doubles, then ints, then the u16 index table)
@@ -68,8 +69,13 @@ __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld
     int bra, k0, k1, kb_lo = 0, kb_n = 0;
     FjSys S;
     if (a.work) {
+        // an item covers at most `span` kets; workgroup (x, y) takes the
+        // y-th of gridDim.y equal slices of item x's kets
         const int4 w = a.work[blockIdx.x];
-        bra = w.x; k0 = w.y; k1 = w.w;
+        const int end = min(w.w, w.y + span), sub = (span + (int)gridDim.y - 1) / (int)gridDim.y;
+        bra = w.x;
+        k0 = w.y + (int)blockIdx.y * sub;
+        k1 = min(end, k0 + sub);
         S = a.sys[w.z & 0xffff];
         kb_lo = (w.z >> 16) & 0xff;
         kb_n = (w.z >> 24) & 0xff;
@@ -77,9 +83,8 @@ __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld
         bra = (int)blockIdx.x;
         k0 = (int)blockIdx.y * span;
         S = a.one;
-        k1 = a.ket.n;
+        k1 = min(a.ket.n, k0 + span);
     }
-    k1 = min(k1, k0 + span);   // an item covers at most `span` kets
     const int kmin = a.diag * bra;
     if (k0 >= k1 || (!a.ket_list && k0 + span <= kmin)) return;   // uniform
     int kb0, kab;

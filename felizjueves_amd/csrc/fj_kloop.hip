@@ -75,7 +75,12 @@ int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, 
     if (threads <= 0) threads = FJ_DEFAULT_THREADS;
     if (threads % 64 || threads > FJ_MAX_THREADS) return FJ_ERR_THREADS;
     if (n_blocks <= 0 || a.ket.n <= 0) return 0;
-    const dim3 grid((unsigned)n_blocks, a.work ? 1u : (unsigned)((a.ket.n + threads - 1) / threads));
+    // with a work list, each item's kets are cut into up to `threads` slices,
+    // one workgroup each, so that a launch of few long items still fills the
+    // chip (a workgroup takes its kets one after another)
+    const int split = a.work ? std::max(1, std::min(threads, FJ_FL_WORKGROUPS / n_blocks)) : 0;
+    const dim3 grid((unsigned)n_blocks,
+                    a.work ? (unsigned)split : (unsigned)((a.ket.n + threads - 1) / threads));
     a.n_bx = (int)grid.x;
     a.n_items = (int)(grid.x * grid.y);
     const FjFlLds lay = fj_floop_lds(la, lb, lc, ld);
