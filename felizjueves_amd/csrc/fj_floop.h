@@ -21,6 +21,16 @@
 // primitives the blocks go to F with one global atomic per element.
 // Every barrier sits in control flow that is uniform over the wavefront.
 //
+// fj_fbloop_k is the batched form for the classes whose whole W fits LDS
+// several times over (fj_fbloop_plan: B >= 2): it takes a ket's primitive
+// quartets B at a time, every phase over the batch at once (lanes over the
+// E tables and seeds of the batch, over (b, t) in each tower level, over
+// (b, tile) in W), so a batch pays the L + 3 barriers and the serial Boys
+// and E-table latencies that the one-at-a-time form pays per primitive
+// quartet; x is summed over the batch in registers (lanes over (slice,
+// tile of (ab, cd)), each slice a share of the batch) before the digestion,
+// which runs once per slice and batch instead of once per primitive quartet.
+//
 // The includer provides fj_device.h and fj_kloop.h.
 #pragma once
 
@@ -61,37 +71,148 @@ __host__ __device__ inline FjFlLds fj_floop_lds(int la, int lb, int lc, int ld) 
     return o;
 }
 
-__global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
-    FJ_DYN_SHARED(fj_floop_lds_block);
-    const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
-    // the work item: (bra pair, first ket, system | bra chunk, ket end), or
-    // (x, y * span, the single system, n_ket) without a work list
-    int bra, k0, k1, kb_lo = 0, kb_n = 0;
+// The workgroup's share of a launch: the bra pair, its kets [k0, k1), the
+// system and the bra's primitive-pair range [kb0, kb0 + kab). The work item
+// is (bra pair, first ket, system | bra chunk, ket end), or (x, y * span,
+// the single system, n_ket) without a work list; an item covers at most
+// `span` kets and, with a work list, workgroup (x, y) takes the y-th of
+// gridDim.y equal slices of item x's kets. False (uniformly) when there is
+// nothing to do.
+struct FjFlItem {
+    int bra, k0, k1, kmin, kb0, kab;
     FjSys S;
+};
+
+__device__ __forceinline__ bool fj_fl_item(const FjFArgs& a, int span, FjFlItem& it) {
+    int kb_lo = 0, kb_n = 0;
     if (a.work) {
-        // an item covers at most `span` kets; workgroup (x, y) takes the
-        // y-th of gridDim.y equal slices of item x's kets
         const int4 w = a.work[blockIdx.x];
         const int end = min(w.w, w.y + span), sub = (span + (int)gridDim.y - 1) / (int)gridDim.y;
-        bra = w.x;
-        k0 = w.y + (int)blockIdx.y * sub;
-        k1 = min(end, k0 + sub);
-        S = a.sys[w.z & 0xffff];
+        it.bra = w.x;
+        it.k0 = w.y + (int)blockIdx.y * sub;
+        it.k1 = min(end, it.k0 + sub);
+        it.S = a.sys[w.z & 0xffff];
         kb_lo = (w.z >> 16) & 0xff;
         kb_n = (w.z >> 24) & 0xff;
     } else {
-        bra = (int)blockIdx.x;
-        k0 = (int)blockIdx.y * span;
-        S = a.one;
-        k1 = min(a.ket.n, k0 + span);
+        it.bra = (int)blockIdx.x;
+        it.k0 = (int)blockIdx.y * span;
+        it.S = a.one;
+        it.k1 = min(a.ket.n, it.k0 + span);
     }
-    const int kmin = a.diag * bra;
-    if (k0 >= k1 || (!a.ket_list && k0 + span <= kmin)) return;   // uniform
-    int kb0, kab;
-    fj_side_range(a.bra, bra, kb0, kab);
-    if (kab > FJ_MAX_PRIM_PAIRS || kb_lo >= kab) return;
-    kb0 += kb_lo;
-    kab = kb_n ? min(kab - kb_lo, kb_n) : kab - kb_lo;
+    it.kmin = a.diag * it.bra;
+    if (it.k0 >= it.k1 || (!a.ket_list && it.k0 + span <= it.kmin)) return false;
+    fj_side_range(a.bra, it.bra, it.kb0, it.kab);
+    if (it.kab > FJ_MAX_PRIM_PAIRS || kb_lo >= it.kab) return false;
+    it.kb0 += kb_lo;
+    it.kab = kb_n ? min(it.kab - kb_lo, kb_n) : it.kab - kb_lo;
+    return true;
+}
+
+// One ket of the item: its pair index, primitive-pair range [kc0, kc0 +
+// kcd), shells, AO offsets and the J / K digestion scales of the quartet's
+// degeneracy. False (uniformly) when it is below the diagonal, screened out
+// (G_ab G_cd times the largest density block it digests into, under tau)
+// or empty.
+struct FjFlKet {
+    int k, kc0, kcd, sc, sd, oc, od;
+    double s_j, s_k;
+};
+
+__device__ __forceinline__ bool fj_fl_ket(const FjFArgs& a, const FjFlItem& it, int kk, int sa, int sb,
+                                          double Gb, FjFlKet& q) {
+    int kc_lo = 0, kc_n = 0;
+    q.k = kk;
+    if (a.ket_list) {
+        const int v = a.ket_list[kk];
+        q.k = v & 0x1ffff;
+        kc_lo = (v >> 17) & 0x7f;
+        kc_n = (v >> 24) & 0x7f;
+    }
+    const int k = q.k;
+    if (k < it.kmin) return false;
+    q.sc = a.ket.sh[2 * k];
+    q.sd = a.ket.sh[2 * k + 1];
+    const int ns = it.S.n_sh;
+    const double* __restrict__ Dsh = it.S.Dsh;
+    double dmax = 2.0 * Dsh[sa * ns + sb];
+    dmax = fmax(dmax, 2.0 * Dsh[q.sc * ns + q.sd]);
+    dmax = fmax(dmax, 0.5 * Dsh[sa * ns + q.sc]);
+    dmax = fmax(dmax, 0.5 * Dsh[sa * ns + q.sd]);
+    dmax = fmax(dmax, 0.5 * Dsh[sb * ns + q.sc]);
+    dmax = fmax(dmax, 0.5 * Dsh[sb * ns + q.sd]);
+    if (Gb * (a.ket.G[k] * dmax) < a.tau) return false;
+    fj_side_range(a.ket, k, q.kc0, q.kcd);
+    if (q.kcd > FJ_MAX_PRIM_PAIRS || q.kcd <= kc_lo) return false;
+    q.kc0 += kc_lo;
+    q.kcd = kc_n ? min(q.kcd - kc_lo, kc_n) : q.kcd - kc_lo;
+    double deg = 1.0;
+    if (sa == sb) deg *= 0.5;
+    if (q.sc == q.sd) deg *= 0.5;
+    if (a.diag && k == it.bra) deg *= 0.5;
+    q.s_j = 2.0 * deg;
+    q.s_k = -0.5 * deg;
+    q.oc = a.ket.ao[2 * k];
+    q.od = a.ket.ao[2 * k + 1];
+    return true;
+}
+
+// The six blocks of a quartet in LDS, in the order ab, cd, ac, ad, bc, bd:
+// w the weighted density blocks (J scale on ab and cd, K scale on the
+// others), f the F accumulators. fj_fl_load fills w and zeroes f;
+// fj_fl_flush adds f into F with one global atomic per nonzero element.
+struct FjFlBlocks {
+    double* w[6];
+    double* f[6];
+};
+
+__device__ __forceinline__ void fj_fl_load(const FjFlBlocks& B, const double* __restrict__ D, int n_ao,
+                                           int oa, int ob, int oc, int od, int na, int nb, int nc,
+                                           int nd, double s_j, double s_k, int lane, int nthr) {
+    const int r0[6] = {oa, oc, oa, oa, ob, ob}, c0[6] = {ob, od, oc, od, oc, od};
+    const int nr[6] = {na, nc, na, na, nb, nb}, ncol[6] = {nb, nd, nc, nd, nc, nd};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double sc = i < 2 ? s_j : s_k;
+        for (int j = lane; j < nr[i] * ncol[i]; j += nthr) {
+            B.w[i][j] = sc * D[(size_t)(r0[i] + j / ncol[i]) * n_ao + c0[i] + j % ncol[i]];
+            B.f[i][j] = 0.0;
+        }
+    }
+}
+
+__device__ __forceinline__ void fj_fl_flush(const FjFlBlocks& B, double* F, int n_ao, int oa, int ob,
+                                            int oc, int od, int na, int nb, int nc, int nd, int lane,
+                                            int nthr) {
+    const int r0[6] = {oa, oc, oa, oa, ob, ob}, c0[6] = {ob, od, oc, od, oc, od};
+    const int nr[6] = {na, nc, na, na, nb, nb}, ncol[6] = {nb, nd, nc, nd, nc, nd};
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        for (int j = lane; j < nr[i] * ncol[i]; j += nthr)
+            if (B.f[i][j] != 0.0)
+                unsafeAtomicAdd(F + (size_t)(r0[i] + j / ncol[i]) * n_ao + c0[i] + j % ncol[i], B.f[i][j]);
+}
+
+// Digest one element x of (ab|cd), ab = (ia, jb), cd = (kc, ld_), into the
+// six LDS blocks.
+__device__ __forceinline__ void fj_fl_digest(const FjFlBlocks& B, int nb, int nc, int nd, int ab,
+                                             int cdi, double x) {
+    const int ia = ab / nb, jb = ab - ia * nb, kc = cdi / nd, ld_ = cdi - kc * nd;
+    atomicAdd(&B.f[0][ab], B.w[1][cdi] * x);
+    atomicAdd(&B.f[1][cdi], B.w[0][ab] * x);
+    atomicAdd(&B.f[2][ia * nc + kc], B.w[5][jb * nd + ld_] * x);
+    atomicAdd(&B.f[3][ia * nd + ld_], B.w[4][jb * nc + kc] * x);
+    atomicAdd(&B.f[4][jb * nc + kc], B.w[3][ia * nd + ld_] * x);
+    atomicAdd(&B.f[5][jb * nd + ld_], B.w[2][ia * nc + kc] * x);
+}
+
+__global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
+    FJ_DYN_SHARED(fj_floop_lds_block);
+    const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
+    FjFlItem it;
+    if (!fj_fl_item(a, span, it)) return;   // uniform
+    const int bra = it.bra, kb0 = it.kb0, kab = it.kab;
+    const FjSys& S = it.S;
 
     const int lp = la + lb, lq = lc + ld, L = lp + lq;
     const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
@@ -110,18 +231,11 @@ __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld
     double* const eab = reinterpret_cast<double*>(lds + lay.eab);
     double* const ecd = reinterpret_cast<double*>(lds + lay.ecd);
     double* const W = reinterpret_cast<double*>(lds + lay.W);
-    double* const wab = reinterpret_cast<double*>(lds + lay.w[0]);
-    double* const wcd = reinterpret_cast<double*>(lds + lay.w[1]);
-    double* const wac = reinterpret_cast<double*>(lds + lay.w[2]);
-    double* const wad = reinterpret_cast<double*>(lds + lay.w[3]);
-    double* const wbc = reinterpret_cast<double*>(lds + lay.w[4]);
-    double* const wbd = reinterpret_cast<double*>(lds + lay.w[5]);
-    double* const fab = reinterpret_cast<double*>(lds + lay.f[0]);
-    double* const fcd = reinterpret_cast<double*>(lds + lay.f[1]);
-    double* const fac = reinterpret_cast<double*>(lds + lay.f[2]);
-    double* const fad = reinterpret_cast<double*>(lds + lay.f[3]);
-    double* const fbc = reinterpret_cast<double*>(lds + lay.f[4]);
-    double* const fbd = reinterpret_cast<double*>(lds + lay.f[5]);
+    FjFlBlocks blk;
+    for (int i = 0; i < 6; ++i) {
+        blk.w[i] = reinterpret_cast<double*>(lds + lay.w[i]);
+        blk.f[i] = reinterpret_cast<double*>(lds + lay.f[i]);
+    }
     double* const wf = reinterpret_cast<double*>(lds + lay.wf);
     int* const hc = reinterpret_cast<int*>(lds + lay.hc);
     int* const tw = reinterpret_cast<int*>(lds + lay.tw);
@@ -140,65 +254,15 @@ __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld
     const double Gb = a.bra.G[bra];
     const double ABx = lb ? a.bra.sep[3 * bra] : 0.0, ABy = lb ? a.bra.sep[3 * bra + 1] : 0.0,
                  ABz = lb ? a.bra.sep[3 * bra + 2] : 0.0;
-    const int n_ao = S.n_ao, ns = S.n_sh;
-    const double* __restrict__ D = S.D;
     __syncthreads();
 
-    for (int kk = k0; kk < k1; ++kk) {
-        int k = kk, kc_lo = 0, kc_n = 0;
-        if (a.ket_list) {
-            const int v = a.ket_list[kk];
-            k = v & 0x1ffff;
-            kc_lo = (v >> 17) & 0x7f;
-            kc_n = (v >> 24) & 0x7f;
-        }
-        if (k < kmin) continue;   // uniform: every lane reads the same ket
-        const int sc = a.ket.sh[2 * k], sd = a.ket.sh[2 * k + 1];
-        const double* __restrict__ Dsh = S.Dsh;
-        double dmax = 2.0 * Dsh[sa * ns + sb];
-        dmax = fmax(dmax, 2.0 * Dsh[sc * ns + sd]);
-        dmax = fmax(dmax, 0.5 * Dsh[sa * ns + sc]);
-        dmax = fmax(dmax, 0.5 * Dsh[sa * ns + sd]);
-        dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sc]);
-        dmax = fmax(dmax, 0.5 * Dsh[sb * ns + sd]);
-        if (Gb * (a.ket.G[k] * dmax) < a.tau) continue;
-        int kc0, kcd;
-        fj_side_range(a.ket, k, kc0, kcd);
-        if (kcd > FJ_MAX_PRIM_PAIRS || kcd <= kc_lo) continue;
-        kc0 += kc_lo;
-        kcd = kc_n ? min(kcd - kc_lo, kc_n) : kcd - kc_lo;
-        double deg = 1.0;
-        if (sa == sb) deg *= 0.5;
-        if (sc == sd) deg *= 0.5;
-        if (a.diag && k == bra) deg *= 0.5;
-        const double s_j = 2.0 * deg, s_k = -0.5 * deg;
-        const int oc = a.ket.ao[2 * k], od = a.ket.ao[2 * k + 1];
+    for (int kk = it.k0; kk < it.k1; ++kk) {
+        FjFlKet q;
+        if (!fj_fl_ket(a, it, kk, sa, sb, Gb, q)) continue;   // uniform: every lane reads the same ket
+        const int k = q.k, kc0 = q.kc0, kcd = q.kcd;
         const double CDx = ld ? a.ket.sep[3 * k] : 0.0, CDy = ld ? a.ket.sep[3 * k + 1] : 0.0,
                      CDz = ld ? a.ket.sep[3 * k + 2] : 0.0;
-        for (int j = lane; j < nab; j += nthr) {
-            wab[j] = s_j * D[(size_t)(oa + j / nb) * n_ao + ob + j % nb];
-            fab[j] = 0.0;
-        }
-        for (int j = lane; j < ncd; j += nthr) {
-            wcd[j] = s_j * D[(size_t)(oc + j / nd) * n_ao + od + j % nd];
-            fcd[j] = 0.0;
-        }
-        for (int j = lane; j < na * nc; j += nthr) {
-            wac[j] = s_k * D[(size_t)(oa + j / nc) * n_ao + oc + j % nc];
-            fac[j] = 0.0;
-        }
-        for (int j = lane; j < na * nd; j += nthr) {
-            wad[j] = s_k * D[(size_t)(oa + j / nd) * n_ao + od + j % nd];
-            fad[j] = 0.0;
-        }
-        for (int j = lane; j < nb * nc; j += nthr) {
-            wbc[j] = s_k * D[(size_t)(ob + j / nc) * n_ao + oc + j % nc];
-            fbc[j] = 0.0;
-        }
-        for (int j = lane; j < nb * nd; j += nthr) {
-            wbd[j] = s_k * D[(size_t)(ob + j / nd) * n_ao + od + j % nd];
-            fbd[j] = 0.0;
-        }
+        fj_fl_load(blk, S.D, S.n_ao, oa, ob, q.oc, q.od, na, nb, nc, nd, q.s_j, q.s_k, lane, nthr);
         // the side of higher angular momentum in the outer loop: its E tables
         // are built once per outer primitive pair, the other side's per pair
         // of primitive pairs
@@ -326,21 +390,9 @@ __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld
 #pragma unroll
                             for (int i = 0; i < 4; ++i) {
                                 if (m0 + i >= nr) continue;
-                                const int ab = r0 + m0 + i;
-                                const int ia = ab / nb, jb = ab % nb;
 #pragma unroll
-                                for (int j = 0; j < 4; ++j) {
-                                    const int cdi = n0 + j;
-                                    if (cdi >= ncd) continue;
-                                    const int kc = cdi / nd, ld_ = cdi % nd;
-                                    const double x = UU * c[i][j];
-                                    atomicAdd(&fab[ab], wcd[cdi] * x);
-                                    atomicAdd(&fcd[cdi], wab[ab] * x);
-                                    atomicAdd(&fac[ia * nc + kc], wbd[jb * nd + ld_] * x);
-                                    atomicAdd(&fad[ia * nd + ld_], wbc[jb * nc + kc] * x);
-                                    atomicAdd(&fbc[jb * nc + kc], wad[ia * nd + ld_] * x);
-                                    atomicAdd(&fbd[jb * nd + ld_], wac[ia * nc + kc] * x);
-                                }
+                                for (int j = 0; j < 4; ++j)
+                                    if (n0 + j < ncd) fj_fl_digest(blk, nb, nc, nd, r0 + m0 + i, n0 + j, UU * c[i][j]);
                             }
                         }
                     }
@@ -349,18 +401,292 @@ __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld
             }
         }
         // the ket's six blocks into F (the caller closes F + F^T)
-        for (int j = lane; j < nab; j += nthr)
-            if (fab[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nb) * n_ao + ob + j % nb, fab[j]);
-        for (int j = lane; j < ncd; j += nthr)
-            if (fcd[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oc + j / nd) * n_ao + od + j % nd, fcd[j]);
-        for (int j = lane; j < na * nc; j += nthr)
-            if (fac[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nc) * n_ao + oc + j % nc, fac[j]);
-        for (int j = lane; j < na * nd; j += nthr)
-            if (fad[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(oa + j / nd) * n_ao + od + j % nd, fad[j]);
-        for (int j = lane; j < nb * nc; j += nthr)
-            if (fbc[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(ob + j / nc) * n_ao + oc + j % nc, fbc[j]);
-        for (int j = lane; j < nb * nd; j += nthr)
-            if (fbd[j] != 0.0) unsafeAtomicAdd(S.F + (size_t)(ob + j / nd) * n_ao + od + j % nd, fbd[j]);
+        fj_fl_flush(blk, S.F, S.n_ao, oa, ob, q.oc, q.od, na, nb, nc, nd, lane, nthr);
+        __syncthreads();   // before the next ket rewrites the blocks
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The batched form (fj_fbloop_k): B primitive quartets of one ket per pass.
+
+#define FJ_FB_BUDGET (64 * 1024)   // LDS bytes per workgroup
+#define FJ_FB_BMAX 64              // primitive quartets per batch
+#define FJ_FB_SC 4                 // per primitive quartet: PQ (3), U_ab U_cd
+
+// The batched kernel's LDS for batch B: B copies of the per-primitive-quartet
+// arrays (the two tower rows, the two sides' axis E tables, W [nab][nq], the
+// seed and the scalars), then the six density and six F blocks, the index
+// tables, and the total.
+struct FjFbLds {
+    int B, nh, nW, axa, axc;
+    int R0, R1, eab, ecd, W, wf, sc, w[6], f[6], hc, tw, pqi, bytes;
+};
+
+__host__ __device__ inline FjFbLds fj_fbloop_layout(int la, int lb, int lc, int ld, int B) {
+    auto herm = [](int n) { return n * (n + 1) * (n + 2) / 6; };
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+    const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
+    const int L = la + lb + lc + ld;
+    const int np = herm(la + lb + 1), nq = herm(lc + ld + 1);
+    FjFbLds o;
+    o.B = B;
+    o.nh = herm(L + 1);
+    o.nW = na * nb * nq;
+    o.axa = (la + 1) * (lb + 1) * (la + lb + 1);
+    o.axc = (lc + 1) * (ld + 1) * (lc + ld + 1);
+    const int blk[6] = {na * nb, nc * nd, na * nc, na * nd, nb * nc, nb * nd};
+    int at = 0;
+    auto take = [&](int n, int size) { const int r = at; at += ((n * size + 7) / 8) * 8; return r; };
+    o.R0 = take(B * o.nh, 8);
+    o.R1 = take(B * o.nh, 8);
+    o.eab = take(B * 3 * o.axa, 8);
+    o.ecd = take(B * 3 * o.axc, 8);
+    o.W = take(B * o.nW, 8);
+    o.wf = take(B * (L + 1), 8);
+    o.sc = take(B * FJ_FB_SC, 8);
+    for (int i = 0; i < 6; ++i) o.w[i] = take(blk[i], 8);
+    for (int i = 0; i < 6; ++i) o.f[i] = take(blk[i], 8);
+    o.hc = take(o.nh, 4);
+    o.tw = take(o.nh, 4);
+    o.pqi = take(np * nq, 2);
+    o.bytes = at;
+    return o;
+}
+
+// B for a class: the largest that fits FJ_FB_BUDGET, at most FJ_FB_BMAX and
+// no more than the lanes need to have one W tile each. B < 2 means the class
+// runs on the one-at-a-time kernel.
+__host__ __device__ inline FjFbLds fj_fbloop_plan(int la, int lb, int lc, int ld, int threads) {
+    const FjFbLds one = fj_fbloop_layout(la, lb, lc, ld, 1), two = fj_fbloop_layout(la, lb, lc, ld, 2);
+    const int per_b = two.bytes - one.bytes, fixed = one.bytes - per_b;
+    const int nab = ((la + 1) * (la + 2) / 2) * ((lb + 1) * (lb + 2) / 2);
+    const int nq = one.nW / nab;
+    const int tiles = ((nab + 3) / 4) * ((nq + 3) / 4);
+    int B = (FJ_FB_BUDGET - fixed) / per_b;
+    const int want = (threads + tiles - 1) / tiles;
+    if (B > want) B = want;
+    if (B > FJ_FB_BMAX) B = FJ_FB_BMAX;
+    if (B < 1) B = 1;
+    return fj_fbloop_layout(la, lb, lc, ld, B);
+}
+
+__global__ void __launch_bounds__(256) fj_fbloop_k(int la, int lb, int lc, int ld, int span, int B,
+                                                   FjFArgs a) {
+    FJ_DYN_SHARED(fj_fbloop_lds_block);
+    const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
+    FjFlItem it;
+    if (!fj_fl_item(a, span, it)) return;   // uniform
+    const int bra = it.bra, kb0 = it.kb0, kab = it.kab;
+    const FjSys& S = it.S;
+
+    const int lp = la + lb, lq = lc + ld, L = lp + lq;
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+    const int nc = (lc + 1) * (lc + 2) / 2, nd = (ld + 1) * (ld + 2) / 2;
+    const int nab = na * nb, ncd = nc * nd;
+    const int np = fj_herm_base(lp + 1), nq = fj_herm_base(lq + 1);
+    const int ha = fj_herm_base(la), hb = fj_herm_base(lb), hcc = fj_herm_base(lc),
+              hd = fj_herm_base(ld);
+    const int ta = lp + 1, tc = lq + 1;   // t extents of the two E tables
+
+    const FjFbLds lay = fj_fbloop_layout(la, lb, lc, ld, B);
+    const int nh = lay.nh, nW = lay.nW, axa = lay.axa, axc = lay.axc;
+    char* const lds = reinterpret_cast<char*>(fj_fbloop_lds_block);
+    double* const Rr0 = reinterpret_cast<double*>(lds + lay.R0);
+    double* const Rr1 = reinterpret_cast<double*>(lds + lay.R1);
+    double* const eab = reinterpret_cast<double*>(lds + lay.eab);
+    double* const ecd = reinterpret_cast<double*>(lds + lay.ecd);
+    double* const W = reinterpret_cast<double*>(lds + lay.W);
+    double* const wf = reinterpret_cast<double*>(lds + lay.wf);
+    double* const scl = reinterpret_cast<double*>(lds + lay.sc);
+    FjFlBlocks blk;
+    for (int i = 0; i < 6; ++i) {
+        blk.w[i] = reinterpret_cast<double*>(lds + lay.w[i]);
+        blk.f[i] = reinterpret_cast<double*>(lds + lay.f[i]);
+    }
+    int* const hc = reinterpret_cast<int*>(lds + lay.hc);
+    int* const tw = reinterpret_cast<int*>(lds + lay.tw);
+    unsigned short* const pqi = reinterpret_cast<unsigned short*>(lds + lay.pqi);
+    fj_kl_steps_n(tw, hc, nh, lane, nthr);
+    for (int j = lane; j < np * nq; j += nthr) {
+        const int p = j / nq, q = j - p * nq;
+        const int cp = fj_herm_decode(p), cq = fj_herm_decode(q);
+        pqi[j] = (unsigned short)fj_herm_index((cp & 15) + (cq & 15),
+                                               ((cp >> 4) & 15) + ((cq >> 4) & 15),
+                                               (cp >> 8) + (cq >> 8));
+    }
+    const int sa = a.bra.sh[2 * bra], sb = a.bra.sh[2 * bra + 1];
+    const int oa = a.bra.ao[2 * bra], ob = a.bra.ao[2 * bra + 1];
+    const double Gb = a.bra.G[bra];
+    const double ABx = lb ? a.bra.sep[3 * bra] : 0.0, ABy = lb ? a.bra.sep[3 * bra + 1] : 0.0,
+                 ABz = lb ? a.bra.sep[3 * bra + 2] : 0.0;
+    __syncthreads();
+
+    const int tq = (nq + 3) >> 2, tW = ((nab + 3) >> 2) * tq;        // W tiles per primitive quartet
+    const int tcd = (ncd + 3) >> 2, tX = ((nab + 3) >> 2) * tcd;     // x tiles
+    for (int kk = it.k0; kk < it.k1; ++kk) {
+        FjFlKet q;
+        if (!fj_fl_ket(a, it, kk, sa, sb, Gb, q)) continue;   // uniform: every lane reads the same ket
+        const int k = q.k, kc0 = q.kc0, kcd = q.kcd;
+        const double CDx = ld ? a.ket.sep[3 * k] : 0.0, CDy = ld ? a.ket.sep[3 * k + 1] : 0.0,
+                     CDz = ld ? a.ket.sep[3 * k + 2] : 0.0;
+        fj_fl_load(blk, S.D, S.n_ao, oa, ob, q.oc, q.od, na, nb, nc, nd, q.s_j, q.s_k, lane, nthr);
+        const int npq = kab * kcd;   // the ket's primitive quartets, bra-major
+        for (int i0 = 0; i0 < npq; i0 += B) {
+            const int nbe = min(B, npq - i0);
+            // the batch's axis E tables, lanes over (b, side and axis)
+            for (int j = lane; j < 6 * nbe; j += nthr) {
+                const int b = j / 6, x = j - 6 * b, i = i0 + b;
+                if (x < 3) {
+                    const double* __restrict__ rb = a.bra.prim + 8 * (size_t)(kb0 + i / kcd);
+                    const double sep = x == 0 ? ABx : (x == 1 ? ABy : ABz);
+                    fj_kl_eaxis_from(eab + (b * 3 + x) * axa, la, lb, rb[6], rb[3 + x], rb[3 + x] + sep, 1.0);
+                } else {
+                    const double* __restrict__ rk = a.ket.prim + 8 * (size_t)(kc0 + i % kcd);
+                    const double sep = x == 3 ? CDx : (x == 4 ? CDy : CDz);
+                    fj_kl_eaxis_from(ecd + (b * 3 + x - 3) * axc, lc, ld, rk[6], rk[x], rk[x] + sep, 1.0);
+                }
+            }
+            // the seeds [0]^(m) and the scalars, lanes over b
+            for (int b = lane; b < nbe; b += nthr) {
+                const int i = i0 + b;
+                const double* __restrict__ rb = a.bra.prim + 8 * (size_t)(kb0 + i / kcd);
+                const double* __restrict__ rk = a.ket.prim + 8 * (size_t)(kc0 + i % kcd);
+                const double p = rb[6], qe = rk[6];
+                const double PQx = rb[0] - rk[0], PQy = rb[1] - rk[1], PQz = rb[2] - rk[2];
+                const double rho = p * qe / (p + qe);
+                double* const w = wf + b * (L + 1);
+                fj_kl_seed(w, rho, rho * (PQx * PQx + PQy * PQy + PQz * PQz), L);
+                ((L & 1) ? Rr1 : Rr0)[b * nh] = w[L];
+                double* const c = scl + b * FJ_FB_SC;
+                c[0] = PQx; c[1] = PQy; c[2] = PQz;
+                c[3] = rb[7] * rk[7];
+            }
+            __syncthreads();
+            // the towers, levels m = L - 1 .. 0, lanes over (b, t); [t]^(0) ends in R0
+            for (int m = L - 1; m >= 0; --m) {
+                double* __restrict__ cur = (m & 1) ? Rr1 : Rr0;
+                const double* __restrict__ prv = (m & 1) ? Rr0 : Rr1;
+                const int n_m = fj_herm_base(L - m + 1);
+                for (int j = lane; j < nbe * n_m; j += nthr) {
+                    const int b = j / n_m, t = j - b * n_m;
+                    double v;
+                    if (t == 0) {
+                        v = wf[b * (L + 1) + m];
+                    } else {
+                        const int w = tw[t];
+                        const int d = (w >> 20) & 3, c1 = w >> 22;
+                        const double* __restrict__ pv = prv + b * nh;
+                        v = scl[b * FJ_FB_SC + d] * pv[w & 1023];
+                        if (c1 > 0) v += (double)c1 * pv[(w >> 10) & 1023];
+                    }
+                    cur[b * nh + t] = v;
+                }
+                __syncthreads();
+            }
+            // W[b][ab][q] = U U sum_p E^{ab}_p [p + q], lanes over (b, 4 x 4 tile of (ab, q))
+            for (int j = lane; j < nbe * tW; j += nthr) {
+                const int b = j / tW, t = j - b * tW;
+                const int m0 = (t / tq) << 2, n0 = (t - (t / tq) * tq) << 2;
+                const double* __restrict__ E = eab + b * 3 * axa;
+                const double* __restrict__ R = Rr0 + b * nh;
+                const double* ex[4];
+                const double* ey[4];
+                const double* ez[4];
+                int qq[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int ab = min(m0 + i, nab - 1);
+                    const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
+                    ex[i] = E + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
+                    ey[i] = E + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
+                    ez[i] = E + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
+                    qq[i] = min(n0 + i, nq - 1);
+                }
+                double c[4][4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) c[i][jj] = 0.0;
+                for (int pi = 0; pi < np; ++pi) {
+                    const int cp = hc[pi];
+                    const int px = cp & 15, py = (cp >> 4) & 15, pz = cp >> 8;
+                    const unsigned short* __restrict__ pr = pqi + pi * nq;
+                    double ea[4], rv[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) ea[i] = ex[i][px] * ey[i][py] * ez[i][pz];
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) rv[jj] = R[pr[qq[jj]]];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) c[i][jj] += ea[i] * rv[jj];
+                }
+                const double uu = scl[b * FJ_FB_SC + 3];
+                double* __restrict__ Wb = W + b * nW;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                        if (m0 + i < nab && n0 + jj < nq) Wb[(m0 + i) * nq + n0 + jj] = uu * c[i][jj];
+            }
+            __syncthreads();
+            // x(ab, cd) = sum_b sum_q W[b][ab][q] (-1)^|q| E^{cd}_q(b), lanes over
+            // (slice, 4 x 4 tile of (ab, cd)); slice s sums b = s, s + ns, ... and
+            // digests its share once
+            {
+                const int ns = max(1, min(nbe, nthr / tX));
+                for (int j = lane; j < ns * tX; j += nthr) {
+                    const int sl = j / tX, t = j - sl * tX;
+                    const int m0 = (t / tcd) << 2, n0 = (t - (t / tcd) * tcd) << 2;
+                    int ox[4], oy[4], oz[4], wr[4];
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) {
+                        const int cdi = min(n0 + jj, ncd - 1);
+                        const int cc = hc[hcc + cdi / nd], cdd = hc[hd + cdi % nd];
+                        ox[jj] = ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
+                        oy[jj] = axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
+                        oz[jj] = 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
+                        wr[jj] = min(m0 + jj, nab - 1) * nq;
+                    }
+                    double c[4][4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) c[i][jj] = 0.0;
+                    for (int b = sl; b < nbe; b += ns) {
+                        const double* __restrict__ Wb = W + b * nW;
+                        const double* __restrict__ E = ecd + b * 3 * axc;
+                        for (int qi = 0; qi < nq; ++qi) {
+                            const int cq = hc[qi];
+                            const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
+                            const bool neg = (qx + qy + qz) & 1;
+                            double w[4], e[4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) w[i] = Wb[wr[i] + qi];
+#pragma unroll
+                            for (int jj = 0; jj < 4; ++jj) {
+                                const double v = E[ox[jj] + qx] * E[oy[jj] + qy] * E[oz[jj] + qz];
+                                e[jj] = neg ? -v : v;
+                            }
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                for (int jj = 0; jj < 4; ++jj) c[i][jj] += w[i] * e[jj];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        if (m0 + i >= nab) continue;
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj)
+                            if (n0 + jj < ncd) fj_fl_digest(blk, nb, nc, nd, m0 + i, n0 + jj, c[i][jj]);
+                    }
+                }
+            }
+            __syncthreads();   // the batch's tables and W are rewritten next
+        }
+        // the ket's six blocks into F (the caller closes F + F^T)
+        fj_fl_flush(blk, S.F, S.n_ao, oa, ob, q.oc, q.od, na, nb, nc, nd, lane, nthr);
         __syncthreads();   // before the next ket rewrites the blocks
     }
 }

@@ -83,7 +83,15 @@ int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, 
                     a.work ? (unsigned)split : (unsigned)((a.ket.n + threads - 1) / threads));
     a.n_bx = (int)grid.x;
     a.n_items = (int)(grid.x * grid.y);
-    const FjFlLds lay = fj_floop_lds(la, lb, lc, ld);
-    hipLaunchKernelGGL(fj_floop_k, grid, dim3(FJ_FL_BLOCK), (size_t)lay.bytes, s, la, lb, lc, ld, threads, a);
+    // the batched kernel where B >= 2 primitive quartets fit its LDS budget,
+    // else one primitive quartet at a time
+    const FjFbLds fb = fj_fbloop_plan(la, lb, lc, ld, FJ_FL_BLOCK);
+    if (fb.B >= 2) {
+        hipLaunchKernelGGL(fj_fbloop_k, grid, dim3(FJ_FL_BLOCK), (size_t)fb.bytes, s, la, lb, lc, ld, threads,
+                           fb.B, a);
+    } else {
+        const FjFlLds lay = fj_floop_lds(la, lb, lc, ld);
+        hipLaunchKernelGGL(fj_floop_k, grid, dim3(FJ_FL_BLOCK), (size_t)lay.bytes, s, la, lb, lc, ld, threads, a);
+    }
     return (int)hipGetLastError();
 }

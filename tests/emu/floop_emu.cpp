@@ -1,11 +1,13 @@
 // Emulate the loop-form fused F kernel (felizjueves_amd/csrc/fj_floop.h) on
 // the CPU over walker launches (upstream ABI: no work list, workgroup
 // (bra, ket block)) read from a binary file; writes F after every launch.
-// 64 threads, one per lane, one workgroup at a time. TEST INFRASTRUCTURE.
+// argv[3] threads (default 64), one per lane, one workgroup at a time.
+// TEST INFRASTRUCTURE.
 #include "fj_device.h"   // the emulation header, copied under this name
 #include KLOOP_HEADER
 #include FLOOP_HEADER
 #include <fstream>
+#include <string>
 #include <thread>
 #include <vector>
 thread_local uint3e threadIdx;
@@ -27,7 +29,11 @@ int main(int argc, char** argv) {
     int hdr[2];   // n_launch n_ao
     f.read((char*)hdr, sizeof hdr);
     const int n_ao = hdr[1], span = 64;
-    struct L { int c[4]; FjFArgs a; double* lds; };
+    // argv[4] "one": every class on the one-at-a-time kernel; else the
+    // launcher's choice (the batched kernel where fj_fbloop_plan gives B >= 2)
+    const bool one_only = argc > 4 && std::string(argv[4]) == "one";
+    const int block = argc > 3 ? atoi(argv[3]) : 64;
+    struct L { int c[4]; FjFArgs a; double* lds; int B; };
     std::vector<L> Ls(hdr[0]);
     double* F = (double*)calloc((size_t)n_ao * n_ao, 8);
     for (auto& l : Ls) {
@@ -37,7 +43,9 @@ int main(int argc, char** argv) {
         f.read((char*)&tau, 8);
         for (int i = 0; i < 4; ++i) l.c[i] = h[i];
         // the launch's dynamic LDS at its exact size, so ASan sees an overrun
-        l.lds = (double*)malloc(fj_floop_lds(h[0], h[1], h[2], h[3]).bytes);
+        const FjFbLds fb = fj_fbloop_plan(h[0], h[1], h[2], h[3], block);
+        l.B = one_only ? 0 : (fb.B >= 2 ? fb.B : 0);
+        l.lds = (double*)malloc(l.B ? fb.bytes : fj_floop_lds(h[0], h[1], h[2], h[3]).bytes);
         double* bp = rd<double>(f); double* AB = h[1] ? rd<double>(f) : nullptr;
         double* Gab = rd<double>(f); int* bsh = rd<int>(f); int* bao = rd<int>(f);
         double* kp = rd<double>(f); double* CD = h[3] ? rd<double>(f) : nullptr;
@@ -52,7 +60,7 @@ int main(int argc, char** argv) {
         a.ws = nullptr; a.ws_doubles = 0; a.n_fp64 = 0; a.tier = nullptr;
         a.n_bx = h[4]; a.n_items = h[4] * ((h[5] + span - 1) / span);
     }
-    const int threads = argc > 3 ? atoi(argv[3]) : 64;
+    const int threads = block;
     std::barrier<> bar(threads);
     g_bar = &bar;
     blockDim = {(unsigned)threads, 1, 1};
@@ -70,7 +78,10 @@ int main(int argc, char** argv) {
                             g_dyn_lds = l.lds;
                         }
                         bar.arrive_and_wait();
-                        fj_floop_k(l.c[0], l.c[1], l.c[2], l.c[3], span, l.a);
+                        if (l.B)
+                            fj_fbloop_k(l.c[0], l.c[1], l.c[2], l.c[3], span, l.B, l.a);
+                        else
+                            fj_floop_k(l.c[0], l.c[1], l.c[2], l.c[3], span, l.a);
                         bar.arrive_and_wait();
                     }
             }

@@ -35,17 +35,24 @@ def _shell(l, centre, exps, raw):  # noqa: E741
     return Shell(l, tuple(centre), tuple(exps), tuple(c))
 
 
-def _system():
+def _system(contracted: bool = False):
     """Two atoms 1.9 bohr apart with s, p, d, f each (the first atom's s
-    contracted over two primitives); D a random symmetric matrix."""
+    contracted over two primitives); D a random symmetric matrix. With
+    `contracted`, every s shell has three primitives and every p, d and f
+    shell two, so that a quartet has up to 81 primitive quartets."""
     A, B = (0.0, 0.0, 0.0), (0.7, -1.1, 1.3)
-    shells = [_shell(0, A, (3.1, 0.45), (0.6, 0.5))]
+
+    def sh(l, X, e):  # noqa: E741
+        if not contracted:
+            return _shell(l, X, (e,), (1.0,))
+        if l == 0:
+            return _shell(0, X, (4.0 * e, e, 0.3 * e), (0.3, 0.5, 0.4))
+        return _shell(l, X, (2.5 * e, e), (0.4, 0.7))
+    shells = [sh(0, A, 0.45) if contracted else _shell(0, A, (3.1, 0.45), (0.6, 0.5))]
     for X, sc in ((A, 1.0), (B, 0.8)):
         if X is B:
-            shells.append(_shell(0, X, (0.5,), (1.0,)))
-        shells += [_shell(1, X, (1.3 * sc,), (1.0,)),
-                   _shell(2, X, (0.9 * sc,), (1.0,)),
-                   _shell(3, X, (0.7 * sc,), (1.0,))]
+            shells.append(sh(0, X, 0.5))
+        shells += [sh(1, X, 1.3 * sc), sh(2, X, 0.9 * sc), sh(3, X, 0.7 * sc)]
     basis = Basis(shells)
     rng = np.random.default_rng(5)
     M = rng.normal(size=(basis.n_ao, basis.n_ao))
@@ -149,13 +156,18 @@ def test_loop_k_kernel_on_cpu_vs_oracle():
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
-def test_loop_f_kernel_on_cpu_vs_oracle():
-    """The loop-form fused F kernel on every walker launch of the s,p,d,f
+@pytest.mark.parametrize("mode,contracted", [("one", False), ("auto", True)])
+def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted):
+    """The loop-form fused F kernels on every walker launch of the s,p,d,f
     basis (every canonical class through (ff|ff)), F against the oracle's
     launch-by-launch witness (the same screen and digestion, integrals by
-    McMurchie-Davidson)."""
+    McMurchie-Davidson). "one": every class on the one-at-a-time kernel;
+    "auto": the launcher's choice, the batched kernel on every class whose
+    batch holds two or more primitive quartets, on the contracted basis
+    (kets of up to 36 primitive quartets: full batches, remainders and
+    several batches per ket)."""
     from felizjueves_amd import pairs as P
-    basis, D = _system()
+    basis, D = _system(contracted)
     plan = P.f_launches(basis, D, tau=0.0, n=1)
     Dsh = P.density_blocks(basis, D)
     launches = plan.launches
@@ -200,8 +212,8 @@ def test_loop_f_kernel_on_cpu_vs_oracle():
                 w(a["ket_ao"], np.int32)
                 w(a["Dsh"], np.float64)
                 w(a["D"], np.float64)
-        # two wavefronts per workgroup, as the library launches it
-        F = _run(exe, td, "F.bin", "128").reshape(basis.n_ao, basis.n_ao)
+        # four wavefronts per workgroup, as the library launches it
+        F = _run(exe, td, "F.bin", "256", mode).reshape(basis.n_ao, basis.n_ao)
     err = np.abs(F - Fo).max()
     print("loop-form F vs oracle", err, "scale", np.abs(Fo).max(), "launches", len(launches), flush=True)
     assert err <= 1e-12 * max(1.0, np.abs(Fo).max())
