@@ -73,6 +73,15 @@ double* fj_library_workspace(long long bytes);
 // A kernel's dynamic LDS block (its size given at launch).
 #define FJ_DYN_SHARED(name) extern __shared__ double name[]
 
+// One FP64 matrix-core step of a wavefront, D = A B + C with A 16 x 4, B
+// 4 x 16, C and D 16 x 16: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[l / 16 + 4 v][l % 16] in element v. Every
+// lane of the wavefront calls it together.
+typedef double fj_v4d __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ fj_v4d fj_mfma_f64_16x16x4(double a, double b, fj_v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 // An empty asm statement that the compiler must assume redefines the
 // per-lane pointer p (a VGPR pair) and the uniform stride s (an SGPR pair):
 // values derived from them are recomputed after it, not hoisted above it.

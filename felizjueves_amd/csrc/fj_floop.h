@@ -13,12 +13,12 @@
 // live ket, the weighted density blocks and the six F accumulators sit in
 // LDS, and per primitive quartet the lanes build the E tables and the
 // Hermite tower, then, for the bra rows ab in chunks of FJ_FL_CH,
-//   W[ab][q] = sum_p E^{ab}_p [p + q]           (lanes over 4 x 4 tiles of (ab, q))
-//   x(ab,cd) = U U sum_q W[ab][q] (-1)^|q| E^{cd}_q   (lanes over 4 x 4 tiles of (ab, cd))
-// and digest x at once (LDS atomics into the six blocks). In a tile the
-// E products come from the per-axis tables and are shared across the tile's
-// row or column: eight operands feed sixteen multiply-adds. After the ket's
-// primitives the blocks go to F with one global atomic per element.
+//   W[ab][q] = sum_p E^{ab}_p [p + q]                  (16 x 16 tiles of (ab, q))
+//   x(ab,cd) = U U sum_q W[ab][q] (-1)^|q| E^{cd}_q     (16 x 16 tiles of (ab, cd))
+// on the FP64 matrix cores (a wavefront per tile, four Hermite indices per
+// step; each lane forms its A and B elements from the axis tables, the tower
+// and W), and digest x at once (LDS atomics into the six blocks). After the
+// ket's primitives the blocks go to F with one global atomic per element.
 // Every barrier sits in control flow that is uniform over the wavefront.
 //
 // fj_fbloop_k is the batched form for the classes whose whole W fits LDS
@@ -209,6 +209,7 @@ __device__ __forceinline__ void fj_fl_digest(const FjFlBlocks& B, int nb, int nc
 __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
     FJ_DYN_SHARED(fj_floop_lds_block);
     const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
+    const int wave = lane >> 6, lw = lane & 63, nw = nthr >> 6;   // wavefronts of the workgroup
     FjFlItem it;
     if (!fj_fl_item(a, span, it)) return;   // uniform
     const int bra = it.bra, kb0 = it.kb0, kab = it.kab;
@@ -299,100 +300,117 @@ __global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld
                 const double UU = rb[7] * rk[7];
                 for (int r0 = 0; r0 < nab; r0 += FJ_FL_CH) {
                     const int nr = min(FJ_FL_CH, nab - r0);
-                    // W[r][q] = sum_p E^{ab}_p [p + q] in 4 x 4 register tiles of
-                    // (r, q): per p four E products (from the axis tables) and
-                    // four tower entries make sixteen multiply-adds
+                    // W[r][q] = sum_p E^{ab}_p [p + q] on the matrix cores: 16 x 16
+                    // tiles of (r, q), p in steps of 4, wavefront w taking tiles w +
+                    // 2 nw i and w + 2 nw i + nw together (two independent chains);
+                    // a lane's A element is an E product from the axis tables, its
+                    // B element a tower entry
                     {
-                        const int tq = (nq + 3) >> 2, nt = ((nr + 3) >> 2) * tq;
-                        for (int t = lane; t < nt; t += nthr) {
-                            const int m0 = (t / tq) << 2, n0 = (t - (t / tq) * tq) << 2;
-                            const double* ex[4];
-                            const double* ey[4];
-                            const double* ez[4];
-                            int qq[4];
+                        const int tqw = (nq + 15) >> 4, nt = ((nr + 15) >> 4) * tqw;
+                        for (int t0 = wave; t0 < nt; t0 += 2 * nw) {   // uniform over the wavefront
+                            const int t1 = t0 + nw < nt ? t0 + nw : t0;
+                            const int rt0 = t0 / tqw, qt0 = t0 - rt0 * tqw, rt1 = t1 / tqw, qt1 = t1 - rt1 * tqw;
+                            const int ra0 = (rt0 << 4) + (lw & 15), qb0 = (qt0 << 4) + (lw & 15);
+                            const int ra1 = (rt1 << 4) + (lw & 15), qb1 = (qt1 << 4) + (lw & 15);
+                            // a lane's A element needs its row valid, its B element its column
+                            const bool two = t1 != t0;
+                            const bool rok0 = ra0 < nr, qok0 = qb0 < nq, rok1 = two && ra1 < nr, qok1 = two && qb1 < nq;
+                            const double* ex[2];
+                            const double* ey[2];
+                            const double* ez[2];
+                            int qc[2];
 #pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                const int ab = r0 + min(m0 + i, nr - 1);
+                            for (int u = 0; u < 2; ++u) {
+                                const int ab = r0 + min(u ? ra1 : ra0, nr - 1);
                                 const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
-                                ex[i] = eab + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
-                                ey[i] = eab + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
-                                ez[i] = eab + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
-                                qq[i] = min(n0 + i, nq - 1);
+                                ex[u] = eab + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
+                                ey[u] = eab + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
+                                ez[u] = eab + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
+                                qc[u] = min(u ? qb1 : qb0, nq - 1);
                             }
-                            double c[4][4];
-#pragma unroll
-                            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                                for (int j = 0; j < 4; ++j) c[i][j] = 0.0;
-                            for (int pi = 0; pi < np; ++pi) {
-                                const int cp = hc[pi];
+                            fj_v4d c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+                            for (int p0 = 0; p0 < np; p0 += 4) {
+                                const int p = p0 + (lw >> 4), pc = min(p, np - 1);
+                                const bool pok = p < np;
+                                const int cp = hc[pc];
                                 const int px = cp & 15, py = (cp >> 4) & 15, pz = cp >> 8;
-                                const unsigned short* __restrict__ pr = pqi + pi * nq;
-                                double ea[4], rv[4];
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) ea[i] = ex[i][px] * ey[i][py] * ez[i][pz];
-#pragma unroll
-                                for (int j = 0; j < 4; ++j) rv[j] = R0[pr[qq[j]]];
-#pragma unroll
-                                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                                    for (int j = 0; j < 4; ++j) c[i][j] += ea[i] * rv[j];
+                                const unsigned short* __restrict__ pr = pqi + pc * nq;
+                                const double a0 = rok0 && pok ? ex[0][px] * ey[0][py] * ez[0][pz] : 0.0;
+                                const double b0 = qok0 && pok ? R0[pr[qc[0]]] : 0.0;
+                                const double a1 = rok1 && pok ? ex[1][px] * ey[1][py] * ez[1][pz] : 0.0;
+                                const double b1 = qok1 && pok ? R0[pr[qc[1]]] : 0.0;
+                                c0 = fj_mfma_f64_16x16x4(a0, b0, c0);
+                                c1 = fj_mfma_f64_16x16x4(a1, b1, c1);
                             }
 #pragma unroll
-                            for (int i = 0; i < 4; ++i)
+                            for (int v = 0; v < 4; ++v) {
+                                const int r = (rt0 << 4) + (lw >> 4) + 4 * v;
+                                if (r < nr && qb0 < nq) W[r * nq + qb0] = c0[v];
+                            }
+                            if (t1 != t0) {
 #pragma unroll
-                                for (int j = 0; j < 4; ++j)
-                                    if (m0 + i < nr && n0 + j < nq) W[(m0 + i) * nq + n0 + j] = c[i][j];
+                                for (int v = 0; v < 4; ++v) {
+                                    const int r = (rt1 << 4) + (lw >> 4) + 4 * v;
+                                    if (r < nr && qb1 < nq) W[r * nq + qb1] = c1[v];
+                                }
+                            }
                         }
                     }
                     __syncthreads();
-                    // x(ab, cd) = U U sum_q W[r][q] (-1)^|q| E^{cd}_q in 4 x 4 tiles
-                    // of (r, cd), each element digested at once into the six blocks
+                    // x(ab, cd) = U U sum_q W[r][q] (-1)^|q| E^{cd}_q on the matrix
+                    // cores, 16 x 16 tiles of (r, cd) two at a time as above, each
+                    // element digested at once into the six blocks
                     {
-                        const int tcd = (ncd + 3) >> 2, nt = ((nr + 3) >> 2) * tcd;
-                        for (int t = lane; t < nt; t += nthr) {
-                            const int m0 = (t / tcd) << 2, n0 = (t - (t / tcd) * tcd) << 2;
-                            const double* fx[4];
-                            const double* fy[4];
-                            const double* fz[4];
-                            int wr[4];
+                        const int tcw = (ncd + 15) >> 4, nt = ((nr + 15) >> 4) * tcw;
+                        for (int t0 = wave; t0 < nt; t0 += 2 * nw) {   // uniform over the wavefront
+                            const int t1 = t0 + nw < nt ? t0 + nw : t0;
+                            const int rt0 = t0 / tcw, ct0 = t0 - rt0 * tcw, rt1 = t1 / tcw, ct1 = t1 - rt1 * tcw;
+                            const int ra0 = (rt0 << 4) + (lw & 15), cb0 = (ct0 << 4) + (lw & 15);
+                            const int ra1 = (rt1 << 4) + (lw & 15), cb1 = (ct1 << 4) + (lw & 15);
+                            const bool two = t1 != t0;
+                            const bool rok0 = ra0 < nr, cok0 = cb0 < ncd, rok1 = two && ra1 < nr, cok1 = two && cb1 < ncd;
+                            const double* fx[2];
+                            const double* fy[2];
+                            const double* fz[2];
+                            int wr[2];
 #pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                const int cdi = min(n0 + j, ncd - 1);
+                            for (int u = 0; u < 2; ++u) {
+                                const int cdi = min(u ? cb1 : cb0, ncd - 1);
                                 const int cc = hc[hcc + cdi / nd], cdd = hc[hd + cdi % nd];
-                                fx[j] = ecd + ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
-                                fy[j] = ecd + axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
-                                fz[j] = ecd + 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
-                                wr[j] = min(m0 + j, nr - 1) * nq;
+                                fx[u] = ecd + ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
+                                fy[u] = ecd + axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
+                                fz[u] = ecd + 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
+                                wr[u] = min(u ? ra1 : ra0, nr - 1) * nq;
                             }
-                            double c[4][4];
-#pragma unroll
-                            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                                for (int j = 0; j < 4; ++j) c[i][j] = 0.0;
-                            for (int qi = 0; qi < nq; ++qi) {
-                                const int cq = hc[qi];
+                            fj_v4d c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+                            for (int q0 = 0; q0 < nq; q0 += 4) {
+                                const int qq = q0 + (lw >> 4), qc = min(qq, nq - 1);
+                                const bool qok = qq < nq;
+                                const int cq = hc[qc];
                                 const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
                                 const bool neg = (qx + qy + qz) & 1;
-                                double w[4], e[4];
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) w[i] = W[wr[i] + qi];
-#pragma unroll
-                                for (int j = 0; j < 4; ++j) {
-                                    const double v = fx[j][qx] * fy[j][qy] * fz[j][qz];
-                                    e[j] = neg ? -v : v;
-                                }
-#pragma unroll
-                                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                                    for (int j = 0; j < 4; ++j) c[i][j] += w[i] * e[j];
+                                const double e0 = fx[0][qx] * fy[0][qy] * fz[0][qz];
+                                const double e1 = fx[1][qx] * fy[1][qy] * fz[1][qz];
+                                const double a0 = rok0 && qok ? W[wr[0] + qc] : 0.0;
+                                const double b0 = cok0 && qok ? (neg ? -e0 : e0) : 0.0;
+                                const double a1 = rok1 && qok ? W[wr[1] + qc] : 0.0;
+                                const double b1 = cok1 && qok ? (neg ? -e1 : e1) : 0.0;
+                                c0 = fj_mfma_f64_16x16x4(a0, b0, c0);
+                                c1 = fj_mfma_f64_16x16x4(a1, b1, c1);
                             }
 #pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                if (m0 + i >= nr) continue;
+                            for (int v = 0; v < 4; ++v) {
+                                const int r = (rt0 << 4) + (lw >> 4) + 4 * v;
+                                if (r < nr && cb0 < ncd) fj_fl_digest(blk, nb, nc, nd, r0 + r, cb0, UU * c0[v]);
+                            }
+                            if (t1 != t0) {
 #pragma unroll
-                                for (int j = 0; j < 4; ++j)
-                                    if (n0 + j < ncd) fj_fl_digest(blk, nb, nc, nd, r0 + m0 + i, n0 + j, UU * c[i][j]);
+                                for (int v = 0; v < 4; ++v) {
+                                    const int r = (rt1 << 4) + (lw >> 4) + 4 * v;
+                                    if (r < nr && cb1 < ncd) fj_fl_digest(blk, nb, nc, nd, r0 + r, cb1, UU * c1[v]);
+                                }
                             }
                         }
                     }

@@ -57,6 +57,29 @@ inline double __shfl_xor(double v, int o, int w) {
     g_bar->arrive_and_wait();
     return r;
 }
+// The FP64 matrix-core step D = A B + C (16 x 4 by 4 x 16) of one
+// wavefront: lane l gives A[l % 16][l / 16] and B[l / 16][l % 16] and gets
+// D[l / 16 + 4 v][l % 16] in element v (the layout measured on gfx950).
+// The wavefront's lanes meet at its own barrier (g_wbar, set by the driver),
+// so every lane of a wavefront must call it together, as on the GPU.
+typedef double fj_v4d __attribute__((vector_size(32)));
+extern std::barrier<>* g_wbar[4];
+extern double g_mfa[256], g_mfb[256];
+inline fj_v4d fj_mfma_f64_16x16x4(double a, double b, fj_v4d c) {
+    const int l = (int)threadIdx.x, w = l >> 6, lw = l & 63, base = w << 6;
+    g_wbar[w]->arrive_and_wait();   // the wavefront's previous step has been read
+    g_mfa[l] = a;
+    g_mfb[l] = b;
+    g_wbar[w]->arrive_and_wait();
+    const int j = lw & 15;
+    for (int v = 0; v < 4; ++v) {
+        const int i = (lw >> 4) + 4 * v;
+        double s = c[v];
+        for (int k = 0; k < 4; ++k) s += g_mfa[base + i + 16 * k] * g_mfb[base + 16 * k + j];
+        c[v] = s;
+    }
+    return c;
+}
 inline void unsafeAtomicAdd(double* p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
 inline double atomicAdd(double* p, double v) { return std::atomic_ref<double>(*p).fetch_add(v); }
 inline void fj_side_range(const FjSide& s, int i, int& k0, int& k) {

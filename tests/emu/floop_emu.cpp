@@ -13,6 +13,8 @@
 thread_local uint3e threadIdx;
 uint3e blockIdx, blockDim, gridDim;
 std::barrier<>* g_bar;
+std::barrier<>* g_wbar[4];
+double g_mfa[256], g_mfb[256];
 double g_shfl[256];
 double* g_dyn_lds;
 
@@ -63,6 +65,10 @@ int main(int argc, char** argv) {
     const int threads = block;
     std::barrier<> bar(threads);
     g_bar = &bar;
+    // one barrier per wavefront, for the matrix-core steps
+    std::barrier<> wb0(64), wb1(64), wb2(64), wb3(64);
+    g_wbar[0] = &wb0; g_wbar[1] = &wb1; g_wbar[2] = &wb2; g_wbar[3] = &wb3;
+    if (threads % 64 || threads > 256) { fprintf(stderr, "threads %d\n", threads); return 1; }
     blockDim = {(unsigned)threads, 1, 1};
     std::vector<std::thread> ts;
     for (int t = 0; t < threads; ++t)

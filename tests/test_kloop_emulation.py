@@ -156,8 +156,8 @@ def test_loop_k_kernel_on_cpu_vs_oracle():
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
-@pytest.mark.parametrize("mode,contracted", [("one", False), ("auto", True)])
-def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted):
+@pytest.mark.parametrize("mode,contracted,threads", [("one", False, "64"), ("auto", True, "256")])
+def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted, threads):
     """The loop-form fused F kernels on every walker launch of the s,p,d,f
     basis (every canonical class through (ff|ff)), F against the oracle's
     launch-by-launch witness (the same screen and digestion, integrals by
@@ -165,7 +165,10 @@ def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted):
     "auto": the launcher's choice, the batched kernel on every class whose
     batch holds two or more primitive quartets, on the contracted basis
     (kets of up to 36 primitive quartets: full batches, remainders and
-    several batches per ket)."""
+    several batches per ket). "auto" runs four wavefronts per workgroup, as
+    the library launches the kernels; "one" a single wavefront (the kernels'
+    strides and wavefront loops take the width as an argument; one
+    wavefront keeps the emulation of every matrix-core step short)."""
     from felizjueves_amd import pairs as P
     basis, D = _system(contracted)
     plan = P.f_launches(basis, D, tau=0.0, n=1)
@@ -212,8 +215,7 @@ def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted):
                 w(a["ket_ao"], np.int32)
                 w(a["Dsh"], np.float64)
                 w(a["D"], np.float64)
-        # four wavefronts per workgroup, as the library launches it
-        F = _run(exe, td, "F.bin", "256", mode).reshape(basis.n_ao, basis.n_ao)
+        F = _run(exe, td, "F.bin", threads, mode).reshape(basis.n_ao, basis.n_ao)
     err = np.abs(F - Fo).max()
     print("loop-form F vs oracle", err, "scale", np.abs(Fo).max(), "launches", len(launches), flush=True)
     assert err <= 1e-12 * max(1.0, np.abs(Fo).max())
