@@ -206,7 +206,10 @@ __device__ __forceinline__ void fj_fl_digest(const FjFlBlocks& B, int nb, int nc
     atomicAdd(&B.f[5][jb * nd + ld_], B.w[2][ia * nc + kc] * x);
 }
 
-__global__ void __launch_bounds__(256) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
+// Three wavefronts per SIMD asked for (166 VGPRs, no scratch): the kernel is
+// latency-bound, and where the class's LDS allows, a CU holds three
+// workgroups instead of two.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) fj_floop_k(int la, int lb, int lc, int ld, int span, FjFArgs a) {
     FJ_DYN_SHARED(fj_floop_lds_block);
     const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
     const int wave = lane >> 6, lw = lane & 63, nw = nthr >> 6;   // wavefronts of the workgroup
