@@ -31,11 +31,14 @@ int main(int argc, char** argv) {
     int hdr[2];   // n_launch n_ao
     f.read((char*)hdr, sizeof hdr);
     const int n_ao = hdr[1], span = 64;
-    // argv[4] "one": every class on the one-at-a-time kernel; else the
-    // launcher's choice (the batched kernel where fj_fbloop_plan gives B >= 2)
-    const bool one_only = argc > 4 && std::string(argv[4]) == "one";
+    // argv[4] "one": every class on the one-at-a-time kernel; "batched":
+    // only the classes the launcher gives the batched kernel (fj_fbloop_plan:
+    // B >= 2), the others skipped; else the launcher's choice for every
+    // class. Which launches ran goes to argv[2] + ".ran" (int32 per launch).
+    const std::string mode = argc > 4 ? argv[4] : "auto";
+    const bool one_only = mode == "one", batched_only = mode == "batched";
     const int block = argc > 3 ? atoi(argv[3]) : 64;
-    struct L { int c[4]; FjFArgs a; double* lds; int B; };
+    struct L { int c[4]; FjFArgs a; double* lds; int B; int run; };
     std::vector<L> Ls(hdr[0]);
     double* F = (double*)calloc((size_t)n_ao * n_ao, 8);
     for (auto& l : Ls) {
@@ -47,6 +50,7 @@ int main(int argc, char** argv) {
         // the launch's dynamic LDS at its exact size, so ASan sees an overrun
         const FjFbLds fb = fj_fbloop_plan(h[0], h[1], h[2], h[3], block);
         l.B = one_only ? 0 : (fb.B >= 2 ? fb.B : 0);
+        l.run = !batched_only || l.B > 0;
         l.lds = (double*)malloc(l.B ? fb.bytes : fj_floop_lds(h[0], h[1], h[2], h[3]).bytes);
         double* bp = rd<double>(f); double* AB = h[1] ? rd<double>(f) : nullptr;
         double* Gab = rd<double>(f); int* bsh = rd<int>(f); int* bao = rd<int>(f);
@@ -75,6 +79,7 @@ int main(int argc, char** argv) {
         ts.emplace_back([&, t] {
             threadIdx = {(unsigned)t, 0, 0};
             for (auto& l : Ls) {
+                if (!l.run) continue;
                 const unsigned gx = l.a.bra.n, gy = (l.a.ket.n + span - 1) / span;
                 for (unsigned y = 0; y < gy; ++y)
                     for (unsigned x = 0; x < gx; ++x) {
@@ -95,6 +100,8 @@ int main(int argc, char** argv) {
     for (auto& t : ts) t.join();
     std::ofstream o(argv[2], std::ios::binary);
     o.write((char*)F, 8LL * n_ao * n_ao);
+    std::ofstream r(std::string(argv[2]) + ".ran", std::ios::binary);
+    for (auto& l : Ls) r.write((char*)&l.run, sizeof(int));
     fprintf(stderr, "done\n");
     return 0;
 }

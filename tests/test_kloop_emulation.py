@@ -156,19 +156,20 @@ def test_loop_k_kernel_on_cpu_vs_oracle():
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
-@pytest.mark.parametrize("mode,contracted,threads", [("one", False, "64"), ("auto", True, "256")])
+@pytest.mark.parametrize("mode,contracted,threads", [("one", False, "64"), ("batched", True, "256")])
 def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted, threads):
     """The loop-form fused F kernels on every walker launch of the s,p,d,f
     basis (every canonical class through (ff|ff)), F against the oracle's
     launch-by-launch witness (the same screen and digestion, integrals by
     McMurchie-Davidson). "one": every class on the one-at-a-time kernel;
-    "auto": the launcher's choice, the batched kernel on every class whose
-    batch holds two or more primitive quartets, on the contracted basis
+    "batched": the classes the launcher gives the batched kernel (a batch
+    of two or more primitive quartets fits its LDS), on the contracted basis
     (kets of up to 36 primitive quartets: full batches, remainders and
-    several batches per ket). "auto" runs four wavefronts per workgroup, as
-    the library launches the kernels; "one" a single wavefront (the kernels'
-    strides and wavefront loops take the width as an argument; one
-    wavefront keeps the emulation of every matrix-core step short)."""
+    several batches per ket), F against the oracle's launches of the same
+    classes. "batched" runs four wavefronts per workgroup, as the library
+    launches the kernels; "one" a single wavefront (the kernels' strides and
+    wavefront loops take the width as an argument; one wavefront keeps the
+    emulation of every matrix-core step short)."""
     from felizjueves_amd import pairs as P
     basis, D = _system(contracted)
     plan = P.f_launches(basis, D, tau=0.0, n=1)
@@ -176,6 +177,7 @@ def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted, threads):
     launches = plan.launches
     assert max(max(L.cls) for L in launches) == 3
     Fo = np.zeros((basis.n_ao, basis.n_ao))
+    parts = []   # each launch's F from the oracle
     with tempfile.TemporaryDirectory() as td:
         td = Path(td)
         (td / "gen").mkdir(exist_ok=True)
@@ -197,7 +199,9 @@ def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted, threads):
                 x.tofile(f)
             for L in launches:
                 a = P.launch_args(basis, plan, L, D, Dsh)
-                O.f_launch(L.cls, a, Fo)
+                Fl = np.zeros_like(Fo)
+                O.f_launch(L.cls, a, Fl)
+                parts.append(Fl)
                 np.array([*L.cls, a["n_bra"], a["n_ket"], a["k_ab"], a["k_cd"], a["n_sh"],
                           a["n_ao"], a["diag"]], dtype=np.int32).tofile(f)
                 np.array([a["tau"]], dtype=np.float64).tofile(f)
@@ -216,6 +220,13 @@ def test_loop_f_kernel_on_cpu_vs_oracle(mode, contracted, threads):
                 w(a["Dsh"], np.float64)
                 w(a["D"], np.float64)
         F = _run(exe, td, "F.bin", threads, mode).reshape(basis.n_ao, basis.n_ao)
+        ran = np.fromfile(td / "F.bin.ran", dtype=np.int32)
+    assert len(ran) == len(launches)
+    if mode == "batched":   # both kernels' classes are present; only the batched ones ran
+        assert 0 < ran.sum() < len(ran)
+    else:
+        assert ran.all()
+    Fo = sum(Fl for Fl, r in zip(parts, ran) if r)
     err = np.abs(F - Fo).max()
     print("loop-form F vs oracle", err, "scale", np.abs(Fo).max(), "launches", len(launches), flush=True)
     assert err <= 1e-12 * max(1.0, np.abs(Fo).max())
