@@ -24,12 +24,15 @@
 // fj_fbloop_k is the batched form for the classes whose whole W fits LDS
 // several times over (fj_fbloop_plan: B >= 2): it takes a ket's primitive
 // quartets B at a time, every phase over the batch at once (lanes over the
-// E tables and seeds of the batch, over (b, t) in each tower level, over
-// (b, tile) in W), so a batch pays the L + 3 barriers and the serial Boys
-// and E-table latencies that the one-at-a-time form pays per primitive
-// quartet; x is summed over the batch in registers (lanes over (slice,
-// tile of (ab, cd)), each slice a share of the batch) before the digestion,
-// which runs once per slice and batch instead of once per primitive quartet.
+// E tables and seeds of the batch, over (b, t) in each tower level, the
+// matrix cores over (b, tile) in W), so a batch pays the L + 3 barriers and
+// the serial Boys and E-table latencies that the one-at-a-time form pays per
+// primitive quartet; x is one matrix-core GEMM over the batch (K = (b, q)),
+// digested once per batch instead of once per primitive quartet. Classes
+// with fewer than FJ_FB_MFMA_ROWS bra functions, whose 16-row tiles would
+// leave the matrix cores mostly idle, contract in 4 x 4 register tiles
+// instead (W lanes over (b, tile); x summed over the batch in registers,
+// lanes over (slice, tile), each slice digesting its share once).
 //
 // The includer provides fj_device.h and fj_kloop.h.
 #pragma once
@@ -433,6 +436,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
 #define FJ_FB_BUDGET (64 * 1024)   // LDS bytes per workgroup
 #define FJ_FB_BMAX 64              // primitive quartets per batch
 #define FJ_FB_SC 4                 // per primitive quartet: PQ (3), U_ab U_cd
+#define FJ_FB_MFMA_ROWS 8          // bra functions from which the contractions use the
+                                   // matrix cores (16-row tiles; fewer rows leave them idle)
 
 // The batched kernel's LDS for batch B: B copies of the per-primitive-quartet
 // arrays (the two tower rows, the two sides' axis E tables, W [nab][nq], the
@@ -474,19 +479,14 @@ __host__ __device__ inline FjFbLds fj_fbloop_layout(int la, int lb, int lc, int 
     return o;
 }
 
-// B for a class: the largest that fits FJ_FB_BUDGET, at most FJ_FB_BMAX and
-// no more than the lanes need to have one W tile each. B < 2 means the class
-// runs on the one-at-a-time kernel.
-__host__ __device__ inline FjFbLds fj_fbloop_plan(int la, int lb, int lc, int ld, int threads) {
+// B for a class: the largest that fits FJ_FB_BUDGET, at most bmax (a
+// larger batch pays the per-batch barriers and latencies over more primitive
+// quartets). B < 2 means the class runs on the one-at-a-time kernel.
+__host__ __device__ inline FjFbLds fj_fbloop_plan(int la, int lb, int lc, int ld, int bmax = FJ_FB_BMAX) {
     const FjFbLds one = fj_fbloop_layout(la, lb, lc, ld, 1), two = fj_fbloop_layout(la, lb, lc, ld, 2);
     const int per_b = two.bytes - one.bytes, fixed = one.bytes - per_b;
-    const int nab = ((la + 1) * (la + 2) / 2) * ((lb + 1) * (lb + 2) / 2);
-    const int nq = one.nW / nab;
-    const int tiles = ((nab + 3) / 4) * ((nq + 3) / 4);
     int B = (FJ_FB_BUDGET - fixed) / per_b;
-    const int want = (threads + tiles - 1) / tiles;
-    if (B > want) B = want;
-    if (B > FJ_FB_BMAX) B = FJ_FB_BMAX;
+    if (B > bmax) B = bmax;
     if (B < 1) B = 1;
     return fj_fbloop_layout(la, lb, lc, ld, B);
 }
@@ -495,6 +495,7 @@ __global__ void __launch_bounds__(256) fj_fbloop_k(int la, int lb, int lc, int l
                                                    FjFArgs a) {
     FJ_DYN_SHARED(fj_fbloop_lds_block);
     const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
+    const int wave = lane >> 6, lw = lane & 63, nw = nthr >> 6;   // wavefronts of the workgroup
     FjFlItem it;
     if (!fj_fl_item(a, span, it)) return;   // uniform
     const int bra = it.bra, kb0 = it.kb0, kab = it.kab;
@@ -542,8 +543,6 @@ __global__ void __launch_bounds__(256) fj_fbloop_k(int la, int lb, int lc, int l
                  ABz = lb ? a.bra.sep[3 * bra + 2] : 0.0;
     __syncthreads();
 
-    const int tq = (nq + 3) >> 2, tW = ((nab + 3) >> 2) * tq;        // W tiles per primitive quartet
-    const int tcd = (ncd + 3) >> 2, tX = ((nab + 3) >> 2) * tcd;     // x tiles
     for (int kk = it.k0; kk < it.k1; ++kk) {
         FjFlKet q;
         if (!fj_fl_ket(a, it, kk, sa, sb, Gb, q)) continue;   // uniform: every lane reads the same ket
@@ -604,103 +603,224 @@ __global__ void __launch_bounds__(256) fj_fbloop_k(int la, int lb, int lc, int l
                 }
                 __syncthreads();
             }
-            // W[b][ab][q] = U U sum_p E^{ab}_p [p + q], lanes over (b, 4 x 4 tile of (ab, q))
-            for (int j = lane; j < nbe * tW; j += nthr) {
-                const int b = j / tW, t = j - b * tW;
-                const int m0 = (t / tq) << 2, n0 = (t - (t / tq) * tq) << 2;
-                const double* __restrict__ E = eab + b * 3 * axa;
-                const double* __restrict__ R = Rr0 + b * nh;
-                const double* ex[4];
-                const double* ey[4];
-                const double* ez[4];
-                int qq[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int ab = min(m0 + i, nab - 1);
-                    const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
-                    ex[i] = E + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
-                    ey[i] = E + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
-                    ez[i] = E + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
-                    qq[i] = min(n0 + i, nq - 1);
-                }
-                double c[4][4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) c[i][jj] = 0.0;
-                for (int pi = 0; pi < np; ++pi) {
-                    const int cp = hc[pi];
-                    const int px = cp & 15, py = (cp >> 4) & 15, pz = cp >> 8;
-                    const unsigned short* __restrict__ pr = pqi + pi * nq;
-                    double ea[4], rv[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) ea[i] = ex[i][px] * ey[i][py] * ez[i][pz];
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) rv[jj] = R[pr[qq[jj]]];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) c[i][jj] += ea[i] * rv[jj];
-                }
-                const double uu = scl[b * FJ_FB_SC + 3];
-                double* __restrict__ Wb = W + b * nW;
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj)
-                        if (m0 + i < nab && n0 + jj < nq) Wb[(m0 + i) * nq + n0 + jj] = uu * c[i][jj];
-            }
-            __syncthreads();
-            // x(ab, cd) = sum_b sum_q W[b][ab][q] (-1)^|q| E^{cd}_q(b), lanes over
-            // (slice, 4 x 4 tile of (ab, cd)); slice s sums b = s, s + ns, ... and
-            // digests its share once
-            {
-                const int ns = max(1, min(nbe, nthr / tX));
-                for (int j = lane; j < ns * tX; j += nthr) {
-                    const int sl = j / tX, t = j - sl * tX;
-                    const int m0 = (t / tcd) << 2, n0 = (t - (t / tcd) * tcd) << 2;
-                    int ox[4], oy[4], oz[4], wr[4];
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) {
-                        const int cdi = min(n0 + jj, ncd - 1);
-                        const int cc = hc[hcc + cdi / nd], cdd = hc[hd + cdi % nd];
-                        ox[jj] = ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
-                        oy[jj] = axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
-                        oz[jj] = 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
-                        wr[jj] = min(m0 + jj, nab - 1) * nq;
-                    }
-                    double c[4][4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) c[i][jj] = 0.0;
-                    for (int b = sl; b < nbe; b += ns) {
-                        const double* __restrict__ Wb = W + b * nW;
-                        const double* __restrict__ E = ecd + b * 3 * axc;
-                        for (int qi = 0; qi < nq; ++qi) {
-                            const int cq = hc[qi];
-                            const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
-                            const bool neg = (qx + qy + qz) & 1;
-                            double w[4], e[4];
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) w[i] = Wb[wr[i] + qi];
-#pragma unroll
-                            for (int jj = 0; jj < 4; ++jj) {
-                                const double v = E[ox[jj] + qx] * E[oy[jj] + qy] * E[oz[jj] + qz];
-                                e[jj] = neg ? -v : v;
+            if (nab >= FJ_FB_MFMA_ROWS) {   // uniform: the class decides
+                // W[b][ab][q] = U U sum_p E^{ab}_p [p + q] on the matrix cores:
+                // 16 x 16 tiles of (ab, q) per b, p in steps of 4, wavefront w
+                // taking tiles w + 2 nw i and w + 2 nw i + nw (two chains)
+                {
+                    const int tqw = (nq + 15) >> 4, tpb = ((nab + 15) >> 4) * tqw, nt = nbe * tpb;
+                    for (int t0 = wave; t0 < nt; t0 += 2 * nw) {   // uniform over the wavefront
+                        const int t1 = t0 + nw < nt ? t0 + nw : t0;
+                        const bool two = t1 != t0;
+                        int bq[2], rt[2], qb[2], qc[2];
+                        bool rok[2], qok[2];
+                        const double* ex[2];
+                        const double* ey[2];
+                        const double* ez[2];
+                        const double* Rb[2];
+    #pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            const int t = u ? t1 : t0, b = t / tpb, tt = t - b * tpb, r_t = tt / tqw;
+                            const int ra = (r_t << 4) + (lw & 15);
+                            bq[u] = b;
+                            rt[u] = r_t;
+                            qb[u] = ((tt - r_t * tqw) << 4) + (lw & 15);
+                            rok[u] = (u == 0 || two) && ra < nab;
+                            qok[u] = (u == 0 || two) && qb[u] < nq;
+                            qc[u] = min(qb[u], nq - 1);
+                            const int ab = min(ra, nab - 1);
+                            const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
+                            const double* __restrict__ E = eab + b * 3 * axa;
+                            ex[u] = E + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
+                            ey[u] = E + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
+                            ez[u] = E + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
+                            Rb[u] = Rr0 + b * nh;
+                        }
+                        fj_v4d c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+                        for (int p0 = 0; p0 < np; p0 += 4) {
+                            const int p = p0 + (lw >> 4), pc = min(p, np - 1);
+                            const bool pok = p < np;
+                            const int cp = hc[pc];
+                            const int px = cp & 15, py = (cp >> 4) & 15, pz = cp >> 8;
+                            const unsigned short* __restrict__ pr = pqi + pc * nq;
+                            const double a0 = rok[0] && pok ? ex[0][px] * ey[0][py] * ez[0][pz] : 0.0;
+                            const double b0 = qok[0] && pok ? Rb[0][pr[qc[0]]] : 0.0;
+                            const double a1 = rok[1] && pok ? ex[1][px] * ey[1][py] * ez[1][pz] : 0.0;
+                            const double b1 = qok[1] && pok ? Rb[1][pr[qc[1]]] : 0.0;
+                            c0 = fj_mfma_f64_16x16x4(a0, b0, c0);
+                            c1 = fj_mfma_f64_16x16x4(a1, b1, c1);
+                        }
+    #pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            if (u && !two) continue;
+                            const double uu = scl[bq[u] * FJ_FB_SC + 3];
+                            double* __restrict__ Wb = W + bq[u] * nW;
+    #pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int r = (rt[u] << 4) + (lw >> 4) + 4 * v;
+                                if (r < nab && qb[u] < nq) Wb[r * nq + qb[u]] = uu * (u ? c1[v] : c0[v]);
                             }
-#pragma unroll
-                            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                                for (int jj = 0; jj < 4; ++jj) c[i][jj] += w[i] * e[jj];
                         }
                     }
-#pragma unroll
+                }
+                __syncthreads();
+                // x(ab, cd) = sum_b sum_q W[b][ab][q] (-1)^|q| E^{cd}_q(b) on the
+                // matrix cores: 16 x 16 tiles of (ab, cd), the batch and q one K
+                // dimension, two chains per wavefront; each tile digested once
+                {
+                    const int tcw = (ncd + 15) >> 4, nt = ((nab + 15) >> 4) * tcw;
+                    for (int t0 = wave; t0 < nt; t0 += 2 * nw) {   // uniform over the wavefront
+                        const int t1 = t0 + nw < nt ? t0 + nw : t0;
+                        const bool two = t1 != t0;
+                        int rt[2], cb[2], ox[2], oy[2], oz[2], wr[2];
+                        bool rok[2], cok[2];
+    #pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            const int t = u ? t1 : t0, r_t = t / tcw;
+                            const int ra = (r_t << 4) + (lw & 15);
+                            rt[u] = r_t;
+                            cb[u] = ((t - r_t * tcw) << 4) + (lw & 15);
+                            rok[u] = (u == 0 || two) && ra < nab;
+                            cok[u] = (u == 0 || two) && cb[u] < ncd;
+                            const int cdi = min(cb[u], ncd - 1);
+                            const int cc = hc[hcc + cdi / nd], cdd = hc[hd + cdi % nd];
+                            ox[u] = ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
+                            oy[u] = axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
+                            oz[u] = 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
+                            wr[u] = min(ra, nab - 1) * nq;
+                        }
+                        fj_v4d c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+                        for (int b = 0; b < nbe; ++b) {
+                            const double* __restrict__ Wb = W + b * nW;
+                            const double* __restrict__ E = ecd + b * 3 * axc;
+                            for (int q0 = 0; q0 < nq; q0 += 4) {
+                                const int qq = q0 + (lw >> 4), qc = min(qq, nq - 1);
+                                const bool qok = qq < nq;
+                                const int cq = hc[qc];
+                                const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
+                                const bool neg = (qx + qy + qz) & 1;
+                                const double e0 = E[ox[0] + qx] * E[oy[0] + qy] * E[oz[0] + qz];
+                                const double e1 = E[ox[1] + qx] * E[oy[1] + qy] * E[oz[1] + qz];
+                                const double a0 = rok[0] && qok ? Wb[wr[0] + qc] : 0.0;
+                                const double b0 = cok[0] && qok ? (neg ? -e0 : e0) : 0.0;
+                                const double a1 = rok[1] && qok ? Wb[wr[1] + qc] : 0.0;
+                                const double b1 = cok[1] && qok ? (neg ? -e1 : e1) : 0.0;
+                                c0 = fj_mfma_f64_16x16x4(a0, b0, c0);
+                                c1 = fj_mfma_f64_16x16x4(a1, b1, c1);
+                            }
+                        }
+    #pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            if (u && !two) continue;
+    #pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int r = (rt[u] << 4) + (lw >> 4) + 4 * v;
+                                if (r < nab && cb[u] < ncd) fj_fl_digest(blk, nb, nc, nd, r, cb[u], u ? c1[v] : c0[v]);
+                            }
+                        }
+                    }
+                }
+            } else {
+                // W[b][ab][q] = U U sum_p E^{ab}_p [p + q], lanes over (b, 4 x 4 tile of (ab, q))
+                const int tq = (nq + 3) >> 2, tW = ((nab + 3) >> 2) * tq;
+                for (int j = lane; j < nbe * tW; j += nthr) {
+                    const int b = j / tW, t = j - b * tW;
+                    const int m0 = (t / tq) << 2, n0 = (t - (t / tq) * tq) << 2;
+                    const double* __restrict__ E = eab + b * 3 * axa;
+                    const double* __restrict__ R = Rr0 + b * nh;
+                    const double* ex[4];
+                    const double* ey[4];
+                    const double* ez[4];
+                    int qq[4];
+    #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        if (m0 + i >= nab) continue;
-#pragma unroll
+                        const int ab = min(m0 + i, nab - 1);
+                        const int ca = hc[ha + ab / nb], cb = hc[hb + ab % nb];
+                        ex[i] = E + ((ca & 15) * (lb + 1) + (cb & 15)) * ta;
+                        ey[i] = E + axa + (((ca >> 4) & 15) * (lb + 1) + ((cb >> 4) & 15)) * ta;
+                        ez[i] = E + 2 * axa + ((ca >> 8) * (lb + 1) + (cb >> 8)) * ta;
+                        qq[i] = min(n0 + i, nq - 1);
+                    }
+                    double c[4][4];
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) c[i][jj] = 0.0;
+                    for (int pi = 0; pi < np; ++pi) {
+                        const int cp = hc[pi];
+                        const int px = cp & 15, py = (cp >> 4) & 15, pz = cp >> 8;
+                        const unsigned short* __restrict__ pr = pqi + pi * nq;
+                        double ea[4], rv[4];
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i) ea[i] = ex[i][px] * ey[i][py] * ez[i][pz];
+    #pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) rv[jj] = R[pr[qq[jj]]];
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                            for (int jj = 0; jj < 4; ++jj) c[i][jj] += ea[i] * rv[jj];
+                    }
+                    const double uu = scl[b * FJ_FB_SC + 3];
+                    double* __restrict__ Wb = W + b * nW;
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i)
+    #pragma unroll
                         for (int jj = 0; jj < 4; ++jj)
-                            if (n0 + jj < ncd) fj_fl_digest(blk, nb, nc, nd, m0 + i, n0 + jj, c[i][jj]);
+                            if (m0 + i < nab && n0 + jj < nq) Wb[(m0 + i) * nq + n0 + jj] = uu * c[i][jj];
+                }
+                __syncthreads();
+                // x(ab, cd) = sum_b sum_q W[b][ab][q] (-1)^|q| E^{cd}_q(b), lanes over
+                // (slice, 4 x 4 tile of (ab, cd)); slice s sums b = s, s + ns, ... and
+                // digests its share once
+                {
+                    const int tcd = (ncd + 3) >> 2, tX = ((nab + 3) >> 2) * tcd;
+                    const int ns = max(1, min(nbe, nthr / tX));
+                    for (int j = lane; j < ns * tX; j += nthr) {
+                        const int sl = j / tX, t = j - sl * tX;
+                        const int m0 = (t / tcd) << 2, n0 = (t - (t / tcd) * tcd) << 2;
+                        int ox[4], oy[4], oz[4], wr[4];
+    #pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) {
+                            const int cdi = min(n0 + jj, ncd - 1);
+                            const int cc = hc[hcc + cdi / nd], cdd = hc[hd + cdi % nd];
+                            ox[jj] = ((cc & 15) * (ld + 1) + (cdd & 15)) * tc;
+                            oy[jj] = axc + (((cc >> 4) & 15) * (ld + 1) + ((cdd >> 4) & 15)) * tc;
+                            oz[jj] = 2 * axc + ((cc >> 8) * (ld + 1) + (cdd >> 8)) * tc;
+                            wr[jj] = min(m0 + jj, nab - 1) * nq;
+                        }
+                        double c[4][4];
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                            for (int jj = 0; jj < 4; ++jj) c[i][jj] = 0.0;
+                        for (int b = sl; b < nbe; b += ns) {
+                            const double* __restrict__ Wb = W + b * nW;
+                            const double* __restrict__ E = ecd + b * 3 * axc;
+                            for (int qi = 0; qi < nq; ++qi) {
+                                const int cq = hc[qi];
+                                const int qx = cq & 15, qy = (cq >> 4) & 15, qz = cq >> 8;
+                                const bool neg = (qx + qy + qz) & 1;
+                                double w[4], e[4];
+    #pragma unroll
+                                for (int i = 0; i < 4; ++i) w[i] = Wb[wr[i] + qi];
+    #pragma unroll
+                                for (int jj = 0; jj < 4; ++jj) {
+                                    const double v = E[ox[jj] + qx] * E[oy[jj] + qy] * E[oz[jj] + qz];
+                                    e[jj] = neg ? -v : v;
+                                }
+    #pragma unroll
+                                for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                                    for (int jj = 0; jj < 4; ++jj) c[i][jj] += w[i] * e[jj];
+                            }
+                        }
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            if (m0 + i >= nab) continue;
+    #pragma unroll
+                            for (int jj = 0; jj < 4; ++jj)
+                                if (n0 + jj < ncd) fj_fl_digest(blk, nb, nc, nd, m0 + i, n0 + jj, c[i][jj]);
+                        }
                     }
                 }
             }

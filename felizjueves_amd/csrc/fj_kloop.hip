@@ -85,7 +85,7 @@ int fj_floop_launch(hipStream_t s, int la, int lb, int lc, int ld, int threads, 
     a.n_items = (int)(grid.x * grid.y);
     // the batched kernel where B >= 2 primitive quartets fit its LDS budget,
     // else one primitive quartet at a time
-    const FjFbLds fb = fj_fbloop_plan(la, lb, lc, ld, FJ_FL_BLOCK);
+    const FjFbLds fb = fj_fbloop_plan(la, lb, lc, ld);
     if (fb.B >= 2) {
         hipLaunchKernelGGL(fj_fbloop_k, grid, dim3(FJ_FL_BLOCK), (size_t)fb.bytes, s, la, lb, lc, ld, threads,
                            fb.B, a);

@@ -64,18 +64,22 @@ inline double __shfl_xor(double v, int o, int w) {
 // so every lane of a wavefront must call it together, as on the GPU.
 typedef double fj_v4d __attribute__((vector_size(32)));
 extern std::barrier<>* g_wbar[4];
-extern double g_mfa[256], g_mfb[256];
+extern double g_mfa[2][256], g_mfb[2][256];
+extern thread_local int g_mf_parity;
 inline fj_v4d fj_mfma_f64_16x16x4(double a, double b, fj_v4d c) {
-    const int l = (int)threadIdx.x, w = l >> 6, lw = l & 63, base = w << 6;
-    g_wbar[w]->arrive_and_wait();   // the wavefront's previous step has been read
-    g_mfa[l] = a;
-    g_mfb[l] = b;
+    // two exchange buffers used in turn: a lane passes step k's barrier only
+    // after writing step k, so after reading step k - 1 -- the buffer step
+    // k + 1 writes is free once step k's barrier is passed
+    const int l = (int)threadIdx.x, w = l >> 6, lw = l & 63, base = w << 6, ph = g_mf_parity;
+    g_mf_parity ^= 1;
+    g_mfa[ph][l] = a;
+    g_mfb[ph][l] = b;
     g_wbar[w]->arrive_and_wait();
     const int j = lw & 15;
     for (int v = 0; v < 4; ++v) {
         const int i = (lw >> 4) + 4 * v;
         double s = c[v];
-        for (int k = 0; k < 4; ++k) s += g_mfa[base + i + 16 * k] * g_mfb[base + 16 * k + j];
+        for (int k = 0; k < 4; ++k) s += g_mfa[ph][base + i + 16 * k] * g_mfb[ph][base + 16 * k + j];
         c[v] = s;
     }
     return c;

@@ -14,7 +14,8 @@ thread_local uint3e threadIdx;
 uint3e blockIdx, blockDim, gridDim;
 std::barrier<>* g_bar;
 std::barrier<>* g_wbar[4];
-double g_mfa[256], g_mfb[256];
+double g_mfa[2][256], g_mfb[2][256];
+thread_local int g_mf_parity;
 double g_shfl[256];
 double* g_dyn_lds;
 
@@ -37,6 +38,9 @@ int main(int argc, char** argv) {
     // class. Which launches ran goes to argv[2] + ".ran" (int32 per launch).
     const std::string mode = argc > 4 ? argv[4] : "auto";
     const bool one_only = mode == "one", batched_only = mode == "batched";
+    // argv[5]: the largest batch (default the launcher's), so that a small
+    // basis still has full batches, remainders and several batches per ket
+    const int bmax = argc > 5 ? atoi(argv[5]) : FJ_FB_BMAX;
     const int block = argc > 3 ? atoi(argv[3]) : 64;
     struct L { int c[4]; FjFArgs a; double* lds; int B; int run; };
     std::vector<L> Ls(hdr[0]);
@@ -48,7 +52,7 @@ int main(int argc, char** argv) {
         f.read((char*)&tau, 8);
         for (int i = 0; i < 4; ++i) l.c[i] = h[i];
         // the launch's dynamic LDS at its exact size, so ASan sees an overrun
-        const FjFbLds fb = fj_fbloop_plan(h[0], h[1], h[2], h[3], block);
+        const FjFbLds fb = fj_fbloop_plan(h[0], h[1], h[2], h[3], bmax);
         l.B = one_only ? 0 : (fb.B >= 2 ? fb.B : 0);
         l.run = !batched_only || l.B > 0;
         l.lds = (double*)malloc(l.B ? fb.bytes : fj_floop_lds(h[0], h[1], h[2], h[3]).bytes);
@@ -78,8 +82,11 @@ int main(int argc, char** argv) {
     for (int t = 0; t < threads; ++t)
         ts.emplace_back([&, t] {
             threadIdx = {(unsigned)t, 0, 0};
-            for (auto& l : Ls) {
+            for (size_t li = 0; li < Ls.size(); ++li) {
+                auto& l = Ls[li];
                 if (!l.run) continue;
+                if (t == 0 && getenv("FJ_EMU_PROGRESS"))
+                    fprintf(stderr, "launch %zu class %d%d%d%d B %d\n", li, l.c[0], l.c[1], l.c[2], l.c[3], l.B);
                 const unsigned gx = l.a.bra.n, gy = (l.a.ket.n + span - 1) / span;
                 for (unsigned y = 0; y < gy; ++y)
                     for (unsigned x = 0; x < gx; ++x) {
