@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
 // ---------------------------------------------------------------------------
 // The batched form (fj_fbloop_k): B primitive quartets of one ket per pass.
 
-#define FJ_FB_BUDGET (64 * 1024)   // LDS bytes per workgroup
+#define FJ_FB_BUDGET (52 * 1024)   // LDS bytes per workgroup: three fit a CU's 160 KB
 #define FJ_FB_BMAX 64              // primitive quartets per batch
 #define FJ_FB_SC 4                 // per primitive quartet: PQ (3), U_ab U_cd
 #define FJ_FB_MFMA_ROWS 8          // bra functions from which the contractions use the
@@ -491,7 +491,9 @@ __host__ __device__ inline FjFbLds fj_fbloop_plan(int la, int lb, int lc, int ld
     return fj_fbloop_layout(la, lb, lc, ld, B);
 }
 
-__global__ void __launch_bounds__(256) fj_fbloop_k(int la, int lb, int lc, int ld, int span, int B,
+// Three wavefronts per SIMD asked for (168 VGPRs, 48 bytes of scratch per
+// lane): with the 52 KB budget three workgroups share a CU.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) fj_fbloop_k(int la, int lb, int lc, int ld, int span, int B,
                                                    FjFArgs a) {
     FJ_DYN_SHARED(fj_fbloop_lds_block);
     const int lane = (int)threadIdx.x, nthr = (int)blockDim.x;
