@@ -8,8 +8,10 @@
 //
 // The straight-line K kernels (gen_mmd.emit_k) grow as L^6 in source; this
 // one is a single kernel whose loops read small index tables. One workgroup
-// of two wavefronts (FJ_KL_BLOCK lanes) per bra record; the kets of a bra
-// are taken one at a time by the whole workgroup, cooperating through LDS:
+// of two wavefronts (FJ_KL_BLOCK lanes) per bra record and slice of the
+// kets (gridDim.y slices, so that a launch of few bras fills the chip); the
+// kets of a slice are taken one at a time by the whole workgroup,
+// cooperating through LDS:
 //
 //   per-axis E tables   E^{ij}_t of the bra (once) and of the ket, one lane
 //                       per axis (upstream mmd/hermite.py:45-79)
@@ -32,6 +34,7 @@
 #define FJ_KL_NC 10                        // Cartesians of an f shell
 #define FJ_KL_AX 112                       // one axis's E table: 4 x 4 x 7
 #define FJ_KL_BLOCK 128                    // lanes per workgroup of the K kernel
+#define FJ_KL_WORKGROUPS 8192              // the K launch's bras x ket slices, at most
 
 // Hermite functions with |t| < n.
 __device__ __forceinline__ int fj_herm_base(int n) { return n * (n + 1) * (n + 2) / 6; }
@@ -277,7 +280,10 @@ __global__ void __launch_bounds__(256) fj_kloop_k(
     const int dcb = dc[bra];
     __syncthreads();
 
-    for (int k = 0; k < n_ket; ++k) {
+    // the kets: the y-th of gridDim.y equal slices (one slice: all of them)
+    const int ksub = (n_ket + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int k_lo = (int)blockIdx.y * ksub, k_hi = min(n_ket, k_lo + ksub);
+    for (int k = k_lo; k < k_hi; ++k) {
         if (ip * Iq[k] * maxD < tau) break;   // kets sorted by Iq: the rest are smaller
         const int blk = dcb + dd[k];
         if (blk < 0 || blk >= n_dc) continue;   // the host never names such a block
@@ -372,7 +378,12 @@ __global__ void __launch_bounds__(256) fj_kloop_k(
         const int a = j / nb, b = j - a * nb;
         double s = 0.0;
         for (int c = 0; c < nc; ++c) s += kac[(a * nc + c) * nb + b];
-        K[(size_t)bra * (na * nb) + j] = sa * s;
+        // one slice writes its bra's block; several add their shares into
+        // the block the launcher cleared
+        if (gridDim.y == 1)
+            K[(size_t)bra * (na * nb) + j] = sa * s;
+        else
+            unsafeAtomicAdd(K + (size_t)bra * (na * nb) + j, sa * s);
     }
 }
 

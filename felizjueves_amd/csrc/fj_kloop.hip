@@ -7,6 +7,12 @@
 #include "fj_kloop.h"
 #include "fj_floop.h"
 
+// x[0 .. n) = 0, grid-stride (vector stores).
+__global__ void fj_kl_zero_k(double* __restrict__ x, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
 int fj_kloop_launch(hipStream_t s, int la, int lc, int lb, int ld, int threads, int n_bra, int n_ket,
                     int n_dc, double tau, double maxD, const double* P, const double* zeta_ac,
                     const double* XPA, const double* XPC, const double* s_ac, const double* Ip,
@@ -17,15 +23,24 @@ int fj_kloop_launch(hipStream_t s, int la, int lc, int lb, int ld, int threads, 
         ld < 0 || ld > FJ_KL_LMAX)
         return FJ_ERR_ARG;
     // threads is checked as for the straight-line kernels; this kernel runs
-    // one wavefront per workgroup whatever it asks
+    // FJ_KL_BLOCK lanes per workgroup whatever it asks
     if (threads <= 0) threads = 256;
     if (threads % 64 || threads > 256) return FJ_ERR_THREADS;
     if (n_bra < 0 || n_ket < 0 || n_dc < 0) return FJ_ERR_SHAPE;
     if (n_bra == 0) return 0;
+    // the kets cut into `split` slices, one workgroup per (bra, slice); with
+    // more than one the slices add into K, cleared first by a fill kernel
+    // (no memset node, so the launch can be captured in a graph)
+    const int split = std::max(1, std::min(n_ket, FJ_KL_WORKGROUPS / n_bra));
+    if (split > 1) {
+        const long long n = (long long)n_bra * ((la + 1) * (la + 2) / 2) * ((lb + 1) * (lb + 2) / 2);
+        const int blocks = (int)std::min<long long>((n + 255) / 256, 1024);
+        hipLaunchKernelGGL(fj_kl_zero_k, dim3((unsigned)blocks), dim3(256), 0, s, K, n);
+    }
     const FjKlLds lay = fj_kloop_lds(la, lc, lb, ld);
-    hipLaunchKernelGGL(fj_kloop_k, dim3((unsigned)n_bra), dim3(FJ_KL_BLOCK), (size_t)lay.bytes, s, la, lc, lb, ld, n_bra, n_ket,
-                       n_dc, tau, maxD, P, zeta_ac, XPA, XPC, s_ac, Ip, dc, Q, eta, XQB, XQD, s_bd, Iq,
-                       dd, Dcd, K);
+    hipLaunchKernelGGL(fj_kloop_k, dim3((unsigned)n_bra, (unsigned)split), dim3(FJ_KL_BLOCK), (size_t)lay.bytes, s,
+                       la, lc, lb, ld, n_bra, n_ket, n_dc, tau, maxD, P, zeta_ac, XPA, XPC, s_ac, Ip, dc, Q,
+                       eta, XQB, XQD, s_bd, Iq, dd, Dcd, K);
     return (int)hipGetLastError();
 }
 

@@ -48,6 +48,13 @@ int main(int argc, char** argv) {
         L.lds = (double*)malloc(fj_kloop_lds(L.h[0], L.h[1], L.h[2], L.h[3]).bytes);
     }
     const int threads = argc > 3 ? atoi(argv[3]) : 64;
+    // argv[4]: ket slices per bra (gridDim.y; default 1); with more than
+    // one the slices add into K, cleared first, as the launcher does
+    const int split = argc > 4 ? atoi(argv[4]) : 1;
+    for (auto& L : Ls) {
+        const int na = (L.h[0] + 1) * (L.h[0] + 2) / 2, nb = (L.h[2] + 1) * (L.h[2] + 2) / 2;
+        for (size_t i = 0; i < (size_t)L.h[4] * na * nb; ++i) L.K[i] = 0.0;
+    }
     std::barrier<> bar(threads);
     g_bar = &bar;
     blockDim = {(unsigned)threads, 1, 1};
@@ -56,18 +63,19 @@ int main(int argc, char** argv) {
         ts.emplace_back([&, t] {
             threadIdx = {(unsigned)t, 0, 0};
             for (auto& L : Ls)
-                for (int b = 0; b < L.h[4]; ++b) {
-                    if (t == 0) {
-                        blockIdx = {(unsigned)b, 0, 0};
-                        gridDim = {(unsigned)L.h[4], 1, 1};
-                        g_dyn_lds = L.lds;
+                for (int y = 0; y < split; ++y)
+                    for (int b = 0; b < L.h[4]; ++b) {
+                        if (t == 0) {
+                            blockIdx = {(unsigned)b, (unsigned)y, 0};
+                            gridDim = {(unsigned)L.h[4], (unsigned)split, 1};
+                            g_dyn_lds = L.lds;
+                        }
+                        bar.arrive_and_wait();
+                        fj_kloop_k(L.h[0], L.h[1], L.h[2], L.h[3], L.h[4], L.h[5], L.h[6], L.tau, L.maxD,
+                                   L.P, L.z, L.XA, L.XC, L.s, L.Ip, L.dc, L.Q, L.eta, L.XB, L.XD, L.sb,
+                                   L.Iq, L.dd, L.Dcd, L.K);
+                        bar.arrive_and_wait();
                     }
-                    bar.arrive_and_wait();
-                    fj_kloop_k(L.h[0], L.h[1], L.h[2], L.h[3], L.h[4], L.h[5], L.h[6], L.tau, L.maxD,
-                               L.P, L.z, L.XA, L.XC, L.s, L.Ip, L.dc, L.Q, L.eta, L.XB, L.XD, L.sb,
-                               L.Iq, L.dd, L.Dcd, L.K);
-                    bar.arrive_and_wait();
-                }
         });
     for (auto& t : ts) t.join();
     std::ofstream o(argv[2], std::ios::binary);

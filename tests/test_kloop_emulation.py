@@ -139,7 +139,9 @@ def test_loop_k_kernel_on_cpu_vs_oracle():
         td = Path(td)
         exe = _build(td, "kloop_emu.cpp")
         _dump(td / "case.bin", basis, D, tables, launches, 0.0, maxD)
-        rows = _run(exe, td, "K.bin", "128")   # two wavefronts, as the library launches it
+        # two wavefronts, as the library launches it; the kets cut into three
+        # slices per bra, adding into the cleared K
+        rows = _run(exe, td, "K.bin", "128", "3")
     K = np.zeros((basis.n_ao, basis.n_ao))
     o = 0
     for la, lc, lb, ld, bs in launches:
