@@ -241,82 +241,53 @@ def split_jk_block(cpu: bool = True, cluster: bool = True) -> dict:
     """The split HGP J and K kernels (hjk.py: one kernel per orbit
     representative, every ordered quartet of the canonical stream, the
     upstream's split design) on the three-atom system and the 10-atom
-    cluster: ms per J and per K build (HIP events around all launches of a
-    build, the launch arguments staged on the device beforehand), the
+    cluster: ms per J and per K build (hjk.HJKBuild: the launch arguments
+    staged on the device beforehand, spread over hjk.STREAMS streams; HIP
+    events around three replays of the build captured as one HIP graph, and
+    around three eager builds), the
     quartets passing the in-kernel screen and their algorithmic FLOPs
     (hgp.cost of the kernel's class: VRR per primitive quartet, HRR and
     digestion per quartet) over that time; on the three-atom system
     max|dJ|, max|dK| against the oracle's ordered sweep, whose time (J and
     K together, host cores) is the CPU baseline."""
     import torch
-    from felizjueves_amd import hjk, native
-    from felizjueves_amd.hgp import cost
+    from felizjueves_amd import hjk
     from felizjueves_amd.systems import config1, config2
     out = {}
     work = [("c1_3atom", config1())] + ([("cluster_10atom", config2(n_atoms=10))] if cluster else [])
     for key, s in work:
         r = {"workload": s.name, "n_ao": s.basis.n_ao}
-        run = hjk.GpuHJKRunner()
         X = {}
         for screen in ("j", "k"):
             t = time.perf_counter()
             plan = hjk.hjk_launches(s.basis, s.D, tau=1e-12, screen=screen)
             t_plan = time.perf_counter() - t
-            D = np.asarray(s.D, dtype=np.float64)
-            # the launch arguments on the device: each (batch, orientation)
-            # side once, D and its block maxima once
-            dD = native.to_dev(np.ascontiguousarray(D.ravel()))
-            dDsh = native.to_dev(np.ascontiguousarray(plan.dmax.ravel()))
-            sides: dict = {}
-
-            def side(bi, sw):
-                if (bi, sw) not in sides:
-                    sides[(bi, sw)] = {k: (native.to_dev(v) if isinstance(v, np.ndarray) else v)
-                                       for k, v in hjk.side_records(s.basis, plan, plan.batches[bi], sw).items()}
-                return sides[(bi, sw)]
-            staged, n_q, fl = [], 0, 0.0
-            for L in plan.launches:
-                (bi, bs), (ki, ks) = hjk.kernel_sides(L)
-                b, k = side(bi, bs), side(ki, ks)
-                a = {"n_bra": len(b["G"]), "n_ket": len(k["G"]), "k_ab": b["k"], "k_cd": k["k"],
-                     "n_sh": s.basis.n_sh, "n_ao": s.basis.n_ao, "perm": L.orientation.perm,
-                     "tau": plan.tau, "bra_prim": b["prim"], "G_ab": b["G"], "bra_sh": b["sh"],
-                     "bra_ao": b["ao"], "ket_prim": k["prim"], "G_cd": k["G"], "ket_sh": k["sh"],
-                     "ket_ao": k["ao"], "Dsh": dDsh, "D": dD}
-                if b["lb"]:
-                    a["AB"] = b["sep"]
-                if k["lb"]:
-                    a["CD"] = k["sep"]
-                staged.append((("hj" if screen == "j" else "hk", L.cls), a["n_bra"], a))
-                bra, ket = plan.batches[L.bra], plan.batches[L.ket]
-                gb = np.array([p_.G for p_ in bra.pairs])
-                gk = np.array([q.G for q in ket.pairs])
-                # the screen's density block: D_cd (J) or D_bd (K) per quartet
-                if screen == "j":
-                    dm = np.array([plan.dmax[q.a, q.b] for q in ket.pairs])[None, :]
-                else:
-                    dm = plan.dmax[np.array([p_.b for p_ in bra.pairs])[:, None],
-                                   np.array([q.b for q in ket.pairs])[None, :]]
-                live = int(np.count_nonzero(gb[:, None] * gk[None, :] * dm >= 1e-12))
-                n_q += live
-                fl += live * cost(L.cls).flops(bra.degree, ket.degree)
-            Xd = run.zeros(s.basis.n_ao)
-            for kern, nb, a in staged:
-                run(kern, nb, (64, 1), a, Xd)
+            b = hjk.HJKBuild(s.basis, s.D, screen=screen, plan=plan)
+            b.run(hjk.STREAMS)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             reps = 3
             e0.record()
             for _ in range(reps):
-                Xd.zero_()
-                for kern, nb, a in staged:
-                    run(kern, nb, (64, 1), a, Xd)
+                b.run(hjk.STREAMS)
+            e1.record()
+            torch.cuda.synchronize()
+            ms_eager = e0.elapsed_time(e1) / reps
+            b.capture(hjk.STREAMS)
+            b.replay()
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                b.replay()
             e1.record()
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1) / reps
-            X[screen] = Xd.cpu().numpy()
-            r[screen] = {"ms_per_build": ms, "launches": len(plan.launches), "quartets_visited":
-                         plan.counts()["quartets"], "quartets_computed": n_q, "plan_s": t_plan,
+            X[screen] = b.X.cpu().numpy()
+            fl = b.flops
+            r[screen] = {"ms_per_build": ms, "ms_per_build_eager": ms_eager,
+                         "launches": len(plan.launches), "quartets_visited":
+                         plan.counts()["quartets"], "quartets_computed": b.quartets_computed,
+                         "plan_s": t_plan,
                          "roofline": {"bound": "fp64_valu", "achieved": fl / ms / 1e9,
                                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                       "frac": fl / ms / 1e9 / FP64_PEAK_TFLOPS, "gflop": fl / 1e9}}

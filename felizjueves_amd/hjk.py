@@ -339,3 +339,127 @@ class GpuHJKRunner:
         import torch
         torch.cuda.synchronize()
         return X.cpu().numpy()
+
+
+STREAMS = 4   # streams of a staged build (hardware queues per process on the MI355X boxes)
+
+
+class HJKBuild:
+    """A split J or K build staged on the device for repeated use: the
+    plan's launch arguments on the device (each (batch, orientation) side
+    once, D and its block maxima once), `run()` issuing every launch into
+    one n_ao x n_ao accumulator on one stream, and `capture()` / `replay()`
+    the same build as one HIP graph (the accumulator cleared by torch's fill
+    kernel, no memset node). The build is for the density it was staged
+    with; the result is walk_hjk's (nothing closed or mirrored).
+
+    `quartets_computed` and `flops` count the quartets that pass the
+    in-kernel screen (G_ab G_cd times the density block, D_cd for J and D_bd
+    for K) and their algorithmic FLOPs (hgp.cost of the kernel's class)."""
+
+    def __init__(self, basis: Basis, D, *, tau: float = 1e-12, screen: str = "j",
+                 threads: int = 64, plan: HPlan | None = None):
+        from . import native
+        from .hgp import cost
+        D = _density(basis, D)
+        self.plan = plan if plan is not None else hjk_launches(basis, D, tau=tau, screen=screen)
+        if self.plan.screen != screen:
+            raise ValueError("the plan is for the other screen")
+        self.basis, self.screen, self.threads = basis, screen, threads
+        self.kind = "hj" if screen == "j" else "hk"
+        self.runner = GpuHJKRunner()
+        plan = self.plan
+        dD = native.to_dev(np.ascontiguousarray(D.ravel()))
+        dDsh = native.to_dev(np.ascontiguousarray(plan.dmax.ravel()))
+        sides: dict = {}
+
+        def side(bi, sw):
+            if (bi, sw) not in sides:
+                sides[(bi, sw)] = {k: (native.to_dev(v) if isinstance(v, np.ndarray) else v)
+                                   for k, v in side_records(basis, plan, plan.batches[bi], sw).items()}
+            return sides[(bi, sw)]
+        self.staged, n_q, fl = [], 0, 0.0
+        for L in plan.launches:
+            (bi, bs), (ki, ks) = kernel_sides(L)
+            b, k = side(bi, bs), side(ki, ks)
+            a = {"n_bra": len(b["G"]), "n_ket": len(k["G"]), "k_ab": b["k"], "k_cd": k["k"],
+                 "n_sh": basis.n_sh, "n_ao": basis.n_ao, "perm": L.orientation.perm,
+                 "tau": plan.tau, "bra_prim": b["prim"], "G_ab": b["G"], "bra_sh": b["sh"],
+                 "bra_ao": b["ao"], "ket_prim": k["prim"], "G_cd": k["G"], "ket_sh": k["sh"],
+                 "ket_ao": k["ao"], "Dsh": dDsh, "D": dD}
+            if b["lb"]:
+                a["AB"] = b["sep"]
+            if k["lb"]:
+                a["CD"] = k["sep"]
+            self.staged.append(((self.kind, L.cls), a))
+            bra, ket = plan.batches[L.bra], plan.batches[L.ket]
+            gb = np.array([p.G for p in bra.pairs])
+            gk = np.array([q.G for q in ket.pairs])
+            if screen == "j":
+                dm = np.array([plan.dmax[q.a, q.b] for q in ket.pairs])[None, :]
+            else:
+                dm = plan.dmax[np.array([p.b for p in bra.pairs])[:, None],
+                               np.array([q.b for q in ket.pairs])[None, :]]
+            live = int(np.count_nonzero(gb[:, None] * gk[None, :] * dm >= plan.tau))
+            n_q += live
+            fl += live * cost(L.cls).flops(bra.degree, ket.degree)
+        self.quartets_computed, self.flops = n_q, fl
+        self.X = self.runner.zeros(basis.n_ao)
+        self.graph = None
+
+    def run(self, n_streams: int = 1):
+        """Every launch of the build into X (cleared first); returns X (on
+        the device). The launches are small and independent (they add into X
+        with atomics), so with n_streams > 1 they are spread over that many
+        side streams, forked from and joined to the current stream: the
+        classes that use the library's one workspace (gen_hip.ws_doubles > 0)
+        all on the first, whose launches must not overlap, the others
+        round-robin over the rest."""
+        import torch
+        self.X.zero_()
+        if n_streams <= 1:
+            for kern, a in self.staged:
+                self.runner(kern, a["n_bra"], (self.threads, 1), a, self.X)
+            return self.X
+        from .gen_hip import ws_doubles
+        if not hasattr(self, "_ws"):
+            self._ws = {kern[1]: ws_doubles(kern[1]) > 0 for kern, _ in self.staged}
+        main = torch.cuda.current_stream()
+        if len(getattr(self, "_streams", ())) != n_streams:
+            self._streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        for st in self._streams:
+            st.wait_stream(main)
+        i = 0
+        for kern, a in self.staged:
+            if self._ws[kern[1]]:
+                st = self._streams[0]
+            else:
+                st = self._streams[1 + i % (n_streams - 1)]
+                i += 1
+            with torch.cuda.stream(st):
+                self.runner(kern, a["n_bra"], (self.threads, 1), a, self.X)
+        for st in self._streams:
+            main.wait_stream(st)
+        return self.X
+
+    def capture(self, n_streams: int = 1):
+        """Capture one build into a HIP graph on a side stream (after an
+        eager build, which also sizes the library workspace the (dd|dd)
+        kernels use)."""
+        import torch
+        self.run(n_streams)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.run(n_streams)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        return self.X
