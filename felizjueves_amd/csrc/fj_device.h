@@ -59,16 +59,18 @@ __device__ __forceinline__ bool fj_item_fp64(const FjFArgs& a, int i) {
 
 // Workgroups of a workspace class: at most FJ_WS_MAX_BLOCKS in one launch;
 // the upstream-ABI entries (no workspace argument) use a library-owned one
-// sized for FJ_LIB_WS_BLOCKS.
+// per stream, sized for FJ_LIB_WS_BLOCKS.
 #define FJ_WS_MAX_BLOCKS 1024
 #define FJ_LIB_WS_BLOCKS 256
 // The fixed device array of fj_f_noalloc, per GPU: 64 MiB (the (dd|dd)
 // kernel's 2,113 doubles per lane fit 62 workgroups of 64 lanes).
 #define FJ_NOALLOC_SCRATCH_BYTES (64LL << 20)
 
-// The library-owned workspace of at least `bytes` on the current device
-// (allocated on first use, grown when asked for more; NULL on failure).
-double* fj_library_workspace(long long bytes);
+// The library-owned workspace of at least `bytes` for launches on stream s
+// of the current device: one per (device, stream), so launches on different
+// streams may overlap, those on one stream run in order (allocated on first
+// use, grown when asked for more; NULL on failure).
+double* fj_library_workspace(hipStream_t s, long long bytes);
 
 // A kernel's dynamic LDS block (its size given at launch).
 #define FJ_DYN_SHARED(name) extern __shared__ double name[]

@@ -8,29 +8,38 @@
 
 extern "C" int fj_version(void) { return FJ_VERSION; }
 
-// One grow-only workspace per device for the upstream-ABI entries of the
-// workspace classes. Callers of those entries on one device must not
-// overlap in time (they share it); the native builder passes its own.
-double* fj_library_workspace(long long bytes) {
+// One grow-only workspace per (device, stream) for the upstream-ABI entries
+// of the workspace classes. Launches on one stream run in order and may
+// share it; launches on different streams get different ones (the native
+// builder passes its own). At most FJ_LIB_WS_SLOTS (device, stream) pairs.
+#define FJ_LIB_WS_SLOTS 64
+double* fj_library_workspace(hipStream_t s, long long bytes) {
     static std::mutex mu;
-    static double* buf[64] = {};
-    static long long have[64] = {};
+    static struct { int dev; hipStream_t stream; double* buf; long long have; } slot[FJ_LIB_WS_SLOTS];
+    static int n_slot = 0;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || bytes <= 0) return nullptr;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || bytes <= 0) return nullptr;
     std::lock_guard<std::mutex> lock(mu);
-    if (have[dev] < bytes) {
-        if (buf[dev]) {
+    int i = 0;
+    while (i < n_slot && !(slot[i].dev == dev && slot[i].stream == s)) ++i;
+    if (i == n_slot) {
+        if (n_slot == FJ_LIB_WS_SLOTS) return nullptr;
+        slot[i] = {dev, s, nullptr, 0};
+        ++n_slot;
+    }
+    if (slot[i].have < bytes) {
+        if (slot[i].buf) {
             if (hipDeviceSynchronize() != hipSuccess) return nullptr;
-            (void)hipFree(buf[dev]);
-            buf[dev] = nullptr;
-            have[dev] = 0;
+            (void)hipFree(slot[i].buf);
+            slot[i].buf = nullptr;
+            slot[i].have = 0;
         }
         void* p = nullptr;
         if (hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;
-        buf[dev] = static_cast<double*>(p);
-        have[dev] = bytes;
+        slot[i].buf = static_cast<double*>(p);
+        slot[i].have = bytes;
     }
-    return buf[dev];
+    return slot[i].buf;
 }
 
 // The 8-element ERI symmetry group as shell-position maps g: the class g.cls

@@ -322,7 +322,7 @@ def emit_class(cls) -> str:
             f"    }}")
         f_ws = (f"    {{\n"
                 f"        const long long need = (long long)8 * {W} * threads * FJ_LIB_WS_BLOCKS;\n"
-                f"        a.ws = fj_library_workspace(need);\n"
+                f"        a.ws = fj_library_workspace(s, need);\n"
                 f"        if (!a.ws) return FJ_ERR_WORKSPACE;\n"
                 f"        a.ws_doubles = need / 8;\n"
                 f"    }}")
@@ -492,8 +492,8 @@ static int fj_launch_{name}(hipStream_t s, int threads, int precision, FjFArgs a
 //@@UPSTREAM@@
 // The upstream-ABI launch with every argument (AB / CD may be NULL when lb /
 // ld is 0) and an optional workspace: ws == NULL takes the library-owned one
-// (allocated on first use), else the persistent grid is cut to the slots ws
-// holds and nothing is allocated.
+// of stream s (allocated on first use), else the persistent grid is cut to
+// the slots ws holds and nothing is allocated.
 int fj_f_any_{name}(hipStream_t s, int threads, int n_bra, int n_ket, int k_ab, int k_cd,
                     int n_sh, int n_ao, int diag, double tau,
                     const double* bra_prim, const double* AB, const double* G_ab, const int* bra_sh,
@@ -615,7 +615,7 @@ def emit_split_class(rep, mask: int) -> str:
         launch_ws = (
             f"    {{\n"
             f"        const long long need = (long long)8 * {W} * threads * FJ_LIB_WS_BLOCKS;\n"
-            f"        a.ws = fj_library_workspace(need);\n"
+            f"        a.ws = fj_library_workspace(s, need);\n"
             f"        if (!a.ws) return FJ_ERR_WORKSPACE;\n"
             f"        a.ws_doubles = need / 8;\n"
             f"        const long long blocks = a.ws_doubles / {W} / threads;\n"

@@ -410,33 +410,23 @@ class HJKBuild:
     def run(self, n_streams: int = 1):
         """Every launch of the build into X (cleared first); returns X (on
         the device). The launches are small and independent (they add into X
-        with atomics), so with n_streams > 1 they are spread over that many
-        side streams, forked from and joined to the current stream: the
-        classes that use the library's one workspace (gen_hip.ws_doubles > 0)
-        all on the first, whose launches must not overlap, the others
-        round-robin over the rest."""
+        with atomics), so with n_streams > 1 they go round-robin to that many
+        side streams, forked from and joined to the current stream (the
+        library keeps one workspace per stream for the classes that need
+        one, so launches on different streams may overlap)."""
         import torch
         self.X.zero_()
         if n_streams <= 1:
             for kern, a in self.staged:
                 self.runner(kern, a["n_bra"], (self.threads, 1), a, self.X)
             return self.X
-        from .gen_hip import ws_doubles
-        if not hasattr(self, "_ws"):
-            self._ws = {kern[1]: ws_doubles(kern[1]) > 0 for kern, _ in self.staged}
         main = torch.cuda.current_stream()
         if len(getattr(self, "_streams", ())) != n_streams:
             self._streams = [torch.cuda.Stream() for _ in range(n_streams)]
         for st in self._streams:
             st.wait_stream(main)
-        i = 0
-        for kern, a in self.staged:
-            if self._ws[kern[1]]:
-                st = self._streams[0]
-            else:
-                st = self._streams[1 + i % (n_streams - 1)]
-                i += 1
-            with torch.cuda.stream(st):
+        for i, (kern, a) in enumerate(self.staged):
+            with torch.cuda.stream(self._streams[i % n_streams]):
                 self.runner(kern, a["n_bra"], (self.threads, 1), a, self.X)
         for st in self._streams:
             main.wait_stream(st)
