@@ -337,7 +337,8 @@ def df_block(cpu: bool = True) -> dict:
     three-center) with max|d| against libcint, and on the 10-atom cluster
     with a synthetic even-tempered s..g aux set (20 shells per atom,
     1,400 aux functions): ms per call of the launches (HIP events; the
-    tables staged on the device beforehand by df.DfIntegrals), integrals
+    tables staged on the device beforehand by df.DfIntegrals; the cluster's
+    (mn|P) classes over four streams), integrals
     per second and the kernel's roofline; the oracle's time on the metric
     fixture is the CPU baseline."""
     import torch
@@ -388,7 +389,7 @@ def df_block(cpu: bool = True) -> dict:
     st = df.DfIntegrals(aux, s.basis)
     Mc, Ec = st.metric(), st.eri3()
     ms_m = timed_call(lambda: st.metric(Mc))
-    ms_3 = timed_call(lambda: st.eri3(Ec))
+    ms_3 = timed_call(lambda: st.eri3(Ec, n_streams=4))
     fl3, flm = df.flops_eri3(s.basis, aux), df.flops_metric(aux)
     out["cluster_10atom"] = {"workload": s.name + " + even-tempered s..g aux", "n_ao": s.basis.n_ao,
                              "n_aux": aux.n_aux, "metric_ms": ms_m, "eri3_ms": ms_3,

@@ -247,8 +247,11 @@ class DfIntegrals:
                                  _p(out), C.c_int(max_rp)), "fj_metric_loop_rp")
         return out
 
-    def eri3(self, out=None, stream=None):
-        """(mn|P) [n_ao, n_ao, n_aux] into `out` (a new device array if None)."""
+    def eri3(self, out=None, stream=None, n_streams: int = 1):
+        """(mn|P) [n_ao, n_ao, n_aux] into `out` (a new device array if None).
+        Each class writes its own cells, so with n_streams > 1 the classes
+        go round-robin (costliest first) to that many side streams, forked
+        from and joined to `stream` (the current stream if None)."""
         import torch
         if self.basis is None:
             raise ValueError("no AO basis staged")
@@ -256,8 +259,17 @@ class DfIntegrals:
         n_ao = self.basis.n_ao
         if out is None:
             out = torch.empty((n_ao, n_ao, aux.n_aux), dtype=torch.float64, device=native.device())
-        h = C.c_void_p(native.stream_handle(stream))
-        for (la, lb, lp), n, max_rp, ar in self.ecls:
+        main = stream if stream is not None else torch.cuda.current_stream()
+        if n_streams > 1:
+            if len(getattr(self, "_streams", ())) != n_streams:
+                self._streams = [torch.cuda.Stream() for _ in range(n_streams)]
+            for st in self._streams:
+                st.wait_stream(main)
+            order = sorted(self.ecls, key=lambda c: -c[1] * c[2] * (c[0][0] + c[0][1] + c[0][2] + 1) ** 4)
+        else:
+            order = self.ecls
+        for i, ((la, lb, lp), n, max_rp, ar) in enumerate(order):
+            h = C.c_void_p(native.stream_handle(self._streams[i % n_streams] if n_streams > 1 else main))
             native.check(self.fe(h, C.c_int(la), C.c_int(lb), C.c_int(lp), C.c_int(self.threads),
                                  C.c_int(n), C.c_int(self.n_bra), C.c_int(self.n_rec), C.c_int(n_ao),
                                  C.c_int(aux.n_aux),
@@ -266,6 +278,9 @@ class DfIntegrals:
                                  *[_p(ar[k]) for k in ("ab_rec0", "ab_rec1", "p_rec0", "p_rec1", "off_a",
                                                        "off_b", "off_p")], _p(out), C.c_int(max_rp)),
                          "fj_eri3_loop_rp")
+        if n_streams > 1:
+            for st in self._streams:
+                main.wait_stream(st)
         return out
 
 

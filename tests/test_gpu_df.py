@@ -34,3 +34,15 @@ def test_eri3_vs_libcint(name):
     print(name, "shape", X.shape, "rel err vs libcint", err, "vs oracle", eo, flush=True)
     assert err < 1e-12 and eo < 1e-12
     assert np.array_equal(X, X.transpose(1, 0, 2))
+
+
+def test_eri3_streams_match_one_stream():
+    """The staged (mn|P) with its classes over four streams equals the
+    one-stream launch bit for bit (every class writes its own cells)."""
+    import torch
+    z, aux, basis = _load("df_prim_spd_f_auxi_3c")
+    st = df.DfIntegrals(aux, basis)
+    one = st.eri3().cpu().numpy()
+    four = st.eri3(n_streams=4)
+    torch.cuda.synchronize()
+    assert np.array_equal(one, four.cpu().numpy())
