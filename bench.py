@@ -179,6 +179,9 @@ def stream_block(s) -> dict:
     return out
 
 
+GRADIENT_STREAMS = 4   # the gradient's class launches over this many streams
+
+
 def gradient_block(steps: int, cpu: bool = True) -> dict:
     """The fused two-electron gradient (gradient.py, fj_gf_jobs: one merged
     launch per class) on the three-atom s,s,p,d system and the 10-atom
@@ -199,13 +202,13 @@ def gradient_block(steps: int, cpu: bool = True) -> dict:
         run = GR.GpuGfRunner(st, len(s.coords), s.basis.n_ao)
         Dd = native.to_dev(s.D)
         stream = torch.cuda.current_stream()
-        run.run(Dd, stream)
+        run.run(Dd, stream, n_streams=GRADIENT_STREAMS)
         torch.cuda.synchronize()
         n = steps if key == "c1_3atom" else 3
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(n):
-            run.run(Dd, stream)
+            run.run(Dd, stream, n_streams=GRADIENT_STREAMS)
         e1.record(stream)
         torch.cuda.synchronize()
         t = e0.elapsed_time(e1) * 1e-3 / n

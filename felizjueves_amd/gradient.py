@@ -268,19 +268,36 @@ class GpuGfRunner:
         self.fj_gf_jobs = lib.fj_gf_jobs
         self.fj_gf_jobs.restype = C.c_int
 
-    def run(self, D_dev, stream=None):
-        """G [n_atom, 3] on the device (float64), zeroed then accumulated."""
+    def run(self, D_dev, stream=None, n_streams: int = 1):
+        """G [n_atom, 3] on the device (float64), zeroed then accumulated.
+        The job-table launches add into G with atomics and are independent,
+        so with n_streams > 1 (job mode) they go round-robin, costliest
+        class first, to that many side streams forked from and joined to
+        `stream` (the current stream if None)."""
+        import torch
         self.G.zero_()
-        h = C.c_void_p(native.stream_handle(stream))
+        main = stream if stream is not None else torch.cuda.current_stream()
+        h = C.c_void_p(native.stream_handle(main))
         if self.mode == "jobs":
             c = self.cat
-            for cls, jd, n_jobs, n_q, max_pq in self.classes:
+            order = self.classes
+            if n_streams > 1:
+                if len(getattr(self, "_streams", ())) != n_streams:
+                    self._streams = [torch.cuda.Stream() for _ in range(n_streams)]
+                for st in self._streams:
+                    st.wait_stream(main)
+                order = sorted(self.classes, key=lambda r: -r[3] * flops_per_prim_quartet(r[0]))
+            for i, (cls, jd, n_jobs, n_q, max_pq) in enumerate(order):
                 th = self.threads.get(cls, default_threads(cls))
+                hs = C.c_void_p(native.stream_handle(self._streams[i % n_streams])) if n_streams > 1 else h
                 native.check(self.fj_gf_jobs(
-                    h, *[C.c_int(x) for x in cls], C.c_int(th), C.c_int(n_jobs), _p(jd),
+                    hs, *[C.c_int(x) for x in cls], C.c_int(th), C.c_int(n_jobs), _p(jd),
                     C.c_longlong(n_q), C.c_int(max_pq), C.c_int(self.n_atom), C.c_int(self.n_ao),
                     _p(c["prim"]), _p(c["sep"]), _p(c["sh"]), _p(c["ao"]), _p(c["at"]), _p(D_dev),
                     _p(self.G)), "fj_gf_jobs")
+            if n_streams > 1:
+                for st in self._streams:
+                    main.wait_stream(st)
             return self.G
         for L in self.s.launches:
             b, k = self.s.sides[L.bra], self.s.sides[L.ket]

@@ -107,3 +107,22 @@ def test_gradient_entry_refusals():
     import torch
     torch.cuda.synchronize()
     assert np.isfinite(G.cpu().numpy()).all()
+
+
+def test_gradient_streams_match_one_stream():
+    """The merged class launches over four streams give the one-stream
+    gradient (atomic sums in another order: within 1e-13 of its scale)."""
+    import torch
+    from felizjueves_amd import native
+    s = config1()
+    at = GR.shell_atoms(s.basis, s.coords)
+    st = GR.gf_launches(s.basis, at)
+    run = GR.GpuGfRunner(st, len(s.coords), s.basis.n_ao)
+    Dd = native.to_dev(s.D)
+    one = run.run(Dd).cpu().numpy()
+    four = run.run(Dd, n_streams=4)
+    torch.cuda.synchronize()
+    four = four.cpu().numpy()
+    d = np.abs(four - one).max()
+    print("four streams vs one", d, flush=True)
+    assert d <= 1e-13 * np.abs(one).max()
