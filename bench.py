@@ -508,7 +508,7 @@ def jk_block(steps, warmup):
             state["J"] = cj.run(Dd)
 
         def run_k():
-            state["K"] = ck.run(Dd)
+            state["K"] = ck.run(Dd, n_streams=4)
         tj = timed(run_j, steps, warmup) / steps
         tk = timed(run_k, steps, warmup) / steps
         G = FockBuilder(s.basis, s.D, tau=1e-12).build(Dd)
@@ -616,7 +616,7 @@ def cof_block(a) -> None:
     G = fb.build(Dd).cpu().numpy()
     cj, ck = jk.CoulombChain(b), jk.ExchangeBuild(b)
     tj = timed(lambda: cj.run(Dd), a.steps, a.warmup)
-    tk = timed(lambda: ck.run(Dd), max(1, a.steps // 4), 1)
+    tk = timed(lambda: ck.run(Dd, n_streams=4), max(1, a.steps // 4), 1)
     ts = timed(lambda: jk.schwarz_matrix(b), a.steps, a.warmup)
     J, K = cj.run(Dd).cpu().numpy(), ck.run(Dd).cpu().numpy()
     torch.cuda.synchronize()

@@ -325,7 +325,12 @@ class ExchangeBuild:
         self.kets = {k: up(t) for k, t in host["kets"].items()}
         self.dblk = {k: up(t) for k, t in host["dblk"].items()}
 
-    def run(self, D):
+    def run(self, D, n_streams: int = 1):
+        """K on the device. Each launch's K kernel writes its bra rows into a
+        scratch and the scatter adds them into K with atomics, so with
+        n_streams > 1 the launches (kernel and scatter together) go
+        round-robin to that many side streams, each with its own scratch,
+        forked from and joined to the current stream."""
         import torch
         b = self.basis
         dev = native.device()
@@ -334,28 +339,45 @@ class ExchangeBuild:
         maxD = float(Dd.abs().max().item())
         Dflat = Dd.view(-1)
         Dcd = {k: Dflat[t["idx"]].contiguous() for k, t in self.dblk.items()}
-        # one scratch for every launch's rows: the K kernel writes each bra
-        # row whole, and the scatter reads it before the next launch on the
-        # same stream writes it again
+        # one scratch per stream for every launch's rows: the K kernel
+        # writes each bra row whole, and the scatter reads it before the
+        # next launch on the same stream writes it again
         n_rows = max(self.bras[(la, lc)]["n"] * n_cart(la) * n_cart(lb)
                      for la, lc, lb, _, _ in self.order)
-        scratch = torch.empty(n_rows, dtype=torch.float64, device=dev)
-        for la, lc, lb, ld, bs in self.order:
+        main = torch.cuda.current_stream()
+        streams = [main] if n_streams <= 1 else [torch.cuda.Stream() for _ in range(n_streams)]
+        scratch = [torch.empty(n_rows, dtype=torch.float64, device=dev) for _ in streams]
+        if n_streams > 1:
+            for st in streams:
+                st.wait_stream(main)
+        for i, (la, lc, lb, ld, bs) in enumerate(self.order):
             t, k = self.bras[(la, lc)], self.kets[(lb, ld, bs)]
-            Kr = scratch[:t["n"] * n_cart(la) * n_cart(lb)]
-            # the K mixed lever: the FP32 tower on the straight-line classes;
-            # the f classes' loop-form kernel stays FP64
-            kn = "k32" if self.mixed and max(la, lc, lb, ld) <= K_STRAIGHT_LMAX else "k"
-            _call(f"fj_{kn}_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
-                  C.c_int(self.dblk[(lc, ld)]["n"]), C.c_double(self.tau),
-                  C.c_double(maxD), _c(t["P"]), _c(t["z"]), _c(t["XA"]), _c(t["XC"]),
-                  _c(t["s"]), _c(t["Ip"]), _c(t[("dc", ld)]), _c(k["Q"]), _c(k["eta"]),
-                  _c(k["XB"]), _c(k["XD"]), _c(k["s"]), _c(k["Iq"]), _c(k["pos"]),
-                  _c(Dcd[(lc, ld)]), _c(Kr))
-            _call("fj_k_scatter", C.c_int(t["n"]), C.c_int(n_cart(la)),
-                  C.c_int(n_cart(lb)), C.c_int(b.n_ao), _c(t["oa"]),
-                  C.c_int(int(b.shell_ao[bs])), _c(Kr), _c(K))
+            with torch.cuda.stream(streams[i % len(streams)]):
+                self._launch(la, lc, lb, ld, bs, t, k, maxD, Dcd,
+                             scratch[i % len(streams)][:t["n"] * n_cart(la) * n_cart(lb)], K)
+        if n_streams > 1:
+            # the current stream waits for every side stream, so the
+            # tensors allocated on it are not reused before they are done
+            for st in streams:
+                main.wait_stream(st)
         return K
+
+    def _launch(self, la, lc, lb, ld, bs, t, k, maxD, Dcd, Kr, K):
+        """One launch (the K kernel into Kr, then its scatter into K) on the
+        current stream."""
+        b = self.basis
+        # the K mixed lever: the FP32 tower on the straight-line classes;
+        # the f classes' loop-form kernel stays FP64
+        kn = "k32" if self.mixed and max(la, lc, lb, ld) <= K_STRAIGHT_LMAX else "k"
+        _call(f"fj_{kn}_{la}{lc}{lb}{ld}", C.c_int(256), C.c_int(t["n"]), C.c_int(k["n"]),
+              C.c_int(self.dblk[(lc, ld)]["n"]), C.c_double(self.tau),
+              C.c_double(maxD), _c(t["P"]), _c(t["z"]), _c(t["XA"]), _c(t["XC"]),
+              _c(t["s"]), _c(t["Ip"]), _c(t[("dc", ld)]), _c(k["Q"]), _c(k["eta"]),
+              _c(k["XB"]), _c(k["XD"]), _c(k["s"]), _c(k["Iq"]), _c(k["pos"]),
+              _c(Dcd[(lc, ld)]), _c(Kr))
+        _call("fj_k_scatter", C.c_int(t["n"]), C.c_int(n_cart(la)),
+              C.c_int(n_cart(lb)), C.c_int(b.n_ao), _c(t["oa"]),
+              C.c_int(int(b.shell_ao[bs])), _c(Kr), _c(K))
 
 
 def fock_jk(basis: Basis, D: np.ndarray, tau: float = 0.0, mixed: bool = False):

@@ -89,12 +89,19 @@ def test_j_chain_vs_libcint(name):
 
 @pytest.mark.parametrize("name", FIX_J)
 def test_k_vs_libcint(name):
+    """K against libcint's vk, its launches on one stream and spread over
+    four (each stream with its own row scratch, the scatters adding into K
+    with atomics: equal within 1e-13 of the scale)."""
     fx = load_fixture(name)
-    K = _host(jk.ExchangeBuild(fx.basis).run(fx.density))
+    eb = jk.ExchangeBuild(fx.basis)
+    K = _host(eb.run(fx.density))
+    K4 = _host(eb.run(fx.density, n_streams=4))
     ref = fx.arrays["vk"]
     err = np.abs(K - ref).max()
-    print(name, "max |K - vk|", err, flush=True)
+    d4 = np.abs(K4 - K).max()
+    print(name, "max |K - vk|", err, "four streams vs one", d4, flush=True)
     assert err < 1e-11 * max(1.0, np.abs(ref).max())
+    assert d4 <= 1e-13 * max(1.0, np.abs(K).max())
 
 
 @pytest.mark.parametrize("name", FIX_J)
