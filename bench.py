@@ -612,8 +612,8 @@ def cof_block(a) -> None:
     ref = fx.arrays["vj"] - 0.5 * fx.arrays["vk"]
     fb = FockBuilder(b, D, tau=1e-12)
     n_f = sum(1 for L in fb.launches if 3 in L.cls)
-    t = timed(lambda: fb.build(Dd), a.steps, a.warmup)
-    G = fb.build(Dd).cpu().numpy()
+    t = timed(lambda: fb.build(Dd, n_streams=a.streams), a.steps, a.warmup)
+    G = fb.build(Dd, n_streams=a.streams).cpu().numpy()
     cj, ck = jk.CoulombChain(b), jk.ExchangeBuild(b)
     tj = timed(lambda: cj.run(Dd), a.steps, a.warmup)
     tk = timed(lambda: ck.run(Dd, n_streams=4), max(1, a.steps // 4), 1)
@@ -625,7 +625,7 @@ def cof_block(a) -> None:
         "value": a.steps / t, "unit": "Fock builds/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * t / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "committed fixture",
-        "config": {"workload": "co-ccpvtz", "n_ao": b.n_ao, "tau": 1e-12},
+        "config": {"workload": "co-ccpvtz", "n_ao": b.n_ao, "tau": 1e-12, "n_streams": a.streams},
         "launches": len(fb.launches), "f_class_launches": n_f,
         "quartets_per_build": fb.shell_quartets(),
         "max_dG_vs_libcint": float(np.abs(G - ref).max()),
@@ -638,8 +638,8 @@ def cof_block(a) -> None:
 
 
 def c4f_block(a) -> None:
-    """C4 molecules with their f shells, each built alone (FP64, one stream,
-    eager): one warm-up and `--steps` timed builds per molecule, beside the
+    """C4 molecules with their f shells, each built alone (FP64, the
+    launches dealt to --streams streams, eager): one warm-up and `--steps` timed builds per molecule, beside the
     same molecule built through d. The whole 64-molecule batch with f is not
     timed: the f classes run on the loop-form kernels, which make the
     11-atom molecule 17 times slower than through d, and the batch holds
@@ -659,7 +659,7 @@ def c4f_block(a) -> None:
             Dd = native.to_dev(s.D)
             log(f"c4f: {s.name} lmax {lmax}: n_ao {s.basis.n_ao}, {len(fb.launches)} launches, "
                 f"plan {plan_s:.1f}s; building")
-            t = timed(lambda: fb.build(Dd), a.steps, 1)
+            t = timed(lambda: fb.build(Dd, n_streams=a.streams), a.steps, 1)
             torch.cuda.synchronize()
             log(f"c4f: {s.name} lmax {lmax}: {1e3 * t / a.steps:.1f} ms per build")
             row.update({"name": s.name, f"n_ao_l{lmax}": s.basis.n_ao,
@@ -674,7 +674,7 @@ def c4f_block(a) -> None:
         "n_gpus": 1, "steps": a.steps, "warmup": 1, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "c4-single-molecules-with-f", "molecules": a.molecules, "tau": 1e-12,
-                   "n_streams": 1, "graph": False},
+                   "n_streams": a.streams, "graph": False},
         "molecules": rows,
     }
     print(json.dumps(out), flush=True)
@@ -682,7 +682,7 @@ def c4f_block(a) -> None:
 
 def c4f_batch_block(a) -> None:
     """Chosen C4 molecules with their f shells in one batch builder (one
-    launch per class pair for all of them, one stream, eager; the f classes
+    launch per class pair for all of them, --streams streams, eager; the f classes
     on the loop-form kernel): batched builds/s, and the first molecule
     against its own single build."""
     import torch
@@ -696,8 +696,8 @@ def c4f_batch_block(a) -> None:
     Ds = [native.to_dev(s.D) for s in systems]
     log(f"c4f-batch: {[s.name for s in systems]}: {len(fb.launches)} launches, "
         f"quartets {fb.shell_quartets():.3e}, plan {plan_s:.1f}s; building")
-    t = timed(lambda: fb.build(Ds), a.steps, 1)
-    Gs = [G.cpu().numpy() for G in fb.build(Ds)]
+    t = timed(lambda: fb.build(Ds, n_streams=a.streams), a.steps, 1)
+    Gs = [G.cpu().numpy() for G in fb.build(Ds, n_streams=a.streams)]
     torch.cuda.synchronize()
     log(f"c4f-batch: {1e3 * t / a.steps:.0f} ms per batched build; checking molecule {a.molecules[0]}")
     s0 = systems[0]
@@ -709,7 +709,7 @@ def c4f_batch_block(a) -> None:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "c4-molecules-with-f-batched", "molecules": a.molecules,
                    "atoms": [s.basis.n_sh // 5 for s in systems], "tau": 1e-12,
-                   "n_streams": 1, "graph": False},
+                   "n_streams": a.streams, "graph": False},
         "n_ao": [s.basis.n_ao for s in systems], "launches": len(fb.launches),
         "f_class_launches": sum(1 for L in fb.launches if 3 in L.cls),
         "quartets_per_build": fb.shell_quartets(), "plan_seconds": plan_s,
@@ -727,6 +727,8 @@ def main() -> None:
     ap.add_argument("--selector", default=str(TUNED / "selector"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the c3 / c5 blocks")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="cof / c4f / c4f-batch: streams the builder's launches are dealt to")
     ap.add_argument("--molecules", type=int, nargs="+", default=[7],
                     help="c4f: which C4 molecules (index i: seed 4 + i)")
     ap.add_argument("--workload", choices=("c2", "c4d", "cof", "c4f", "c4f-batch", "gradient",
