@@ -1275,9 +1275,12 @@ int fj_schwarz_loop(fj_stream_t stream, int la, int lb, int threads, int n_pair,
 /* The K kernel in loop form for any class (la lc | lb ld) through f, the
  * class as arguments, otherwise the fj_k_<la><lc><lb><ld> parameter list
  * (upstream eri/k_exchange/kernel_k.py:314, host k_quartet k.py:395). One
- * wavefront per bra record whatever `threads` asks (checked as for the
- * straight-line kernels); the f classes' entries call it, and it also runs
- * the classes through d for checking one form against the other. */
+ * workgroup of two wavefronts per bra record and slice of its kets
+ * whatever `threads` asks (checked as for the straight-line kernels); with
+ * several slices the launch first clears K's n_bra rows with a fill kernel
+ * and the slices add into them, so K is written either way. The f classes'
+ * entries call it, and it also runs the classes through d for checking one
+ * form against the other. */
 int fj_k_loop(fj_stream_t stream, int la, int lc, int lb, int ld, int threads, int n_bra, int n_ket,
               int n_dc, double tau, double maxD, const double* P, const double* zeta_ac,
               const double* XPA, const double* XPC, const double* s_ac, const double* Ip,
