@@ -433,15 +433,16 @@ class HJKBuild:
         return self.X
 
     def capture(self, n_streams: int = 1):
-        """Capture one build into a HIP graph on a side stream (after an
-        eager build, which also sizes the library workspace the (dd|dd)
-        kernels use)."""
+        """Capture one build into a HIP graph on a side stream, after an
+        eager build on the same streams: the library's workspace is per
+        stream and allocated on first use, which a capture may not do."""
         import torch
-        self.run(n_streams)
-        torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.run(n_streams)
+        torch.cuda.synchronize()
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
                 self.run(n_streams)

@@ -58,21 +58,21 @@ def test_sub_pairs_match_whole_pairs(screen):
     assert d <= 1e-12 * max(1.0, np.abs(X[0]).max())
 
 
-@pytest.mark.parametrize("screen", ["j", "k"])
-def test_staged_build_and_graph_replays(screen):
-    """hjk.HJKBuild on C1 (tau = 0), its launches spread over four streams
-    (the workspace classes on one): the staged eager build equals
-    walk_hjk's one-stream walk, and three graph replays, with eager builds
-    and host synchronizations between them, each equal the eager result
-    and the oracle's every-ordered-quartet sweep."""
+@pytest.mark.parametrize("screen,n_streams", [("j", 4), ("k", 4), ("k", 1)])
+def test_staged_build_and_graph_replays(screen, n_streams):
+    """hjk.HJKBuild on C1 (tau = 0), its launches on one stream or spread
+    over four (the workspace classes with their stream's workspace): the
+    staged eager build equals walk_hjk's one-stream walk, and three graph
+    replays, with eager builds and host synchronizations between them, each
+    equal the eager result and the oracle's every-ordered-quartet sweep."""
     import torch
     s = config1()
     plan = hjk.hjk_launches(s.basis, s.D, tau=0.0, screen=screen)
     w = hjk.walk_hjk(plan, s.basis, s.D, run=hjk.GpuHJKRunner())
     b = hjk.HJKBuild(s.basis, s.D, screen=screen, plan=plan)
-    eager = b.run(4).cpu().numpy()
+    eager = b.run(n_streams).cpu().numpy()
     assert np.abs(eager - w.X).max() <= 1e-13 * max(1.0, np.abs(w.X).max())
-    b.capture(4)
+    b.capture(n_streams)
     J, K = O.fock_ordered(s.basis, s.D)
     ref = J if screen == "j" else K
     for i in range(3):
@@ -83,5 +83,5 @@ def test_staged_build_and_graph_replays(screen):
         print(screen, "replay", i, "vs eager", d_eager, "vs sweep", d_ref, flush=True)
         assert d_eager <= 1e-13 * max(1.0, np.abs(eager).max())
         assert d_ref < 1e-12 * max(1.0, np.abs(ref).max())
-        eager = b.run(4).cpu().numpy()   # eager work between the replays
+        eager = b.run(n_streams).cpu().numpy()   # eager work between the replays
     assert b.quartets_computed > 0 and b.flops > 0
